@@ -1,0 +1,305 @@
+"""ctypes binding of libsccmi.so (include/sccmi.h) and a thin Python mirror of
+the reference's scc.h entry points over one device context.
+
+The product path is the HIP library: if libsccmi.so is missing or no GPU is
+present, constructing an ``SccEngine`` raises; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libsccmi.so")
+
+# scc_status
+SCC_OK = 0
+SCC_ERR_ARG = -1
+SCC_ERR_HIP = -2
+SCC_ERR_OOM = -3
+SCC_ERR_BUDGET = -4
+SCC_ERR_STATE = -5
+SCC_ERR_DEVICE_LOST = -6
+SCC_ERR_CANCELLED = -7
+SCC_ERR_LIMIT = -8
+STATUS_NAMES = {0: "OK", -1: "ARG", -2: "HIP", -3: "OOM", -4: "BUDGET", -5: "STATE",
+                -6: "DEVICE_LOST", -7: "CANCELLED", -8: "LIMIT"}
+
+# scc_pivot_policy
+PIVOT_DEGREE_PRODUCT = 0
+PIVOT_MIN_ID = 1
+PIVOT_LEARNED = 2
+
+# every symbol include/sccmi.h declares
+EXPORTS = [
+    "scc_ctx_create", "scc_ctx_destroy", "scc_last_error", "scc_ctx_bytes", "scc_bytes_needed",
+    "scc_ctx_cancel", "scc_ctx_inject_fault", "scc_graph_upload", "scc_graph_upload_device",
+    "scc_graph_info", "scc_graph_download", "scc_state_reset", "scc_state_set_colors",
+    "scc_trim1", "scc_fb_global", "scc_trim2", "scc_wcc", "scc_fb_partitions",
+    "scc_run_opts_default", "scc_run_method2", "scc_labels_download", "scc_colors_download",
+    "scc_count", "scc_labels_device", "scc_update", "scc_ctx_set_kernel_timing",
+    "scc_ctx_synchronize",
+]
+
+
+class SccError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{STATUS_NAMES.get(code, code)}] {msg}")
+        self.code = code
+
+
+class RunOpts(C.Structure):
+    _fields_ = [("pivot_policy", C.c_int32), ("trim_stop", C.c_int32), ("use_trim2", C.c_int32),
+                ("trim_between", C.c_int32), ("use_wcc", C.c_int32), ("time_phases", C.c_int32)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("trimmed_first", C.c_int64), ("scc_first", C.c_int64),
+                ("pivot_first", C.c_int32), ("pad0", C.c_int32),
+                ("trimmed_second", C.c_int64), ("wcc_parts", C.c_int64),
+                ("fb_rounds", C.c_int64), ("num_sccs", C.c_int64), ("fb_levels", C.c_int64),
+                ("ms_total", C.c_double), ("ms_trim1", C.c_double),
+                ("ms_fb_global", C.c_double), ("ms_trim2", C.c_double), ("ms_wcc", C.c_double),
+                ("ms_fb_parts", C.c_double), ("ms_canon", C.c_double),
+                ("ms_fb_kernels", C.c_double), ("fb_launches", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad0"}
+
+
+class UpdateStats(C.Structure):
+    _fields_ = [("inserted", C.c_int64), ("deleted", C.c_int64), ("affected", C.c_int64),
+                ("num_sccs", C.c_int64), ("ms_total", C.c_double), ("ms_apply", C.c_double),
+                ("ms_region", C.c_double), ("ms_recompute", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def load_library():
+    """Load libsccmi.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    vp, i64, i32 = C.c_void_p, C.c_int64, C.c_int32
+    sig = {
+        "scc_ctx_create": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(vp)]),
+        "scc_ctx_destroy": (C.c_int, [vp]),
+        "scc_last_error": (C.c_char_p, [vp]),
+        "scc_ctx_bytes": (C.c_uint64, [vp]),
+        "scc_bytes_needed": (C.c_uint64, [i32, i64]),
+        "scc_ctx_cancel": (None, [vp]),
+        "scc_ctx_inject_fault": (None, [vp, C.c_int]),
+        "scc_graph_upload": (C.c_int, [vp, i32, i64, vp, vp, vp, vp]),
+        "scc_graph_upload_device": (C.c_int, [vp, i32, i64, vp, vp, vp, vp]),
+        "scc_graph_info": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i64)]),
+        "scc_graph_download": (C.c_int, [vp, vp, vp, vp, vp]),
+        "scc_state_reset": (C.c_int, [vp]),
+        "scc_state_set_colors": (C.c_int, [vp, vp]),
+        "scc_trim1": (C.c_int, [vp, i64, C.POINTER(i64)]),
+        "scc_fb_global": (C.c_int, [vp, C.c_int, i32, C.POINTER(i64), C.POINTER(i32)]),
+        "scc_trim2": (C.c_int, [vp, C.POINTER(i64)]),
+        "scc_wcc": (C.c_int, [vp, C.POINTER(i64)]),
+        "scc_fb_partitions": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(i64)]),
+        "scc_run_opts_default": (None, [C.POINTER(RunOpts)]),
+        "scc_run_method2": (C.c_int, [vp, C.POINTER(RunOpts), C.POINTER(RunStats)]),
+        "scc_labels_download": (C.c_int, [vp, vp, C.c_int]),
+        "scc_colors_download": (C.c_int, [vp, vp]),
+        "scc_count": (C.c_int, [vp, C.POINTER(i64)]),
+        "scc_labels_device": (C.c_int, [vp, C.POINTER(vp)]),
+        "scc_update": (C.c_int, [vp, vp, vp, i64, vp, vp, i64, C.POINTER(UpdateStats)]),
+        "scc_ctx_set_kernel_timing": (C.c_int, [vp, C.c_int]),
+        "scc_ctx_synchronize": (C.c_int, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def default_opts(**kw) -> RunOpts:
+    o = RunOpts()
+    load_library().scc_run_opts_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, int(v))
+    return o
+
+
+class SccEngine:
+    """One device context: upload a graph, run phases or the whole method-2
+    pipeline, download labels.  Method names mirror src/scc.h."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.scc_ctx_create(int(device), int(hbm_budget), C.byref(h))
+        if rc != 0:
+            raise SccError(rc, "scc_ctx_create failed (no HIP device?)")
+        self.h = h
+        self._graph = None
+
+    # -- plumbing
+    def _check(self, rc: int):
+        if rc != 0:
+            raise SccError(rc, self.lib.scc_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.scc_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def bytes(self) -> int:
+        return int(self.lib.scc_ctx_bytes(self.h))
+
+    # -- graph
+    def upload(self, g) -> None:
+        b, a, rb, ra = (_i32(x) for x in (g.begin, g.node_idx, g.r_begin, g.r_node_idx))
+        self._graph = (b, a, rb, ra)
+        self._check(self.lib.scc_graph_upload(self.h, b.shape[0] - 1, a.shape[0], _ptr(b),
+                                              _ptr(a), _ptr(rb), _ptr(ra)))
+        self._graph = None
+
+    def info(self):
+        n = C.c_int32()
+        m = C.c_int64()
+        self._check(self.lib.scc_graph_info(self.h, C.byref(n), C.byref(m)))
+        return n.value, m.value
+
+    def download_graph(self):
+        n, m = self.info()
+        b = np.empty(n + 1, np.int32)
+        a = np.empty(m, np.int32)
+        rb = np.empty(n + 1, np.int32)
+        ra = np.empty(m, np.int32)
+        self._check(self.lib.scc_graph_download(self.h, _ptr(b), _ptr(a), _ptr(rb), _ptr(ra)))
+        from .graphs import Graph
+        return Graph(b, a, rb, ra)
+
+    # -- state (src/scc_main.cc:96-118)
+    def reset(self):
+        self._check(self.lib.scc_state_reset(self.h))
+
+    def set_colors(self, colors):
+        c = _i32(colors)
+        self._check(self.lib.scc_state_set_colors(self.h, _ptr(c)))
+
+    # -- phases, named as in src/scc.h
+    def repeat_global_trim1(self, trim_stop: int = 0) -> int:
+        t = C.c_int64()
+        self._check(self.lib.scc_trim1(self.h, int(trim_stop), C.byref(t)))
+        return t.value
+
+    def do_fw_bw_global_main(self, policy: int = PIVOT_DEGREE_PRODUCT, pivot: int = -1):
+        s = C.c_int64()
+        p = C.c_int32()
+        self._check(self.lib.scc_fb_global(self.h, int(policy), int(pivot), C.byref(s),
+                                           C.byref(p)))
+        return s.value, p.value
+
+    def do_global_trim2_new(self) -> int:
+        t = C.c_int64()
+        self._check(self.lib.scc_trim2(self.h, C.byref(t)))
+        return t.value
+
+    def do_global_wcc(self) -> int:
+        t = C.c_int64()
+        self._check(self.lib.scc_wcc(self.h, C.byref(t)))
+        return t.value
+
+    def start_workers_fw_bw_dfs(self, policy: int = PIVOT_DEGREE_PRODUCT,
+                                trim_between: bool = True) -> int:
+        r = C.c_int64()
+        self._check(self.lib.scc_fb_partitions(self.h, int(policy), int(trim_between),
+                                               C.byref(r)))
+        return r.value
+
+    def run_method2(self, opts: RunOpts | None = None, **kw) -> dict:
+        o = opts if opts is not None else default_opts(**kw)
+        st = RunStats()
+        self._check(self.lib.scc_run_method2(self.h, C.byref(o), C.byref(st)))
+        return st.as_dict()
+
+    def set_kernel_timing(self, on: bool):
+        self._check(self.lib.scc_ctx_set_kernel_timing(self.h, int(on)))
+
+    def synchronize(self):
+        self._check(self.lib.scc_ctx_synchronize(self.h))
+
+    # -- results
+    def labels(self, canonical: bool = True) -> np.ndarray:
+        n, _ = self.info()
+        out = np.empty(n, np.int32)
+        self._check(self.lib.scc_labels_download(self.h, _ptr(out), int(canonical)))
+        return out
+
+    def colors(self) -> np.ndarray:
+        n, _ = self.info()
+        out = np.empty(n, np.int32)
+        self._check(self.lib.scc_colors_download(self.h, _ptr(out)))
+        return out
+
+    def count(self) -> int:
+        c = C.c_int64()
+        self._check(self.lib.scc_count(self.h, C.byref(c)))
+        return c.value
+
+    # -- dynamic updates
+    def update(self, ins_src=None, ins_dst=None, del_src=None, del_dst=None) -> dict:
+        e = np.empty(0, np.int32)
+        isrc = _i32(ins_src if ins_src is not None else e)
+        idst = _i32(ins_dst if ins_dst is not None else e)
+        dsrc = _i32(del_src if del_src is not None else e)
+        ddst = _i32(del_dst if del_dst is not None else e)
+        if isrc.shape != idst.shape or dsrc.shape != ddst.shape:
+            raise ValueError("src/dst length mismatch")
+        st = UpdateStats()
+        self._check(self.lib.scc_update(self.h, _ptr(isrc), _ptr(idst), isrc.shape[0],
+                                        _ptr(dsrc), _ptr(ddst), dsrc.shape[0], C.byref(st)))
+        return st.as_dict()
+
+    # -- fault hooks (host-side failure tests)
+    def cancel(self):
+        self.lib.scc_ctx_cancel(self.h)
+
+    def inject_fault(self, after: int):
+        self.lib.scc_ctx_inject_fault(self.h, int(after))
+
+
+def scc_labels(g, **kw) -> np.ndarray:
+    """Canonical SCC labels of a graph through the GPU engine (one-shot)."""
+    with SccEngine() as e:
+        e.upload(g)
+        e.run_method2(**kw)
+        return e.labels(canonical=True)

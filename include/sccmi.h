@@ -1,0 +1,201 @@
+/*
+ * sccmi.h — C ABI of the MI355X (gfx950) SCC engine, libsccmi.so.
+ *
+ * One context owns one device-resident graph (int32 CSR + reverse CSR), the
+ * per-vertex colour and label arrays, scratch, and one HIP stream.  Every entry
+ * point is called from one host thread per context; several contexts may run
+ * concurrently (each on its own stream).  Calls never exit the process: they
+ * return SCC_OK (0) or a negative scc_status and record a message readable with
+ * scc_last_error().
+ *
+ * Conventions kept from the reference (src/scc.h:11-19, src/scc_color.cc:15-31):
+ *   colour -1 = not yet assigned, -2 = SCC found, any other value = the
+ *   partition a vertex currently belongs to; label = a member id of the
+ *   vertex's SCC (pivot, self, or the smaller vertex of a 2-cycle), -1 before
+ *   it is known.  scc_labels_download(..., canonical=1) returns min-id
+ *   canonical labels, the form every parity check compares.
+ *
+ * Each phase entry point replaces one function of the reference's scc.h and
+ * cites it; scc_run_method2 replaces the whole method-2 sequence.
+ */
+#ifndef SCCMI_H
+#define SCCMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct scc_ctx scc_ctx;
+
+typedef enum scc_status {
+  SCC_OK = 0,
+  SCC_ERR_ARG = -1,         /* bad argument (sizes, null pointers, ids out of range) */
+  SCC_ERR_HIP = -2,         /* a HIP runtime call failed */
+  SCC_ERR_OOM = -3,         /* device allocation failed */
+  SCC_ERR_BUDGET = -4,      /* request exceeds the context's HBM budget */
+  SCC_ERR_STATE = -5,       /* call out of order (e.g. no graph uploaded) */
+  SCC_ERR_DEVICE_LOST = -6, /* device unusable (also forced by the fault hook) */
+  SCC_ERR_CANCELLED = -7,   /* stopped by scc_ctx_cancel or a deadline */
+  SCC_ERR_LIMIT = -8        /* a size limit of the engine is exceeded */
+} scc_status;
+
+/* Pivot policies for forward-backward steps. */
+typedef enum scc_pivot_policy {
+  SCC_PIVOT_DEGREE_PRODUCT = 0, /* max(in-degree x out-degree), ties -> min id (plain heuristic) */
+  SCC_PIVOT_MIN_ID = 1,         /* min id of the partition (src_CUDA/scc_cuda_fb_global.cu:426-440) */
+  SCC_PIVOT_LEARNED = 2         /* scores from the learned scheduler (scc_sched_* below) */
+} scc_pivot_policy;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Create a context on HIP device `device`.  hbm_budget_bytes = 0 means no
+ * budget; otherwise allocations that would exceed it fail with SCC_ERR_BUDGET.
+ * (No reference counterpart: the reference state is process-global,
+ *  src/scc.h:11-13.) */
+int scc_ctx_create(int device, uint64_t hbm_budget_bytes, scc_ctx** out);
+int scc_ctx_destroy(scc_ctx* ctx);
+const char* scc_last_error(const scc_ctx* ctx);
+/* Device bytes currently held by the context. */
+uint64_t scc_ctx_bytes(const scc_ctx* ctx);
+/* Device bytes a graph of n vertices and m edges needs in a context. */
+uint64_t scc_bytes_needed(int32_t n, int64_t m);
+/* Ask a running call on this context to stop at its next phase boundary
+ * (thread-safe).  The call returns SCC_ERR_CANCELLED. */
+void scc_ctx_cancel(scc_ctx* ctx);
+/* Test hook: make the next `after` HIP calls succeed, then report
+ * SCC_ERR_DEVICE_LOST (after < 0 disables).  For host-side failure tests. */
+void scc_ctx_inject_fault(scc_ctx* ctx, int after);
+
+/* ---- graph -------------------------------------------------------------- */
+
+/* Copy an int32 CSR + reverse CSR to the device (rows need not be sorted).
+ * Replaces the H2D load of src_CUDA/scc_cuda_graph.cu:4-25 and takes the arrays
+ * gm_graph exposes (gm_graph/inc/gm_graph.h:154-160).  Validates offsets and
+ * ids; resets the state (colours -1, labels -1). */
+int scc_graph_upload(scc_ctx* ctx, int32_t n, int64_t m, const int32_t* begin,
+                     const int32_t* node_idx, const int32_t* r_begin, const int32_t* r_node_idx);
+/* Same, from device pointers already in HBM (copied device-to-device). */
+int scc_graph_upload_device(scc_ctx* ctx, int32_t n, int64_t m, const int32_t* d_begin,
+                            const int32_t* d_node_idx, const int32_t* d_r_begin,
+                            const int32_t* d_r_node_idx);
+int scc_graph_info(const scc_ctx* ctx, int32_t* n, int64_t* m);
+/* Download the current device CSR (after updates).  Any pointer may be NULL. */
+int scc_graph_download(scc_ctx* ctx, int32_t* begin, int32_t* node_idx, int32_t* r_begin,
+                       int32_t* r_node_idx);
+
+/* Reset colours to -1 and labels to -1 (src/scc_main.cc:96-118 prepare,
+ * src/scc_color.cc:15-31 initialize_color). */
+int scc_state_reset(scc_ctx* ctx);
+/* Set the initial colour of every vertex (partitioned runs: vertices of
+ * different colours are never in one SCC).  Colours must be -1 or in [0, n). */
+int scc_state_set_colors(scc_ctx* ctx, const int32_t* colors);
+
+/* ---- phases (each replaces one function of src/scc.h) -------------------- */
+
+/* repeat_global_trim1 (src/scc.h:43, src/scc_trim1.cc:195-209): sweep until a
+ * sweep trims <= trim_stop vertices (0 = fixpoint).  *trimmed = total. */
+int scc_trim1(scc_ctx* ctx, int64_t trim_stop, int64_t* trimmed);
+/* do_fw_bw_global_main (src/scc.h:81, src/scc_fb_global.cc:133-219): one
+ * forward-backward step from one pivot over all live vertices of colour -1.
+ * pivot_override >= 0 forces the pivot (src/scc_fb_global.cc:140-147). */
+int scc_fb_global(scc_ctx* ctx, int policy, int32_t pivot_override, int64_t* scc_size,
+                  int32_t* pivot_out);
+/* do_global_trim2_new (src/scc.h:57, src/scc_trim2_new.cc:80-101): one pass. */
+int scc_trim2(scc_ctx* ctx, int64_t* removed);
+/* do_global_wcc + create_work_items_from_wcc (src/scc.h:64-65,
+ * src/scc_weak.cc:128-271): recolour live vertices by WCC root. *parts = #roots. */
+int scc_wcc(scc_ctx* ctx, int64_t* parts);
+/* start_workers_fw_bw_dfs (src/scc.h:77, src/scc_fb_seq2.cc:292-351): resolve
+ * every remaining partition by rounds of forward-backward from one pivot per
+ * partition (all partitions at once), with trim1 between rounds when
+ * trim_between != 0.  *rounds = rounds used. */
+int scc_fb_partitions(scc_ctx* ctx, int policy, int trim_between, int64_t* rounds);
+
+/* ---- whole pipeline ------------------------------------------------------ */
+
+typedef struct scc_run_opts {
+  int32_t pivot_policy;    /* scc_pivot_policy for every FB step */
+  int32_t trim_stop;       /* trim1 stop threshold (0 = fixpoint; reference CPU uses 100) */
+  int32_t use_trim2;       /* reference USE_TRIM2 (src/scc_main.cc:46) */
+  int32_t trim_between;    /* trim1 between partition FB rounds */
+  int32_t use_wcc;         /* WCC colouring before the partition rounds */
+  int32_t time_phases;     /* record per-phase HIP-event timings in stats */
+} scc_run_opts;
+
+typedef struct scc_run_stats {
+  int64_t trimmed_first;   /* repeat_global_trim1 count */
+  int64_t scc_first;       /* size of the global FB's SCC */
+  int32_t pivot_first;     /* pivot of the global FB */
+  int32_t pad0;
+  int64_t trimmed_second;  /* trim1 + trim2 + trim1 after the global FB */
+  int64_t wcc_parts;       /* partitions after WCC */
+  int64_t fb_rounds;       /* partition FB rounds */
+  int64_t num_sccs;        /* total SCC count (src/scc_main.cc:147-161) */
+  int64_t fb_levels;       /* BFS levels of the global FB (fw + bw) */
+  double ms_total;         /* host wall time of the run, ms */
+  double ms_trim1, ms_fb_global, ms_trim2, ms_wcc, ms_fb_parts, ms_canon; /* HIP events */
+  double ms_fb_kernels;    /* summed device time of the global FB traversal kernels */
+  int64_t fb_launches;     /* traversal kernel launches in the global FB */
+} scc_run_stats;
+
+/* Default options: degree-product pivot, trim_stop 0, trim2 on, trim between
+ * rounds on, WCC on, no phase timing. */
+void scc_run_opts_default(scc_run_opts* o);
+
+/* do_baseline_global_wcc_fb (src/scc_main.cc:272-351): trim1 -> global FB ->
+ * trim1/trim2/trim1 -> WCC -> partition FB, then canonicalises labels on the
+ * device.  Starts from the current state (call scc_state_reset first for a
+ * from-scratch run).  stats may be NULL. */
+int scc_run_method2(scc_ctx* ctx, const scc_run_opts* opts, scc_run_stats* stats);
+
+/* ---- results -------------------------------------------------------------- */
+
+/* Labels (G_SCC, src/scc.h:12).  canonical != 0 returns min-id labels. */
+int scc_labels_download(scc_ctx* ctx, int32_t* out, int canonical);
+/* Colours (G_Color, src/scc.h:11). */
+int scc_colors_download(scc_ctx* ctx, int32_t* out);
+/* Number of SCCs (vertices whose label is themselves; src/scc_main.cc:147-161). */
+int scc_count(scc_ctx* ctx, int64_t* count);
+/* Device pointers of the canonical label / colour arrays (valid until the next
+ * call on the context), for callers that keep results in HBM. */
+int scc_labels_device(scc_ctx* ctx, const int32_t** d_labels);
+
+/* ---- dynamic updates ------------------------------------------------------ */
+
+typedef struct scc_update_stats {
+  int64_t inserted;        /* edges actually added (absent before) */
+  int64_t deleted;         /* edges actually removed (present before) */
+  int64_t affected;        /* vertices whose SCC was recomputed */
+  int64_t num_sccs;        /* SCC count after the batch */
+  double ms_total;         /* wall time of the call */
+  double ms_apply;         /* CSR rebuild */
+  double ms_region;        /* affected-region search */
+  double ms_recompute;     /* SCC recompute inside the region */
+} scc_update_stats;
+
+/* Apply one batch of edge inserts and deletes to the device graph and bring
+ * the canonical labels up to date, recomputing only SCCs the batch can change.
+ * Replaces the reference's insert path (src/scc_incremental.cc:7-27 with
+ * src/common_main.h:180-314, methods 5/6/7/11) and delete path
+ * (src/scc_decremental.cc:7-24 with src/common_main.h:343-393, methods 8/9/10)
+ * as one exact call.  Requires a completed run (labels known).  Inserting a
+ * present edge or deleting an absent one is a no-op; self-loops are ignored. */
+int scc_update(scc_ctx* ctx, const int32_t* ins_src, const int32_t* ins_dst, int64_t n_ins,
+               const int32_t* del_src, const int32_t* del_dst, int64_t n_del,
+               scc_update_stats* stats);
+
+/* ---- timing helpers (HIP events on the context's stream) ------------------ */
+
+/* Record per-launch durations of the global-FB traversal kernels during the
+ * next runs (enable != 0).  Sum and count are returned in scc_run_stats. */
+int scc_ctx_set_kernel_timing(scc_ctx* ctx, int enable);
+/* Block until the context's stream is idle. */
+int scc_ctx_synchronize(scc_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SCCMI_H */

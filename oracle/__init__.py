@@ -1,0 +1,204 @@
+"""ORACLE — test infrastructure only.
+
+CPU restatement of the reference's OpenMP SCC path (scc_oracle.cc, built into
+liboracle.so by oracle/Makefile) and a launcher for the reference's own binary
+(oracle/_ref/scc_ref, compiled from /root/reference sources by ``make ref``).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
+this package, and only as the checker or the timed CPU baseline.  The product
+path (dynamicgraphs_scc_amd) never imports it.
+
+How each function is pinned (tests/test_oracle.py):
+
+* ``method2`` (src/scc_main.cc:272-351): canonical labels equal the labels the
+  compiled reference binary writes with ``-p`` (golden fixtures in tests/golden,
+  produced by tests/golden/make_golden.py), iterative Tarjan, scipy's
+  strong components, and the planted partition of the planted generator.
+* ``tarjan`` (src/scc_tarjan.cc:99-149): scipy strong components; brute-force
+  mutual reachability on tiny graphs.
+* ``trim1_fixpoint`` (src/scc_trim1.cc:38-110, :195-209): trimmed vertices are
+  singleton SCCs (Tarjan); survivors have a live in- and out-neighbour of their
+  colour; the trimmed set equals an independent Kahn-style degree-peeling.
+* ``trim2_pass`` (src/scc_trim2_new.cc:42-101): every removed pair is a 2-vertex
+  SCC by Tarjan; on graphs made only of isolated 2-cycles every pair is found.
+* ``wcc`` (src/scc_weak.cc:65-196): W[v] equals min of the initial tags over the
+  same-colour forward-reachable set of v (brute-force DFS), and every SCC lies in
+  one class.
+* ``reach`` / ``fb_global`` (src/scc_fb_global.cc:133-219): FW ∩ BW equals the
+  pivot's SCC by Tarjan; reach sets equal scipy breadth-first order sets.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_BIN = os.path.join(HERE, "_ref", "scc_ref")
+
+_lib = None
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+def build(with_ref: bool | None = None) -> None:
+    """Compile liboracle.so (and the reference binary when its sources exist)."""
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+    if with_ref is None:
+        with_ref = os.path.isdir("/root/reference/src")
+    if with_ref:
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} not built (make -C oracle)")
+    lib = C.CDLL(LIB_PATH)
+    lib.oracle_method2.restype = C.c_int
+    lib.oracle_method2.argtypes = [C.c_int32, i32p, i32p, i32p, i32p, i32p, C.c_void_p,
+                                   C.c_int, C.c_int64, C.POINTER(C.c_double), C.c_void_p]
+    lib.oracle_tarjan.restype = C.c_int
+    lib.oracle_tarjan.argtypes = [C.c_int32, i32p, i32p, i32p]
+    lib.oracle_canonicalize.restype = C.c_int
+    lib.oracle_canonicalize.argtypes = [C.c_int32, i32p, i32p]
+    lib.oracle_trim1_fixpoint.restype = C.c_int64
+    lib.oracle_trim1_fixpoint.argtypes = [C.c_int32, i32p, i32p, i32p, i32p, i32p, i32p]
+    lib.oracle_trim2_pass.restype = C.c_int64
+    lib.oracle_trim2_pass.argtypes = [C.c_int32, i32p, i32p, i32p, i32p, i32p, i32p]
+    lib.oracle_wcc.restype = None
+    lib.oracle_wcc.argtypes = [C.c_int32, i32p, i32p, i32p, i32p]
+    lib.oracle_reach.restype = None
+    lib.oracle_reach.argtypes = [C.c_int32, i32p, i32p, i32p, C.c_int32, u8p]
+    lib.oracle_fb_global.restype = C.c_int64
+    lib.oracle_fb_global.argtypes = [C.c_int32, i32p, i32p, i32p, i32p, i32p, i32p, C.c_int32,
+                                     C.c_int32]
+    _lib = lib
+    return lib
+
+
+def _arrs(g):
+    return (np.ascontiguousarray(g.begin, np.int32), np.ascontiguousarray(g.node_idx, np.int32),
+            np.ascontiguousarray(g.r_begin, np.int32),
+            np.ascontiguousarray(g.r_node_idx, np.int32))
+
+
+def method2(g, threads: int = 0, trim_stop: int = 100, want_color=False):
+    """Run the method-2 restatement.  Returns (labels, run_ms, stats[, colors]);
+    labels are the reference's G_SCC (not canonical)."""
+    lib = _load()
+    b, a, rb, ra = _arrs(g)
+    n = g.num_nodes
+    lab = np.empty(n, np.int32)
+    col = np.empty(n, np.int32)
+    st = np.zeros(8, np.int64)
+    ms = C.c_double()
+    rc = lib.oracle_method2(n, b, a, rb, ra, lab, col.ctypes.data, int(threads), int(trim_stop),
+                            C.byref(ms), st.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("oracle method2 left a vertex unlabelled")
+    out = (lab, ms.value, st)
+    return out + (col,) if want_color else out
+
+
+def tarjan(g) -> np.ndarray:
+    lib = _load()
+    b, a, _, _ = _arrs(g)
+    lab = np.empty(g.num_nodes, np.int32)
+    lib.oracle_tarjan(g.num_nodes, b, a, lab)
+    return lab
+
+
+def canonicalize(labels) -> np.ndarray:
+    lib = _load()
+    lab = np.ascontiguousarray(labels, np.int32)
+    out = np.empty_like(lab)
+    if lib.oracle_canonicalize(lab.shape[0], lab, out) != 0:
+        raise ValueError("label out of range")
+    return out
+
+
+def scc_canonical(g, threads: int = 0) -> np.ndarray:
+    """Canonical (min-id) SCC labels via the method-2 restatement."""
+    return canonicalize(method2(g, threads)[0])
+
+
+def trim1_fixpoint(g, color, label):
+    lib = _load()
+    b, a, rb, ra = _arrs(g)
+    return lib.oracle_trim1_fixpoint(g.num_nodes, b, a, rb, ra, color, label)
+
+
+def trim2_pass(g, color, label):
+    lib = _load()
+    b, a, rb, ra = _arrs(g)
+    return lib.oracle_trim2_pass(g.num_nodes, b, a, rb, ra, color, label)
+
+
+def wcc(g, color):
+    lib = _load()
+    b, a, _, _ = _arrs(g)
+    W = np.full(g.num_nodes, -1, np.int32)
+    lib.oracle_wcc(g.num_nodes, b, a, np.ascontiguousarray(color, np.int32), W)
+    return W
+
+
+def reach(g, color, pivot: int, reverse: bool = False) -> np.ndarray:
+    lib = _load()
+    b, a, rb, ra = _arrs(g)
+    out = np.empty(g.num_nodes, np.uint8)
+    off, nb = (rb, ra) if reverse else (b, a)
+    lib.oracle_reach(g.num_nodes, off, nb, np.ascontiguousarray(color, np.int32), int(pivot),
+                     out)
+    return out
+
+
+def fb_global(g, color, label, base: int, pivot: int) -> int:
+    lib = _load()
+    b, a, rb, ra = _arrs(g)
+    return lib.oracle_fb_global(g.num_nodes, b, a, rb, ra, color, label, int(base), int(pivot))
+
+
+# ---- the reference's own binary ------------------------------------------
+
+def have_ref() -> bool:
+    return os.path.exists(REF_BIN)
+
+
+def run_ref(edgelist: str, threads: int = 1, method: int = 2, flag: str = "-p",
+            cwd: str | None = None, timeout: float = 600.0) -> dict:
+    """Run oracle/_ref/scc_ref on a text edge list and parse its stdout contract
+    (src/common_main.h:753-807, src/scc_main.cc:159; scraped the same way as
+    exec_auto.py:10-33).  With -p the labels are read from scc_list.txt
+    (src/scc_post_analyze.cc:31-53) written in ``cwd``."""
+    if not have_ref():
+        raise RuntimeError("reference binary not built (make -C oracle ref)")
+    cwd = cwd or os.getcwd()
+    args = [REF_BIN, edgelist, str(threads), str(method)]
+    if flag:
+        args.append(flag)
+    out = subprocess.run(args, cwd=cwd, capture_output=True, text=True, timeout=timeout,
+                         check=True).stdout
+    res: dict = {"stdout": out}
+    m = re.search(r"running_time\(ms\)=([0-9.eE+-]+)", out)
+    if m:
+        res["runtime_ms"] = float(m.group(1))
+    m = re.search(r"Total # SCCs = (\d+)", out)
+    if m:
+        res["scc_count"] = int(m.group(1))
+    if flag == "-p":
+        dense = np.loadtxt(os.path.join(cwd, "scc_list.txt"), dtype=np.int64, ndmin=2)
+        res["dense"] = dense[:, 1].astype(np.int32)
+    return res
+
+
+def dense_to_canonical(dense: np.ndarray) -> np.ndarray:
+    """scc_list.txt numbers components densely in first-appearance order
+    (src/scc_post_analyze.cc:31-53); map each to its minimum vertex id."""
+    return canonicalize(dense)

@@ -1,0 +1,66 @@
+"""The C ABI libraries load and export every symbol their headers declare.
+No compute calls: these run without a GPU."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+LIB = os.path.join(ROOT, "dynamicgraphs_scc_amd", "lib")
+
+
+def declared(header):
+    with open(os.path.join(ROOT, "include", header)) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    names = re.findall(r"\b([a-z_][a-z0-9_]*)\s*\([^;{]*\)\s*;", text)
+    return sorted(set(n for n in names if n.startswith(("scc_", "sccgen_"))))
+
+
+@pytest.mark.parametrize("header,lib", [("sccmi.h", "libsccmi.so"), ("sccgen.h", "libsccgen.so")])
+def test_library_exports_every_declared_symbol(header, lib):
+    names = declared(header)
+    assert len(names) > 10
+    so = ctypes.CDLL(os.path.join(LIB, lib))
+    missing = [n for n in names if not hasattr(so, n)]
+    assert not missing, f"{lib} lacks {missing}"
+
+
+def test_python_binding_covers_the_header():
+    from dynamicgraphs_scc_amd import engine
+    assert sorted(engine.EXPORTS) == declared("sccmi.h")
+
+
+def test_struct_layouts_match_header():
+    """ctypes mirrors of scc_run_opts / scc_run_stats / scc_update_stats have the
+    sizes the C compiler gives the header's structs."""
+    from dynamicgraphs_scc_amd import engine
+    import subprocess
+    import tempfile
+    src = r"""
+#include <stdio.h>
+#include "sccmi.h"
+int main(){printf("%zu %zu %zu\n", sizeof(scc_run_opts), sizeof(scc_run_stats), sizeof(scc_update_stats));}
+"""
+    with tempfile.TemporaryDirectory() as td:
+        c = os.path.join(td, "s.c")
+        with open(c, "w") as f:
+            f.write(src)
+        exe = os.path.join(td, "s")
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
+    assert [int(x) for x in out] == [ctypes.sizeof(engine.RunOpts), ctypes.sizeof(engine.RunStats),
+                                     ctypes.sizeof(engine.UpdateStats)]
+
+
+def test_no_cpu_fallback_in_product_path():
+    """The product package never imports the oracle."""
+    pkg = os.path.join(ROOT, "dynamicgraphs_scc_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for fn in files:
+            if fn.endswith(".py"):
+                with open(os.path.join(dirpath, fn)) as f:
+                    txt = f.read()
+                assert not re.search(r"^\s*(import|from)\s+oracle\b", txt, re.M), fn

@@ -1,0 +1,131 @@
+"""Host graph library: CSR/reverse-CSR invariants, reference text format,
+seeded generators and the update stream (CPU only)."""
+import os
+
+import numpy as np
+import pytest
+
+from dynamicgraphs_scc_amd import graphs
+from helpers import random_graph, scipy_canonical
+
+
+def check_csr(g, simple: bool):
+    n, m = g.num_nodes, g.num_edges
+    for off, nb in ((g.begin, g.node_idx), (g.r_begin, g.r_node_idx)):
+        assert off[0] == 0 and off[-1] == m
+        assert (np.diff(off) >= 0).all()
+        assert ((nb >= 0) & (nb < n)).all()
+        for v in range(0, n, max(1, n // 500)):
+            row = nb[off[v]:off[v + 1]]
+            assert (np.diff(row) >= 0).all()  # rows sorted (gm_graph.cc:99-100, :276)
+            if simple:
+                assert (np.diff(row) > 0).all()
+    # reverse CSR is the transpose
+    src, dst = g.edges()
+    fwd = np.sort(src.astype(np.int64) * n + dst)
+    rsrc = np.repeat(np.arange(n, dtype=np.int64), np.diff(g.r_begin))
+    rev = np.sort(g.r_node_idx.astype(np.int64) * n + rsrc)
+    assert np.array_equal(fwd, rev)
+    if simple:
+        assert not (src == dst).any()
+        assert len(np.unique(fwd)) == m
+
+
+def test_from_edges_keeps_loops_and_duplicates_like_reference():
+    src = np.array([0, 0, 1, 1, 2, 2, 2], np.int32)
+    dst = np.array([1, 1, 1, 0, 0, 2, 1], np.int32)
+    g = graphs.from_edges(4, src, dst)
+    assert g.num_edges == 7
+    check_csr(g, simple=False)
+    assert list(g.node_idx[g.begin[0]:g.begin[1]]) == [1, 1]
+    d = graphs.from_edges(4, src, dst, dedup=True)
+    assert d.num_edges == 4
+    check_csr(d, simple=True)
+
+
+def test_random_csr_invariants():
+    g = random_graph(5000, 40000, 1)
+    check_csr(g, simple=False)
+    g = random_graph(5000, 40000, 2, self_loops=False, dups=False)
+    check_csr(g, simple=True)
+
+
+def test_edgelist_roundtrip(tmp_path):
+    g = random_graph(300, 1200, 3)
+    p = str(tmp_path / "e.txt")
+    g.write_edgelist(p)
+    with open(p) as f:
+        first = f.readline().split(" ")
+    assert len(first) == 2 and int(first[0]) >= 1  # 1-indexed, single space
+    h = graphs.read_edgelist(p)
+    # N = max id seen (src/common_main.h:76-77)
+    src, dst = g.edges()
+    n_ref = int(max(src.max(), dst.max())) + 1
+    assert h.num_nodes == n_ref
+    assert np.array_equal(h.node_idx, g.node_idx)
+    assert np.array_equal(h.begin, g.begin[:n_ref + 1])
+
+
+def test_rmat_deterministic_and_quadrants():
+    a = graphs.rmat(12, 8, seed=5)
+    b = graphs.rmat(12, 8, seed=5)
+    c = graphs.rmat(12, 8, seed=6)
+    assert np.array_equal(a.node_idx, b.node_idx) and np.array_equal(a.begin, b.begin)
+    assert not np.array_equal(a.node_idx, c.node_idx)
+    check_csr(a, simple=True)
+    # without permutation the top bit of (src, dst) follows (a, b, c, d)
+    raw = graphs.rmat(16, 16, seed=1, permute=False, dedup=False)
+    src, dst = raw.edges()
+    hs, hd = src >> 15, dst >> 15
+    frac = [np.mean((hs == i) & (hd == j)) for i, j in ((0, 0), (0, 1), (1, 0), (1, 1))]
+    assert np.allclose(frac, [0.57, 0.19, 0.19, 0.05], atol=0.005)
+    assert raw.num_edges == 16 * 65536
+
+
+def test_rmat16_sizes_match_survey_shape():
+    g = graphs.rmat(16, 16, seed=1)
+    assert g.num_nodes == 65536
+    assert 950_000 < g.num_edges < 960_000  # ~0.91 kept after dedup
+
+
+def test_rmat_n_keep_rejects_outside_edges():
+    g = graphs.rmat(10, 8, seed=2, n_keep=700)
+    assert g.num_nodes == 700
+    check_csr(g, simple=True)
+
+
+def test_planted_partition_is_the_scc_partition():
+    g, comp = graphs.planted(50000, seed=4, n_medium=1200, with_components=True)
+    check_csr(g, simple=True)
+    assert np.array_equal(scipy_canonical(g), graphs.canonicalize(comp))
+    sizes = np.bincount(comp)
+    assert sizes[0] == 20000  # giant 40 %
+    med = sizes[1:1201]
+    assert med.min() >= 3 and med.max() <= 1000
+
+
+def test_update_stream_semantics():
+    base = graphs.rmat(10, 8, seed=3)
+    us = graphs.UpdateStream(base, seed=5, lo=50, hi=400)
+    src, dst = base.edges()
+    cur = set(zip(src.tolist(), dst.tolist()))
+    for _ in range(6):
+        isrc, idst, dsrc, ddst = us.next()
+        assert 50 <= len(isrc) + len(dsrc) <= 400
+        dels = list(zip(dsrc.tolist(), ddst.tolist()))
+        assert all(e in cur for e in dels) and len(set(dels)) == len(dels)
+        cur -= set(dels)
+        ins = list(zip(isrc.tolist(), idst.tolist()))
+        assert all(e not in cur and e[0] != e[1] for e in ins) and len(set(ins)) == len(ins)
+        cur |= set(ins)
+        assert us.num_edges() == len(cur)
+    g = us.graph()
+    s2, d2 = g.edges()
+    assert set(zip(s2.tolist(), d2.tolist())) == cur
+
+
+def test_canonicalize():
+    lab = np.array([3, 3, 0, 0, 4, 3], np.int32)
+    assert list(graphs.canonicalize(lab)) == [0, 0, 2, 2, 4, 0]
+    with pytest.raises(ValueError):
+        graphs.canonicalize(np.array([5], np.int32))

@@ -1,0 +1,253 @@
+"""Pinning of the CPU oracle (oracle/scc_oracle.cc) before it is trusted.
+
+Each oracle function is checked against something it does not share code or
+formulas with: the reference binary's own labels (golden fixtures), scipy's
+strong components, brute-force transitive closure, the planted generator's
+known partition, and independent re-derivations of each phase's fixpoint.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from dynamicgraphs_scc_amd import graphs
+from helpers import (brute_canonical, golden_cases, load_golden, random_graph,
+                     scipy_canonical)
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_method2_matches_reference_fixture(name):
+    g, canon, count = load_golden(name)
+    lab, _, _ = oracle.method2(g, threads=4)
+    got = oracle.canonicalize(lab)
+    assert np.array_equal(got, canon)
+    # src/scc_main.cc:147-161 counts vertices labelled with themselves
+    assert int((lab == np.arange(g.num_nodes)).sum()) == count
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_tarjan_matches_reference_fixture(name):
+    g, canon, _ = load_golden(name)
+    assert np.array_equal(oracle.canonicalize(oracle.tarjan(g)), canon)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_method2_tarjan_brute_force_tiny(seed):
+    n = 6 + seed * 3
+    g = random_graph(n, 2 * n, seed)
+    want = brute_canonical(g)
+    assert np.array_equal(oracle.canonicalize(oracle.tarjan(g)), want)
+    for threads in (1, 3):
+        assert np.array_equal(oracle.scc_canonical(g, threads), want)
+
+
+@pytest.mark.parametrize("seed,n,ef", [(1, 3000, 1.2), (2, 3000, 2.0), (3, 20000, 3.0),
+                                       (4, 50000, 1.5)])
+def test_method2_vs_scipy_random(seed, n, ef):
+    g = random_graph(n, int(ef * n), seed)
+    assert np.array_equal(oracle.scc_canonical(g, 4), scipy_canonical(g))
+
+
+def test_method2_reference_trim_stop_zero_same_labels():
+    g = graphs.rmat(12, 4, seed=5)
+    a = oracle.canonicalize(oracle.method2(g, 4, trim_stop=100)[0])
+    b = oracle.canonicalize(oracle.method2(g, 4, trim_stop=0)[0])
+    assert np.array_equal(a, b)
+
+
+def test_method2_planted_partition():
+    g, comp = graphs.planted(60000, seed=9, n_medium=1500, with_components=True)
+    assert np.array_equal(oracle.scc_canonical(g, 4), graphs.canonicalize(comp))
+
+
+def test_checker_detects_a_planted_mistake():
+    """A wrong oracle (one label moved) must fail the reference comparison."""
+    g, canon, _ = load_golden("rmat12_s13")
+    lab = oracle.canonicalize(oracle.method2(g, 2)[0])
+    bad = lab.copy()
+    big = np.bincount(lab).argmax()
+    v = int(np.flatnonzero(lab == big)[-1])
+    bad[v] = v  # split one vertex off the giant SCC
+    assert not np.array_equal(bad, canon)
+
+
+def _kahn_peel(g, color):
+    """Independent trim1 fixpoint: queue-based peeling by live same-colour degree."""
+    n = g.num_nodes
+    live = color != -2
+    src, dst = g.edges()
+    ok = (src != dst) & live[src] & live[dst] & (color[src] == color[dst])
+    s, d = src[ok], dst[ok]
+    pairs = np.unique(np.stack([s, d], 1), axis=0) if len(s) else np.zeros((0, 2), np.int32)
+    s, d = pairs[:, 0], pairs[:, 1]
+    outd = np.bincount(s, minlength=n)
+    ind = np.bincount(d, minlength=n)
+    outn = [[] for _ in range(n)]
+    inn = [[] for _ in range(n)]
+    for a, b in zip(s.tolist(), d.tolist()):
+        outn[a].append(b)
+        inn[b].append(a)
+    removed = ~live.copy()
+    stack = [v for v in range(n) if live[v] and (outd[v] == 0 or ind[v] == 0)]
+    for v in stack:
+        removed[v] = True
+    while stack:
+        v = stack.pop()
+        for w in outn[v]:
+            ind[w] -= 1
+            if not removed[w] and ind[w] == 0:
+                removed[w] = True
+                stack.append(w)
+        for w in inn[v]:
+            outd[w] -= 1
+            if not removed[w] and outd[w] == 0:
+                removed[w] = True
+                stack.append(w)
+    return removed & live
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_trim1_fixpoint_equals_degree_peeling(seed):
+    g = random_graph(3000, 4000 + 500 * seed, seed)
+    rng = np.random.default_rng(seed)
+    color = rng.integers(-1, 2, g.num_nodes).astype(np.int32)  # three colour classes
+    want = _kahn_peel(g, color)
+    col = color.copy()
+    lab = np.full(g.num_nodes, -1, np.int32)
+    cnt = oracle.trim1_fixpoint(g, col, lab)
+    got = col == -2
+    assert cnt == want.sum()
+    assert np.array_equal(got, want)
+    assert np.array_equal(lab[got], np.flatnonzero(got))
+    if seed == 0:  # with one colour, trimmed vertices are singleton SCCs
+        col = np.full(g.num_nodes, -1, np.int32)
+        lab = np.full(g.num_nodes, -1, np.int32)
+        oracle.trim1_fixpoint(g, col, lab)
+        canon = scipy_canonical(g)
+        t = np.flatnonzero(col == -2)
+        assert np.array_equal(canon[t], t)
+        assert all((canon == v).sum() == 1 for v in t[:500])
+
+
+def test_trim2_finds_isolated_pairs_and_only_true_pairs():
+    # isolated 2-cycles plus a DAG between them: every 2-cycle is an SCC
+    k = 500
+    rng = np.random.default_rng(3)
+    a = np.arange(0, 2 * k, 2)
+    src = np.concatenate([a, a + 1])
+    dst = np.concatenate([a + 1, a])
+    # extra edges from pair i to pair j > i keep out-uniqueness only for the sink
+    # side, so detection goes through the in-edge test for some pairs
+    i = rng.integers(0, k - 1, 300)
+    j = i + 1 + rng.integers(0, 3, 300)
+    j = np.minimum(j, k - 1)
+    src = np.concatenate([src, 2 * i]).astype(np.int32)
+    dst = np.concatenate([dst, 2 * j + 1]).astype(np.int32)
+    g = graphs.from_edges(2 * k, src, dst)
+    col = np.full(g.num_nodes, -1, np.int32)
+    lab = np.full(g.num_nodes, -1, np.int32)
+    removed = oracle.trim2_pass(g, col, lab)
+    done = col == -2
+    assert removed == done.sum() and removed % 2 == 0
+    canon = scipy_canonical(g)
+    for v in np.flatnonzero(done):
+        members = np.flatnonzero(canon == canon[v])
+        assert len(members) == 2 and lab[v] == members.min()
+    # pairs with no extra out-edge are found through the out-edge test
+    plain = np.setdiff1d(np.arange(k), i)
+    assert done[2 * plain].all()
+
+
+def _wcc_brute(g, color):
+    n = g.num_nodes
+    tag = np.where(np.diff(g.begin) >= 50, np.arange(n) | 0x20000000, np.arange(n))
+    W = np.full(n, -1, np.int64)
+    for v in range(n):
+        if color[v] == -2:
+            continue
+        seen = {v}
+        st = [v]
+        while st:
+            u = st.pop()
+            for k in g.node_idx[g.begin[u]:g.begin[u + 1]]:
+                if k not in seen and color[k] == color[v]:
+                    seen.add(int(k))
+                    st.append(int(k))
+        W[v] = min(tag[list(seen)])
+    return W
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_wcc_fixpoint_is_min_tag_over_forward_reach(seed):
+    g = random_graph(400, 700, seed)
+    # give a few vertices out-degree >= 50 so the tag path is exercised
+    if seed == 2:
+        src, dst = g.edges()
+        hub_src = np.full(60, 7, np.int32)
+        hub_dst = np.arange(100, 160, dtype=np.int32)
+        g = graphs.from_edges(400, np.concatenate([src, hub_src]), np.concatenate([dst, hub_dst]))
+    rng = np.random.default_rng(seed)
+    color = rng.choice(np.array([-2, -1, 5], np.int32), g.num_nodes, p=[0.2, 0.6, 0.2])
+    W = oracle.wcc(g, color)
+    want = _wcc_brute(g, color)
+    live = color != -2
+    assert np.array_equal(W[live], want[live].astype(np.int32))
+    # every SCC of the colour-induced subgraph lies in one class
+    src, dst = g.edges()
+    keep = live[src] & live[dst] & (color[src] == color[dst])
+    canon = scipy_canonical(graphs.from_edges(g.num_nodes, src[keep], dst[keep]))
+    for v in np.flatnonzero(live):
+        same = np.flatnonzero(canon == canon[v])
+        assert (W[same] == W[v]).all()
+
+
+def test_reach_matches_scipy_bfs():
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import breadth_first_order
+    g = random_graph(5000, 9000, 11, self_loops=False, dups=False)
+    col = np.full(g.num_nodes, -1, np.int32)
+    A = csr_matrix((np.ones(g.num_edges), g.node_idx, g.begin), shape=(g.num_nodes,) * 2)
+    for p in (0, 17, 4999):
+        want = np.zeros(g.num_nodes, np.uint8)
+        want[breadth_first_order(A, p, directed=True, return_predecessors=False)] = 1
+        assert np.array_equal(oracle.reach(g, col, p), want)
+        want = np.zeros(g.num_nodes, np.uint8)
+        want[breadth_first_order(A.T.tocsr(), p, directed=True, return_predecessors=False)] = 1
+        assert np.array_equal(oracle.reach(g, col, p, reverse=True), want)
+
+
+def test_fb_global_intersection_is_pivot_scc():
+    g = graphs.rmat(13, 8, seed=21)
+    canon = scipy_canonical(g)
+    big = np.bincount(canon).argmax()
+    for pivot in (int(big), int(np.flatnonzero(canon == big)[-1])):
+        col = np.full(g.num_nodes, -1, np.int32)
+        lab = np.full(g.num_nodes, -1, np.int32)
+        size = oracle.fb_global(g, col, lab, -1, pivot)
+        scc = col == -2
+        assert size == scc.sum() == (canon == big).sum()
+        assert np.array_equal(scc, canon == big)
+        assert (lab[scc] == pivot).all()
+        # forward-only vertices are exactly reach(pivot) minus the SCC
+        fw = oracle.reach(g, np.full(g.num_nodes, -1, np.int32), pivot).astype(bool)
+        bw = oracle.reach(g, np.full(g.num_nodes, -1, np.int32), pivot, reverse=True).astype(bool)
+        assert np.array_equal(fw & bw, scc)
+        fw_only = fw & ~bw
+        if fw_only.any():
+            c = np.unique(col[fw_only])
+            assert len(c) == 1 and c[0] >= 0
+            assert not np.isin(col[~fw_only], c).any()
+
+
+@pytest.mark.skipif(not oracle.have_ref(), reason="reference binary not built here")
+@pytest.mark.parametrize("seed", range(4))
+def test_reference_binary_agrees_with_oracle(tmp_path, seed):
+    g = random_graph(2000, 3000 + 1000 * seed, 100 + seed)
+    path = str(tmp_path / "e.txt")
+    g.write_edgelist(path)
+    res = oracle.run_ref(path, threads=2, method=2, flag="-p", cwd=str(tmp_path))
+    n_ref = res["dense"].shape[0]
+    ref = oracle.dense_to_canonical(res["dense"])
+    # the reference loads N = max id; compare its prefix of the graph
+    src, dst = g.edges()
+    g2 = graphs.from_edges(n_ref, src, dst)
+    assert np.array_equal(oracle.scc_canonical(g2, 2), ref)
