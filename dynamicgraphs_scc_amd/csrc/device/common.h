@@ -20,20 +20,75 @@ constexpr int LIGHT = 16;      // row entries a lane scans alone before the wave
 constexpr int HUGE_ROW = 8192; // rows at least this long are expanded by many blocks
 constexpr int32_t WCC_TAG = 0x20000000, WCC_MASK = 0x1FFFFFFF;  // src/scc_weak.cc:57-62
 
-// Counter slots in the context's device counter array (unsigned long long).
+// Counter slots of DevCtl::ctr (unsigned long long).
 enum Ctr : int {
   C_LIST_OUT = 0,   // compaction output count (live / survivors)
-  C_TRIMMED = 1,    // vertices trimmed by a trim sweep / removed by trim2
-  C_Q_OUT = 2,      // next-frontier count
-  C_CAND_OUT = 3,   // bottom-up candidates left
-  C_SCC = 4,        // SCC members found by classify
-  C_CHANGED = 5,    // WCC iteration changed something
+  C_TRIMMED = 1,    // vertices removed by trim2
+  C_Q_OUT = 2,      // seeds / frontier count outside the level loop
+  C_CAND_OUT = 3,
+  C_SCC = 4,        // SCC members found by classify / extract
+  C_CHANGED = 5,
   C_ROOTS = 6,      // WCC roots
-  C_Q_EDGES = 7,    // sum of traversal-direction degrees of the next frontier
-  C_HUGE = 8,       // frontier vertices deferred to the many-block expansion
-  C_COUNT = 9,      // generic counter (SCC count, ...)
-  C_AUX = 10,
+  C_AUX_T = 7,      // second seed list (dynamic updates)
+  C_AUX_SEL = 8,    // selected-count slot for library selects (int32)
+  C_COUNT = 9,      // SCC count from canonicalisation
+  C_F = 10,         // |F|  forward-reached vertices of a step
+  C_B = 11,         // |B|  backward-reached
+  C_EF = 12,        // sum of out-degrees over F
+  C_EB = 13,        // sum of in-degrees over B
+  C_FB = 14,        // |F n B|
+  C_PIVOT = 15,     // a pivot of the last seeded search (the pivot, for one partition)
   C_NUM = 16
+};
+
+// ---- device-resident control ------------------------------------------------
+// Iterative phases run as a sequence of launches of one kernel.  Each launch
+// reads its input sizes from this block; the last workgroup to finish a launch
+// (counted with an atomic) writes the next launch's inputs and the `done` flag,
+// so the host queues launches in batches and synchronises only to learn that a
+// phase has finished.  Launches after `done` exit at once.
+
+struct LiveCtl {  // the current vertex list: live[sel][0, n)
+  int32_t sel;
+  int32_t pad;
+  unsigned long long n;
+};
+
+struct LoopCtl {  // trim sweeps and WCC iterations
+  int32_t done;
+  int32_t iters;
+  uint32_t blocks_done;
+  int32_t pad;
+  long long stop;               // trim: stop when a sweep trims <= stop
+  unsigned long long acc0, acc1;  // per-launch accumulators
+  unsigned long long total0;      // sum of acc0 over launches
+};
+
+enum : int32_t { MODE_TD = 0, MODE_BU = 1 };
+
+struct LevelCtl {  // one reachability search (all partitions at once)
+  int32_t done;
+  int32_t mode;     // MODE_TD / MODE_BU for the next launch
+  int32_t q_sel;    // frontier q[q_sel] of length nq
+  int32_t c_sel;    // bottom-up candidates: -1 = the live list, else cand[c_sel] (nc)
+  int32_t h_sel;    // rows deferred by the last top-down level: huge[h_sel] (nh)
+  int32_t levels;
+  uint32_t blocks_done;
+  int32_t pad;
+  unsigned long long nq, nh, nc, mf, prev_nq;
+  unsigned long long out_q, out_c, out_h, out_edges;
+  unsigned long long visited;  // vertices admitted over all levels (seeds included)
+  double bu_edges_per_cand;    // Beamer's switch: BU when m_f > n_cand * this
+};
+
+struct DevCtl {
+  LiveCtl live;
+  LoopCtl loop;
+  LevelCtl lv;
+  uint32_t blocks_done;  // (unused; completion counts live in done[])
+  int32_t pad;
+  unsigned long long ctr[C_NUM];
+  uint32_t done[9 * 16];  // last_block2 slots, 64 B apart
 };
 
 // ---- wave64 primitives -----------------------------------------------------
@@ -95,9 +150,12 @@ __device__ __forceinline__ void block_append(bool flag, int32_t value, int32_t* 
   __syncthreads();
 }
 
-// Block-wide sum added to *count with one atomic.  All threads must call.
+// Block-wide sum added to *count with one atomic.  All threads must call.  The
+// atomic's return value is kept, so thread 0 waits for it to be performed
+// before it goes on (the completion count of last_block relies on that).
 __device__ __forceinline__ void block_count(unsigned long long x, unsigned long long* count) {
   __shared__ unsigned long long s_sum[WAVES];
+  __shared__ volatile unsigned long long s_sink;
   x = wave_sum_u64(x);
   if (lane_id() == 0) s_sum[threadIdx.x >> 6] = x;
   __syncthreads();
@@ -105,9 +163,168 @@ __device__ __forceinline__ void block_count(unsigned long long x, unsigned long 
     unsigned long long t = 0;
 #pragma unroll
     for (int i = 0; i < WAVES; i++) t += s_sum[i];
-    if (t) atomicAdd(count, t);
+    if (t) s_sink = atomicAdd(count, t);
   }
   __syncthreads();
+}
+
+// LDS-staged list output.  Lanes push entries into a CAP-entry workgroup buffer
+// (reserved with LDS atomics); the buffer goes to the global list with one
+// global atomic per flush.  A wave whose reservation does not fit writes its
+// entries straight to the global list, and the buffer's valid prefix ends at
+// the first failed reservation.  ID separates buffers of one kernel.
+template <int ID, int CAP = 2048>
+struct LdsOut {
+  __device__ static int32_t* buf() {
+    __shared__ int32_t b[CAP];
+    return b;
+  }
+  __device__ static int* cnt() {
+    __shared__ int c[2];  // [0] reserved, [1] valid limit
+    return c;
+  }
+  __device__ static unsigned long long* gbase() {
+    __shared__ unsigned long long g;
+    return &g;
+  }
+  // all threads, at kernel start
+  __device__ static void init() {
+    if (threadIdx.x == 0) {
+      cnt()[0] = 0;
+      cnt()[1] = CAP;
+    }
+    __syncthreads();
+  }
+  // all threads: copy the buffer out (one global atomic) and empty it
+  __device__ static void flush(int32_t* out, unsigned long long* gcount) {
+    __syncthreads();
+    const int c = min(cnt()[0], cnt()[1]);
+    if (c > 0) {
+      if (threadIdx.x == 0) *gbase() = atomicAdd(gcount, (unsigned long long)c);
+      __syncthreads();
+      const unsigned long long b = *gbase();
+      int32_t* B = buf();
+      for (int i = threadIdx.x; i < c; i += BLOCK) out[b + i] = B[i];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      cnt()[0] = 0;
+      cnt()[1] = CAP;
+    }
+    __syncthreads();
+  }
+  // all threads: flush when fewer than `room` free entries remain
+  __device__ static void maybe_flush(int32_t* out, unsigned long long* gcount, int room) {
+    __syncthreads();
+    const int c = cnt()[0];
+    if (c > CAP - room || cnt()[1] < CAP) flush(out, gcount);
+  }
+  // all lanes of a wave
+  __device__ static void push(bool flag, int32_t v, int32_t* out, unsigned long long* gcount) {
+    const uint64_t m = __ballot(flag);
+    if (m == 0) return;
+    const int k = __popcll(m);
+    int base = 0;
+    if (lane_id() == 0) {
+      base = atomicAdd(&cnt()[0], k);
+      if (base + k > CAP) {
+        atomicMin(&cnt()[1], base);
+        base = -1 - (int)atomicAdd(gcount, (unsigned long long)k);  // global, encoded < 0
+      }
+    }
+    base = __shfl(base, 0);
+    if (flag) {
+      const int r = lanes_below(m);
+      if (base >= 0) buf()[base + r] = v;
+      else out[(unsigned long long)(-1 - base) + r] = v;
+    }
+  }
+};
+
+// Completion count of a launch in two levels: the workgroups of each XCD
+// (blockIdx % 8) count on their own slot, the last of each slot on the top
+// slot, so no address sees more than grid/8 atomics.  Counters reset
+// themselves.  True in every thread of the workgroup that finishes last.
+__device__ __forceinline__ bool last_block2(uint32_t* ctr) {
+  __shared__ int s_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned nb = gridDim.x;
+    const unsigned x = blockIdx.x & 7u;
+    const unsigned in_slot = (nb > x) ? (nb - x + 7u) / 8u : 0u;
+    int last = 0;
+    const unsigned prev = atomicAdd(&ctr[(1 + x) * 16], 1u);
+    if (prev == in_slot - 1) {
+      atomicExch(&ctr[(1 + x) * 16], 0u);
+      const unsigned nslots = nb < 8 ? nb : 8;
+      const unsigned p2 = atomicAdd(&ctr[0], 1u);
+      if (p2 == nslots - 1) {
+        atomicExch(&ctr[0], 0u);
+        last = 1;
+      }
+    }
+    s_last = last;
+  }
+  __syncthreads();
+  return s_last;
+}
+
+// atomicMax of `val` into target[slot] for every lane with act, after the lanes
+// of the wave that share a slot have reduced among themselves (one atomic per
+// distinct slot per wave; skipped when the slot already holds a larger value).
+__device__ __forceinline__ void grouped_max_u64(bool act, int32_t slot, unsigned long long val,
+                                                unsigned long long* target) {
+  const int lane = lane_id();
+  bool pend = act;
+  uint64_t m = __ballot(pend);
+  while (m) {
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    const int32_t S = __shfl(slot, leader);
+    const bool mine = pend && slot == S;
+    unsigned long long x = mine ? val : 0ull;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned long long y = __shfl_xor(x, o);
+      x = y > x ? y : x;
+    }
+    if (lane == leader && x > target[S]) atomicMax(&target[S], x);
+    if (mine) pend = false;
+    m = __ballot(pend);
+  }
+}
+
+// atomicMin counterpart for int32 slots.
+__device__ __forceinline__ void grouped_min_i32(bool act, int64_t slot, int32_t val, int32_t* target) {
+  const int lane = lane_id();
+  bool pend = act;
+  uint64_t m = __ballot(pend);
+  while (m) {
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    const int64_t S = __shfl(slot, leader);
+    const bool mine = pend && slot == S;
+    int32_t x = mine ? val : INT32_MAX;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o));
+    if (lane == leader && x < target[S]) atomicMin(&target[S], x);
+    if (mine) pend = false;
+    m = __ballot(pend);
+  }
+}
+
+// True in every thread of the workgroup that finishes last among the grid's
+// workgroups (after a fence, so it sees every other workgroup's atomics).
+__device__ __forceinline__ bool last_block(uint32_t* blocks_done) {
+  __shared__ int s_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = (atomicAdd(blocks_done, 1u) == gridDim.x - 1);
+  __syncthreads();
+  if (s_last) __threadfence();
+  return s_last;
+}
+
+__device__ __forceinline__ unsigned long long ld_acq(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ bool bit_test(const uint32_t* bm, int32_t v) {
@@ -151,17 +368,24 @@ struct scc_ctx {
   // per-vertex state
   int32_t *color = nullptr, *label = nullptr, *canon = nullptr, *W = nullptr;
   int32_t* live[2] = {nullptr, nullptr};
-  int cur = 0;            // live[cur] is the current live list
-  int64_t n_live = 0;     // its length (host copy)
+  int cur = 0;            // live[cur] is the current live list (host copy of dc->live)
+  int64_t n_live = 0;     // its length (host copy, valid at phase boundaries)
   int32_t* q[2] = {nullptr, nullptr};
   int32_t* cand[2] = {nullptr, nullptr};
-  int32_t* huge = nullptr;          // frontier vertices with huge rows
+  int32_t *huge = nullptr, *huge2 = nullptr;  // deferred huge-row frontier vertices (ping-pong)
   uint32_t *vf = nullptr, *vb = nullptr;  // reach bitmaps
   unsigned long long* pkey = nullptr;     // per-slot pivot key (n+1)
   int32_t* rep = nullptr;                 // per-slot new-partition representatives 3*(n+1)
   int32_t* scratch_i32 = nullptr;         // n+1 ints for canonicalisation / misc
-  unsigned long long* ctr = nullptr;      // device counters
-  unsigned long long* h_ctr = nullptr;    // pinned host mirror
+  sccmi::DevCtl* dc = nullptr;            // device control block
+  sccmi::DevCtl* h_dc = nullptr;          // pinned host mirror
+  unsigned long long* ctr = nullptr;      // == dc->ctr
+  unsigned long long* h_ctr = nullptr;    // == h_dc->ctr
+  int32_t maxdeg = 0;                     // max(out-degree, in-degree), for huge-row chunking
+  int32_t update_policy = 0;              // pivot policy of dynamic recomputes
+  bool colors_split = false;              // colours other than -1/-2 exist (after FB/WCC/set)
+  int32_t lv_hint[2] = {8, 8};            // level launches per search batch (fw, bw)
+  int32_t it_hint = 4;                    // WCC iterations per batch
   bool labels_ready = false;              // canon[] valid for the current graph
   bool ran = false;
 

@@ -3,6 +3,7 @@
 #include <string>
 
 #include "common.h"
+#include "kernels.h"
 
 namespace sccmi {
 
@@ -10,22 +11,29 @@ int set_error(scc_ctx* c, int code, const std::string& msg);
 int hip_check(scc_ctx* c, hipError_t e, const char* what);
 int dev_alloc(scc_ctx* c, void** p, uint64_t bytes, const char* what);
 void dev_free(scc_ctx* c, void* p);
-int sync_ctr(scc_ctx* c);
-int zero_ctr(scc_ctx* c);
+GraphPtrs gp(const scc_ctx* c);
+int pull_ctl(scc_ctx* c);   // control block -> host (syncs the stream)
+int sync_ctr(scc_ctx* c);   // same
+int zero_ctr(scc_ctx* c);   // zero DevCtl::ctr (queued)
+int set_live(scc_ctx* c, int sel, int64_t n);
 int check_cancel(scc_ctx* c);
 int reset_state(scc_ctx* c);
 int alloc_graph(scc_ctx* c, int32_t n, int64_t m_cap);
 
-// phases (operate on the live list live[cur] of length n_live)
+// phases (operate on the live list of the control block)
 int do_trim_compact(scc_ctx* c);  // drop colour -2 entries from the live list
 int do_trim1(scc_ctx* c, int64_t stop, int64_t* trimmed);
 int do_trim2(scc_ctx* c, int64_t* removed);
 int force_pivot(scc_ctx* c, int32_t pivot);
+int run_levels(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed);
 int traverse(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed);
+int traverse_from(scc_ctx* c, bool fwd, bool multi, const int32_t* seeds,
+                  const unsigned long long* d_count, int64_t* levels);
 int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int64_t* scc,
-                int32_t* pivot_out, int64_t* levels, bool timed);
+                int32_t* pivot_out, int64_t* levels, bool timed, int64_t* sets);
 int do_wcc(scc_ctx* c, int64_t* parts);
-int do_fb_parts(scc_ctx* c, int policy, bool trim_between, int64_t* rounds);
+int do_color_round(scc_ctx* c, int64_t* scc);
+int do_fb_parts(scc_ctx* c, int policy, int mode, bool trim_between, int64_t* rounds);
 int do_canon(scc_ctx* c, int64_t* count);
 
 // optional subsystems (dynamic.hip, sched.hip)

@@ -4,12 +4,18 @@
 // Data layout in HBM (per context, n vertices, m edges, all int32):
 //   begin[n+1] adj[m] rbegin[n+1] radj[m]            graph (CSR + reverse CSR)
 //   color[n] label[n] canon[n] W[n]                  per-vertex state
-//   live[2][n] q[2][n] cand[2][n] huge[n]            ping-pong vertex lists
+//   live[2][n] q[2][n] cand[2][n] huge[2][n]         ping-pong vertex lists
 //   vf[n/32] vb[n/32]                                forward / backward reach bitmaps
 //   pkey[n+1] (u64), rep[3(n+1)], scratch[n+1]       per-partition slots
+//   DevCtl                                           device-resident loop control
 // A partition is identified by its colour; colours are -1 (the initial
 // partition, slot n) or the id of a member vertex (slot = that id), so the
 // per-partition arrays need no allocator.
+//
+// Iterative phases (trim sweeps, search levels, WCC iterations) are queued in
+// batches of launches; each launch's last workgroup decides the next step on
+// the device, and the host synchronises once per batch to see whether the
+// phase has finished.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -76,19 +82,32 @@ void dev_free(scc_ctx* c, void* p) {
   }
 }
 
-int sync_ctr(scc_ctx* c) {
-  int rc = hip_check(c, hipMemcpyAsync(c->h_ctr, c->ctr, sizeof(unsigned long long) * C_NUM,
-                                       hipMemcpyDeviceToHost, c->stream),
-                     "counter readback");
+GraphPtrs gp(const scc_ctx* c) { return GraphPtrs{c->begin, c->adj, c->rbegin, c->radj, c->n, c->maxdeg}; }
+
+// Copy the control block to the host and refresh the host copies of the live list.
+int pull_ctl(scc_ctx* c) {
+  int rc = hip_check(c, hipMemcpyAsync(c->h_dc, c->dc, sizeof(DevCtl), hipMemcpyDeviceToHost, c->stream),
+                     "control readback");
   if (rc) return rc;
-  rc = hip_check(c, hipStreamSynchronize(c->stream), "stream sync");
-  if (rc) return rc;
-  return hip_check(c, hipGetLastError(), "kernel launch");
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "stream sync"))) return rc;
+  if ((rc = hip_check(c, hipGetLastError(), "kernel launch"))) return rc;
+  c->cur = c->h_dc->live.sel;
+  c->n_live = (int64_t)c->h_dc->live.n;
+  return SCC_OK;
 }
 
+int sync_ctr(scc_ctx* c) { return pull_ctl(c); }
+
 int zero_ctr(scc_ctx* c) {
-  return hip_check(c, hipMemsetAsync(c->ctr, 0, sizeof(unsigned long long) * C_NUM, c->stream),
-                   "counter reset");
+  launch_ctl_init(c->stream, c->dc, 1, 0, 0, 0, 0.0);
+  return hip_check(c, hipGetLastError(), "counter reset");
+}
+
+int set_live(scc_ctx* c, int sel, int64_t n) {
+  launch_ctl_init(c->stream, c->dc, 2, sel, (unsigned long long)n, 0, 0.0);
+  c->cur = sel;
+  c->n_live = n;
+  return hip_check(c, hipGetLastError(), "live set");
 }
 
 int check_cancel(scc_ctx* c) {
@@ -104,7 +123,7 @@ uint64_t state_bytes(int32_t n) {
   const uint64_t N = (uint64_t)n + 1;
   const uint64_t words = ((uint64_t)n + 31) / 32 + 1;
   return 4 * N * 4            // color label canon W
-         + 7 * N * 4          // live[2] q[2] cand[2] huge
+         + 8 * N * 4          // live[2] q[2] cand[2] huge[2]
          + 2 * words * 4      // vf vb
          + 8 * N + 12 * N     // pkey rep
          + 4 * N;             // scratch
@@ -113,7 +132,7 @@ uint64_t state_bytes(int32_t n) {
 int free_graph(scc_ctx* c) {
   int32_t** arrs[] = {&c->begin, &c->adj, &c->rbegin, &c->radj, &c->color, &c->label, &c->canon,
                       &c->W, &c->live[0], &c->live[1], &c->q[0], &c->q[1], &c->cand[0],
-                      &c->cand[1], &c->huge, &c->rep, &c->scratch_i32};
+                      &c->cand[1], &c->huge, &c->huge2, &c->rep, &c->scratch_i32};
   for (auto p : arrs) {
     dev_free(c, *p);
     *p = nullptr;
@@ -153,6 +172,7 @@ int alloc_graph(scc_ctx* c, int32_t n, int64_t m_cap) {
   ALLOC(c->cand[0], 4 * N);
   ALLOC(c->cand[1], 4 * N);
   ALLOC(c->huge, 4 * N);
+  ALLOC(c->huge2, 4 * N);
   ALLOC(c->vf, 4 * words);
   ALLOC(c->vb, 4 * words);
   ALLOC(c->pkey, 8 * N);
@@ -168,24 +188,22 @@ int reset_state(scc_ctx* c) {
   launch_fill_i32(c->stream, c->color, UNSET, c->n);
   launch_fill_i32(c->stream, c->label, -1, c->n);
   launch_iota(c->stream, c->live[0], c->n);
-  c->cur = 0;
-  c->n_live = c->n;
   c->labels_ready = false;
   c->multi_colors = false;
-  return hip_check(c, hipGetLastError(), "state reset");
+  c->colors_split = false;
+  int rc = zero_ctr(c);
+  if (rc) return rc;
+  return set_live(c, 0, c->n);
 }
 
 // ---- phases -----------------------------------------------------------------------
 
 int do_trim_compact(scc_ctx* c) {
-  int rc;
   if (c->n_live == 0) return SCC_OK;
+  int rc;
   if ((rc = zero_ctr(c))) return rc;
-  launch_compact_live(c->stream, c->live[c->cur], c->n_live, c->color, c->live[c->cur ^ 1], c->ctr);
-  if ((rc = sync_ctr(c))) return rc;
-  c->cur ^= 1;
-  c->n_live = (int64_t)c->h_ctr[C_LIST_OUT];
-  return SCC_OK;
+  launch_compact(c->stream, grid_for(c->n_live), c->dc, c->color, c->live[0], c->live[1]);
+  return pull_ctl(c);
 }
 
 int force_pivot(scc_ctx* c, int32_t pivot) {
@@ -193,157 +211,187 @@ int force_pivot(scc_ctx* c, int32_t pivot) {
   return hip_check(c, hipGetLastError(), "force pivot");
 }
 
+// repeat_global_trim1: sweeps until one trims <= stop.
 int do_trim1(scc_ctx* c, int64_t stop, int64_t* trimmed) {
-  int64_t total = 0;
-  int rc;
-  while (c->n_live > 0) {
-    if ((rc = zero_ctr(c))) return rc;
-    launch_trim1(c->stream, c->begin, c->adj, c->rbegin, c->radj, c->color, c->label,
-                 c->live[c->cur], c->n_live, c->live[c->cur ^ 1], c->ctr);
-    if ((rc = sync_ctr(c))) return rc;
-    const int64_t t = (int64_t)c->h_ctr[C_TRIMMED];
-    c->n_live = (int64_t)c->h_ctr[C_LIST_OUT];
-    c->cur ^= 1;
-    total += t;
-    if (t <= stop) break;
+  if (trimmed) *trimmed = 0;
+  if (c->n_live == 0) return SCC_OK;
+  launch_ctl_init(c->stream, c->dc, 4, 0, 0, (long long)stop, 0.0);
+  const unsigned grid = grid_for(c->n_live);
+  int batch = 3, rc;
+  for (int guard = 0;; guard++) {
+    if ((rc = check_cancel(c))) return rc;
+    for (int i = 0; i < batch; i++)
+      launch_trim1_step(c->stream, grid, c->dc, gp(c), c->color, c->label, c->live[0], c->live[1]);
+    if ((rc = pull_ctl(c))) return rc;
+    if (c->h_dc->loop.done) break;
+    batch = 4;
+    if (guard > (int64_t)c->n + 8) return set_error(c, SCC_ERR_LIMIT, "trim did not converge");
   }
-  if (trimmed) *trimmed = total;
+  if (trimmed) *trimmed = (int64_t)c->h_dc->loop.total0;
   return SCC_OK;
 }
 
 int do_trim2(scc_ctx* c, int64_t* removed) {
+  if (removed) *removed = 0;
+  if (c->n_live == 0) return SCC_OK;
   int rc;
   if ((rc = zero_ctr(c))) return rc;
-  launch_trim2(c->stream, c->begin, c->adj, c->rbegin, c->radj, c->color, c->label,
-               c->live[c->cur], c->n_live, c->ctr);
-  if ((rc = sync_ctr(c))) return rc;
-  if (removed) *removed = (int64_t)c->h_ctr[C_TRIMMED];
+  launch_trim2(c->stream, grid_for(c->n_live), c->dc, gp(c), c->color, c->label, c->live[0], c->live[1]);
+  if ((rc = pull_ctl(c))) return rc;
+  if (removed) *removed = (int64_t)c->h_dc->ctr[C_TRIMMED];
   return SCC_OK;
 }
 
-// Forward (fwd) or backward reachability from every partition's pivot,
-// alternating top-down and bottom-up levels (Beamer's switch rule with the
-// edge estimate m_f against m_u / 14 and back when the frontier shrinks below
-// n_live / 24).
-int traverse(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed) {
+// Run the level launches of one search whose frontier has been seeded.
+int run_levels(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed) {
   const int32_t *off = fwd ? c->begin : c->rbegin, *nb = fwd ? c->adj : c->radj;
   const int32_t *roff = fwd ? c->rbegin : c->begin, *rnb = fwd ? c->radj : c->adj;
   uint32_t* V = fwd ? c->vf : c->vb;
-  int rc;
-  if ((rc = zero_ctr(c))) return rc;
-  launch_seed(c->stream, c->live[c->cur], c->n_live, c->n, c->color, c->pkey, V, c->q[0], c->ctr);
-  if ((rc = sync_ctr(c))) return rc;
-  int64_t nq = (int64_t)c->h_ctr[C_Q_OUT];
-  int qc = 0;
-  const int32_t* cand_in = c->live[c->cur];
-  int64_t ncand = c->n_live;
-  int cc = 0;
-  const double avg_deg = c->n ? (double)c->m / (double)c->n : 0.0;
-  int64_t mf = 0;
-  bool bu = false;
-  int64_t prev_nq = nq;
-  while (nq > 0) {
+  const unsigned grid = grid_for(c->n_live);
+  const int32_t max_chunks = std::max<int32_t>(1, (c->maxdeg + 2047) / 2048);
+  int& hint = c->lv_hint[fwd ? 0 : 1];
+  int batch = std::max(2, hint), rc;
+  for (int guard = 0;; guard++) {
     if ((rc = check_cancel(c))) return rc;
-    if (!bu) {
-      if ((double)mf > (double)ncand * avg_deg / 14.0 && ncand > 0) bu = true;
-    } else if (nq < prev_nq && nq * 24 < c->n_live) {
-      bu = false;
-    }
-    if ((rc = zero_ctr(c))) return rc;
-    if (timed) hipEventRecord(c->ev[0], c->stream);
-    if (!bu) {
-      launch_td(c->stream, off, nb, c->color, V, c->q[qc], nq, c->q[qc ^ 1], c->huge, c->ctr);
-      if ((rc = sync_ctr(c))) return rc;
-      const int64_t nh = (int64_t)c->h_ctr[C_HUGE];
-      if (nh > 0) {
-        launch_td_huge(c->stream, off, nb, c->color, V, c->huge, nh, c->q[qc ^ 1], c->ctr);
-        if ((rc = sync_ctr(c))) return rc;
-      }
-      c->kernel_launches += nh > 0 ? 2 : 1;
-    } else {
-      launch_bu(c->stream, multi, roff, rnb, off, c->color, V, cand_in, ncand, c->cand[cc],
-                c->q[qc ^ 1], c->ctr);
-      if ((rc = sync_ctr(c))) return rc;
-      cand_in = c->cand[cc];
-      ncand = (int64_t)c->h_ctr[C_CAND_OUT];
-      cc ^= 1;
-      c->kernel_launches += 1;
-    }
+    if (timed) (void)hipEventRecord(c->ev[0], c->stream);
+    for (int i = 0; i < batch; i++)
+      launch_level(c->stream, grid, c->dc, multi ? 1 : 0, off, nb, roff, rnb, c->color, V, c->q[0],
+                   c->q[1], c->cand[0], c->cand[1], c->huge, c->huge2, c->live[0], c->live[1],
+                   max_chunks);
+    if (timed) (void)hipEventRecord(c->ev[1], c->stream);
+    if ((rc = pull_ctl(c))) return rc;
     if (timed) {
-      hipEventRecord(c->ev[1], c->stream);
-      hipEventSynchronize(c->ev[1]);
       float ms = 0;
-      hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+      (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
       c->kernel_ms += ms;
+      c->kernel_launches += batch;
     }
-    prev_nq = nq;
-    nq = (int64_t)c->h_ctr[C_Q_OUT];
-    mf = (int64_t)c->h_ctr[C_Q_EDGES];
-    qc ^= 1;
-    if (levels) (*levels)++;
+    if (c->h_dc->lv.done) break;
+    batch = std::max(4, batch);
+    if (guard > 4 * (int64_t)c->n + 64) return set_error(c, SCC_ERR_LIMIT, "search did not converge");
   }
+  const int used = c->h_dc->lv.levels;
+  if (levels) *levels += used;
+  // next time, queue one more launch than this search needed (its last level
+  // admits nothing)
+  hint = std::min(64, used + 1);
   return SCC_OK;
+}
+
+int traverse(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed) {
+  const int32_t* off = fwd ? c->begin : c->rbegin;
+  uint32_t* V = fwd ? c->vf : c->vb;
+  const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
+  launch_ctl_init(c->stream, c->dc, 8, 0, 0, 0, avg / 14.0);
+  launch_seed_pivots(c->stream, grid_for(c->n_live), c->dc, c->n, off, c->color, c->pkey, V, c->q[0],
+                     c->live[0], c->live[1]);
+  return run_levels(c, fwd, multi, levels, timed);
+}
+
+int traverse_from(scc_ctx* c, bool fwd, bool multi, const int32_t* seeds,
+                  const unsigned long long* d_count, int64_t* levels) {
+  const int32_t* off = fwd ? c->begin : c->rbegin;
+  uint32_t* V = fwd ? c->vf : c->vb;
+  const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
+  launch_ctl_init(c->stream, c->dc, 8, 0, 0, 0, avg / 14.0);
+  launch_seed_list(c->stream, grid_for(c->n_live), c->dc, off, c->color, seeds, d_count, V, c->q[0]);
+  return run_levels(c, fwd, multi, levels, false);
 }
 
 // One forward-backward step for every live partition at once (multi) or for
 // the single partition of colour -1 (the global step).
 int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int64_t* scc,
-                int32_t* pivot_out, int64_t* levels, bool timed) {
+                int32_t* pivot_out, int64_t* levels, bool timed, int64_t* sets) {
   int rc;
-  if (c->n_live == 0) {
-    if (scc) *scc = 0;
-    return SCC_OK;
-  }
-  launch_reset_slots(c->stream, c->live[c->cur], c->n_live, c->n, c->color, c->pkey, c->rep);
+  if (scc) *scc = 0;
+  if (c->n_live == 0) return SCC_OK;
+  const unsigned grid = grid_for(c->n_live);
+  if ((rc = zero_ctr(c))) return rc;
+  launch_reset_slots(c->stream, grid, c->dc, c->n, c->color, c->pkey, c->rep, c->live[0], c->live[1]);
   if (pivot_override >= 0) {
     if ((rc = force_pivot(c, pivot_override))) return rc;
   } else {
-    launch_pivot_key(c->stream, c->live[c->cur], c->n_live, c->n, c->color, c->begin, c->rbegin,
-                     policy, c->sched_scores, c->pkey);
+    launch_pivot_key(c->stream, grid, c->dc, gp(c), c->color, policy, c->sched_scores, c->pkey,
+                     c->live[0], c->live[1]);
   }
   const int64_t words = ((int64_t)c->n + 31) / 32;
   launch_fill_u32(c->stream, c->vf, 0u, words);
   launch_fill_u32(c->stream, c->vb, 0u, words);
-  if (pivot_out) {
-    unsigned long long k = 0;
-    if ((rc = hip_check(c, hipMemcpyAsync(&k, c->pkey + c->n, sizeof(k), hipMemcpyDeviceToHost,
-                                          c->stream), "pivot readback")))
-      return rc;
-    if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "stream sync"))) return rc;
-    *pivot_out = k ? (int32_t)(0x7FFFFFFF - (int32_t)(k & 0x7FFFFFFFull)) : -1;
-  }
   if ((rc = traverse(c, true, multi, levels, timed))) return rc;
   if ((rc = traverse(c, false, multi, levels, timed))) return rc;
-  if ((rc = zero_ctr(c))) return rc;
-  launch_classify(c->stream, c->live[c->cur], c->n_live, c->n, c->color, c->label, c->vf, c->vb,
-                  c->pkey, c->rep, c->live[c->cur ^ 1], c->ctr);
-  if ((rc = sync_ctr(c))) return rc;
-  if (scc) *scc = (int64_t)c->h_ctr[C_SCC];
-  c->cur ^= 1;
-  c->n_live = (int64_t)c->h_ctr[C_LIST_OUT];
-  launch_recolor(c->stream, c->live[c->cur], c->n_live, c->n, c->color, c->vf, c->vb, c->rep);
-  return hip_check(c, hipGetLastError(), "recolor");
-}
-
-int do_wcc(scc_ctx* c, int64_t* parts) {
-  int rc;
-  launch_wcc_init(c->stream, c->live[c->cur], c->n_live, c->begin, c->W);
-  for (int it = 0;; it++) {
-    if ((rc = check_cancel(c))) return rc;
-    if ((rc = zero_ctr(c))) return rc;
-    launch_wcc_iter(c->stream, c->live[c->cur], c->n_live, c->begin, c->adj, c->color, c->W, c->ctr);
-    if ((rc = sync_ctr(c))) return rc;
-    if (c->h_ctr[C_CHANGED] == 0) break;
-    if (it > 4 * (int64_t)c->n + 64) return set_error(c, SCC_ERR_LIMIT, "WCC did not converge");
+  if (sets) launch_fb_stats(c->stream, grid, c->dc, gp(c), c->vf, c->vb, c->live[0], c->live[1]);
+  launch_classify(c->stream, grid, c->dc, c->n, c->color, c->label, c->vf, c->vb, c->pkey, c->rep,
+                  c->live[0], c->live[1]);
+  launch_recolor(c->stream, grid, c->dc, c->n, c->color, c->vf, c->vb, c->rep, c->live[0], c->live[1]);
+  if ((rc = pull_ctl(c))) return rc;
+  const unsigned long long* h = c->h_dc->ctr;
+  if (scc) *scc = (int64_t)h[C_SCC];
+  if (pivot_out) *pivot_out = multi ? -1 : (int32_t)h[C_PIVOT];
+  if (sets) {
+    sets[0] = (int64_t)h[C_F];
+    sets[1] = (int64_t)h[C_B];
+    sets[2] = (int64_t)h[C_EF];
+    sets[3] = (int64_t)h[C_EB];
+    sets[4] = (int64_t)h[C_FB];
   }
-  if ((rc = zero_ctr(c))) return rc;
-  launch_wcc_color(c->stream, c->live[c->cur], c->n_live, c->color, c->W, c->ctr);
-  if ((rc = sync_ctr(c))) return rc;
-  if (parts) *parts = (int64_t)c->h_ctr[C_ROOTS];
+  c->colors_split = true;
   return SCC_OK;
 }
 
-int do_fb_parts(scc_ctx* c, int policy, bool trim_between, int64_t* rounds) {
+// WCC colouring of every live partition (W to its fixpoint, colour = root).
+// With seed, the roots also seed the forward search of a colouring round.
+static int wcc_fixpoint(scc_ctx* c, bool seed) {
+  int rc;
+  const unsigned grid = grid_for(c->n_live);
+  launch_ctl_init(c->stream, c->dc, 1 | 4, 0, 0, 0, 0.0);
+  launch_wcc_init(c->stream, grid, c->dc, c->begin, c->W, c->live[0], c->live[1]);
+  int batch = std::max(2, c->it_hint);
+  for (int guard = 0;; guard++) {
+    if ((rc = check_cancel(c))) return rc;
+    for (int i = 0; i < batch; i++)
+      launch_wcc_step(c->stream, grid, c->dc, c->begin, c->adj, c->color, c->W, c->live[0], c->live[1]);
+    if ((rc = pull_ctl(c))) return rc;
+    if (c->h_dc->loop.done) break;
+    batch = std::max(4, batch);
+    if (guard > 4 * (int64_t)c->n + 64) return set_error(c, SCC_ERR_LIMIT, "WCC did not converge");
+  }
+  c->it_hint = std::min(64, c->h_dc->loop.iters + 1);
+  if (seed) {
+    const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
+    launch_ctl_init(c->stream, c->dc, 8, 0, 0, 0, avg / 14.0);
+    launch_fill_u32(c->stream, c->vf, 0u, ((int64_t)c->n + 31) / 32);
+  }
+  launch_wcc_color(c->stream, grid, c->dc, c->color, c->W, seed ? 1 : 0, c->begin, c->vf, c->q[0],
+                   c->live[0], c->live[1]);
+  c->colors_split = true;
+  return hip_check(c, hipGetLastError(), "wcc colour");
+}
+
+int do_wcc(scc_ctx* c, int64_t* parts) {
+  if (parts) *parts = 0;
+  if (c->n_live == 0) return SCC_OK;
+  int rc;
+  if ((rc = wcc_fixpoint(c, false))) return rc;
+  if ((rc = pull_ctl(c))) return rc;
+  if (parts) *parts = (int64_t)c->h_dc->ctr[C_ROOTS];
+  return SCC_OK;
+}
+
+// One colouring round: WCC classes, then each root's forward reach inside its
+// class is the root's SCC (see k_extract).
+int do_color_round(scc_ctx* c, int64_t* scc) {
+  if (scc) *scc = 0;
+  if (c->n_live == 0) return SCC_OK;
+  int rc;
+  if ((rc = wcc_fixpoint(c, true))) return rc;
+  if ((rc = run_levels(c, true, true, nullptr, false))) return rc;
+  launch_extract(c->stream, grid_for(c->n_live), c->dc, c->color, c->label, c->vf, c->live[0], c->live[1]);
+  if ((rc = pull_ctl(c))) return rc;
+  if (scc) *scc = (int64_t)c->h_dc->ctr[C_SCC];
+  return SCC_OK;
+}
+
+int do_fb_parts(scc_ctx* c, int policy, int mode, bool trim_between, int64_t* rounds) {
   int64_t r = 0;
   int rc;
   while (c->n_live > 0) {
@@ -352,7 +400,15 @@ int do_fb_parts(scc_ctx* c, int policy, bool trim_between, int64_t* rounds) {
       if ((rc = do_trim1(c, 0, nullptr))) return rc;
       if (c->n_live == 0) break;
     }
-    if ((rc = do_fb_round(c, policy, true, -1, nullptr, nullptr, nullptr, false))) return rc;
+    if (mode == SCC_PARTS_COLOR) {
+      if ((rc = do_color_round(c, nullptr))) return rc;
+    } else {
+      if ((rc = do_fb_round(c, policy, true, -1, nullptr, nullptr, nullptr, false, nullptr))) return rc;
+      if (mode == SCC_PARTS_BOTH && c->n_live > 0) {
+        if (trim_between && (rc = do_trim1(c, 0, nullptr))) return rc;
+        if ((rc = do_color_round(c, nullptr))) return rc;
+      }
+    }
     r++;
   }
   if (rounds) *rounds = r;
@@ -362,9 +418,9 @@ int do_fb_parts(scc_ctx* c, int policy, bool trim_between, int64_t* rounds) {
 int do_canon(scc_ctx* c, int64_t* count) {
   int rc;
   if ((rc = zero_ctr(c))) return rc;
-  launch_canon(c->stream, c->n, c->label, c->scratch_i32, c->canon, c->ctr);
-  if ((rc = sync_ctr(c))) return rc;
-  if (count) *count = (int64_t)c->h_ctr[C_COUNT];
+  launch_canon(c->stream, c->dc, c->n, c->label, c->scratch_i32, c->canon);
+  if ((rc = pull_ctl(c))) return rc;
+  if (count) *count = (int64_t)c->h_dc->ctr[C_COUNT];
   c->labels_ready = true;
   return SCC_OK;
 }
@@ -386,30 +442,37 @@ int scc_ctx_create(int device, uint64_t hbm_budget_bytes, scc_ctx** out) {
   int rc;
   if ((rc = hip_check(c, hipSetDevice(device), "hipSetDevice")) ||
       (rc = hip_check(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream")) ||
-      (rc = dev_alloc(c, (void**)&c->ctr, sizeof(unsigned long long) * C_NUM, "counters")) ||
-      (rc = hip_check(c, hipHostMalloc((void**)&c->h_ctr, sizeof(unsigned long long) * C_NUM, 0),
-                      "pinned counters"))) {
+      (rc = dev_alloc(c, (void**)&c->dc, sizeof(DevCtl), "control block")) ||
+      (rc = hip_check(c, hipHostMalloc((void**)&c->h_dc, sizeof(DevCtl), 0), "pinned control"))) {
     scc_ctx_destroy(c);
     return rc;
   }
-  for (auto& e : c->ev) hipEventCreate(&e);
+  c->ctr = c->dc->ctr;
+  c->h_ctr = c->h_dc->ctr;
+  std::memset(c->h_dc, 0, sizeof(DevCtl));
+  for (auto& e : c->ev) (void)hipEventCreate(&e);
+  launch_ctl_init(c->stream, c->dc, 1 | 2 | 4 | 8, 0, 0, 0, 0.0);
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "init"))) {
+    scc_ctx_destroy(c);
+    return rc;
+  }
   *out = c;
   return SCC_OK;
 }
 
 int scc_ctx_destroy(scc_ctx* c) {
   if (!c) return SCC_ERR_ARG;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   destroy_dynamic(c);
   destroy_sched(c);
   free_graph(c);
   for (void* p : c->allocs) hipFree(p);
   c->allocs.clear();
-  if (c->h_ctr) hipHostFree(c->h_ctr);
+  if (c->h_dc) (void)hipHostFree(c->h_dc);
   for (auto& e : c->ev)
-    if (e) hipEventDestroy(e);
-  if (c->stream) hipStreamDestroy(c->stream);
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return SCC_OK;
 }
@@ -417,8 +480,7 @@ int scc_ctx_destroy(scc_ctx* c) {
 const char* scc_last_error(const scc_ctx* c) { return c ? c->err.c_str() : "null context"; }
 uint64_t scc_ctx_bytes(const scc_ctx* c) { return c ? c->bytes : 0; }
 uint64_t scc_bytes_needed(int32_t n, int64_t m) {
-  return 8ull * (uint64_t)(m > 0 ? m : 1) + 8ull * ((uint64_t)n + 1) + state_bytes(n) +
-         sizeof(unsigned long long) * C_NUM;
+  return 8ull * (uint64_t)(m > 0 ? m : 1) + 8ull * ((uint64_t)n + 1) + state_bytes(n) + sizeof(DevCtl);
 }
 void scc_ctx_cancel(scc_ctx* c) {
   if (c) c->cancel.store(1);
@@ -445,6 +507,12 @@ static int validate_csr(scc_ctx* c, int32_t n, int64_t m, const int32_t* off, co
   return SCC_OK;
 }
 
+static int32_t max_degree(int32_t n, const int32_t* a, const int32_t* b) {
+  int32_t mx = 0;
+  for (int32_t v = 0; v < n; v++) mx = std::max(mx, std::max(a[v + 1] - a[v], b[v + 1] - b[v]));
+  return mx;
+}
+
 int scc_graph_upload(scc_ctx* c, int32_t n, int64_t m, const int32_t* begin, const int32_t* node_idx,
                      const int32_t* r_begin, const int32_t* r_node_idx) {
   if (!c) return SCC_ERR_ARG;
@@ -456,10 +524,11 @@ int scc_graph_upload(scc_ctx* c, int32_t n, int64_t m, const int32_t* begin, con
   if ((rc = validate_csr(c, n, m, begin, node_idx, "begin/node_idx")) ||
       (rc = validate_csr(c, n, m, r_begin, r_node_idx, "r_begin/r_node_idx")))
     return rc;
-  hipSetDevice(c->device);
+  (void)hipSetDevice(c->device);
   destroy_dynamic(c);
   if ((rc = alloc_graph(c, n, m))) return rc;
   c->m = m;
+  c->maxdeg = max_degree(n, begin, r_begin);
   if ((rc = hip_check(c, hipMemcpyAsync(c->begin, begin, 4ull * (n + 1), hipMemcpyHostToDevice, c->stream), "upload")) ||
       (rc = hip_check(c, hipMemcpyAsync(c->rbegin, r_begin, 4ull * (n + 1), hipMemcpyHostToDevice, c->stream), "upload")) ||
       (m > 0 && (rc = hip_check(c, hipMemcpyAsync(c->adj, node_idx, 4ull * m, hipMemcpyHostToDevice, c->stream), "upload"))) ||
@@ -477,7 +546,7 @@ int scc_graph_upload_device(scc_ctx* c, int32_t n, int64_t m, const int32_t* d_b
   if (n < 0 || m < 0 || m >= INT32_MAX || n >= WCC_MASK)
     return set_error(c, SCC_ERR_LIMIT, "graph exceeds int32 limits");
   int rc;
-  hipSetDevice(c->device);
+  (void)hipSetDevice(c->device);
   destroy_dynamic(c);
   if ((rc = alloc_graph(c, n, m))) return rc;
   c->m = m;
@@ -486,6 +555,14 @@ int scc_graph_upload_device(scc_ctx* c, int32_t n, int64_t m, const int32_t* d_b
       (m > 0 && (rc = hip_check(c, hipMemcpyAsync(c->adj, d_node_idx, 4ull * m, hipMemcpyDeviceToDevice, c->stream), "d2d"))) ||
       (m > 0 && (rc = hip_check(c, hipMemcpyAsync(c->radj, d_r_node_idx, 4ull * m, hipMemcpyDeviceToDevice, c->stream), "d2d"))))
     return rc;
+  {  // degrees for the huge-row chunking
+    std::vector<int32_t> b(n + 1), rb(n + 1);
+    if ((rc = hip_check(c, hipMemcpyAsync(b.data(), c->begin, 4ull * (n + 1), hipMemcpyDeviceToHost, c->stream), "d2h")) ||
+        (rc = hip_check(c, hipMemcpyAsync(rb.data(), c->rbegin, 4ull * (n + 1), hipMemcpyDeviceToHost, c->stream), "d2h")) ||
+        (rc = hip_check(c, hipStreamSynchronize(c->stream), "sync")))
+      return rc;
+    c->maxdeg = max_degree(n, b.data(), rb.data());
+  }
   if ((rc = reset_state(c))) return rc;
   c->ran = false;
   return hip_check(c, hipStreamSynchronize(c->stream), "upload sync");
@@ -502,7 +579,7 @@ int scc_graph_download(scc_ctx* c, int32_t* begin, int32_t* node_idx, int32_t* r
                        int32_t* r_node_idx) {
   if (!c) return SCC_ERR_ARG;
   if (!c->begin) return set_error(c, SCC_ERR_STATE, "no graph");
-  hipSetDevice(c->device);
+  (void)hipSetDevice(c->device);
   int rc = 0;
   if (begin && (rc = hip_check(c, hipMemcpyAsync(begin, c->begin, 4ull * (c->n + 1), hipMemcpyDeviceToHost, c->stream), "download"))) return rc;
   if (r_begin && (rc = hip_check(c, hipMemcpyAsync(r_begin, c->rbegin, 4ull * (c->n + 1), hipMemcpyDeviceToHost, c->stream), "download"))) return rc;
@@ -514,7 +591,7 @@ int scc_graph_download(scc_ctx* c, int32_t* begin, int32_t* node_idx, int32_t* r
 int scc_state_reset(scc_ctx* c) {
   if (!c) return SCC_ERR_ARG;
   if (!c->begin) return set_error(c, SCC_ERR_STATE, "no graph uploaded");
-  hipSetDevice(c->device);
+  (void)hipSetDevice(c->device);
   return reset_state(c);
 }
 
@@ -524,25 +601,30 @@ int scc_state_set_colors(scc_ctx* c, const int32_t* colors) {
   for (int32_t v = 0; v < c->n; v++)
     if (colors[v] < -1 || colors[v] >= c->n)
       return set_error(c, SCC_ERR_ARG, "colour out of range at " + std::to_string(v));
-  hipSetDevice(c->device);
+  (void)hipSetDevice(c->device);
   int rc;
   if ((rc = reset_state(c))) return rc;
   c->multi_colors = false;
   for (int32_t v = 0; v < c->n && !c->multi_colors; v++) c->multi_colors = colors[v] != -1;
-  return hip_check(c, hipMemcpyAsync(c->color, colors, 4ull * c->n, hipMemcpyHostToDevice, c->stream),
-                   "colours upload");
+  c->colors_split = c->multi_colors;
+  if ((rc = hip_check(c, hipMemcpyAsync(c->color, colors, 4ull * c->n, hipMemcpyHostToDevice, c->stream),
+                      "colours upload")))
+    return rc;
+  return hip_check(c, hipStreamSynchronize(c->stream), "colours sync");
 }
 
 #define REQUIRE_GRAPH(c)                                                    \
   do {                                                                      \
     if (!(c)) return SCC_ERR_ARG;                                           \
     if (!(c)->begin) return set_error((c), SCC_ERR_STATE, "no graph uploaded"); \
-    hipSetDevice((c)->device);                                              \
+    (void)hipSetDevice((c)->device);                                              \
   } while (0)
 
 int scc_trim1(scc_ctx* c, int64_t trim_stop, int64_t* trimmed) {
   REQUIRE_GRAPH(c);
   c->labels_ready = false;
+  int rc = do_trim_compact(c);
+  if (rc) return rc;
   return do_trim1(c, trim_stop, trimmed);
 }
 
@@ -553,12 +635,15 @@ int scc_fb_global(scc_ctx* c, int policy, int32_t pivot_override, int64_t* scc_s
   c->labels_ready = false;
   int rc = do_trim_compact(c);
   if (rc) return rc;
-  return do_fb_round(c, policy, false, pivot_override, scc_size, pivot_out, nullptr, false);
+  return do_fb_round(c, policy, c->colors_split, pivot_override, scc_size, pivot_out, nullptr, false,
+                     nullptr);
 }
 
 int scc_trim2(scc_ctx* c, int64_t* removed) {
   REQUIRE_GRAPH(c);
   c->labels_ready = false;
+  int rc = do_trim_compact(c);
+  if (rc) return rc;
   return do_trim2(c, removed);
 }
 
@@ -570,12 +655,17 @@ int scc_wcc(scc_ctx* c, int64_t* parts) {
   return do_wcc(c, parts);
 }
 
-int scc_fb_partitions(scc_ctx* c, int policy, int trim_between, int64_t* rounds) {
+int scc_partitions_resolve(scc_ctx* c, int policy, int mode, int trim_between, int64_t* rounds) {
   REQUIRE_GRAPH(c);
+  if (mode < SCC_PARTS_FB || mode > SCC_PARTS_BOTH) return set_error(c, SCC_ERR_ARG, "bad partition mode");
   c->labels_ready = false;
   int rc = do_trim_compact(c);
   if (rc) return rc;
-  return do_fb_parts(c, policy, trim_between != 0, rounds);
+  return do_fb_parts(c, policy, mode, trim_between != 0, rounds);
+}
+
+int scc_fb_partitions(scc_ctx* c, int policy, int trim_between, int64_t* rounds) {
+  return scc_partitions_resolve(c, policy, SCC_PARTS_FB, trim_between, rounds);
 }
 
 void scc_run_opts_default(scc_run_opts* o) {
@@ -587,6 +677,7 @@ void scc_run_opts_default(scc_run_opts* o) {
   o->trim_between = 1;
   o->use_wcc = 1;
   o->time_phases = 0;
+  o->partition_mode = SCC_PARTS_BOTH;
 }
 
 int scc_run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_out) {
@@ -594,6 +685,8 @@ int scc_run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_o
   scc_run_opts opts;
   if (opts_in) opts = *opts_in;
   else scc_run_opts_default(&opts);
+  if (opts.partition_mode < SCC_PARTS_FB || opts.partition_mode > SCC_PARTS_BOTH)
+    return set_error(c, SCC_ERR_ARG, "bad partition mode");
   scc_run_stats st;
   std::memset(&st, 0, sizeof(st));
   st.pivot_first = -1;
@@ -605,19 +698,19 @@ int scc_run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_o
   hipEvent_t* E = c->ev;
   int rc = 0;
   auto mark = [&](int i) {
-    if (tp) hipEventRecord(E[2 + i], c->stream);
+    if (tp) (void)hipEventRecord(E[2 + i], c->stream);
   };
   auto elapsed = [&](int a, int b) -> double {
     float ms = 0;
     if (tp) {
-      hipEventSynchronize(E[2 + b]);
-      hipEventElapsedTime(&ms, E[2 + a], E[2 + b]);
+      (void)hipEventSynchronize(E[2 + b]);
+      (void)hipEventElapsedTime(&ms, E[2 + a], E[2 + b]);
     }
     return ms;
   };
   do {
     mark(0);
-    if ((rc = do_trim_compact(c))) break;
+    // (trim sweeps drop finished vertices from the list, so no compaction first)
     if ((rc = do_trim1(c, opts.trim_stop, &st.trimmed_first))) break;
     mark(1);
     st.ms_trim1 = elapsed(0, 1);
@@ -625,16 +718,21 @@ int scc_run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_o
     if ((rc = check_cancel(c))) break;
     // the global step runs over colour -1; with caller-set colours it is a
     // partition round
-    const bool multi = c->multi_colors;
-    if ((rc = do_fb_round(c, opts.pivot_policy, multi, -1, &st.scc_first, &st.pivot_first,
-                          &st.fb_levels, c->kernel_timing)))
+    int64_t sets[5] = {0, 0, 0, 0, 0};
+    if ((rc = do_fb_round(c, opts.pivot_policy, c->colors_split, -1, &st.scc_first, &st.pivot_first,
+                          &st.fb_levels, c->kernel_timing, tp || c->kernel_timing ? sets : nullptr)))
       break;
     mark(2);
     st.ms_fb_global = elapsed(1, 2);
+    st.fb_f = sets[0];
+    st.fb_b = sets[1];
+    st.fb_ef = sets[2];
+    st.fb_eb = sets[3];
+    st.fb_fb = sets[4];
     int64_t t = 0;
     if ((rc = do_trim1(c, opts.trim_stop, &t))) break;
     st.trimmed_second = t;
-    if (opts.use_trim2) {
+    if (opts.use_trim2 && c->n_live > 0) {
       if ((rc = do_trim2(c, &t))) break;
       st.trimmed_second += t;
       if ((rc = do_trim1(c, opts.trim_stop, &t))) break;
@@ -649,7 +747,9 @@ int scc_run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_o
     }
     mark(4);
     st.ms_wcc = elapsed(3, 4);
-    if ((rc = do_fb_parts(c, opts.pivot_policy, opts.trim_between != 0, &st.fb_rounds))) break;
+    if ((rc = do_fb_parts(c, opts.pivot_policy, opts.partition_mode, opts.trim_between != 0,
+                          &st.fb_rounds)))
+      break;
     mark(5);
     st.ms_fb_parts = elapsed(4, 5);
   } while (false);
@@ -712,7 +812,7 @@ int scc_ctx_set_kernel_timing(scc_ctx* c, int enable) {
 
 int scc_ctx_synchronize(scc_ctx* c) {
   if (!c) return SCC_ERR_ARG;
-  hipSetDevice(c->device);
+  (void)hipSetDevice(c->device);
   return hip_check(c, hipStreamSynchronize(c->stream), "synchronize");
 }
 

@@ -1,13 +1,22 @@
-// Host launchers of the SCC kernels (defined in kernels.hip).  Each takes the
-// context's stream and device pointers; list lengths are host values.
+// Host launchers of the SCC kernels (defined in kernels.hip).  List-processing
+// kernels read the current vertex list and its length from the device control
+// block, so the host only chooses the grid (an upper bound on the list length).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "common.h"
+
 namespace sccmi {
 
+struct GraphPtrs {
+  const int32_t *begin, *adj, *rbegin, *radj;
+  int32_t n;
+  int32_t maxdeg;
+};
+
 // grid size for a list of n items (grid-stride beyond the cap)
-inline unsigned grid_for(int64_t n, unsigned cap = 8192) {
+inline unsigned grid_for(int64_t n, unsigned cap = 2048) {
   int64_t g = (n + 255) / 256;
   if (g < 1) g = 1;
   if (g > cap) g = cap;
@@ -17,46 +26,52 @@ inline unsigned grid_for(int64_t n, unsigned cap = 8192) {
 void launch_fill_i32(hipStream_t st, int32_t* p, int32_t v, int64_t n);
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, int64_t n);
 void launch_iota(hipStream_t st, int32_t* p, int32_t n);
-void launch_trim1(hipStream_t st, const int32_t* begin, const int32_t* adj, const int32_t* rbegin,
-                  const int32_t* radj, int32_t* color, int32_t* label, const int32_t* list_in,
-                  int64_t n_in, int32_t* list_out, unsigned long long* ctr);
-void launch_trim2(hipStream_t st, const int32_t* begin, const int32_t* adj, const int32_t* rbegin,
-                  const int32_t* radj, int32_t* color, int32_t* label, const int32_t* list_in,
-                  int64_t n_in, unsigned long long* ctr);
-void launch_reset_slots(hipStream_t st, const int32_t* list, int64_t n_in, int32_t n,
-                        const int32_t* color, unsigned long long* pkey, int32_t* rep);
-void launch_pivot_key(hipStream_t st, const int32_t* list, int64_t n_in, int32_t n,
-                      const int32_t* color, const int32_t* begin, const int32_t* rbegin, int policy,
-                      const uint32_t* scores, unsigned long long* pkey);
+// what: bit 0 counters, 1 live list, 2 loop control, 3 level control
+void launch_ctl_init(hipStream_t st, DevCtl* dc, int what, int32_t live_sel, unsigned long long live_n,
+                     long long stop, double bu_ratio);
+void launch_trim1_step(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, int32_t* color,
+                       int32_t* label, int32_t* L0, int32_t* L1);
+void launch_compact(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* color, int32_t* L0,
+                    int32_t* L1);
+void launch_trim2(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, int32_t* color,
+                  int32_t* label, const int32_t* L0, const int32_t* L1);
+void launch_reset_slots(hipStream_t st, unsigned grid, const DevCtl* dc, int32_t n,
+                        const int32_t* color, unsigned long long* pkey, int32_t* rep,
+                        const int32_t* L0, const int32_t* L1);
+void launch_pivot_key(hipStream_t st, unsigned grid, const DevCtl* dc, GraphPtrs G,
+                      const int32_t* color, int policy, const uint32_t* scores,
+                      unsigned long long* pkey, const int32_t* L0, const int32_t* L1);
 void launch_force_pivot(hipStream_t st, int32_t p, int32_t n, const int32_t* color,
                         unsigned long long* pkey);
-void launch_compact_live(hipStream_t st, const int32_t* list, int64_t n_in, const int32_t* color,
-                         int32_t* list_out, unsigned long long* ctr);
-void launch_seed(hipStream_t st, const int32_t* list, int64_t n_in, int32_t n, const int32_t* color,
-                 const unsigned long long* pkey, uint32_t* V, int32_t* q_out,
-                 unsigned long long* ctr);
-void launch_td(hipStream_t st, const int32_t* off, const int32_t* nb, const int32_t* color,
-               uint32_t* V, const int32_t* q_in, int64_t n_qin, int32_t* q_out, int32_t* huge,
-               unsigned long long* ctr);
-void launch_td_huge(hipStream_t st, const int32_t* off, const int32_t* nb, const int32_t* color,
-                    uint32_t* V, const int32_t* huge, int64_t n_huge, int32_t* q_out,
-                    unsigned long long* ctr);
-void launch_bu(hipStream_t st, bool multi, const int32_t* roff, const int32_t* rnb,
-               const int32_t* off, const int32_t* color, uint32_t* V, const int32_t* c_in,
-               int64_t n_cin, int32_t* c_out, int32_t* q_out, unsigned long long* ctr);
-void launch_classify(hipStream_t st, const int32_t* list, int64_t n_in, int32_t n, int32_t* color,
+void launch_seed_pivots(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, const int32_t* off,
+                        const int32_t* color, const unsigned long long* pkey, uint32_t* V,
+                        int32_t* q0, const int32_t* L0, const int32_t* L1);
+void launch_seed_list(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* off,
+                      const int32_t* color, const int32_t* seeds, const unsigned long long* ns,
+                      uint32_t* V, int32_t* q0);
+void launch_level(hipStream_t st, unsigned grid, DevCtl* dc, int multi, const int32_t* off,
+                  const int32_t* nb, const int32_t* roff, const int32_t* rnb, const int32_t* color,
+                  uint32_t* V, int32_t* q0, int32_t* q1, int32_t* c0, int32_t* c1, int32_t* h0,
+                  int32_t* h1, const int32_t* L0, const int32_t* L1, int32_t max_chunks);
+void launch_fb_stats(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, const uint32_t* vf,
+                     const uint32_t* vb, const int32_t* L0, const int32_t* L1);
+void launch_classify(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, int32_t* color,
                      int32_t* label, const uint32_t* vf, const uint32_t* vb,
-                     const unsigned long long* pkey, int32_t* rep, int32_t* list_out,
-                     unsigned long long* ctr);
-void launch_recolor(hipStream_t st, const int32_t* list, int64_t n_in, int32_t n, int32_t* color,
-                    const uint32_t* vf, const uint32_t* vb, const int32_t* rep);
-void launch_wcc_init(hipStream_t st, const int32_t* list, int64_t n_in, const int32_t* begin,
-                     int32_t* W);
-void launch_wcc_iter(hipStream_t st, const int32_t* list, int64_t n_in, const int32_t* begin,
-                     const int32_t* adj, const int32_t* color, int32_t* W, unsigned long long* ctr);
-void launch_wcc_color(hipStream_t st, const int32_t* list, int64_t n_in, int32_t* color,
-                      const int32_t* W, unsigned long long* ctr);
-void launch_canon(hipStream_t st, int32_t n, const int32_t* label, int32_t* mn, int32_t* out,
-                  unsigned long long* ctr);
+                     const unsigned long long* pkey, int32_t* rep, int32_t* L0, int32_t* L1);
+void launch_recolor(hipStream_t st, unsigned grid, const DevCtl* dc, int32_t n, int32_t* color,
+                    const uint32_t* vf, const uint32_t* vb, const int32_t* rep, const int32_t* L0,
+                    const int32_t* L1);
+void launch_wcc_init(hipStream_t st, unsigned grid, const DevCtl* dc, const int32_t* begin,
+                     int32_t* W, const int32_t* L0, const int32_t* L1);
+void launch_wcc_step(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* begin,
+                     const int32_t* adj, const int32_t* color, int32_t* W, const int32_t* L0,
+                     const int32_t* L1);
+void launch_wcc_color(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, const int32_t* W,
+                      int seed, const int32_t* off, uint32_t* V, int32_t* q0, const int32_t* L0,
+                      const int32_t* L1);
+void launch_extract(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, int32_t* label,
+                    const uint32_t* V, int32_t* L0, int32_t* L1);
+void launch_canon(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label, int32_t* mn,
+                  int32_t* out);
 
 }  // namespace sccmi

@@ -7,10 +7,17 @@
 // block-wide collectives (ballot compaction, counters) can be used in the body.
 // Row scans use a lane-per-vertex phase over the first LIGHT entries and then a
 // wave-cooperative phase for the rest of long rows (64 lanes x 4 loads in flight).
+//
+// List inputs come from the device control block (DevCtl): the current vertex
+// list is live[dc->live.sel][0, dc->live.n).  Kernels that shrink the list write
+// the survivors to the other buffer and their last workgroup flips the selector.
 #include "common.h"
 #include "kernels.h"
 
 namespace sccmi {
+
+#define LIVE_IN(dc, L0, L1) ((dc)->live.sel ? (L1) : (L0))
+#define LIVE_OUT(dc, L0, L1) ((dc)->live.sel ? (L0) : (L1))
 
 // ---- row scans -----------------------------------------------------------------
 
@@ -70,23 +77,70 @@ __global__ __launch_bounds__(BLOCK) void k_iota(int32_t* __restrict__ p, int32_t
     p[i] = (int32_t)i;
 }
 
+// Control-block set-up (one thread): the counters, the live list, the loop
+// control and the level control.  what: bit 0 counters, 1 live, 2 loop, 3 level.
+__global__ void k_ctl_init(DevCtl* dc, int what, int32_t live_sel, unsigned long long live_n,
+                           long long stop, double bu_ratio) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (what & 1) {
+    for (int i = 0; i < C_NUM; i++) dc->ctr[i] = 0;
+    dc->blocks_done = 0;
+    for (int i = 0; i < 9 * 16; i++) dc->done[i] = 0;
+  }
+  if (what & 2) {
+    dc->live.sel = live_sel;
+    dc->live.n = live_n;
+  }
+  if (what & 4) {
+    LoopCtl& l = dc->loop;
+    l.done = 0;
+    l.iters = 0;
+    l.blocks_done = 0;
+    l.stop = stop;
+    l.acc0 = l.acc1 = l.total0 = 0;
+  }
+  if (what & 8) {
+    LevelCtl& v = dc->lv;
+    v.done = 0;
+    v.mode = MODE_TD;
+    v.q_sel = 0;
+    v.c_sel = -1;
+    v.h_sel = 0;
+    v.levels = 0;
+    v.blocks_done = 0;
+    v.nq = v.nh = v.nc = v.mf = v.prev_nq = 0;
+    v.out_q = v.out_c = v.out_h = v.out_edges = 0;
+    v.visited = 0;
+    v.bu_edges_per_cand = bu_ratio;
+  }
+}
+
 // ---- trim1 -------------------------------------------------------------------
 // src/scc_trim1.cc:38-110: v of colour c is trimmed (label v, colour -2) when it
 // has no out-neighbour k != v of colour c, or no such in-neighbour.  Survivors
-// are compacted into list_out (replaces create_trim1_compact, :211-275).  Reads
-// of neighbour colours race with trims of the same sweep; trimming is monotone,
-// so the fixpoint is the same (the unique maximal set where every vertex keeps a
-// same-colour in- and out-neighbour).
-__global__ __launch_bounds__(BLOCK) void k_trim1(const int32_t* __restrict__ begin,
-                                                 const int32_t* __restrict__ adj,
-                                                 const int32_t* __restrict__ rbegin,
-                                                 const int32_t* __restrict__ radj,
-                                                 int32_t* __restrict__ color, int32_t* __restrict__ label,
-                                                 const int32_t* __restrict__ list_in, int64_t n_in,
-                                                 int32_t* __restrict__ list_out,
-                                                 unsigned long long* __restrict__ ctr) {
+// are compacted into the other list (replaces create_trim1_compact, :211-275).
+// Reads of neighbour colours race with trims of the same sweep; trimming is
+// monotone, so the fixpoint is the same (the unique maximal set where every
+// vertex keeps a same-colour in- and out-neighbour).  One launch is one sweep;
+// the last workgroup ends the loop when a sweep trims <= stop
+// (src/scc_trim1.cc:195-209 repeat_global_trim1).
+__global__ __launch_bounds__(BLOCK) void k_trim1_step(DevCtl* __restrict__ dc, GraphPtrs G,
+                                                      int32_t* __restrict__ color,
+                                                      int32_t* __restrict__ label,
+                                                      int32_t* __restrict__ L0, int32_t* __restrict__ L1) {
+  __shared__ int s_done;
+  if (threadIdx.x == 0) s_done = dc->loop.done;
+  __syncthreads();
+  if (s_done) return;
+  const int32_t* __restrict__ list_in = LIVE_IN(dc, L0, L1);
+  int32_t* __restrict__ list_out = LIVE_OUT(dc, L0, L1);
+  const int64_t n_in = (int64_t)dc->live.n;
   auto live = [color](int32_t k, int32_t pv, int32_t pc) { return k != pv && color[k] == pc; };
+  unsigned long long trimmed = 0;
+  using Out = LdsOut<0>;
+  Out::init();
   for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
+    Out::maybe_flush(list_out, &dc->loop.acc1, BLOCK);
     const int64_t i = base + threadIdx.x;
     const bool valid = i < n_in;
     const int32_t v = valid ? list_in[i] : 0;
@@ -94,24 +148,65 @@ __global__ __launch_bounds__(BLOCK) void k_trim1(const int32_t* __restrict__ beg
     const bool act = valid && c != DONE;
     int32_t s = 0, e = 0;
     if (act) {
-      s = begin[v];
-      e = begin[v + 1];
+      s = G.begin[v];
+      e = G.begin[v + 1];
     }
-    const bool out_ok = row_any(act, v, c, s, e, adj, live);
+    const bool out_ok = row_any(act, v, c, s, e, G.adj, live);
     const bool act2 = act && out_ok;
     int32_t rs = 0, re = 0;
     if (act2) {
-      rs = rbegin[v];
-      re = rbegin[v + 1];
+      rs = G.rbegin[v];
+      re = G.rbegin[v + 1];
     }
-    const bool in_ok = row_any(act2, v, c, rs, re, radj, live);
+    const bool in_ok = row_any(act2, v, c, rs, re, G.radj, live);
     const bool trim = act && !in_ok;
     if (trim) {
       label[v] = v;
       color[v] = DONE;
+      trimmed++;
     }
-    block_count(trim ? 1ull : 0ull, &ctr[C_TRIMMED]);
-    block_append(act && !trim, v, list_out, &ctr[C_LIST_OUT]);
+    Out::push(act && !trim, v, list_out, &dc->loop.acc1);
+  }
+  Out::flush(list_out, &dc->loop.acc1);
+  block_count(trimmed, &dc->loop.acc0);
+  if (last_block2(dc->done) && threadIdx.x == 0) {
+    LoopCtl& l = dc->loop;
+    const unsigned long long t = ld_acq(&l.acc0), out = ld_acq(&l.acc1);
+    l.total0 += t;
+    l.iters++;
+    l.done = ((long long)t <= l.stop) || out == 0;
+    l.acc0 = l.acc1 = 0;
+    l.blocks_done = 0;
+    dc->live.sel ^= 1;
+    dc->live.n = out;
+  }
+}
+
+// Drop vertices of colour -2 from the live list (one launch).
+__global__ __launch_bounds__(BLOCK) void k_compact(DevCtl* __restrict__ dc, const int32_t* __restrict__ color,
+                                                   int32_t* __restrict__ L0, int32_t* __restrict__ L1) {
+  const int32_t* __restrict__ list_in = LIVE_IN(dc, L0, L1);
+  int32_t* __restrict__ list_out = LIVE_OUT(dc, L0, L1);
+  const int64_t n_in = (int64_t)dc->live.n;
+  using Out = LdsOut<0>;
+  Out::init();
+  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
+    Out::maybe_flush(list_out, &dc->ctr[C_LIST_OUT], BLOCK);
+    const int64_t i = base + threadIdx.x;
+    int32_t v = 0;
+    bool keep = false;
+    if (i < n_in) {
+      v = list_in[i];
+      keep = color[v] != DONE;
+    }
+    Out::push(keep, v, list_out, &dc->ctr[C_LIST_OUT]);
+  }
+  Out::flush(list_out, &dc->ctr[C_LIST_OUT]);
+  if (last_block2(dc->done) && threadIdx.x == 0) {
+    dc->live.n = ld_acq(&dc->ctr[C_LIST_OUT]);
+    dc->live.sel ^= 1;
+    dc->ctr[C_LIST_OUT] = 0;
+    dc->blocks_done = 0;
   }
 }
 
@@ -135,48 +230,46 @@ __device__ __forceinline__ int32_t unique_nbr(const int32_t* __restrict__ off,
 
 // src/scc_trim2_new.cc:42-101: {v, k} with v < k, each the other's unique
 // same-colour out-neighbour (or in-neighbour), is a 2-vertex SCC labelled v.
-__global__ __launch_bounds__(BLOCK) void k_trim2(const int32_t* __restrict__ begin,
-                                                 const int32_t* __restrict__ adj,
-                                                 const int32_t* __restrict__ rbegin,
-                                                 const int32_t* __restrict__ radj,
+__global__ __launch_bounds__(BLOCK) void k_trim2(DevCtl* __restrict__ dc, GraphPtrs G,
                                                  int32_t* __restrict__ color, int32_t* __restrict__ label,
-                                                 const int32_t* __restrict__ list_in, int64_t n_in,
-                                                 unsigned long long* __restrict__ ctr) {
-  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
-    const int64_t i = base + threadIdx.x;
-    int removed = 0;
-    if (i < n_in) {
-      const int32_t v = list_in[i];
-      const int32_t c = color[v];
-      if (c != DONE) {
-        int32_t k = unique_nbr(begin, adj, color, v, c);
-        bool pair = k > v && unique_nbr(begin, adj, color, k, c) == v;
-        if (!pair) {
-          k = unique_nbr(rbegin, radj, color, v, c);
-          pair = k > v && unique_nbr(rbegin, radj, color, k, c) == v;
-        }
-        if (pair) {
-          label[v] = v;
-          label[k] = v;
-          color[v] = DONE;
-          color[k] = DONE;
-          removed = 2;
-        }
-      }
+                                                 const int32_t* __restrict__ L0,
+                                                 const int32_t* __restrict__ L1) {
+  const int32_t* __restrict__ list_in = LIVE_IN(dc, L0, L1);
+  const int64_t n_in = (int64_t)dc->live.n;
+  unsigned long long removed = 0;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_in; i += (int64_t)gridDim.x * BLOCK) {
+    const int32_t v = list_in[i];
+    const int32_t c = color[v];
+    if (c == DONE) continue;
+    int32_t k = unique_nbr(G.begin, G.adj, color, v, c);
+    bool pair = k > v && unique_nbr(G.begin, G.adj, color, k, c) == v;
+    if (!pair) {
+      k = unique_nbr(G.rbegin, G.radj, color, v, c);
+      pair = k > v && unique_nbr(G.rbegin, G.radj, color, k, c) == v;
     }
-    block_count((unsigned long long)removed, &ctr[C_TRIMMED]);
+    if (pair) {
+      label[v] = v;
+      label[k] = v;
+      color[v] = DONE;
+      color[k] = DONE;
+      removed += 2;
+    }
   }
+  block_count(removed, &dc->ctr[C_TRIMMED]);
 }
 
 // ---- pivot selection -----------------------------------------------------------
 
-__global__ __launch_bounds__(BLOCK) void k_reset_slots(const int32_t* __restrict__ list, int64_t n_in,
-                                                       int32_t n, const int32_t* __restrict__ color,
+__global__ __launch_bounds__(BLOCK) void k_reset_slots(const DevCtl* __restrict__ dc, int32_t n,
+                                                       const int32_t* __restrict__ color,
                                                        unsigned long long* __restrict__ pkey,
-                                                       int32_t* __restrict__ rep) {
+                                                       int32_t* __restrict__ rep,
+                                                       const int32_t* __restrict__ L0,
+                                                       const int32_t* __restrict__ L1) {
+  const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
+  const int64_t n_in = (int64_t)dc->live.n;
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_in; i += (int64_t)gridDim.x * BLOCK) {
-    const int32_t v = list[i];
-    const int32_t c = color[v];
+    const int32_t c = color[list[i]];
     if (c == DONE) continue;
     const int32_t s = slot_of(c, n);
     pkey[s] = 0ull;
@@ -191,27 +284,37 @@ __global__ __launch_bounds__(BLOCK) void k_reset_slots(const int32_t* __restrict
 // id tie-break picks the smallest member (src_CUDA/scc_cuda_fb_global.cu:426-440);
 // SCC_PIVOT_LEARNED takes scores[] from the scheduler.  The slot is read first
 // so that most lanes of a large partition skip the atomic.
-__global__ __launch_bounds__(BLOCK) void k_pivot_key(const int32_t* __restrict__ list, int64_t n_in,
-                                                     int32_t n, const int32_t* __restrict__ color,
-                                                     const int32_t* __restrict__ begin,
-                                                     const int32_t* __restrict__ rbegin, int policy,
+__global__ __launch_bounds__(BLOCK) void k_pivot_key(const DevCtl* __restrict__ dc, GraphPtrs G,
+                                                     const int32_t* __restrict__ color, int policy,
                                                      const uint32_t* __restrict__ scores,
-                                                     unsigned long long* __restrict__ pkey) {
-  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_in; i += (int64_t)gridDim.x * BLOCK) {
-    const int32_t v = list[i];
-    const int32_t c = color[v];
-    if (c == DONE) continue;
-    unsigned long long score = 0;
-    if (policy == SCC_PIVOT_DEGREE_PRODUCT) {
-      const unsigned long long p = (unsigned long long)(begin[v + 1] - begin[v]) *
-                                   (unsigned long long)(rbegin[v + 1] - rbegin[v]);
-      score = p > 0x1FFFFFFFFull ? 0x1FFFFFFFFull : p;
-    } else if (policy == SCC_PIVOT_LEARNED) {
-      score = scores[v];
+                                                     unsigned long long* __restrict__ pkey,
+                                                     const int32_t* __restrict__ L0,
+                                                     const int32_t* __restrict__ L1) {
+  const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
+  const int64_t n_in = (int64_t)dc->live.n;
+  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
+    const int64_t i = base + threadIdx.x;
+    bool act = false;
+    int32_t slot = 0;
+    unsigned long long key = 0;
+    if (i < n_in) {
+      const int32_t v = list[i];
+      const int32_t c = color[v];
+      if (c != DONE) {
+        unsigned long long score = 0;
+        if (policy == SCC_PIVOT_DEGREE_PRODUCT) {
+          const unsigned long long p = (unsigned long long)(G.begin[v + 1] - G.begin[v]) *
+                                       (unsigned long long)(G.rbegin[v + 1] - G.rbegin[v]);
+          score = p > 0x1FFFFFFFFull ? 0x1FFFFFFFFull : p;
+        } else if (policy == SCC_PIVOT_LEARNED && scores) {
+          score = scores[v];
+        }
+        key = pivot_key(score, v);
+        slot = slot_of(c, G.n);
+        act = true;
+      }
     }
-    const unsigned long long key = pivot_key(score, v);
-    unsigned long long* slot = &pkey[slot_of(c, n)];
-    if (key > *slot) atomicMax(slot, key);
+    grouped_max_u64(act, slot, key, pkey);
   }
 }
 
@@ -224,30 +327,19 @@ __global__ void k_force_pivot(int32_t p, int32_t n, const int32_t* __restrict__ 
   }
 }
 
-// Drop vertices of colour -2 from a list (order kept inside each workgroup).
-__global__ __launch_bounds__(BLOCK) void k_compact_live(const int32_t* __restrict__ list, int64_t n_in,
-                                                        const int32_t* __restrict__ color,
-                                                        int32_t* __restrict__ list_out,
-                                                        unsigned long long* __restrict__ ctr) {
-  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
-    const int64_t i = base + threadIdx.x;
-    int32_t v = 0;
-    bool keep = false;
-    if (i < n_in) {
-      v = list[i];
-      keep = color[v] != DONE;
-    }
-    block_append(keep, v, list_out, &ctr[C_LIST_OUT]);
-  }
-}
-
-// Seed a traversal: every live vertex that is its partition's pivot marks
-// itself in V and enters the frontier.
-__global__ __launch_bounds__(BLOCK) void k_seed(const int32_t* __restrict__ list, int64_t n_in, int32_t n,
-                                                const int32_t* __restrict__ color,
-                                                const unsigned long long* __restrict__ pkey,
-                                                uint32_t* __restrict__ V, int32_t* __restrict__ q_out,
-                                                unsigned long long* __restrict__ ctr) {
+// Seed a search: every live vertex that is its partition's pivot marks itself
+// in V and enters the frontier q0 (dc->lv.nq).  off gives the traversal-
+// direction degree for the first switch decision.
+__global__ __launch_bounds__(BLOCK) void k_seed_pivots(DevCtl* __restrict__ dc, int32_t n,
+                                                       const int32_t* __restrict__ off,
+                                                       const int32_t* __restrict__ color,
+                                                       const unsigned long long* __restrict__ pkey,
+                                                       uint32_t* __restrict__ V, int32_t* __restrict__ q0,
+                                                       const int32_t* __restrict__ L0,
+                                                       const int32_t* __restrict__ L1) {
+  const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
+  const int64_t n_in = (int64_t)dc->live.n;
+  unsigned long long edges = 0, cnt = 0;
   for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
     const int64_t i = base + threadIdx.x;
     bool piv = false;
@@ -256,10 +348,47 @@ __global__ __launch_bounds__(BLOCK) void k_seed(const int32_t* __restrict__ list
       v = list[i];
       const int32_t c = color[v];
       piv = c != DONE && pivot_of_key(pkey[slot_of(c, n)]) == v;
-      if (piv) atomicOr(&V[v >> 5], 1u << (v & 31));
+      if (piv) {
+        atomicOr(&V[v >> 5], 1u << (v & 31));
+        edges += (unsigned long long)(off[v + 1] - off[v]);
+        cnt++;
+        dc->ctr[C_PIVOT] = (unsigned long long)v;  // the pivot, when there is one partition
+      }
     }
-    wave_append(piv, v, q_out, &ctr[C_Q_OUT]);
+    wave_append(piv, v, q0, &dc->lv.nq);
   }
+  block_count(edges, &dc->lv.mf);
+  block_count(cnt, &dc->lv.visited);
+}
+
+// Seed a search from an explicit vertex list (count at *ns): vertices not yet
+// in V and not of colour -2 enter the frontier once.
+__global__ __launch_bounds__(BLOCK) void k_seed_list(DevCtl* __restrict__ dc, const int32_t* __restrict__ off,
+                                                     const int32_t* __restrict__ color,
+                                                     const int32_t* __restrict__ seeds,
+                                                     const unsigned long long* __restrict__ ns,
+                                                     uint32_t* __restrict__ V, int32_t* __restrict__ q0) {
+  const int64_t n_in = (int64_t)*ns;
+  unsigned long long edges = 0, cnt = 0;
+  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
+    const int64_t i = base + threadIdx.x;
+    bool take = false;
+    int32_t v = 0;
+    if (i < n_in) {
+      v = seeds[i];
+      if (color[v] != DONE) {
+        const uint32_t bit = 1u << (v & 31);
+        take = !(atomicOr(&V[v >> 5], bit) & bit);
+        if (take) {
+          edges += (unsigned long long)(off[v + 1] - off[v]);
+          cnt++;
+        }
+      }
+    }
+    wave_append(take, v, q0, &dc->lv.nq);
+  }
+  block_count(edges, &dc->lv.mf);
+  block_count(cnt, &dc->lv.visited);
 }
 
 // ---- reachability (direction-optimising, partition-aware) ----------------------
@@ -267,7 +396,7 @@ __global__ __launch_bounds__(BLOCK) void k_seed(const int32_t* __restrict__ list
 // search of src/scc_fb_global.cc:32-129 / gm_bfs_template.h, for every
 // partition at once: a vertex is admitted when it is reached from a visited
 // vertex of the same colour (the navigator of src/scc_fb_global.cc:64-68 and
-// :123-127; during a step colours do not change, so "same colour" is the
+// :123-127; during a search colours do not change, so "same colour" is the
 // partition).  Visited state is a bitmap, claimed with atomicOr.
 
 __device__ __forceinline__ bool try_claim(uint32_t* __restrict__ V, const int32_t* __restrict__ color,
@@ -278,176 +407,253 @@ __device__ __forceinline__ bool try_claim(uint32_t* __restrict__ V, const int32_
   return !(atomicOr(&V[k >> 5], bit) & bit);
 }
 
-// Top-down: expand every frontier vertex's row.  Rows of at least HUGE_ROW
-// entries are deferred to k_td_huge (appended to `huge`).  `noff` gives the
-// degree used to estimate the next level's work (C_Q_EDGES).
-__global__ __launch_bounds__(BLOCK) void k_td(const int32_t* __restrict__ off, const int32_t* __restrict__ nb,
-                                              const int32_t* __restrict__ color, uint32_t* __restrict__ V,
-                                              const int32_t* __restrict__ q_in, int64_t n_qin,
-                                              int32_t* __restrict__ q_out, int32_t* __restrict__ huge,
-                                              unsigned long long* __restrict__ ctr) {
+constexpr int TD_CHUNK = 2048;
+
+// One level.  Top-down: expand the frontier (rows of at least HUGE_ROW entries
+// are deferred to the next launch, which cuts them into TD_CHUNK pieces spread
+// over the whole grid).  Bottom-up: every candidate not yet visited scans its
+// opposite-direction row for a visited parent of its colour
+// (gm_bfs_template.h:281-301, :810-829); a parent admitted earlier in the same
+// launch also counts (reachability needs no levels).  The last workgroup
+// applies Beamer's rule (BU when the frontier's edges exceed the candidates'
+// edges / 14, back to TD when the frontier shrinks below n/24) and ends the
+// search when a level admits nothing.  A switch to BU drops deferred rows:
+// their neighbours find them as parents.
+__global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int multi,
+                                                 const int32_t* __restrict__ off, const int32_t* __restrict__ nb,
+                                                 const int32_t* __restrict__ roff, const int32_t* __restrict__ rnb,
+                                                 const int32_t* __restrict__ color, uint32_t* __restrict__ V,
+                                                 int32_t* __restrict__ q0, int32_t* __restrict__ q1,
+                                                 int32_t* __restrict__ c0, int32_t* __restrict__ c1,
+                                                 int32_t* __restrict__ h0, int32_t* __restrict__ h1,
+                                                 const int32_t* __restrict__ L0, const int32_t* __restrict__ L1,
+                                                 int32_t max_chunks) {
+  __shared__ int s_done, s_mode, s_q, s_c, s_h;
+  __shared__ unsigned long long s_nq, s_nh, s_nc;
+  LevelCtl* lv = &dc->lv;
+  if (threadIdx.x == 0) {
+    s_done = lv->done;
+    s_mode = lv->mode;
+    s_q = lv->q_sel;
+    s_c = lv->c_sel;
+    s_h = lv->h_sel;
+    s_nq = lv->nq;
+    s_nh = lv->nh;
+    s_nc = lv->nc;
+  }
+  __syncthreads();
+  if (s_done) return;
   const int lane = lane_id();
   unsigned long long edges = 0;
-  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_qin; base += (int64_t)gridDim.x * BLOCK) {
-    const int64_t i = base + threadIdx.x;
-    const bool valid = i < n_qin;
-    int32_t u = 0, s = 0, e = 0, c = DONE;
-    if (valid) {
-      u = q_in[i];
-      s = off[u];
-      e = off[u + 1];
-      c = color[u];
-    }
-    const int32_t d = e - s;
-    const bool hug = valid && d >= HUGE_ROW;
-    wave_append(hug, u, huge, &ctr[C_HUGE]);
-    const bool act = valid && !hug;
-    const int lim = act ? min(d, LIGHT) : 0;
-    const int mx = wave_max(lim);
-    for (int j = 0; j < mx; j++) {
-      int32_t k = 0;
-      bool cl = false;
-      if (j < lim) {
-        k = nb[s + j];
-        cl = try_claim(V, color, k, c);
-        if (cl) edges += (unsigned long long)(off[k + 1] - off[k]);
+  using QOut = LdsOut<0>;
+  using COut = LdsOut<1>;
+  QOut::init();
+  COut::init();
+  int32_t* q_out_g = s_q ? q0 : q1;
+  if (s_mode == MODE_TD) {
+    const int32_t* __restrict__ q_in = s_q ? q1 : q0;
+    int32_t* __restrict__ q_out = s_q ? q0 : q1;
+    const int32_t* __restrict__ h_in = s_h ? h1 : h0;
+    int32_t* __restrict__ h_out = s_h ? h0 : h1;
+    // (a) rows deferred by the previous level, chunked over the grid
+    const int64_t nh = (int64_t)s_nh;
+    if (nh > 0) {
+      const int64_t total = nh * max_chunks;
+      for (int64_t g = blockIdx.x; g < total; g += gridDim.x) {
+        QOut::maybe_flush(q_out, &lv->out_q, BLOCK);
+        const int32_t u = h_in[g % nh];
+        const int32_t ch = (int32_t)(g / nh);
+        const int32_t s = off[u] + ch * TD_CHUNK;
+        const int32_t e = min(s + TD_CHUNK, off[u + 1]);
+        if (s >= e) continue;  // uniform across the workgroup
+        const int32_t c = color[u];
+        for (int32_t b = s; b < e; b += BLOCK) {
+          const int32_t idx = b + threadIdx.x;
+          int32_t k = 0;
+          bool cl = false;
+          if (idx < e) {
+            k = nb[idx];
+            cl = try_claim(V, color, k, c);
+            if (cl) edges += (unsigned long long)(off[k + 1] - off[k]);
+          }
+          QOut::push(cl, k, q_out, &lv->out_q);
+        }
       }
-      wave_append(cl, k, q_out, &ctr[C_Q_OUT]);
     }
-    uint64_t hm = __ballot(act && d > LIGHT);
-    while (hm) {
-      const int L = __ffsll((unsigned long long)hm) - 1;
-      hm &= hm - 1;
-      const int32_t hs = __shfl(s, L) + LIGHT, he = __shfl(e, L), hc = __shfl(c, L);
-      for (int32_t b = hs; b < he; b += 64) {
-        const int32_t idx = b + lane;
+    // (b) the frontier
+    const int64_t nq = (int64_t)s_nq;
+    for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < nq; base += (int64_t)gridDim.x * BLOCK) {
+      QOut::maybe_flush(q_out, &lv->out_q, BLOCK);
+      const int64_t i = base + threadIdx.x;
+      const bool valid = i < nq;
+      int32_t u = 0, s = 0, e = 0, c = DONE;
+      if (valid) {
+        u = q_in[i];
+        s = off[u];
+        e = off[u + 1];
+        c = color[u];
+      }
+      const int32_t d = e - s;
+      const bool hug = valid && d >= HUGE_ROW;
+      wave_append(hug, u, h_out, &lv->out_h);
+      const bool act = valid && !hug;
+      const int lim = act ? min(d, LIGHT) : 0;
+      const int mx = wave_max(lim);
+      for (int j = 0; j < mx; j++) {
         int32_t k = 0;
         bool cl = false;
-        if (idx < he) {
-          k = nb[idx];
-          cl = try_claim(V, color, k, hc);
+        if (j < lim) {
+          k = nb[s + j];
+          cl = try_claim(V, color, k, c);
           if (cl) edges += (unsigned long long)(off[k + 1] - off[k]);
         }
-        wave_append(cl, k, q_out, &ctr[C_Q_OUT]);
+        QOut::push(cl, k, q_out, &lv->out_q);
       }
-    }
-  }
-  edges = wave_sum_u64(edges);
-  if (lane == 0 && edges) atomicAdd(&ctr[C_Q_EDGES], edges);
-}
-
-// Top-down expansion of the deferred huge rows: the rows are cut into chunks of
-// CHUNK entries and the grid's workgroups take chunks round-robin.
-constexpr int TD_CHUNK = 4096;
-__global__ __launch_bounds__(BLOCK) void k_td_huge(const int32_t* __restrict__ off,
-                                                   const int32_t* __restrict__ nb,
-                                                   const int32_t* __restrict__ color, uint32_t* __restrict__ V,
-                                                   const int32_t* __restrict__ huge, int64_t n_huge,
-                                                   int32_t* __restrict__ q_out,
-                                                   unsigned long long* __restrict__ ctr) {
-  int64_t total = 0;
-  for (int64_t h = 0; h < n_huge; h++) {
-    const int32_t u = huge[h];
-    total += (off[u + 1] - off[u] + TD_CHUNK - 1) / TD_CHUNK;
-  }
-  unsigned long long edges = 0;
-  for (int64_t g = blockIdx.x; g < total; g += gridDim.x) {
-    int64_t acc = 0, h = 0;
-    int32_t u = 0, nch = 0;
-    for (;; h++) {
-      u = huge[h];
-      nch = (off[u + 1] - off[u] + TD_CHUNK - 1) / TD_CHUNK;
-      if (g < acc + nch) break;
-      acc += nch;
-    }
-    const int32_t s = off[u] + (int32_t)(g - acc) * TD_CHUNK;
-    const int32_t e = min(s + TD_CHUNK, off[u + 1]);
-    const int32_t c = color[u];
-    for (int32_t b = s; b < e; b += BLOCK) {
-      const int32_t idx = b + threadIdx.x;
-      int32_t k = 0;
-      bool cl = false;
-      if (idx < e) {
-        k = nb[idx];
-        cl = try_claim(V, color, k, c);
-        if (cl) edges += (unsigned long long)(off[k + 1] - off[k]);
-      }
-      wave_append(cl, k, q_out, &ctr[C_Q_OUT]);
-    }
-  }
-  edges = wave_sum_u64(edges);
-  if (lane_id() == 0 && edges) atomicAdd(&ctr[C_Q_EDGES], edges);
-}
-
-// Bottom-up: every candidate not yet visited scans its opposite-direction row
-// for a visited parent of its colour (gm_bfs_template.h:281-301, :810-829).
-// A parent visited earlier in the same pass also counts (reachability needs no
-// levels).  Newly visited vertices form the next frontier; the rest stay
-// candidates.  MULTI=false when only one partition is being searched, where a
-// visited parent is in the partition by construction.
-template <bool MULTI>
-__global__ __launch_bounds__(BLOCK) void k_bu(const int32_t* __restrict__ roff, const int32_t* __restrict__ rnb,
-                                              const int32_t* __restrict__ off,
-                                              const int32_t* __restrict__ color, uint32_t* __restrict__ V,
-                                              const int32_t* __restrict__ c_in, int64_t n_cin,
-                                              int32_t* __restrict__ c_out, int32_t* __restrict__ q_out,
-                                              unsigned long long* __restrict__ ctr) {
-  auto reached = [V, color](int32_t k, int32_t pv, int32_t pc) {
-    (void)pv;
-    if (!((V[k >> 5] >> (k & 31)) & 1u)) return false;
-    return !MULTI || color[k] == pc;
-  };
-  unsigned long long edges = 0;
-  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_cin; base += (int64_t)gridDim.x * BLOCK) {
-    const int64_t i = base + threadIdx.x;
-    const bool valid = i < n_cin;
-    int32_t v = 0, c = DONE, s = 0, e = 0;
-    bool act = false;
-    if (valid) {
-      v = c_in[i];
-      if (!bit_test(V, v)) {
-        c = color[v];
-        act = c != DONE;
-        if (act) {
-          s = roff[v];
-          e = roff[v + 1];
+      uint64_t hm = __ballot(act && d > LIGHT);
+      while (hm) {
+        const int L = __ffsll((unsigned long long)hm) - 1;
+        hm &= hm - 1;
+        const int32_t hs = __shfl(s, L) + LIGHT, he = __shfl(e, L), hc = __shfl(c, L);
+        for (int32_t b = hs; b < he; b += 64) {
+          const int32_t idx = b + lane;
+          int32_t k = 0;
+          bool cl = false;
+          if (idx < he) {
+            k = nb[idx];
+            cl = try_claim(V, color, k, hc);
+            if (cl) edges += (unsigned long long)(off[k + 1] - off[k]);
+          }
+          QOut::push(cl, k, q_out, &lv->out_q);
         }
       }
     }
-    const bool found = row_any(act, v, c, s, e, rnb, reached);
-    if (found) {
-      atomicOr(&V[v >> 5], 1u << (v & 31));
-      edges += (unsigned long long)(off[v + 1] - off[v]);
+  } else {
+    const int32_t* __restrict__ c_in = (s_c < 0) ? LIVE_IN(dc, L0, L1) : (s_c ? c1 : c0);
+    const int64_t ncin = (s_c < 0) ? (int64_t)dc->live.n : (int64_t)s_nc;
+    int32_t* __restrict__ c_out = (s_c == 0) ? c1 : c0;
+    int32_t* __restrict__ q_out = s_q ? q0 : q1;
+    auto reached = [V, color, multi](int32_t k, int32_t pv, int32_t pc) {
+      (void)pv;
+      if (!((V[k >> 5] >> (k & 31)) & 1u)) return false;
+      return !multi || color[k] == pc;
+    };
+    for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < ncin; base += (int64_t)gridDim.x * BLOCK) {
+      QOut::maybe_flush(q_out, &lv->out_q, BLOCK);
+      COut::maybe_flush(c_out, &lv->out_c, BLOCK);
+      const int64_t i = base + threadIdx.x;
+      const bool valid = i < ncin;
+      int32_t v = 0, c = DONE, s = 0, e = 0;
+      bool act = false;
+      if (valid) {
+        v = c_in[i];
+        if (!bit_test(V, v)) {
+          c = color[v];
+          act = c != DONE;
+          if (act) {
+            s = roff[v];
+            e = roff[v + 1];
+          }
+        }
+      }
+      const bool found = row_any(act, v, c, s, e, rnb, reached);
+      if (found) {
+        atomicOr(&V[v >> 5], 1u << (v & 31));
+        edges += (unsigned long long)(off[v + 1] - off[v]);
+      }
+      QOut::push(found, v, q_out, &lv->out_q);
+      COut::push(act && !found, v, c_out, &lv->out_c);
     }
-    block_append(found, v, q_out, &ctr[C_Q_OUT]);
-    block_append(act && !found, v, c_out, &ctr[C_CAND_OUT]);
+    COut::flush(c_out, &lv->out_c);
   }
-  block_count(edges, &ctr[C_Q_EDGES]);
+  QOut::flush(q_out_g, &lv->out_q);
+  block_count(edges, &lv->out_edges);
+  if (last_block2(dc->done) && threadIdx.x == 0) {
+    const unsigned long long oq = ld_acq(&lv->out_q), oc = ld_acq(&lv->out_c);
+    const unsigned long long oh = ld_acq(&lv->out_h), oe = ld_acq(&lv->out_edges);
+    const unsigned long long nlive = dc->live.n;
+    lv->visited += oq;
+    lv->levels++;
+    if (s_mode == MODE_TD) {
+      lv->done = (oq == 0 && oh == 0);
+      lv->h_sel ^= 1;
+      lv->nh = oh;
+    } else {
+      lv->done = (oq == 0);
+      lv->c_sel = (s_c == 0) ? 1 : 0;
+      lv->nc = oc;
+      lv->nh = 0;
+    }
+    lv->q_sel ^= 1;
+    lv->prev_nq = lv->nq;
+    lv->nq = oq;
+    lv->mf = oe;
+    if (s_mode == MODE_TD) {
+      const double ncand = (lv->c_sel < 0) ? (double)nlive : (double)lv->nc;
+      if ((double)oe > ncand * lv->bu_edges_per_cand && ncand > 0) {
+        lv->mode = MODE_BU;
+        lv->nh = 0;
+      }
+    } else if (oq < lv->prev_nq && oq * 24 < nlive) {
+      lv->mode = MODE_TD;
+    }
+    lv->out_q = lv->out_c = lv->out_h = lv->out_edges = 0;
+    lv->blocks_done = 0;
+  }
 }
 
-template __global__ void k_bu<true>(const int32_t*, const int32_t*, const int32_t*, const int32_t*,
-                                    uint32_t*, const int32_t*, int64_t, int32_t*, int32_t*,
-                                    unsigned long long*);
-template __global__ void k_bu<false>(const int32_t*, const int32_t*, const int32_t*, const int32_t*,
-                                     uint32_t*, const int32_t*, int64_t, int32_t*, int32_t*,
-                                     unsigned long long*);
+// Sizes of a step's reach sets for the traffic model: |F|, |B|, E_F (out-degrees
+// over F), E_B (in-degrees over B), |F n B|.
+__global__ __launch_bounds__(BLOCK) void k_fb_stats(DevCtl* __restrict__ dc, GraphPtrs G,
+                                                    const uint32_t* __restrict__ vf,
+                                                    const uint32_t* __restrict__ vb,
+                                                    const int32_t* __restrict__ L0,
+                                                    const int32_t* __restrict__ L1) {
+  const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
+  const int64_t n_in = (int64_t)dc->live.n;
+  unsigned long long f = 0, b = 0, ef = 0, eb = 0, fb = 0;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_in; i += (int64_t)gridDim.x * BLOCK) {
+    const int32_t v = list[i];
+    const bool inf = bit_test(vf, v), inb = bit_test(vb, v);
+    if (inf) {
+      f++;
+      ef += (unsigned long long)(G.begin[v + 1] - G.begin[v]);
+    }
+    if (inb) {
+      b++;
+      eb += (unsigned long long)(G.rbegin[v + 1] - G.rbegin[v]);
+    }
+    fb += (inf && inb) ? 1 : 0;
+  }
+  block_count(f, &dc->ctr[C_F]);
+  block_count(b, &dc->ctr[C_B]);
+  block_count(ef, &dc->ctr[C_EF]);
+  block_count(eb, &dc->ctr[C_EB]);
+  block_count(fb, &dc->ctr[C_FB]);
+}
 
 // ---- classification after a forward-backward step -------------------------------
 // src/scc_fb_global.cc:99-113: forward and backward reached -> SCC of the pivot
 // (label pivot, colour -2).  The rest split into fw-only / bw-only / neither
 // (src/scc_fb_seq2.cc:245-285); each new part's colour is its minimum member,
 // gathered here with atomicMin and applied by k_recolor.
-__global__ __launch_bounds__(BLOCK) void k_classify(const int32_t* __restrict__ list, int64_t n_in,
-                                                    int32_t n, int32_t* __restrict__ color,
-                                                    int32_t* __restrict__ label,
+__global__ __launch_bounds__(BLOCK) void k_classify(DevCtl* __restrict__ dc, int32_t n,
+                                                    int32_t* __restrict__ color, int32_t* __restrict__ label,
                                                     const uint32_t* __restrict__ vf,
                                                     const uint32_t* __restrict__ vb,
                                                     const unsigned long long* __restrict__ pkey,
-                                                    int32_t* __restrict__ rep, int32_t* __restrict__ list_out,
-                                                    unsigned long long* __restrict__ ctr) {
+                                                    int32_t* __restrict__ rep, int32_t* __restrict__ L0,
+                                                    int32_t* __restrict__ L1) {
+  const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
+  int32_t* __restrict__ list_out = LIVE_OUT(dc, L0, L1);
+  const int64_t n_in = (int64_t)dc->live.n;
+  unsigned long long nscc = 0;
+  using Out = LdsOut<0>;
+  Out::init();
   for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
+    Out::maybe_flush(list_out, &dc->ctr[C_LIST_OUT], BLOCK);
     const int64_t i = base + threadIdx.x;
-    bool keep = false, scc = false;
+    bool keep = false;
+    int64_t rslot = 0;
     int32_t v = 0;
     if (i < n_in) {
       v = list[i];
@@ -458,28 +664,38 @@ __global__ __launch_bounds__(BLOCK) void k_classify(const int32_t* __restrict__ 
         if (f && b) {
           label[v] = pivot_of_key(pkey[s]);
           color[v] = DONE;
-          scc = true;
+          nscc++;
         } else {
-          const int t = f ? 0 : (b ? 1 : 2);
-          atomicMin(&rep[3 * (int64_t)s + t], v);
+          rslot = 3 * (int64_t)s + (f ? 0 : (b ? 1 : 2));
           keep = true;
         }
       }
     }
-    block_count(scc ? 1ull : 0ull, &ctr[C_SCC]);
-    block_append(keep, v, list_out, &ctr[C_LIST_OUT]);
+    grouped_min_i32(keep, rslot, v, rep);
+    Out::push(keep, v, list_out, &dc->ctr[C_LIST_OUT]);
+  }
+  Out::flush(list_out, &dc->ctr[C_LIST_OUT]);
+  block_count(nscc, &dc->ctr[C_SCC]);
+  if (last_block2(dc->done) && threadIdx.x == 0) {
+    dc->live.n = ld_acq(&dc->ctr[C_LIST_OUT]);
+    dc->live.sel ^= 1;
+    dc->ctr[C_LIST_OUT] = 0;
+    dc->blocks_done = 0;
   }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_recolor(const int32_t* __restrict__ list, int64_t n_in, int32_t n,
+__global__ __launch_bounds__(BLOCK) void k_recolor(const DevCtl* __restrict__ dc, int32_t n,
                                                    int32_t* __restrict__ color,
                                                    const uint32_t* __restrict__ vf,
                                                    const uint32_t* __restrict__ vb,
-                                                   const int32_t* __restrict__ rep) {
+                                                   const int32_t* __restrict__ rep,
+                                                   const int32_t* __restrict__ L0,
+                                                   const int32_t* __restrict__ L1) {
+  const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
+  const int64_t n_in = (int64_t)dc->live.n;
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_in; i += (int64_t)gridDim.x * BLOCK) {
     const int32_t v = list[i];
-    const int32_t c = color[v];
-    const int32_t s = slot_of(c, n);
+    const int32_t s = slot_of(color[v], n);
     const bool f = bit_test(vf, v), b = bit_test(vb, v);
     const int t = f ? 0 : (b ? 1 : 2);
     color[v] = rep[3 * (int64_t)s + t];
@@ -488,9 +704,12 @@ __global__ __launch_bounds__(BLOCK) void k_recolor(const int32_t* __restrict__ l
 
 // ---- WCC colouring --------------------------------------------------------------
 // src/scc_weak.cc:128-150: W[v] = v, tagged when out-degree >= 50.
-__global__ __launch_bounds__(BLOCK) void k_wcc_init(const int32_t* __restrict__ list, int64_t n_in,
+__global__ __launch_bounds__(BLOCK) void k_wcc_init(const DevCtl* __restrict__ dc,
                                                     const int32_t* __restrict__ begin,
-                                                    int32_t* __restrict__ W) {
+                                                    int32_t* __restrict__ W, const int32_t* __restrict__ L0,
+                                                    const int32_t* __restrict__ L1) {
+  const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
+  const int64_t n_in = (int64_t)dc->live.n;
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_in; i += (int64_t)gridDim.x * BLOCK) {
     const int32_t v = list[i];
     W[v] = (begin[v + 1] - begin[v] >= 50) ? (v | WCC_TAG) : v;
@@ -501,15 +720,22 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_init(const int32_t* __restrict__ 
 // the minimum W over same-colour out-neighbours, then jump to the root's value
 // when the root is not self-rooted.  Each vertex writes only its own W and W
 // only decreases, so racing reads change the path to the fixpoint, not the
-// fixpoint (W[v] = min tag over v's same-colour forward-reachable set).
-__global__ __launch_bounds__(BLOCK) void k_wcc_iter(const int32_t* __restrict__ list, int64_t n_in,
-                                                    const int32_t* __restrict__ begin,
+// fixpoint (W[v] = min tag over v's same-colour forward-reachable set).  One
+// launch is one iteration; the last workgroup ends the loop after an iteration
+// that changed nothing.
+__global__ __launch_bounds__(BLOCK) void k_wcc_step(DevCtl* __restrict__ dc, const int32_t* __restrict__ begin,
                                                     const int32_t* __restrict__ adj,
                                                     const int32_t* __restrict__ color,
-                                                    int32_t* __restrict__ W,
-                                                    unsigned long long* __restrict__ ctr) {
+                                                    int32_t* __restrict__ W, const int32_t* __restrict__ L0,
+                                                    const int32_t* __restrict__ L1) {
+  __shared__ int s_done;
+  if (threadIdx.x == 0) s_done = dc->loop.done;
+  __syncthreads();
+  if (s_done) return;
+  const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
+  const int64_t n_in = (int64_t)dc->live.n;
   const int lane = lane_id();
-  bool changed_any = false;
+  unsigned long long changed = 0;
   for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
     const int64_t i = base + threadIdx.x;
     const bool valid = i < n_in;
@@ -519,14 +745,13 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_iter(const int32_t* __restrict__ 
       c = color[v];
       s = begin[v];
       e = begin[v + 1];
-      w0 = m = __hip_atomic_load(&W[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      w0 = m = W[v];
     }
     const bool act = valid && c != DONE;
-    // lane-serial over short rows, wave-cooperative min over long ones
     const int lim = act ? min(e - s, 64) : 0;
     for (int j = 0; j < lim; j++) {
       const int32_t k = adj[s + j];
-      if (color[k] == c) m = min(m, __hip_atomic_load(&W[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      if (color[k] == c) m = min(m, W[k]);
     }
     uint64_t hm = __ballot(act && e - s > 64);
     while (hm) {
@@ -538,8 +763,7 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_iter(const int32_t* __restrict__ 
         const int32_t idx = b + lane;
         if (idx < he) {
           const int32_t k = adj[idx];
-          if (color[k] == hc)
-            hmin = min(hmin, __hip_atomic_load(&W[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+          if (color[k] == hc) hmin = min(hmin, W[k]);
         }
       }
 #pragma unroll
@@ -549,69 +773,163 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_iter(const int32_t* __restrict__ 
     if (act) {
       const int32_t r = m & WCC_MASK;
       if (r != v) {
-        const int32_t wr = __hip_atomic_load(&W[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int32_t wr = W[r];
         if ((wr & WCC_MASK) != r) m = min(m, wr);
       }
       if (m < w0) {
-        __hip_atomic_store(&W[v], m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        changed_any = true;
+        W[v] = m;
+        changed++;
       }
     }
   }
-  if (__ballot(changed_any) != 0 && lane == 0) atomicAdd(&ctr[C_CHANGED], 1ull);
+  block_count(changed, &dc->loop.acc0);
+  if (last_block2(dc->done) && threadIdx.x == 0) {
+    LoopCtl& l = dc->loop;
+    const unsigned long long ch = ld_acq(&l.acc0);
+    l.total0 += ch;
+    l.iters++;
+    l.done = ch == 0;
+    l.acc0 = 0;
+    l.blocks_done = 0;
+  }
 }
 
-// src/scc_weak.cc:166-194: each class takes its root's id as colour (a fresh,
-// unique colour: the root is a member of the class); counts roots.
-__global__ __launch_bounds__(BLOCK) void k_wcc_color(const int32_t* __restrict__ list, int64_t n_in,
-                                                     int32_t* __restrict__ color,
-                                                     const int32_t* __restrict__ W,
-                                                     unsigned long long* __restrict__ ctr) {
+// src/scc_weak.cc:166-194: each class takes its root's id as colour (a unique
+// colour: the root is a member of its class); counts roots.  With seed != 0 the
+// roots also seed a forward search (V, q0, dc->lv) for root-SCC extraction.
+__global__ __launch_bounds__(BLOCK) void k_wcc_color(DevCtl* __restrict__ dc, int32_t* __restrict__ color,
+                                                     const int32_t* __restrict__ W, int seed,
+                                                     const int32_t* __restrict__ off,
+                                                     uint32_t* __restrict__ V, int32_t* __restrict__ q0,
+                                                     const int32_t* __restrict__ L0,
+                                                     const int32_t* __restrict__ L1) {
+  const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
+  const int64_t n_in = (int64_t)dc->live.n;
+  unsigned long long roots = 0, edges = 0;
+  using Out = LdsOut<0>;
+  Out::init();
   for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
+    if (seed) Out::maybe_flush(q0, &dc->lv.nq, BLOCK);
     const int64_t i = base + threadIdx.x;
     bool root = false;
+    int32_t v = 0;
     if (i < n_in) {
-      const int32_t v = list[i];
+      v = list[i];
       if (color[v] != DONE) {
         const int32_t r = W[v] & WCC_MASK;
         color[v] = r;
         root = r == v;
+        if (root) {
+          roots++;
+          if (seed) {
+            atomicOr(&V[v >> 5], 1u << (v & 31));
+            edges += (unsigned long long)(off[v + 1] - off[v]);
+          }
+        }
       }
     }
-    block_count(root ? 1ull : 0ull, &ctr[C_ROOTS]);
+    if (seed) Out::push(root, v, q0, &dc->lv.nq);
+  }
+  if (seed) Out::flush(q0, &dc->lv.nq);
+  block_count(roots, &dc->ctr[C_ROOTS]);
+  if (seed) {
+    block_count(edges, &dc->lv.mf);
+    block_count(roots, &dc->lv.visited);
+  }
+}
+
+// Root-SCC extraction after a colouring: every class member reaches its root
+// (W is a minimum over the forward-reachable set and the path to the root stays
+// in the class), so the root's forward reach inside its class is its SCC.
+// Reached vertices take the root (their colour) as label and leave the list.
+__global__ __launch_bounds__(BLOCK) void k_extract(DevCtl* __restrict__ dc, int32_t* __restrict__ color,
+                                                   int32_t* __restrict__ label,
+                                                   const uint32_t* __restrict__ V, int32_t* __restrict__ L0,
+                                                   int32_t* __restrict__ L1) {
+  const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
+  int32_t* __restrict__ list_out = LIVE_OUT(dc, L0, L1);
+  const int64_t n_in = (int64_t)dc->live.n;
+  unsigned long long nscc = 0;
+  using Out = LdsOut<0>;
+  Out::init();
+  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
+    Out::maybe_flush(list_out, &dc->ctr[C_LIST_OUT], BLOCK);
+    const int64_t i = base + threadIdx.x;
+    bool keep = false;
+    int32_t v = 0;
+    if (i < n_in) {
+      v = list[i];
+      const int32_t c = color[v];
+      if (c != DONE) {
+        if (bit_test(V, v)) {
+          label[v] = c;
+          color[v] = DONE;
+          nscc++;
+        } else {
+          keep = true;
+        }
+      }
+    }
+    Out::push(keep, v, list_out, &dc->ctr[C_LIST_OUT]);
+  }
+  Out::flush(list_out, &dc->ctr[C_LIST_OUT]);
+  block_count(nscc, &dc->ctr[C_SCC]);
+  if (last_block2(dc->done) && threadIdx.x == 0) {
+    dc->live.n = ld_acq(&dc->ctr[C_LIST_OUT]);
+    dc->live.sel ^= 1;
+    dc->ctr[C_LIST_OUT] = 0;
+    dc->blocks_done = 0;
   }
 }
 
 // ---- canonical labels -----------------------------------------------------------
+// mn[l] = min vertex labelled l.  Most labels are shared by one vertex; the
+// forward-backward pivots label whole SCCs, so lanes of a wave that share a
+// label first reduce among themselves and one lane issues the atomic (test
+// first: the slot usually already holds a smaller id).
 __global__ __launch_bounds__(BLOCK) void k_canon_min(int32_t n, const int32_t* __restrict__ label,
                                                      int32_t* __restrict__ mn) {
-  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < n; v += (int64_t)gridDim.x * BLOCK) {
-    const int32_t l = label[v];
-    if ((uint32_t)l < (uint32_t)n) atomicMin(&mn[l], (int32_t)v);
-  }
-}
-
-__global__ __launch_bounds__(BLOCK) void k_canon_apply(int32_t n, const int32_t* __restrict__ label,
-                                                       const int32_t* __restrict__ mn,
-                                                       int32_t* __restrict__ out,
-                                                       unsigned long long* __restrict__ ctr) {
+  const int lane = lane_id();
   for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n; base += (int64_t)gridDim.x * BLOCK) {
     const int64_t v = base + threadIdx.x;
-    bool self = false;
-    if (v < n) {
-      const int32_t l = label[v];
-      const int32_t c = ((uint32_t)l < (uint32_t)n) ? mn[l] : -1;
-      out[v] = c;
-      self = c == (int32_t)v;
+    int32_t l = -1;
+    if (v < n) l = label[v];
+    bool pending = (uint32_t)l < (uint32_t)n;
+    if (pending && l == (int32_t)v) {
+      // self-labelled: contention only with the rest of its SCC
+      if ((int32_t)v < mn[l]) atomicMin(&mn[l], (int32_t)v);
+      pending = false;
     }
-    block_count(self ? 1ull : 0ull, &ctr[C_COUNT]);
+    uint64_t act = __ballot(pending);
+    while (act) {
+      const int leader = __ffsll((unsigned long long)act) - 1;
+      const int32_t L = __shfl(l, leader);
+      const bool mine = pending && l == L;
+      int32_t x = mine ? (int32_t)v : INT32_MAX;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o));
+      if (lane == leader && x < mn[L]) atomicMin(&mn[L], x);
+      if (mine) pending = false;
+      act = __ballot(pending);
+    }
   }
 }
 
-}  // namespace sccmi
+__global__ __launch_bounds__(BLOCK) void k_canon_apply(DevCtl* __restrict__ dc, int32_t n,
+                                                       const int32_t* __restrict__ label,
+                                                       const int32_t* __restrict__ mn,
+                                                       int32_t* __restrict__ out) {
+  unsigned long long self = 0;
+  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < n; v += (int64_t)gridDim.x * BLOCK) {
+    const int32_t l = label[v];
+    const int32_t c = ((uint32_t)l < (uint32_t)n) ? mn[l] : -1;
+    out[v] = c;
+    self += (c == (int32_t)v) ? 1 : 0;
+  }
+  block_count(self, &dc->ctr[C_COUNT]);
+}
 
 // ---- host launchers -------------------------------------------------------------
-namespace sccmi {
 
 #define LAUNCH(kern, grid, st, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, st, __VA_ARGS__)
 
@@ -624,92 +942,91 @@ void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, int64_t n) {
 void launch_iota(hipStream_t st, int32_t* p, int32_t n) {
   if (n > 0) LAUNCH(k_iota, grid_for(n, 4096), st, p, n);
 }
-void launch_trim1(hipStream_t st, const int32_t* begin, const int32_t* adj, const int32_t* rbegin,
-                  const int32_t* radj, int32_t* color, int32_t* label, const int32_t* list_in,
-                  int64_t n_in, int32_t* list_out, unsigned long long* ctr) {
-  if (n_in > 0)
-    LAUNCH(k_trim1, grid_for(n_in), st, begin, adj, rbegin, radj, color, label, list_in, n_in,
-           list_out, ctr);
+void launch_ctl_init(hipStream_t st, DevCtl* dc, int what, int32_t live_sel, unsigned long long live_n,
+                     long long stop, double bu_ratio) {
+  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, dc, what, live_sel, live_n, stop, bu_ratio);
 }
-void launch_trim2(hipStream_t st, const int32_t* begin, const int32_t* adj, const int32_t* rbegin,
-                  const int32_t* radj, int32_t* color, int32_t* label, const int32_t* list_in,
-                  int64_t n_in, unsigned long long* ctr) {
-  if (n_in > 0)
-    LAUNCH(k_trim2, grid_for(n_in), st, begin, adj, rbegin, radj, color, label, list_in, n_in, ctr);
+void launch_trim1_step(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, int32_t* color,
+                       int32_t* label, int32_t* L0, int32_t* L1) {
+  LAUNCH(k_trim1_step, grid, st, dc, G, color, label, L0, L1);
 }
-void launch_reset_slots(hipStream_t st, const int32_t* list, int64_t n_in, int32_t n,
-                        const int32_t* color, unsigned long long* pkey, int32_t* rep) {
-  if (n_in > 0) LAUNCH(k_reset_slots, grid_for(n_in), st, list, n_in, n, color, pkey, rep);
+void launch_compact(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* color, int32_t* L0,
+                    int32_t* L1) {
+  LAUNCH(k_compact, grid, st, dc, color, L0, L1);
 }
-void launch_pivot_key(hipStream_t st, const int32_t* list, int64_t n_in, int32_t n,
-                      const int32_t* color, const int32_t* begin, const int32_t* rbegin, int policy,
-                      const uint32_t* scores, unsigned long long* pkey) {
-  if (n_in > 0)
-    LAUNCH(k_pivot_key, grid_for(n_in), st, list, n_in, n, color, begin, rbegin, policy, scores,
-           pkey);
+void launch_trim2(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, int32_t* color,
+                  int32_t* label, const int32_t* L0, const int32_t* L1) {
+  LAUNCH(k_trim2, grid, st, dc, G, color, label, L0, L1);
+}
+void launch_reset_slots(hipStream_t st, unsigned grid, const DevCtl* dc, int32_t n,
+                        const int32_t* color, unsigned long long* pkey, int32_t* rep,
+                        const int32_t* L0, const int32_t* L1) {
+  LAUNCH(k_reset_slots, grid, st, dc, n, color, pkey, rep, L0, L1);
+}
+void launch_pivot_key(hipStream_t st, unsigned grid, const DevCtl* dc, GraphPtrs G,
+                      const int32_t* color, int policy, const uint32_t* scores,
+                      unsigned long long* pkey, const int32_t* L0, const int32_t* L1) {
+  LAUNCH(k_pivot_key, grid, st, dc, G, color, policy, scores, pkey, L0, L1);
 }
 void launch_force_pivot(hipStream_t st, int32_t p, int32_t n, const int32_t* color,
                         unsigned long long* pkey) {
   LAUNCH(k_force_pivot, 1, st, p, n, color, pkey);
 }
-void launch_compact_live(hipStream_t st, const int32_t* list, int64_t n_in, const int32_t* color,
-                         int32_t* list_out, unsigned long long* ctr) {
-  if (n_in > 0) LAUNCH(k_compact_live, grid_for(n_in), st, list, n_in, color, list_out, ctr);
+void launch_seed_pivots(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, const int32_t* off,
+                        const int32_t* color, const unsigned long long* pkey, uint32_t* V,
+                        int32_t* q0, const int32_t* L0, const int32_t* L1) {
+  LAUNCH(k_seed_pivots, grid, st, dc, n, off, color, pkey, V, q0, L0, L1);
 }
-void launch_seed(hipStream_t st, const int32_t* list, int64_t n_in, int32_t n, const int32_t* color,
-                 const unsigned long long* pkey, uint32_t* V, int32_t* q_out,
-                 unsigned long long* ctr) {
-  if (n_in > 0) LAUNCH(k_seed, grid_for(n_in), st, list, n_in, n, color, pkey, V, q_out, ctr);
+void launch_seed_list(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* off,
+                      const int32_t* color, const int32_t* seeds, const unsigned long long* ns,
+                      uint32_t* V, int32_t* q0) {
+  LAUNCH(k_seed_list, grid, st, dc, off, color, seeds, ns, V, q0);
 }
-void launch_td(hipStream_t st, const int32_t* off, const int32_t* nb, const int32_t* color,
-               uint32_t* V, const int32_t* q_in, int64_t n_qin, int32_t* q_out, int32_t* huge,
-               unsigned long long* ctr) {
-  if (n_qin > 0) LAUNCH(k_td, grid_for(n_qin), st, off, nb, color, V, q_in, n_qin, q_out, huge, ctr);
+void launch_level(hipStream_t st, unsigned grid, DevCtl* dc, int multi, const int32_t* off,
+                  const int32_t* nb, const int32_t* roff, const int32_t* rnb, const int32_t* color,
+                  uint32_t* V, int32_t* q0, int32_t* q1, int32_t* c0, int32_t* c1, int32_t* h0,
+                  int32_t* h1, const int32_t* L0, const int32_t* L1, int32_t max_chunks) {
+  LAUNCH(k_level, grid, st, dc, multi, off, nb, roff, rnb, color, V, q0, q1, c0, c1, h0, h1, L0, L1,
+         max_chunks);
 }
-void launch_td_huge(hipStream_t st, const int32_t* off, const int32_t* nb, const int32_t* color,
-                    uint32_t* V, const int32_t* huge, int64_t n_huge, int32_t* q_out,
-                    unsigned long long* ctr) {
-  if (n_huge > 0) LAUNCH(k_td_huge, 2048, st, off, nb, color, V, huge, n_huge, q_out, ctr);
+void launch_fb_stats(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, const uint32_t* vf,
+                     const uint32_t* vb, const int32_t* L0, const int32_t* L1) {
+  LAUNCH(k_fb_stats, grid, st, dc, G, vf, vb, L0, L1);
 }
-void launch_bu(hipStream_t st, bool multi, const int32_t* roff, const int32_t* rnb,
-               const int32_t* off, const int32_t* color, uint32_t* V, const int32_t* c_in,
-               int64_t n_cin, int32_t* c_out, int32_t* q_out, unsigned long long* ctr) {
-  if (n_cin <= 0) return;
-  if (multi)
-    LAUNCH(k_bu<true>, grid_for(n_cin), st, roff, rnb, off, color, V, c_in, n_cin, c_out, q_out, ctr);
-  else
-    LAUNCH(k_bu<false>, grid_for(n_cin), st, roff, rnb, off, color, V, c_in, n_cin, c_out, q_out, ctr);
-}
-void launch_classify(hipStream_t st, const int32_t* list, int64_t n_in, int32_t n, int32_t* color,
+void launch_classify(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, int32_t* color,
                      int32_t* label, const uint32_t* vf, const uint32_t* vb,
-                     const unsigned long long* pkey, int32_t* rep, int32_t* list_out,
-                     unsigned long long* ctr) {
-  if (n_in > 0)
-    LAUNCH(k_classify, grid_for(n_in), st, list, n_in, n, color, label, vf, vb, pkey, rep, list_out,
-           ctr);
+                     const unsigned long long* pkey, int32_t* rep, int32_t* L0, int32_t* L1) {
+  LAUNCH(k_classify, grid, st, dc, n, color, label, vf, vb, pkey, rep, L0, L1);
 }
-void launch_recolor(hipStream_t st, const int32_t* list, int64_t n_in, int32_t n, int32_t* color,
-                    const uint32_t* vf, const uint32_t* vb, const int32_t* rep) {
-  if (n_in > 0) LAUNCH(k_recolor, grid_for(n_in), st, list, n_in, n, color, vf, vb, rep);
+void launch_recolor(hipStream_t st, unsigned grid, const DevCtl* dc, int32_t n, int32_t* color,
+                    const uint32_t* vf, const uint32_t* vb, const int32_t* rep, const int32_t* L0,
+                    const int32_t* L1) {
+  LAUNCH(k_recolor, grid, st, dc, n, color, vf, vb, rep, L0, L1);
 }
-void launch_wcc_init(hipStream_t st, const int32_t* list, int64_t n_in, const int32_t* begin,
-                     int32_t* W) {
-  if (n_in > 0) LAUNCH(k_wcc_init, grid_for(n_in), st, list, n_in, begin, W);
+void launch_wcc_init(hipStream_t st, unsigned grid, const DevCtl* dc, const int32_t* begin,
+                     int32_t* W, const int32_t* L0, const int32_t* L1) {
+  LAUNCH(k_wcc_init, grid, st, dc, begin, W, L0, L1);
 }
-void launch_wcc_iter(hipStream_t st, const int32_t* list, int64_t n_in, const int32_t* begin,
-                     const int32_t* adj, const int32_t* color, int32_t* W, unsigned long long* ctr) {
-  if (n_in > 0) LAUNCH(k_wcc_iter, grid_for(n_in), st, list, n_in, begin, adj, color, W, ctr);
+void launch_wcc_step(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* begin,
+                     const int32_t* adj, const int32_t* color, int32_t* W, const int32_t* L0,
+                     const int32_t* L1) {
+  LAUNCH(k_wcc_step, grid, st, dc, begin, adj, color, W, L0, L1);
 }
-void launch_wcc_color(hipStream_t st, const int32_t* list, int64_t n_in, int32_t* color,
-                      const int32_t* W, unsigned long long* ctr) {
-  if (n_in > 0) LAUNCH(k_wcc_color, grid_for(n_in), st, list, n_in, color, W, ctr);
+void launch_wcc_color(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, const int32_t* W,
+                      int seed, const int32_t* off, uint32_t* V, int32_t* q0, const int32_t* L0,
+                      const int32_t* L1) {
+  LAUNCH(k_wcc_color, grid, st, dc, color, W, seed, off, V, q0, L0, L1);
 }
-void launch_canon(hipStream_t st, int32_t n, const int32_t* label, int32_t* mn, int32_t* out,
-                  unsigned long long* ctr) {
+void launch_extract(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, int32_t* label,
+                    const uint32_t* V, int32_t* L0, int32_t* L1) {
+  LAUNCH(k_extract, grid, st, dc, color, label, V, L0, L1);
+}
+void launch_canon(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label, int32_t* mn,
+                  int32_t* out) {
   if (n <= 0) return;
   LAUNCH(k_fill_i32, grid_for(n, 4096), st, mn, INT32_MAX, (int64_t)n);
   LAUNCH(k_canon_min, grid_for(n), st, n, label, mn);
-  LAUNCH(k_canon_apply, grid_for(n), st, n, label, mn, out, ctr);
+  LAUNCH(k_canon_apply, grid_for(n, 4096), st, dc, n, label, mn, out);
 }
 
 #undef LAUNCH

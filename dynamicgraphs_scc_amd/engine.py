@@ -27,6 +27,11 @@ SCC_ERR_LIMIT = -8
 STATUS_NAMES = {0: "OK", -1: "ARG", -2: "HIP", -3: "OOM", -4: "BUDGET", -5: "STATE",
                 -6: "DEVICE_LOST", -7: "CANCELLED", -8: "LIMIT"}
 
+# scc_partition_mode
+PARTS_FB = 0
+PARTS_COLOR = 1
+PARTS_BOTH = 2
+
 # scc_pivot_policy
 PIVOT_DEGREE_PRODUCT = 0
 PIVOT_MIN_ID = 1
@@ -38,6 +43,7 @@ EXPORTS = [
     "scc_ctx_cancel", "scc_ctx_inject_fault", "scc_graph_upload", "scc_graph_upload_device",
     "scc_graph_info", "scc_graph_download", "scc_state_reset", "scc_state_set_colors",
     "scc_trim1", "scc_fb_global", "scc_trim2", "scc_wcc", "scc_fb_partitions",
+    "scc_partitions_resolve",
     "scc_run_opts_default", "scc_run_method2", "scc_labels_download", "scc_colors_download",
     "scc_count", "scc_labels_device", "scc_update", "scc_ctx_set_kernel_timing",
     "scc_ctx_synchronize",
@@ -52,7 +58,8 @@ class SccError(RuntimeError):
 
 class RunOpts(C.Structure):
     _fields_ = [("pivot_policy", C.c_int32), ("trim_stop", C.c_int32), ("use_trim2", C.c_int32),
-                ("trim_between", C.c_int32), ("use_wcc", C.c_int32), ("time_phases", C.c_int32)]
+                ("trim_between", C.c_int32), ("use_wcc", C.c_int32), ("time_phases", C.c_int32),
+                ("partition_mode", C.c_int32)]
 
 
 class RunStats(C.Structure):
@@ -63,7 +70,9 @@ class RunStats(C.Structure):
                 ("ms_total", C.c_double), ("ms_trim1", C.c_double),
                 ("ms_fb_global", C.c_double), ("ms_trim2", C.c_double), ("ms_wcc", C.c_double),
                 ("ms_fb_parts", C.c_double), ("ms_canon", C.c_double),
-                ("ms_fb_kernels", C.c_double), ("fb_launches", C.c_int64)]
+                ("ms_fb_kernels", C.c_double), ("fb_launches", C.c_int64),
+                ("fb_f", C.c_int64), ("fb_b", C.c_int64), ("fb_ef", C.c_int64),
+                ("fb_eb", C.c_int64), ("fb_fb", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad0"}
@@ -110,6 +119,7 @@ def load_library():
         "scc_trim2": (C.c_int, [vp, C.POINTER(i64)]),
         "scc_wcc": (C.c_int, [vp, C.POINTER(i64)]),
         "scc_fb_partitions": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(i64)]),
+        "scc_partitions_resolve": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(i64)]),
         "scc_run_opts_default": (None, [C.POINTER(RunOpts)]),
         "scc_run_method2": (C.c_int, [vp, C.POINTER(RunOpts), C.POINTER(RunStats)]),
         "scc_labels_download": (C.c_int, [vp, vp, C.c_int]),
@@ -243,6 +253,13 @@ class SccEngine:
         r = C.c_int64()
         self._check(self.lib.scc_fb_partitions(self.h, int(policy), int(trim_between),
                                                C.byref(r)))
+        return r.value
+
+    def resolve_partitions(self, mode: int = PARTS_BOTH, policy: int = PIVOT_DEGREE_PRODUCT,
+                           trim_between: bool = True) -> int:
+        r = C.c_int64()
+        self._check(self.lib.scc_partitions_resolve(self.h, int(policy), int(mode),
+                                                    int(trim_between), C.byref(r)))
         return r.value
 
     def run_method2(self, opts: RunOpts | None = None, **kw) -> dict:

@@ -113,6 +113,8 @@ int scc_wcc(scc_ctx* ctx, int64_t* parts);
  * partition (all partitions at once), with trim1 between rounds when
  * trim_between != 0.  *rounds = rounds used. */
 int scc_fb_partitions(scc_ctx* ctx, int policy, int trim_between, int64_t* rounds);
+/* The partition phase with an explicit scc_partition_mode. */
+int scc_partitions_resolve(scc_ctx* ctx, int policy, int mode, int trim_between, int64_t* rounds);
 
 /* ---- whole pipeline ------------------------------------------------------ */
 
@@ -123,7 +125,16 @@ typedef struct scc_run_opts {
   int32_t trim_between;    /* trim1 between partition FB rounds */
   int32_t use_wcc;         /* WCC colouring before the partition rounds */
   int32_t time_phases;     /* record per-phase HIP-event timings in stats */
+  int32_t partition_mode;  /* scc_partition_mode: how the partition phase resolves what is left */
 } scc_run_opts;
+
+/* Partition phase (after the global step and WCC).  Every round resolves all
+ * remaining partitions at once. */
+typedef enum scc_partition_mode {
+  SCC_PARTS_FB = 0,     /* one forward-backward step per partition per round (src/scc_fb_seq2.cc) */
+  SCC_PARTS_COLOR = 1,  /* WCC colouring, then each class's root SCC by a forward search */
+  SCC_PARTS_BOTH = 2    /* a forward-backward step, then a colouring step, per round */
+} scc_partition_mode;
 
 typedef struct scc_run_stats {
   int64_t trimmed_first;   /* repeat_global_trim1 count */
@@ -139,10 +150,13 @@ typedef struct scc_run_stats {
   double ms_trim1, ms_fb_global, ms_trim2, ms_wcc, ms_fb_parts, ms_canon; /* HIP events */
   double ms_fb_kernels;    /* summed device time of the global FB traversal kernels */
   int64_t fb_launches;     /* traversal kernel launches in the global FB */
+  /* reach sets of the global FB (filled when time_phases or kernel timing is on):
+   * |F|, |B|, out-degrees summed over F, in-degrees summed over B, |F n B| */
+  int64_t fb_f, fb_b, fb_ef, fb_eb, fb_fb;
 } scc_run_stats;
 
 /* Default options: degree-product pivot, trim_stop 0, trim2 on, trim between
- * rounds on, WCC on, no phase timing. */
+ * rounds on, WCC on, no phase timing, SCC_PARTS_BOTH. */
 void scc_run_opts_default(scc_run_opts* o);
 
 /* do_baseline_global_wcc_fb (src/scc_main.cc:272-351): trim1 -> global FB ->
