@@ -90,7 +90,7 @@ def cpu_baseline(g, cores: int, want_ref: bool = True):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--edge-factor", type=float, default=16)
@@ -98,6 +98,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-ref", action="store_true", help="CPU baseline from the restatement only")
     ap.add_argument("--policy", default="degree", choices=["degree", "minid"])
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (reference point for "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
@@ -108,6 +111,15 @@ def main():
     from dynamicgraphs_scc_amd import graphs
     from dynamicgraphs_scc_amd.engine import (PIVOT_DEGREE_PRODUCT, PIVOT_MIN_ID, SccEngine,
                                               default_opts)
+
+    if args.copy_calibration_mib:
+        import torch
+        nb = args.copy_calibration_mib << 20
+        a = torch.ones(nb // 4, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {nb} bytes read + {nb} bytes written")
 
     t = time.time()
     g = graphs.rmat(args.scale, args.edge_factor, seed=args.seed)
@@ -145,10 +157,18 @@ def main():
     b_fb = 20 * F + 8 * EF + 20 * B + 8 * EB + 4 * FB
     t_fb_ms = statistics.median(kt)
     achieved = b_fb / (t_fb_ms / 1e3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "rmat22_fb_traffic.json")
+    if os.path.exists(tfile) and args.scale == 22:
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "global forward-backward traversal (k_td, k_td_huge, k_bu)",
-                "algorithmic_bytes": int(b_fb), "launches_per_step": int(statistics.median(launches)),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "k_level (global forward-backward reachability levels, both directions)",
+                "algorithmic_bytes": int(b_fb),
+                "algorithmic_bytes_per_launch": int(b_fb / max(1, statistics.median(launches))),
+                "avg_launch_us": round(1e3 * t_fb_ms / max(1, statistics.median(launches)), 3),
+                "launches_per_step": int(statistics.median(launches)),
                 "kernel_ms_per_step": round(t_fb_ms, 4),
                 "sets": {"F": F, "B": B, "E_F": EF, "E_B": EB, "FnB": FB}}
     phases = {k: round(st[k], 4) for k in ("ms_trim1", "ms_fb_global", "ms_trim2", "ms_wcc",

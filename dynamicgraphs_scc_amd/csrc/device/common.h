@@ -384,6 +384,7 @@ struct scc_ctx {
   int32_t maxdeg = 0;                     // max(out-degree, in-degree), for huge-row chunking
   int32_t update_policy = 0;              // pivot policy of dynamic recomputes
   bool colors_split = false;              // colours other than -1/-2 exist (after FB/WCC/set)
+  bool fresh = false;                     // state just reset: all colours -1, nothing done
   int32_t lv_hint[2] = {8, 8};            // level launches per search batch (fw, bw)
   int32_t it_hint = 4;                    // WCC iterations per batch
   bool labels_ready = false;              // canon[] valid for the current graph

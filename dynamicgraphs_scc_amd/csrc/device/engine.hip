@@ -191,6 +191,7 @@ int reset_state(scc_ctx* c) {
   c->labels_ready = false;
   c->multi_colors = false;
   c->colors_split = false;
+  c->fresh = true;
   int rc = zero_ctr(c);
   if (rc) return rc;
   return set_live(c, 0, c->n);
@@ -199,6 +200,7 @@ int reset_state(scc_ctx* c) {
 // ---- phases -----------------------------------------------------------------------
 
 int do_trim_compact(scc_ctx* c) {
+  c->fresh = false;
   if (c->n_live == 0) return SCC_OK;
   int rc;
   if ((rc = zero_ctr(c))) return rc;
@@ -220,8 +222,11 @@ int do_trim1(scc_ctx* c, int64_t stop, int64_t* trimmed) {
   int batch = 3, rc;
   for (int guard = 0;; guard++) {
     if ((rc = check_cancel(c))) return rc;
-    for (int i = 0; i < batch; i++)
-      launch_trim1_step(c->stream, grid, c->dc, gp(c), c->color, c->label, c->live[0], c->live[1]);
+    for (int i = 0; i < batch; i++) {
+      launch_trim1_step(c->stream, grid, c->dc, gp(c), c->color, c->label, c->live[0], c->live[1],
+                        c->fresh ? 1 : 0);
+      c->fresh = false;
+    }
     if ((rc = pull_ctl(c))) return rc;
     if (c->h_dc->loop.done) break;
     batch = 4;
@@ -232,6 +237,7 @@ int do_trim1(scc_ctx* c, int64_t stop, int64_t* trimmed) {
 }
 
 int do_trim2(scc_ctx* c, int64_t* removed) {
+  c->fresh = false;
   if (removed) *removed = 0;
   if (c->n_live == 0) return SCC_OK;
   int rc;
@@ -284,7 +290,7 @@ int traverse(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed) {
   const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
   launch_ctl_init(c->stream, c->dc, 8, 0, 0, 0, avg / 14.0);
   launch_seed_pivots(c->stream, grid_for(c->n_live), c->dc, c->n, off, c->color, c->pkey, V, c->q[0],
-                     c->live[0], c->live[1]);
+                     c->huge, c->live[0], c->live[1]);
   return run_levels(c, fwd, multi, levels, timed);
 }
 
@@ -302,6 +308,7 @@ int traverse_from(scc_ctx* c, bool fwd, bool multi, const int32_t* seeds,
 // the single partition of colour -1 (the global step).
 int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int64_t* scc,
                 int32_t* pivot_out, int64_t* levels, bool timed, int64_t* sets) {
+  c->fresh = false;
   int rc;
   if (scc) *scc = 0;
   if (c->n_live == 0) return SCC_OK;
@@ -341,6 +348,7 @@ int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int6
 // WCC colouring of every live partition (W to its fixpoint, colour = root).
 // With seed, the roots also seed the forward search of a colouring round.
 static int wcc_fixpoint(scc_ctx* c, bool seed) {
+  c->fresh = false;
   int rc;
   const unsigned grid = grid_for(c->n_live);
   launch_ctl_init(c->stream, c->dc, 1 | 4, 0, 0, 0, 0.0);
@@ -607,6 +615,7 @@ int scc_state_set_colors(scc_ctx* c, const int32_t* colors) {
   c->multi_colors = false;
   for (int32_t v = 0; v < c->n && !c->multi_colors; v++) c->multi_colors = colors[v] != -1;
   c->colors_split = c->multi_colors;
+  c->fresh = false;
   if ((rc = hip_check(c, hipMemcpyAsync(c->color, colors, 4ull * c->n, hipMemcpyHostToDevice, c->stream),
                       "colours upload")))
     return rc;

@@ -30,7 +30,7 @@ void launch_iota(hipStream_t st, int32_t* p, int32_t n);
 void launch_ctl_init(hipStream_t st, DevCtl* dc, int what, int32_t live_sel, unsigned long long live_n,
                      long long stop, double bu_ratio);
 void launch_trim1_step(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, int32_t* color,
-                       int32_t* label, int32_t* L0, int32_t* L1);
+                       int32_t* label, int32_t* L0, int32_t* L1, int fresh);
 void launch_compact(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* color, int32_t* L0,
                     int32_t* L1);
 void launch_trim2(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, int32_t* color,
@@ -45,7 +45,7 @@ void launch_force_pivot(hipStream_t st, int32_t p, int32_t n, const int32_t* col
                         unsigned long long* pkey);
 void launch_seed_pivots(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, const int32_t* off,
                         const int32_t* color, const unsigned long long* pkey, uint32_t* V,
-                        int32_t* q0, const int32_t* L0, const int32_t* L1);
+                        int32_t* q0, int32_t* h0, const int32_t* L0, const int32_t* L1);
 void launch_seed_list(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* off,
                       const int32_t* color, const int32_t* seeds, const unsigned long long* ns,
                       uint32_t* V, int32_t* q0);

@@ -127,7 +127,8 @@ __global__ void k_ctl_init(DevCtl* dc, int what, int32_t live_sel, unsigned long
 __global__ __launch_bounds__(BLOCK) void k_trim1_step(DevCtl* __restrict__ dc, GraphPtrs G,
                                                       int32_t* __restrict__ color,
                                                       int32_t* __restrict__ label,
-                                                      int32_t* __restrict__ L0, int32_t* __restrict__ L1) {
+                                                      int32_t* __restrict__ L0, int32_t* __restrict__ L1,
+                                                      int fresh) {
   __shared__ int s_done;
   if (threadIdx.x == 0) s_done = dc->loop.done;
   __syncthreads();
@@ -135,7 +136,11 @@ __global__ __launch_bounds__(BLOCK) void k_trim1_step(DevCtl* __restrict__ dc, G
   const int32_t* __restrict__ list_in = LIVE_IN(dc, L0, L1);
   int32_t* __restrict__ list_out = LIVE_OUT(dc, L0, L1);
   const int64_t n_in = (int64_t)dc->live.n;
-  auto live = [color](int32_t k, int32_t pv, int32_t pc) { return k != pv && color[k] == pc; };
+  // fresh: every vertex still has colour -1 and none is done, so any neighbour
+  // other than v itself is live (a Jacobi sweep; the fixpoint is the same)
+  auto live = [color, fresh](int32_t k, int32_t pv, int32_t pc) {
+    return k != pv && (fresh || color[k] == pc);
+  };
   unsigned long long trimmed = 0;
   using Out = LdsOut<0>;
   Out::init();
@@ -314,7 +319,37 @@ __global__ __launch_bounds__(BLOCK) void k_pivot_key(const DevCtl* __restrict__ 
         act = true;
       }
     }
-    grouped_max_u64(act, slot, key, pkey);
+    // one slot for the whole workgroup (the global step): reduce in LDS first
+    __shared__ int32_t s_slot;
+    __shared__ int s_mixed;
+    __shared__ unsigned long long s_best[WAVES];
+    if (threadIdx.x == 0) s_mixed = 0;
+    const uint64_t am = __ballot(act);
+    const int32_t ws = am ? __shfl(slot, __ffsll((unsigned long long)am) - 1) : -1;
+    __syncthreads();
+    if (lane_id() == 0 && am) s_slot = ws;
+    __syncthreads();
+    const int32_t bs = s_slot;
+    if (__ballot(act && slot != bs) != 0 && lane_id() == 0) s_mixed = 1;
+    __syncthreads();
+    if (!s_mixed) {
+      unsigned long long x = act ? key : 0ull;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long y = __shfl_xor(x, o);
+        x = y > x ? y : x;
+      }
+      if (lane_id() == 0) s_best[threadIdx.x >> 6] = x;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        unsigned long long b = 0;
+        for (int w = 0; w < WAVES; w++) b = s_best[w] > b ? s_best[w] : b;
+        if (b && b > pkey[bs]) atomicMax(&pkey[bs], b);
+      }
+      __syncthreads();
+    } else {
+      grouped_max_u64(act, slot, key, pkey);
+    }
   }
 }
 
@@ -335,6 +370,7 @@ __global__ __launch_bounds__(BLOCK) void k_seed_pivots(DevCtl* __restrict__ dc, 
                                                        const int32_t* __restrict__ color,
                                                        const unsigned long long* __restrict__ pkey,
                                                        uint32_t* __restrict__ V, int32_t* __restrict__ q0,
+                                                       int32_t* __restrict__ h0,
                                                        const int32_t* __restrict__ L0,
                                                        const int32_t* __restrict__ L1) {
   const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
@@ -355,7 +391,10 @@ __global__ __launch_bounds__(BLOCK) void k_seed_pivots(DevCtl* __restrict__ dc, 
         dc->ctr[C_PIVOT] = (unsigned long long)v;  // the pivot, when there is one partition
       }
     }
-    wave_append(piv, v, q0, &dc->lv.nq);
+    // rows of at least HUGE_ROW entries go straight to the chunked expansion
+    const bool hug = piv && (off[v + 1] - off[v]) >= HUGE_ROW;
+    wave_append(piv && !hug, v, q0, &dc->lv.nq);
+    wave_append(hug, v, h0, &dc->lv.nh);
   }
   block_count(edges, &dc->lv.mf);
   block_count(cnt, &dc->lv.visited);
@@ -583,6 +622,8 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
       lv->nc = oc;
       lv->nh = 0;
     }
+    // one partition: every live vertex reached means nothing is left to find
+    if (!multi && lv->visited >= nlive) lv->done = 1;
     lv->q_sel ^= 1;
     lv->prev_nq = lv->nq;
     lv->nq = oq;
@@ -894,12 +935,9 @@ __global__ __launch_bounds__(BLOCK) void k_canon_min(int32_t n, const int32_t* _
     const int64_t v = base + threadIdx.x;
     int32_t l = -1;
     if (v < n) l = label[v];
-    bool pending = (uint32_t)l < (uint32_t)n;
-    if (pending && l == (int32_t)v) {
-      // self-labelled: contention only with the rest of its SCC
-      if ((int32_t)v < mn[l]) atomicMin(&mn[l], (int32_t)v);
-      pending = false;
-    }
+    // mn starts as the identity (a label is always a vertex labelled with
+    // itself), so self-labelled vertices have nothing to contribute
+    bool pending = (uint32_t)l < (uint32_t)n && l != (int32_t)v;
     uint64_t act = __ballot(pending);
     while (act) {
       const int leader = __ffsll((unsigned long long)act) - 1;
@@ -947,8 +985,8 @@ void launch_ctl_init(hipStream_t st, DevCtl* dc, int what, int32_t live_sel, uns
   hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, dc, what, live_sel, live_n, stop, bu_ratio);
 }
 void launch_trim1_step(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, int32_t* color,
-                       int32_t* label, int32_t* L0, int32_t* L1) {
-  LAUNCH(k_trim1_step, grid, st, dc, G, color, label, L0, L1);
+                       int32_t* label, int32_t* L0, int32_t* L1, int fresh) {
+  LAUNCH(k_trim1_step, grid, st, dc, G, color, label, L0, L1, fresh);
 }
 void launch_compact(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* color, int32_t* L0,
                     int32_t* L1) {
@@ -974,8 +1012,8 @@ void launch_force_pivot(hipStream_t st, int32_t p, int32_t n, const int32_t* col
 }
 void launch_seed_pivots(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, const int32_t* off,
                         const int32_t* color, const unsigned long long* pkey, uint32_t* V,
-                        int32_t* q0, const int32_t* L0, const int32_t* L1) {
-  LAUNCH(k_seed_pivots, grid, st, dc, n, off, color, pkey, V, q0, L0, L1);
+                        int32_t* q0, int32_t* h0, const int32_t* L0, const int32_t* L1) {
+  LAUNCH(k_seed_pivots, grid, st, dc, n, off, color, pkey, V, q0, h0, L0, L1);
 }
 void launch_seed_list(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* off,
                       const int32_t* color, const int32_t* seeds, const unsigned long long* ns,
@@ -1024,7 +1062,7 @@ void launch_extract(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, i
 void launch_canon(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label, int32_t* mn,
                   int32_t* out) {
   if (n <= 0) return;
-  LAUNCH(k_fill_i32, grid_for(n, 4096), st, mn, INT32_MAX, (int64_t)n);
+  LAUNCH(k_iota, grid_for(n, 4096), st, mn, n);
   LAUNCH(k_canon_min, grid_for(n), st, n, label, mn);
   LAUNCH(k_canon_apply, grid_for(n, 4096), st, dc, n, label, mn, out);
 }

@@ -241,19 +241,54 @@ sccgen_graph* sccgen_read_edgelist(const char* path) {
   return build_graph((int32_t)maxid, (int64_t)s.size(), s.data(), d.data(), false);
 }
 
+namespace {
+inline char* put_uint(char* p, uint32_t x) {
+  char tmp[12];
+  int k = 0;
+  do {
+    tmp[k++] = (char)('0' + x % 10);
+    x /= 10;
+  } while (x);
+  while (k) *p++ = tmp[--k];
+  return p;
+}
+}  // namespace
+
 int sccgen_write_edgelist(const sccgen_graph* g, const char* path) {
-  FILE* f = std::fopen(path, "w");
+  FILE* f = std::fopen(path, "wb");
   if (!f) {
     set_err(std::string("cannot open ") + path);
     return -1;
   }
-  std::vector<char> buf(1 << 20);
-  std::setvbuf(f, buf.data(), _IOFBF, buf.size());
-  for (int32_t v = 0; v < g->n; v++)
-    for (int32_t i = g->begin[v]; i < g->begin[v + 1]; i++)
-      std::fprintf(f, "%d %d\n", v + 1, g->adj[i] + 1);
+  // rows are formatted in parallel chunks, written in order
+  const int32_t n = g->n;
+  const int32_t chunk = 1 << 16;
+  const int32_t nchunks = (n + chunk - 1) / chunk;
+  int rc = 0;
+  for (int32_t c0 = 0; c0 < nchunks && rc == 0; c0 += 64) {
+    const int32_t c1 = std::min(nchunks, c0 + 64);
+    std::vector<std::string> out(c1 - c0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int32_t c = c0; c < c1; c++) {
+      const int32_t v0 = c * chunk, v1 = std::min(n, v0 + chunk);
+      std::string& s = out[c - c0];
+      s.resize((size_t)(g->begin[v1] - g->begin[v0]) * 24);
+      char* p = &s[0];
+      for (int32_t v = v0; v < v1; v++)
+        for (int32_t i = g->begin[v]; i < g->begin[v + 1]; i++) {
+          p = put_uint(p, (uint32_t)v + 1);
+          *p++ = ' ';
+          p = put_uint(p, (uint32_t)g->adj[i] + 1);
+          *p++ = '\n';
+        }
+      s.resize(p - &s[0]);
+    }
+    for (auto& s : out)
+      if (std::fwrite(s.data(), 1, s.size(), f) != s.size()) rc = -1;
+  }
   std::fclose(f);
-  return 0;
+  if (rc) set_err("write failed");
+  return rc;
 }
 
 sccgen_graph* sccgen_rmat(int scale, double edge_factor, double a, double b, double c,

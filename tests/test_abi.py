@@ -64,3 +64,36 @@ def test_no_cpu_fallback_in_product_path():
                 with open(os.path.join(dirpath, fn)) as f:
                     txt = f.read()
                 assert not re.search(r"^\s*(import|from)\s+oracle\b", txt, re.M), fn
+
+
+def test_shim_defines_every_scc_h_entry_point():
+    """A C++ program that takes the address of every function scc.h declares
+    links against libsccshim.so (a missing definition fails the link) and runs
+    without touching the GPU."""
+    import subprocess
+    import tempfile
+    with open(os.path.join(ROOT, "include", "scc.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    names = sorted(set(re.findall(r"^\s*(?:void|int|long long)\s+([a-z_][a-zA-Z0-9_]*)\s*\(", text, re.M)))
+    assert {"repeat_global_trim1", "do_fw_bw_global_main", "start_workers_fw_bw_dfs",
+            "do_global_wcc", "output_scc_list", "insert_edges"} <= set(names)
+    body = "\n".join(f'  printf("%d\\n", (int)(reinterpret_cast<void*>(&{n}) != nullptr));' for n in names)
+    src = f"""#include <stdio.h>
+#include "scc.h"
+int main() {{
+{body}
+  printf("%d\\n", G_num_nodes);
+  return 0;
+}}
+"""
+    with tempfile.TemporaryDirectory() as td:
+        cc = os.path.join(td, "t.cpp")
+        with open(cc, "w") as f:
+            f.write(src)
+        exe = os.path.join(td, "t")
+        subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), cc, "-o", exe,
+                        "-L", LIB, "-lsccshim", "-lsccmi", "-lsccgen", f"-Wl,-rpath,{LIB}"], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
+    assert out == ["1"] * len(names) + ["0"]

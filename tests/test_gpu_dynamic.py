@@ -1,0 +1,99 @@
+"""GPU parity of dynamic updates: after every insert/delete batch the engine's
+canonical labels equal a from-scratch recompute (oracle) on the same edge set,
+and the device CSR equals the host-tracked graph."""
+import numpy as np
+import pytest
+
+import oracle
+from dynamicgraphs_scc_amd import graphs
+from dynamicgraphs_scc_amd.engine import SccEngine, SccError
+from helpers import scipy_canonical
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_graph(engine, want):
+    got = engine.download_graph()
+    assert np.array_equal(got.begin, want.begin)
+    assert np.array_equal(got.node_idx, want.node_idx)
+    assert np.array_equal(got.r_begin, want.r_begin)
+    assert np.array_equal(got.r_node_idx, want.r_node_idx)
+
+
+@pytest.mark.parametrize("kind,seed", [("rmat", 1), ("rmat", 2), ("planted", 3)])
+def test_update_stream_matches_recompute(engine, kind, seed):
+    if kind == "rmat":
+        base = graphs.rmat(12, 6, seed=seed)
+    else:
+        base = graphs.planted(20000, seed=seed, n_medium=400)
+    engine.upload(base)
+    engine.run_method2()
+    us = graphs.UpdateStream(base, seed=seed + 10, lo=20, hi=3000)
+    for b in range(8):
+        isrc, idst, dsrc, ddst = us.next()
+        print(f"  batch {b}: +{len(isrc)} -{len(dsrc)}", flush=True)
+        st = engine.update(isrc, idst, dsrc, ddst)
+        cur = us.graph()
+        assert st["inserted"] == len(isrc) and st["deleted"] == len(dsrc)
+        want = oracle.scc_canonical(cur, 8)
+        assert np.array_equal(engine.labels(), want), f"batch {b}"
+        assert st["num_sccs"] == len(np.unique(want))
+        if b % 3 == 0:
+            _check_graph(engine, cur)
+
+
+def test_insert_merges_and_delete_splits(engine):
+    # two 3-cycles joined one way; an inserted back edge merges them, deleting it splits again
+    src = np.array([0, 1, 2, 3, 4, 5, 2], np.int32)
+    dst = np.array([1, 2, 0, 4, 5, 3, 3], np.int32)
+    g = graphs.from_edges(7, src, dst)
+    engine.upload(g)
+    engine.run_method2()
+    assert list(engine.labels()) == [0, 0, 0, 3, 3, 3, 6]
+    st = engine.update(ins_src=[5], ins_dst=[0])
+    assert st["inserted"] == 1
+    assert list(engine.labels()) == [0, 0, 0, 0, 0, 0, 6]
+    st = engine.update(del_src=[2], del_dst=[3])
+    assert st["deleted"] == 1
+    assert list(engine.labels()) == [0, 0, 0, 3, 3, 3, 6]
+    # no-ops: present insert, absent delete, self-loop
+    st = engine.update(ins_src=[0, 4], ins_dst=[1, 4], del_src=[6], del_dst=[0])
+    assert st["inserted"] == 0 and st["deleted"] == 0
+    assert list(engine.labels()) == [0, 0, 0, 3, 3, 3, 6]
+
+
+def test_same_batch_delete_and_reinsert_is_noop(engine):
+    g = graphs.rmat(10, 8, seed=4)
+    engine.upload(g)
+    engine.run_method2()
+    before = engine.labels()
+    src, dst = g.edges()
+    st = engine.update(src[:50], dst[:50], src[:50], dst[:50])
+    assert st["deleted"] == 0
+    assert np.array_equal(engine.labels(), before)
+    _check_graph(engine, g)
+
+
+def test_large_batches_rmat16(engine):
+    base = graphs.rmat(16, 16, seed=7)
+    engine.upload(base)
+    engine.run_method2()
+    us = graphs.UpdateStream(base, seed=8, lo=1000, hi=20000)
+    for b in range(4):
+        isrc, idst, dsrc, ddst = us.next()
+        engine.update(isrc, idst, dsrc, ddst)
+    cur = us.graph()
+    assert np.array_equal(engine.labels(), scipy_canonical(cur))
+
+
+def test_update_requires_labels_and_valid_ids():
+    g = graphs.rmat(8, 4, seed=1)
+    with SccEngine() as e:
+        e.upload(g)
+        with pytest.raises(SccError) as ei:
+            e.update([0], [1])
+        assert ei.value.code == -5
+        e.run_method2()
+        with pytest.raises(SccError) as ei:
+            e.update([0], [g.num_nodes])
+        assert ei.value.code == -1
