@@ -468,7 +468,15 @@ extern "C" int scc_update(scc_ctx* c, const int32_t* ins_src, const int32_t* ins
                           scc_update_stats* stats) {
   if (!c) return SCC_ERR_ARG;
   if (!c->begin) return set_error(c, SCC_ERR_STATE, "no graph uploaded");
-  if (!c->labels_ready) return set_error(c, SCC_ERR_STATE, "scc_update needs current labels (run first)");
+  (void)hipSetDevice(c->device);
+  if (!c->labels_ready) {
+    // phase-by-phase callers: labels are a full decomposition once no vertex is live
+    int r0 = do_trim_compact(c);
+    if (r0) return r0;
+    if (c->n_live != 0)
+      return set_error(c, SCC_ERR_STATE, "scc_update needs a finished decomposition (run first)");
+    if ((r0 = do_canon(c, nullptr))) return r0;
+  }
   if (n_ins < 0 || n_del < 0 || (n_ins && (!ins_src || !ins_dst)) || (n_del && (!del_src || !del_dst)))
     return set_error(c, SCC_ERR_ARG, "bad batch");
   for (int64_t i = 0; i < n_ins; i++)

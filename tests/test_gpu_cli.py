@@ -20,8 +20,10 @@ SCC = os.path.join(ROOT, "dynamicgraphs_scc_amd", "lib", "scc")
 
 def run_scc(args, cwd):
     print("  scc", args, flush=True)
-    return subprocess.run([SCC] + [str(a) for a in args], cwd=cwd, capture_output=True, text=True,
-                          timeout=120, check=True).stdout
+    r = subprocess.run([SCC] + [str(a) for a in args], cwd=cwd, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, f"scc {args} exited {r.returncode}: {r.stdout[-2000:]} {r.stderr[-2000:]}"
+    return r.stdout
 
 
 def parse(out):

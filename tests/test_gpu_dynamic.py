@@ -97,3 +97,27 @@ def test_update_requires_labels_and_valid_ids():
         with pytest.raises(SccError) as ei:
             e.update([0], [g.num_nodes])
         assert ei.value.code == -1
+
+
+def test_update_after_phase_by_phase_run():
+    """Labels left by the separate phase calls count as current once no vertex is
+    live; a half-finished decomposition is refused."""
+    g = graphs.rmat(11, 6, seed=4)
+    src, dst = g.edges()
+    with SccEngine() as e:
+        e.upload(g)
+        e.repeat_global_trim1()
+        with pytest.raises(SccError) as ei:
+            e.update([0], [1])
+        assert ei.value.code == -5
+        e.do_fw_bw_global_main()
+        e.do_global_trim2_new()
+        e.repeat_global_trim1()
+        e.do_global_wcc()
+        e.start_workers_fw_bw_dfs()
+        e.update([5, 9], [9, 5], [src[0]], [dst[0]])
+        keep = np.ones(len(src), bool)
+        keep[0] = False
+        want = graphs.from_edges(g.num_nodes, np.concatenate([src[keep], [5, 9]]),
+                                 np.concatenate([dst[keep], [9, 5]]), dedup=True)
+        assert np.array_equal(e.labels(), scipy_canonical(want))
