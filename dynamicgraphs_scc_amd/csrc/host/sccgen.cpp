@@ -291,6 +291,101 @@ int sccgen_write_edgelist(const sccgen_graph* g, const char* path) {
   return rc;
 }
 
+namespace {
+constexpr uint32_t GM_MAGIC = 0x03939999u;
+inline uint32_t bswap32(uint32_t x) { return __builtin_bswap32(x); }
+}  // namespace
+
+sccgen_graph* sccgen_read_gm_binary(const char* path) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) {
+    set_err(std::string("cannot open ") + path);
+    return nullptr;
+  }
+  std::fseek(f, 0, SEEK_END);
+  const long size = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  if (size < 20) {
+    std::fclose(f);
+    set_err("gm binary: file too short for its header");
+    return nullptr;
+  }
+  std::vector<uint32_t> w((size_t)size / 4);
+  const size_t got = std::fread(w.data(), 4, w.size(), f);
+  std::fclose(f);
+  if (got != w.size()) {
+    set_err("gm binary: read failed");
+    return nullptr;
+  }
+  // header words are big-endian; an old writer left them in host order
+  bool flip;
+  if (bswap32(w[0]) == GM_MAGIC) flip = true;
+  else if (w[0] == GM_MAGIC) flip = false;
+  else {
+    set_err("gm binary: magic word mismatch");
+    return nullptr;
+  }
+  auto word = [&](size_t i) -> uint32_t { return flip ? bswap32(w[i]) : w[i]; };
+  if (word(1) != 4 || word(2) != 4) {
+    set_err("gm binary: only 4-byte node and edge ids are supported");
+    return nullptr;
+  }
+  const int64_t n = (int32_t)word(3), m = (int32_t)word(4);
+  if (n < 0 || m < 0 || (uint64_t)(5 + (n + 1) + m) * 4 > (uint64_t)size) {
+    set_err("gm binary: sizes in the header do not fit the file");
+    return nullptr;
+  }
+  const size_t b0 = 5, a0 = 5 + (size_t)n + 1;
+  std::vector<int32_t> src(m), dst(m);
+  std::atomic<int> bad{0};
+  if ((int32_t)word(b0) != 0 || (int64_t)(int32_t)word(b0 + n) != m) bad = 1;
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t v = 0; v < n; v++) {
+    const int64_t lo = (int32_t)word(b0 + v), hi = (int32_t)word(b0 + v + 1);
+    if (lo < 0 || hi < lo || hi > m) {
+      bad = 1;
+      continue;
+    }
+    for (int64_t i = lo; i < hi; i++) {
+      const int32_t k = (int32_t)word(a0 + i);
+      if (k < 0 || k >= n) bad = 1;
+      src[i] = (int32_t)v;
+      dst[i] = k;
+    }
+  }
+  if (bad) {
+    set_err("gm binary: malformed CSR (offsets or ids out of range)");
+    return nullptr;
+  }
+  return build_graph((int32_t)n, m, src.data(), dst.data(), false);
+}
+
+int sccgen_write_gm_binary(const sccgen_graph* g, const char* path) {
+  FILE* f = std::fopen(path, "wb");
+  if (!f) {
+    set_err(std::string("cannot open ") + path);
+    return -1;
+  }
+  const int64_t n = g->n, m = g->begin[g->n];
+  std::vector<uint32_t> w(5 + (size_t)n + 1 + (size_t)m);
+  w[0] = bswap32(GM_MAGIC);
+  w[1] = bswap32(4);
+  w[2] = bswap32(4);
+  w[3] = bswap32((uint32_t)n);
+  w[4] = bswap32((uint32_t)m);
+#pragma omp parallel for schedule(static)
+  for (int64_t v = 0; v <= n; v++) w[5 + v] = bswap32((uint32_t)g->begin[v]);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < m; i++) w[6 + n + i] = bswap32((uint32_t)g->adj[i]);
+  const bool ok = std::fwrite(w.data(), 4, w.size(), f) == w.size();
+  std::fclose(f);
+  if (!ok) {
+    set_err("write failed");
+    return -1;
+  }
+  return 0;
+}
+
 sccgen_graph* sccgen_rmat(int scale, double edge_factor, double a, double b, double c,
                           uint64_t seed, int permute, int dedup, int64_t n_keep) {
   if (scale < 1 || scale > 30 || a < 0 || b < 0 || c < 0 || a + b + c > 1.0) {

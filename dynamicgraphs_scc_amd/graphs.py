@@ -44,6 +44,10 @@ def _load():
     lib.sccgen_read_edgelist.argtypes = [C.c_char_p]
     lib.sccgen_write_edgelist.restype = C.c_int
     lib.sccgen_write_edgelist.argtypes = [vp, C.c_char_p]
+    lib.sccgen_read_gm_binary.restype = vp
+    lib.sccgen_read_gm_binary.argtypes = [C.c_char_p]
+    lib.sccgen_write_gm_binary.restype = C.c_int
+    lib.sccgen_write_gm_binary.argtypes = [vp, C.c_char_p]
     lib.sccgen_rmat.restype = vp
     lib.sccgen_rmat.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                 C.c_uint64, C.c_int, C.c_int, C.c_int64]
@@ -123,6 +127,17 @@ class Graph:
         finally:
             lib.sccgen_free(h)
 
+    def write_gm_binary(self, path: str) -> None:
+        """Write the forward CSR as a gm binary graph (big-endian,
+        gm_graph/src/gm_graph_binary_loader.cc:207-248)."""
+        lib = _load()
+        h = _to_handle(self)
+        try:
+            if lib.sccgen_write_gm_binary(h, path.encode()) != 0:
+                raise RuntimeError(_err(lib))
+        finally:
+            lib.sccgen_free(h)
+
 
 def _from_handle(h) -> Graph:
     lib = _load()
@@ -167,6 +182,13 @@ def read_edgelist(path: str) -> Graph:
     single-space separated, N = max id."""
     lib = _load()
     return _from_handle(lib.sccgen_read_edgelist(path.encode()))
+
+
+def read_gm_binary(path: str) -> Graph:
+    """Load a gm binary graph (gm_graph/src/gm_graph_binary_loader.cc:48-205):
+    rows sorted and the reverse CSR built, as load_binary does."""
+    lib = _load()
+    return _from_handle(lib.sccgen_read_gm_binary(path.encode()))
 
 
 RMAT_ABCD = (0.57, 0.19, 0.19, 0.05)

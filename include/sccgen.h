@@ -50,6 +50,18 @@ sccgen_graph* sccgen_read_edgelist(const char* path);
  * order).  Returns 0 on success. */
 int sccgen_write_edgelist(const sccgen_graph* g, const char* path);
 
+/* gm binary CSR (reference: gm_graph/src/gm_graph_binary_loader.cc:21-45,
+ * load :48-205, store :207-248): big-endian words
+ *   magic 0x03939999 | sizeof(node_t)=4 | sizeof(edge_t)=4 | N | M |
+ *   begin[N+1] | node_idx[M].
+ * The reader also takes the older little-endian ("flipped") files the
+ * reference accepts, rejects 8-byte ids (the reference's int32 build does too),
+ * sorts each row and builds the reverse CSR, as load_binary does.  Returns NULL
+ * on error (sccgen_last_error says why). */
+sccgen_graph* sccgen_read_gm_binary(const char* path);
+/* Write the forward CSR in that format (big-endian).  Returns 0 on success. */
+int sccgen_write_gm_binary(const sccgen_graph* g, const char* path);
+
 /* ---- seeded synthetic generators -------------------------------------- */
 
 /* Directed R-MAT: n = 2^scale vertices, round(edge_factor * n) sampled edges

@@ -81,6 +81,17 @@ class gm_graph {
     frozen_ = true;
     ++version_;
   }
+  // gm binary CSR files (gm_graph/src/gm_graph_binary_loader.cc:48-248)
+  bool load_binary(const char* filename) {
+    sccgen_graph* h = sccgen_read_gm_binary(filename);
+    if (!h) return false;
+    adopt(h);
+    return true;
+  }
+  bool store_binary(const char* filename) {
+    if (!freeze()) return false;
+    return sccgen_write_gm_binary(h_, filename) == 0;
+  }
   void clear() {
     if (h_) sccgen_free(h_);
     h_ = nullptr;

@@ -129,3 +129,60 @@ def test_canonicalize():
     assert list(graphs.canonicalize(lab)) == [0, 0, 2, 2, 4, 0]
     with pytest.raises(ValueError):
         graphs.canonicalize(np.array([5], np.int32))
+
+
+def test_gm_binary_layout_and_round_trip(tmp_path):
+    """The file is the big-endian word layout gm_graph_binary_loader.cc:21-45
+    documents (begin has N+1 entries, :145-157), built here independently with
+    numpy; reading it back restores the CSR and its transpose."""
+    import struct
+    g = graphs.rmat(9, 6, seed=4)
+    p = str(tmp_path / "g.bin")
+    g.write_gm_binary(p)
+    raw = open(p, "rb").read()
+    want = (struct.pack(">5I", 0x03939999, 4, 4, g.num_nodes, g.num_edges)
+            + g.begin.astype(">i4").tobytes() + g.node_idx.astype(">i4").tobytes())
+    assert raw == want
+    h = graphs.read_gm_binary(p)
+    for a, b in zip(h.arrays(), g.arrays()):
+        assert np.array_equal(a, b)
+
+
+def test_gm_binary_reader_sorts_rows_and_takes_flipped_files(tmp_path):
+    import struct
+    # rows deliberately unsorted, with a duplicate and a self-loop, little-endian
+    begin = np.array([0, 3, 4, 4, 6], np.int32)
+    adj = np.array([3, 1, 1, 1, 0, 3], np.int32)
+    body = begin.astype("<i4").tobytes() + adj.astype("<i4").tobytes()
+    p = str(tmp_path / "flip.bin")
+    open(p, "wb").write(struct.pack("<5I", 0x03939999, 4, 4, 4, 6) + body)
+    h = graphs.read_gm_binary(p)
+    assert list(h.begin) == [0, 3, 4, 4, 6]
+    assert list(h.node_idx) == [1, 1, 3, 1, 0, 3]
+    src, dst = h.edges()
+    ref = graphs.from_edges(4, src, dst)
+    assert np.array_equal(h.r_begin, ref.r_begin) and np.array_equal(h.r_node_idx, ref.r_node_idx)
+
+
+@pytest.mark.parametrize("case", ["magic", "size8", "short", "offsets", "ids"])
+def test_gm_binary_reader_rejects_malformed(tmp_path, case):
+    import struct
+    n, m = 3, 2
+    head = [0x03939999, 4, 4, n, m]
+    begin = [0, 1, 2, 2]
+    adj = [1, 2]
+    if case == "magic":
+        head[0] = 0x12345678
+    elif case == "size8":
+        head[1] = 8
+    elif case == "offsets":
+        begin = [0, 2, 1, 2]
+    elif case == "ids":
+        adj = [1, 7]
+    data = struct.pack(">5I", *head) + struct.pack(f">{n + 1}i", *begin) + struct.pack(f">{m}i", *adj)
+    if case == "short":
+        data = data[:-4]
+    p = str(tmp_path / "bad.bin")
+    open(p, "wb").write(data)
+    with pytest.raises(RuntimeError):
+        graphs.read_gm_binary(p)
