@@ -11,6 +11,14 @@ achieved = B_FB / (summed device time of its traversal kernels, HIP events on
 the engine's stream), with B_FB = 20|F| + 8E_F + 20|B| + 8E_B + 4|F n B|.
 cpu_baseline: the reference's own OpenMP binary (oracle/_ref/scc_ref) when it
 was built, else the oracle's OpenMP restatement, on this host's cores.
+
+dynamic: the serving workload (R-MAT scale 23 base resident in HBM, 2,000
+insert/delete batches with sizes log-uniform in [1k, 100k], half uniform
+inserts and half uniform deletes of present edges, Poisson arrivals at a fixed
+rate) through the request server; latency = arrival to labels final; labels
+read back every 100 batches and checked against a from-scratch decomposition
+of the same edge set.  Then 500 batches of exactly 10k edges for the
+10k-batch latency figure.
 """
 from __future__ import annotations
 
@@ -87,6 +95,95 @@ def cpu_baseline(g, cores: int, want_ref: bool = True):
     return res
 
 
+def _sleep_until(srv, t_ms):
+    while True:
+        d = t_ms - srv.now_ms()
+        if d <= 0:
+            return
+        if d > 2.0:
+            time.sleep((d - 1.0) / 1e3)
+
+
+def _stream(srv, gid, n, batches, rate_per_s, seed, check_at=()):
+    """Submit batches at Poisson arrival times; LABELS right after check points."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    arr = np.cumsum(rng.exponential(1e3 / rate_per_s, len(batches)))
+    srv.reset_stats()
+    t0 = srv.now_ms() + 20.0
+    reads = {}
+    for i, (b, at) in enumerate(zip(batches, arr)):
+        _sleep_until(srv, t0 + at)
+        srv.update(gid, *b, arrival_ms=t0 + float(at))
+        if i in check_at:
+            reads[i] = srv.labels(gid, n)
+        if (i + 1) % 250 == 0:
+            log(f"  dynamic: submitted {i + 1}/{len(batches)}")
+    if not srv.drain(600_000):
+        raise RuntimeError("update stream did not drain")
+    return srv.stats(), reads
+
+
+def dynamic_leg(args):
+    import hashlib
+
+    from dynamicgraphs_scc_amd import graphs
+    from dynamicgraphs_scc_amd.engine import SccEngine
+    from dynamicgraphs_scc_amd.serve import SccServer
+
+    t = time.time()
+    base = graphs.rmat(args.dyn_scale, 16, seed=23)
+    n = base.num_nodes
+    log(f"dynamic: base R-MAT-{args.dyn_scale} n={n} m={base.num_edges} in {time.time() - t:.1f}s")
+    us = graphs.UpdateStream(base, seed=5, lo=1000, hi=100_000)
+    batches, want = [], {}
+    ver = SccEngine(device=0)
+    t = time.time()
+    for i in range(args.dyn_batches):
+        batches.append(us.next())
+        if (i + 1) % args.dyn_check_every == 0:
+            gi = us.graph()
+            ver.upload(gi)
+            ver.run_method2()
+            want[i] = hashlib.sha256(ver.labels().tobytes()).hexdigest()
+            log(f"  dynamic: reference labels after batch {i + 1} ({time.time() - t:.1f}s)")
+    ver.close()
+    us.set_sizes(10_000, 10_000)
+    fixed = [us.next() for _ in range(args.dyn_fixed_batches)]
+    us.close()
+    n_upd = sum(len(b[0]) + len(b[2]) for b in batches)
+    log(f"dynamic: {len(batches)} batches ({n_upd} edges) generated in {time.time() - t:.1f}s")
+
+    with SccServer(device=0, slots=2, max_batch=100_000) as srv:
+        gid, tk = srv.load(base)
+        srv.result(tk)
+        del base
+        st, reads = _stream(srv, gid, n, batches, args.dyn_rate, seed=6, check_at=set(want))
+        ok = 0
+        for i, (tk, buf) in reads.items():
+            srv.result(tk)
+            ok += hashlib.sha256(buf.tobytes()).hexdigest() == want[i]
+        log(f"dynamic: p50 {st['update_p50_ms']:.2f} p99 {st['update_p99_ms']:.2f} ms, "
+            f"{st['updates_per_s'] / 1e6:.2f} M updates/s, checks {ok}/{len(want)}")
+        st10, _ = _stream(srv, gid, n, fixed, args.dyn_fixed_rate, seed=7)
+        log(f"dynamic 10k: p50 {st10['update_p50_ms']:.2f} p99 {st10['update_p99_ms']:.2f} ms")
+    return {
+        "workload": f"rmat{args.dyn_scale}_ef16_seed23 base, {len(batches)} batches "
+                    f"log-uniform [1k,100k] edges (stream seed 5), Poisson arrivals seed 6",
+        "rate_batches_per_s": args.dyn_rate,
+        "offered_updates_per_s": round(n_upd / len(batches) * args.dyn_rate, 1),
+        "p50_ms": round(st["update_p50_ms"], 3), "p99_ms": round(st["update_p99_ms"], 3),
+        "max_ms": round(st["update_max_ms"], 3), "mean_ms": round(st["update_mean_ms"], 3),
+        "updates_per_s": round(st["updates_per_s"], 1),
+        "edges_applied": int(st["update_edges"]),
+        "checks_passed": ok, "checks": len(want),
+        "batch10k": {"batches": len(fixed), "rate_batches_per_s": args.dyn_fixed_rate,
+                     "p50_ms": round(st10["update_p50_ms"], 3),
+                     "p99_ms": round(st10["update_p99_ms"], 3),
+                     "max_ms": round(st10["update_max_ms"], 3)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +198,13 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (reference point for "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE runs)")
+    ap.add_argument("--no-dyn", action="store_true", help="skip the dynamic serving leg")
+    ap.add_argument("--dyn-scale", type=int, default=23)
+    ap.add_argument("--dyn-batches", type=int, default=2000)
+    ap.add_argument("--dyn-check-every", type=int, default=100)
+    ap.add_argument("--dyn-rate", type=float, default=60.0, help="Poisson arrival rate, batches/s")
+    ap.add_argument("--dyn-fixed-batches", type=int, default=500)
+    ap.add_argument("--dyn-fixed-rate", type=float, default=100.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
@@ -175,12 +279,19 @@ def main():
                                            "ms_fb_parts", "ms_canon")}
     log(f"step {ms_step:.3f} ms  GTEPS {gteps:.2f}  phases {phases}  fb_kernels {t_fb_ms:.3f} ms")
 
+    eng.close()
     cpu = None
     if not args.no_cpu:
         cores = host_cores()
         t = time.time()
         cpu = cpu_baseline(g, cores, want_ref=not args.no_ref)
         log(f"cpu baseline {cpu['kind']}: {cpu['value']:.3f} GTEPS in {time.time() - t:.1f}s")
+
+    dyn = None
+    if not args.no_dyn:
+        t = time.time()
+        dyn = dynamic_leg(args)
+        log(f"dynamic leg in {time.time() - t:.1f}s")
 
     out = {
         "metric": "SCC GTEPS at R-MAT-22, learned vs plain scheduler; p50/p99 dyn-update latency",
@@ -195,9 +306,9 @@ def main():
         "cpu_baseline": cpu,
         "phases_ms": phases,
         "learned_gteps": None,
+        "dynamic": dyn,
     }
     print(json.dumps(out), flush=True)
-    eng.close()
 
 
 if __name__ == "__main__":

@@ -364,6 +364,10 @@ struct scc_ctx {
   int64_t m = 0;
   int64_t m_cap = 0;  // edge capacity of adj/radj
   int32_t *begin = nullptr, *adj = nullptr, *rbegin = nullptr, *radj = nullptr;
+  // Row ends.  Uploaded graphs are packed (row v ends where v+1 begins) and
+  // leave these NULL; the update engine keeps rows with spare room behind them
+  // (see dynamic.hip) and sets them.
+  int32_t *end = nullptr, *rend = nullptr;
 
   // per-vertex state
   int32_t *color = nullptr, *label = nullptr, *canon = nullptr, *W = nullptr;
@@ -407,3 +411,9 @@ struct scc_ctx {
   std::vector<void*> allocs;
   std::vector<uint64_t> alloc_sizes;
 };
+
+namespace sccmi {
+// Row-end arrays of the forward and reverse CSR.
+inline const int32_t* row_end(const scc_ctx* c) { return c->end ? c->end : c->begin + 1; }
+inline const int32_t* rrow_end(const scc_ctx* c) { return c->rend ? c->rend : c->rbegin + 1; }
+}  // namespace sccmi

@@ -14,12 +14,20 @@
 // BW(T) is searched only inside FW(S) (a path from w in FW(S) to T stays in
 // FW(S)), the same restriction the reference's backward search uses.
 //
-// The device CSR and reverse CSR are rebuilt per batch with rows kept sorted:
-// every surviving old entry moves to newbegin[u] + (i - begin[u]) -
-// #deleted-before + #inserted-before, every insert to its merged rank; all
-// positions come from binary searches, so no row is merged sequentially.
+// Graph storage under updates ("gapped" CSR, per direction): row u occupies
+// adj[begin[u] .. end[u]) inside its own room adj[begin[u] .. cap[u]); rooms
+// are laid out in vertex order with 2 + len/8 spare entries each, followed by
+// an overflow area.  The first update converts the packed upload to this
+// layout.  A batch then touches only the rows it changes: their old entries
+// are staged, and each row is rewritten sorted (survivors at
+// i - #deleted-before + #inserted-before, inserts at their merged rank, all by
+// binary search), in its own room when it fits, else in a new room taken from
+// the overflow area.  When the overflow area runs out, the layout is rebuilt
+// with fresh spare room.  Cost per batch is O(batch + entries of changed rows),
+// not O(edges).
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 
@@ -31,6 +39,17 @@ namespace sccmi {
 
 namespace {
 
+// Device counters of the update engine (own block, apart from DevCtl).
+enum DynCtr : int {
+  D_ROWS = 0,      // changed rows collected
+  D_BUMP_F = 1,    // next free entry of the forward overflow area
+  D_BUMP_R = 2,    // same, reverse
+  D_OVERFLOW = 3,  // a room could not be taken
+  D_MAXLEN = 4,    // longest changed row after the batch
+  D_STAGED = 5,    // entries staged from the changed rows
+  D_NUM = 8
+};
+
 struct Dyn {
   int64_t cap_b = 0;  // capacity of each batch list
   unsigned long long *k_in = nullptr, *k_tmp = nullptr;
@@ -38,17 +57,22 @@ struct Dyn {
   unsigned long long *ins_sorted = nullptr;
   uint8_t* flags = nullptr;
   int32_t *src = nullptr, *dst = nullptr;  // staging of one list
-  int32_t *rdel_f = nullptr, *rins_f = nullptr, *rdel_r = nullptr, *rins_r = nullptr;  // per row
-  int32_t *nb_f = nullptr, *nb_r = nullptr;  // new offsets (n+1)
-  int32_t *deg = nullptr;                    // n+1 scan input
+  int32_t *rdel = nullptr, *rins = nullptr;  // per-row counts of the direction being applied (n+1)
+  int32_t* rowmark = nullptr;                // per row: 1 + index in the changed-row list, else 0
   int32_t* aff = nullptr;                    // per old-SCC flag
-  int32_t *adj2 = nullptr, *radj2 = nullptr;
-  int64_t cap2 = 0;
-  void* tmp = nullptr;
+  int32_t *cap_f = nullptr, *cap_r = nullptr;  // room ends of the gapped layout
+  int64_t size_f = 0, size_r = 0;              // entries allocated for adj / radj
+  int64_t cap_rows = 0;                        // capacity of the changed-row arrays
+  int32_t *rows = nullptr, *tmpoff = nullptr, *newbeg = nullptr, *newcap = nullptr, *lens = nullptr;
+  int4* rng = nullptr;  // per changed row: its delete range, its insert range
+  int32_t* stage = nullptr;  // staged old entries of changed rows
+  int64_t stage_cap = 0;
+  unsigned long long* dctr = nullptr;    // DynCtr (device)
+  unsigned long long* h_dctr = nullptr;  // pinned mirror
+  void* tmp = nullptr;  // library temp storage
   size_t tmp_bytes = 0;
   int32_t n = -1;
-  int32_t* h_stage = nullptr;  // pinned staging for batch upload (4 * cap_b ints)
-  int64_t n_del = 0, n_ins = 0;  // final counts (forward lists)
+  int32_t* h_stage = nullptr;  // pinned staging for batch upload (2 * cap_b ints)
 };
 
 constexpr unsigned long long BAD_KEY = ~0ull;
@@ -89,6 +113,7 @@ __global__ __launch_bounds__(BLOCK) void k_make_keys(const int32_t* __restrict__
 __global__ __launch_bounds__(BLOCK) void k_flag_del(const unsigned long long* __restrict__ keys, int64_t k,
                                                     const unsigned long long* __restrict__ ins, int64_t ni,
                                                     const int32_t* __restrict__ begin,
+                                                    const int32_t* __restrict__ end,
                                                     const int32_t* __restrict__ adj,
                                                     uint8_t* __restrict__ flags) {
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < k; i += (int64_t)gridDim.x * BLOCK) {
@@ -96,7 +121,7 @@ __global__ __launch_bounds__(BLOCK) void k_flag_del(const unsigned long long* __
     bool keep = key != BAD_KEY && (i == 0 || keys[i - 1] != key);
     if (keep) {
       const int32_t u = key_hi(key), v = key_lo(key);
-      const int32_t s = begin[u], e = begin[u + 1];
+      const int32_t s = begin[u], e = end[u];
       const int32_t p = lb_i32(adj, s, e, v);
       keep = p < e && adj[p] == v;
       if (keep && ni > 0) {
@@ -111,6 +136,7 @@ __global__ __launch_bounds__(BLOCK) void k_flag_del(const unsigned long long* __
 // Flag sorted insert keys that are absent from G (self-loops already BAD_KEY).
 __global__ __launch_bounds__(BLOCK) void k_flag_ins(const unsigned long long* __restrict__ keys, int64_t k,
                                                     const int32_t* __restrict__ begin,
+                                                    const int32_t* __restrict__ end,
                                                     const int32_t* __restrict__ adj,
                                                     uint8_t* __restrict__ flags) {
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < k; i += (int64_t)gridDim.x * BLOCK) {
@@ -118,7 +144,7 @@ __global__ __launch_bounds__(BLOCK) void k_flag_ins(const unsigned long long* __
     bool keep = key != BAD_KEY && (i == 0 || keys[i - 1] != key);
     if (keep) {
       const int32_t u = key_hi(key), v = key_lo(key);
-      const int32_t s = begin[u], e = begin[u + 1];
+      const int32_t s = begin[u], e = end[u];
       const int32_t p = lb_i32(adj, s, e, v);
       keep = !(p < e && adj[p] == v);
     }
@@ -140,98 +166,230 @@ __global__ __launch_bounds__(BLOCK) void k_row_count(const unsigned long long* _
     atomicAdd(&cnt[key_hi(keys[i])], val);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_row_zero(const unsigned long long* __restrict__ keys, int64_t k,
-                                                    int32_t* __restrict__ a, int32_t* __restrict__ b) {
-  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < k; i += (int64_t)gridDim.x * BLOCK) {
-    const int32_t u = key_hi(keys[i]);
-    a[u] = 0;
-    b[u] = 0;
+__global__ void k_dctr_zero(unsigned long long* __restrict__ dctr, int mask) {
+  const int i = threadIdx.x;
+  if (i < D_NUM && ((mask >> i) & 1)) dctr[i] = 0;
+}
+__global__ void k_store_total(const int32_t* __restrict__ p, unsigned long long* __restrict__ out) {
+  if (threadIdx.x == 0) *out = (unsigned long long)(uint32_t)p[0];
+}
+
+// ---- gapped CSR ------------------------------------------------------------------
+__device__ __forceinline__ int32_t room_of(int32_t len) { return len + 2 + len / 8; }
+
+// Changed rows: every row with a delete or an insert enters the list once.
+__global__ __launch_bounds__(BLOCK) void k_collect_rows(const unsigned long long* __restrict__ keys, int64_t k,
+                                                        int32_t* __restrict__ rowmark,
+                                                        int32_t* __restrict__ rows,
+                                                        unsigned long long* __restrict__ count) {
+  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < k; base += (int64_t)gridDim.x * BLOCK) {
+    const int64_t i = base + threadIdx.x;
+    bool take = false;
+    int32_t u = 0;
+    if (i < k) {
+      u = key_hi(keys[i]);
+      if (i == 0 || key_hi(keys[i - 1]) != u) take = atomicCAS(&rowmark[u], 0, -1) == 0;
+    }
+    wave_append(take, u, rows, count);
   }
 }
 
-// deg[u] = old degree - deleted + inserted (deg[n] = 0 for the scan's total)
-__global__ __launch_bounds__(BLOCK) void k_new_deg(int32_t n, const int32_t* __restrict__ begin,
-                                                   const int32_t* __restrict__ rdel,
-                                                   const int32_t* __restrict__ rins,
-                                                   int32_t* __restrict__ deg) {
-  for (int64_t u = (int64_t)blockIdx.x * BLOCK + threadIdx.x; u <= n; u += (int64_t)gridDim.x * BLOCK)
-    deg[u] = (u < n) ? (begin[u + 1] - begin[u] - rdel[u] + rins[u]) : 0;
-}
-
-// Move surviving old entries.  A workgroup owns 256 consecutive rows and walks
-// their entries block-strided; the row of an entry is found by binary search
-// over the group's offsets staged in LDS.
-constexpr int RB_ROWS = 256;
-__global__ __launch_bounds__(BLOCK) void k_rebuild_old(int32_t n, const int32_t* __restrict__ begin,
-                                                       const int32_t* __restrict__ adj,
-                                                       const int32_t* __restrict__ nbeg,
-                                                       int32_t* __restrict__ adj2,
-                                                       const int32_t* __restrict__ rdel,
-                                                       const int32_t* __restrict__ rins,
-                                                       const unsigned long long* __restrict__ del, int64_t nd,
-                                                       const unsigned long long* __restrict__ ins, int64_t ni) {
-  __shared__ int32_t s_off[RB_ROWS + 1];
-  __shared__ int32_t s_nb[RB_ROWS];
-  __shared__ int32_t s_d0[RB_ROWS], s_d1[RB_ROWS], s_i0[RB_ROWS], s_i1[RB_ROWS];
-  __shared__ uint8_t s_ch[RB_ROWS];
-  for (int64_t r0 = (int64_t)blockIdx.x * RB_ROWS; r0 < n; r0 += (int64_t)gridDim.x * RB_ROWS) {
-    const int32_t rows = (int32_t)min<int64_t>(RB_ROWS, n - r0);
-    for (int t = threadIdx.x; t <= rows; t += BLOCK) s_off[t] = begin[r0 + t];
-    for (int t = threadIdx.x; t < rows; t += BLOCK) {
-      const int32_t u = (int32_t)(r0 + t);
-      s_nb[t] = nbeg[u];
-      const bool ch = rdel[u] != 0 || rins[u] != 0;
-      s_ch[t] = ch;
-      if (ch) {
-        const unsigned long long lo = (unsigned long long)(uint32_t)u << 32;
-        const unsigned long long hi = (unsigned long long)(uint32_t)(u + 1) << 32;
-        s_d0[t] = (int32_t)lb_u64(del, 0, nd, lo);
-        s_d1[t] = (int32_t)lb_u64(del, 0, nd, hi);
-        s_i0[t] = (int32_t)lb_u64(ins, 0, ni, lo);
-        s_i1[t] = (int32_t)lb_u64(ins, 0, ni, hi);
+// Per changed row: staged length, new room (in place, or from the overflow
+// area when the row outgrows its room), longest row.
+__global__ __launch_bounds__(BLOCK) void k_row_plan(const int32_t* __restrict__ rows, int64_t nr,
+                                                    const int32_t* __restrict__ begin,
+                                                    const int32_t* __restrict__ end,
+                                                    const int32_t* __restrict__ cap,
+                                                    const int32_t* __restrict__ rdel,
+                                                    const int32_t* __restrict__ rins,
+                                                    int32_t* __restrict__ rowmark, int32_t* __restrict__ lens,
+                                                    int32_t* __restrict__ newbeg, int32_t* __restrict__ newcap,
+                                                    const unsigned long long* __restrict__ del, int64_t nd,
+                                                    const unsigned long long* __restrict__ ins, int64_t ni,
+                                                    int4* __restrict__ rng,
+                                                    unsigned long long* __restrict__ dctr, int bump_slot,
+                                                    long long size) {
+  for (int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x; j < nr; j += (int64_t)gridDim.x * BLOCK) {
+    const int32_t u = rows[j];
+    // the row's ranges in the sorted delete / insert lists
+    const unsigned long long klo = (unsigned long long)(uint32_t)u << 32;
+    const unsigned long long khi = (unsigned long long)(uint32_t)(u + 1) << 32;
+    int4 r;
+    r.x = (int32_t)lb_u64(del, 0, nd, klo);
+    r.y = (int32_t)lb_u64(del, r.x, nd, khi);
+    r.z = (int32_t)lb_u64(ins, 0, ni, klo);
+    r.w = (int32_t)lb_u64(ins, r.z, ni, khi);
+    rng[j] = r;
+    const int32_t b = begin[u], len = end[u] - b;
+    const int32_t nlen = len - rdel[u] + rins[u];
+    rowmark[u] = (int32_t)j + 1;
+    lens[j] = len;
+    if (j == 0) lens[nr] = 0;  // the scan's last input
+    if (nlen <= cap[u] - b) {
+      newbeg[j] = b;
+      newcap[j] = cap[u];
+    } else {
+      const int32_t room = room_of(nlen);
+      const unsigned long long at = atomicAdd(&dctr[bump_slot], (unsigned long long)room);
+      if ((long long)(at + room) > size) {
+        atomicOr(&dctr[D_OVERFLOW], 1ull);
+        newbeg[j] = b;
+        newcap[j] = cap[u];
+      } else {
+        newbeg[j] = (int32_t)at;
+        newcap[j] = (int32_t)(at + room);
       }
     }
+    atomicMax(&dctr[D_MAXLEN], (unsigned long long)nlen);
+  }
+}
+
+// Copy rows into a destination laid out by doff (an exclusive scan over the
+// list): row j (vertex rows[j], or j when rows is NULL) moves
+// src[sbeg[u] .. send[u]) to dst[doff[j] ..].  A workgroup takes 256 rows and
+// walks their destination range; the row of a position is found by binary
+// search over the group's offsets staged in LDS.  Positions past a row's
+// length (spare room) are skipped.
+constexpr int RB_ROWS = 256;
+__global__ __launch_bounds__(BLOCK) void k_gather_rows(const int32_t* __restrict__ rows, int64_t nr,
+                                                       const int32_t* __restrict__ sbeg,
+                                                       const int32_t* __restrict__ send,
+                                                       const int32_t* __restrict__ src,
+                                                       const int32_t* __restrict__ doff,
+                                                       int32_t* __restrict__ dst) {
+  __shared__ int32_t s_off[RB_ROWS + 1];
+  __shared__ int32_t s_src[RB_ROWS], s_len[RB_ROWS];
+  for (int64_t j0 = (int64_t)blockIdx.x * RB_ROWS; j0 < nr; j0 += (int64_t)gridDim.x * RB_ROWS) {
+    const int32_t cnt = (int32_t)min<int64_t>(RB_ROWS, nr - j0);
+    const int32_t o0 = doff[j0];
+    for (int t = threadIdx.x; t <= cnt; t += BLOCK) s_off[t] = doff[j0 + t] - o0;
+    for (int t = threadIdx.x; t < cnt; t += BLOCK) {
+      const int32_t u = rows ? rows[j0 + t] : (int32_t)(j0 + t);
+      s_src[t] = sbeg[u];
+      s_len[t] = send[u] - sbeg[u];
+    }
     __syncthreads();
-    const int32_t e0 = s_off[0], e1 = s_off[rows];
-    for (int32_t e = e0 + threadIdx.x; e < e1; e += BLOCK) {
-      // row r with s_off[r] <= e < s_off[r+1]
-      int32_t lo = 0, hi = rows;
+    const int32_t total = s_off[cnt];
+    for (int32_t e = threadIdx.x; e < total; e += BLOCK) {
+      int32_t lo = 0, hi = cnt;
       while (hi - lo > 1) {
         const int32_t mid = (lo + hi) >> 1;
         if (s_off[mid] <= e) lo = mid;
         else hi = mid;
       }
-      const int32_t r = lo;
-      const int32_t k = adj[e];
-      int32_t pos = s_nb[r] + (e - s_off[r]);
-      if (s_ch[r]) {
-        const unsigned long long key = ((unsigned long long)(uint32_t)(r0 + r) << 32) | (uint32_t)k;
-        const int64_t dp = lb_u64(del, s_d0[r], s_d1[r], key);
-        if (dp < s_d1[r] && del[dp] == key) continue;  // deleted
-        pos += -(int32_t)(dp - s_d0[r]) + (int32_t)(lb_u64(ins, s_i0[r], s_i1[r], key) - s_i0[r]);
-      }
-      adj2[pos] = k;
+      const int32_t i = e - s_off[lo];
+      if (i < s_len[lo]) dst[o0 + e] = src[s_src[lo] + i];
     }
     __syncthreads();
   }
 }
 
-// Place inserted entries at their merged rank.
-__global__ __launch_bounds__(BLOCK) void k_rebuild_ins(const int32_t* __restrict__ begin,
-                                                       const int32_t* __restrict__ adj,
-                                                       const int32_t* __restrict__ nbeg,
-                                                       int32_t* __restrict__ adj2,
-                                                       const unsigned long long* __restrict__ del, int64_t nd,
-                                                       const unsigned long long* __restrict__ ins, int64_t ni) {
+// Index r of the changed row holding staged entry e: tmpoff[r] <= e < tmpoff[r+1].
+__device__ __forceinline__ int32_t row_of(const int32_t* __restrict__ tmpoff, int32_t nr, int32_t e) {
+  int32_t lo = 0, hi = nr;
+  while (hi - lo > 1) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (tmpoff[mid] <= e) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Stage the old entries of the changed rows (entry-parallel over the grid).
+__global__ __launch_bounds__(BLOCK) void k_stage_rows(const int32_t* __restrict__ rows, int32_t nr,
+                                                      const int32_t* __restrict__ tmpoff,
+                                                      const int32_t* __restrict__ begin,
+                                                      const int32_t* __restrict__ adj,
+                                                      int32_t* __restrict__ stage) {
+  const int32_t total = tmpoff[nr];
+  for (int32_t e = blockIdx.x * BLOCK + threadIdx.x; e < total; e += gridDim.x * BLOCK) {
+    const int32_t r = row_of(tmpoff, nr, e);
+    stage[e] = adj[begin[rows[r]] + (e - tmpoff[r])];
+  }
+}
+
+// Survivors of the staged rows, written sorted to their new rooms:
+// old index - #deleted before it + #inserted before it.
+__global__ __launch_bounds__(BLOCK) void k_merge_rows(const int32_t* __restrict__ rows, int32_t nr,
+                                                      const int32_t* __restrict__ tmpoff,
+                                                      const int32_t* __restrict__ stage,
+                                                      const int32_t* __restrict__ newbeg,
+                                                      const int4* __restrict__ rng,
+                                                      const unsigned long long* __restrict__ del,
+                                                      const unsigned long long* __restrict__ ins,
+                                                      int32_t* __restrict__ adj) {
+  const int32_t total = tmpoff[nr];
+  for (int32_t e = blockIdx.x * BLOCK + threadIdx.x; e < total; e += gridDim.x * BLOCK) {
+    const int32_t r = row_of(tmpoff, nr, e);
+    const int4 g = rng[r];
+    const int32_t k = stage[e];
+    const unsigned long long key = ((unsigned long long)(uint32_t)rows[r] << 32) | (uint32_t)k;
+    int32_t dp = g.x;
+    if (g.y > g.x) {
+      dp = (int32_t)lb_u64(del, g.x, g.y, key);
+      if (dp < g.y && del[dp] == key) continue;  // deleted
+    }
+    const int32_t ip = g.w > g.z ? (int32_t)lb_u64(ins, g.z, g.w, key) : g.z;
+    adj[newbeg[r] + (e - tmpoff[r]) - (dp - g.x) + (ip - g.z)] = k;
+  }
+}
+
+// Inserted entries at their merged rank in the new room.
+__global__ __launch_bounds__(BLOCK) void k_place_ins(const unsigned long long* __restrict__ ins, int64_t ni,
+                                                     const int32_t* __restrict__ rowmark,
+                                                     const int32_t* __restrict__ tmpoff,
+                                                     const int32_t* __restrict__ stage,
+                                                     const int32_t* __restrict__ newbeg,
+                                                     const int4* __restrict__ rng,
+                                                     const unsigned long long* __restrict__ del,
+                                                     int32_t* __restrict__ adj) {
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < ni; i += (int64_t)gridDim.x * BLOCK) {
     const unsigned long long key = ins[i];
     const int32_t u = key_hi(key), x = key_lo(key);
-    const unsigned long long lo = (unsigned long long)(uint32_t)u << 32;
-    const int64_t t = i - lb_u64(ins, 0, i, lo);
-    const int64_t d0 = lb_u64(del, 0, nd, lo);
-    const int64_t dl = lb_u64(del, d0, nd, key) - d0;
-    const int32_t ol = lb_i32(adj, begin[u], begin[u + 1], x) - begin[u];
-    adj2[nbeg[u] + t + ol - dl] = x;
+    const int32_t j = rowmark[u] - 1;
+    const int4 g = rng[j];
+    const int32_t t = (int32_t)i - g.z;
+    const int32_t dl = (int32_t)lb_u64(del, g.x, g.y, key) - g.x;
+    const int32_t s0 = tmpoff[j], s1 = tmpoff[j + 1];
+    const int32_t ol = lb_i32(stage, s0, s1, x) - s0;
+    adj[newbeg[j] + t + ol - dl] = x;
+  }
+}
+
+// Install the new rows and clear the per-row scratch.
+__global__ __launch_bounds__(BLOCK) void k_commit_rows(const int32_t* __restrict__ rows, int64_t nr,
+                                                       const int32_t* __restrict__ newbeg,
+                                                       const int32_t* __restrict__ newcap,
+                                                       int32_t* __restrict__ rdel, int32_t* __restrict__ rins,
+                                                       int32_t* __restrict__ rowmark, int32_t* __restrict__ begin,
+                                                       int32_t* __restrict__ end, int32_t* __restrict__ cap) {
+  for (int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x; j < nr; j += (int64_t)gridDim.x * BLOCK) {
+    const int32_t u = rows[j];
+    const int32_t nlen = end[u] - begin[u] - rdel[u] + rins[u];
+    begin[u] = newbeg[j];
+    end[u] = newbeg[j] + nlen;
+    cap[u] = newcap[j];
+    rdel[u] = 0;
+    rins[u] = 0;
+    rowmark[u] = 0;
+  }
+}
+
+// Room sizes of every row for a fresh layout (rooms[n] = 0 for the scan).
+__global__ __launch_bounds__(BLOCK) void k_room_sizes(int32_t n, const int32_t* __restrict__ begin,
+                                                      const int32_t* __restrict__ end,
+                                                      int32_t* __restrict__ rooms) {
+  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v <= n; v += (int64_t)gridDim.x * BLOCK)
+    rooms[v] = v < n ? room_of(end[v] - begin[v]) : 0;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_room_ends(int32_t n, const int32_t* __restrict__ begin,
+                                                     const int32_t* __restrict__ end,
+                                                     const int32_t* __restrict__ nbeg,
+                                                     int32_t* __restrict__ nend, int32_t* __restrict__ ncap) {
+  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < n; v += (int64_t)gridDim.x * BLOCK) {
+    nend[v] = nbeg[v] + (end[v] - begin[v]);
+    ncap[v] = nbeg[v + 1];
   }
 }
 
@@ -336,6 +494,18 @@ __global__ __launch_bounds__(BLOCK) void k_clear_aff(const unsigned long long* _
 
 Dyn* dyn_of(scc_ctx* c) { return (Dyn*)c->dyn; }
 
+// Library temp storage of at least `bytes`.
+int ensure_tmp(scc_ctx* c, Dyn* d, size_t bytes) {
+  if (bytes <= d->tmp_bytes) return SCC_OK;
+  dev_free(c, d->tmp);
+  d->tmp = nullptr;
+  d->tmp_bytes = 0;
+  int rc = dev_alloc(c, &d->tmp, bytes, "library temp");
+  if (rc) return rc;
+  d->tmp_bytes = bytes;
+  return SCC_OK;
+}
+
 int ensure_dyn(scc_ctx* c, int64_t batch) {
   Dyn* d = dyn_of(c);
   if (!d) {
@@ -343,12 +513,16 @@ int ensure_dyn(scc_ctx* c, int64_t batch) {
     c->dyn = d;
   }
   int rc;
+  if (!d->dctr) {
+    if ((rc = dev_alloc(c, (void**)&d->dctr, 8ull * D_NUM, "update counters"))) return rc;
+    if ((rc = hip_check(c, hipHostMalloc((void**)&d->h_dctr, 8ull * D_NUM, 0), "pinned counters")))
+      return rc;
+  }
   if (d->n != c->n) {
-    // per-row arrays
-    int32_t** rows[] = {&d->rdel_f, &d->rins_f, &d->rdel_r, &d->rins_r, &d->nb_f, &d->nb_r, &d->deg,
-                        &d->aff};
+    int32_t** rows[] = {&d->rdel, &d->rins, &d->rowmark, &d->aff};
     for (auto p : rows) {
       dev_free(c, *p);
+      *p = nullptr;
       if ((rc = dev_alloc(c, (void**)p, 4ull * ((uint64_t)c->n + 1), "dynamic row arrays"))) return rc;
       launch_fill_i32(c->stream, *p, 0, c->n + 1);
     }
@@ -360,11 +534,14 @@ int ensure_dyn(scc_ctx* c, int64_t batch) {
                                  &d->ins_sorted};
     for (auto p : ks) {
       dev_free(c, *p);
+      *p = nullptr;
       if ((rc = dev_alloc(c, (void**)p, 8ull * cap, "batch keys"))) return rc;
     }
     dev_free(c, d->flags);
     dev_free(c, d->src);
     dev_free(c, d->dst);
+    d->flags = nullptr;
+    d->src = d->dst = nullptr;
     if ((rc = dev_alloc(c, (void**)&d->flags, cap, "batch flags")) ||
         (rc = dev_alloc(c, (void**)&d->src, 4ull * cap, "batch src")) ||
         (rc = dev_alloc(c, (void**)&d->dst, 4ull * cap, "batch dst")))
@@ -373,28 +550,47 @@ int ensure_dyn(scc_ctx* c, int64_t batch) {
     d->h_stage = nullptr;
     if ((rc = hip_check(c, hipHostMalloc((void**)&d->h_stage, 8ull * cap, 0), "pinned staging")))
       return rc;
-    // temp storage for sort / select / scan at this size
-    size_t a = 0, b = 0, s = 0, m = 0;
-    hipcub::DeviceRadixSort::SortKeys(nullptr, a, d->k_in, d->k_tmp, (int)cap);
-    hipcub::DeviceSelect::Flagged(nullptr, b, d->k_in, d->flags, d->del_f, (int32_t*)nullptr, (int)cap);
-    hipcub::DeviceScan::ExclusiveSum(nullptr, s, d->deg, d->nb_f, c->n + 1);
-    m = std::max(std::max(a, b), s) + 256;
-    if (m > d->tmp_bytes) {
-      dev_free(c, d->tmp);
-      if ((rc = dev_alloc(c, &d->tmp, m, "cub temp"))) return rc;
-      d->tmp_bytes = m;
+    // changed rows: at most one per delete plus one per insert
+    const int64_t cr = 2 * cap + 1;
+    int32_t** ra[] = {&d->rows, &d->tmpoff, &d->newbeg, &d->newcap, &d->lens};
+    for (auto p : ra) {
+      dev_free(c, *p);
+      *p = nullptr;
+      if ((rc = dev_alloc(c, (void**)p, 4ull * (cr + 1), "changed-row arrays"))) return rc;
     }
+    dev_free(c, d->rng);
+    d->rng = nullptr;
+    if ((rc = dev_alloc(c, (void**)&d->rng, 16ull * (cr + 1), "changed-row ranges"))) return rc;
+    d->cap_rows = cr;
+    size_t a = 0, b = 0, s = 0;
+    (void)hipcub::DeviceRadixSort::SortKeys(nullptr, a, d->k_in, d->k_tmp, (int)cap);
+    (void)hipcub::DeviceSelect::Flagged(nullptr, b, d->k_in, d->flags, d->del_f, (int32_t*)nullptr, (int)cap);
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, s, d->lens, d->tmpoff, (int)(cr + 1));
+    if ((rc = ensure_tmp(c, d, std::max(std::max(a, b), s) + 256))) return rc;
     d->cap_b = cap;
-  } else {
-    size_t s = 0;
-    hipcub::DeviceScan::ExclusiveSum(nullptr, s, d->deg, d->nb_f, c->n + 1);
-    if (s + 256 > d->tmp_bytes) {
-      dev_free(c, d->tmp);
-      if ((rc = dev_alloc(c, &d->tmp, s + 256, "cub temp"))) return rc;
-      d->tmp_bytes = s + 256;
-    }
   }
   return SCC_OK;
+}
+
+// What ensure_dyn, the gapped layout and its rebuilds allocate, bounded from
+// above, for a graph growing to m_max edges.
+uint64_t gapped_entries(int64_t n, int64_t m) {
+  const uint64_t t = (uint64_t)m + 2ull * (uint64_t)n + (uint64_t)m / 8;
+  return t + std::max<uint64_t>(t / 8, 1u << 20);
+}
+uint64_t update_bytes(int64_t n, int64_t m, int64_t m_max, int64_t max_batch) {
+  const uint64_t N = (uint64_t)n + 1;
+  const int64_t mm = std::max(m, m_max);
+  const uint64_t cap_b = (uint64_t)std::max<int64_t>(max_batch, 1024);
+  const uint64_t g = gapped_entries(n, mm) + 4 * cap_b;
+  uint64_t b = 4 * 4 * N;                       // per-row counters, marks, old-SCC flags
+  b += 2 * 3 * 4 * N;                           // begin / end / room ends of a rebuilt layout
+  b += 2 * 2 * 4 * g;                           // adj + radj, old and new during a rebuild
+  b += 4 * (uint64_t)mm;                        // staged rows (at most every entry)
+  b += cap_b * (7 * 8 + 1 + 4 + 4);             // keys, flags, src, dst
+  b += 5 * 4 * (2 * cap_b + 2);                 // changed-row arrays
+  b += 2 * 8 * cap_b + 8 * N + (1u << 20);      // sort / select / scan temp
+  return b + 4096;
 }
 
 int sort_keys(scc_ctx* c, Dyn* d, unsigned long long* in, unsigned long long* out, int64_t k) {
@@ -421,22 +617,132 @@ int select_flagged(scc_ctx* c, Dyn* d, unsigned long long* in, unsigned long lon
   return SCC_OK;
 }
 
-// Rebuild one direction: off/nb -> noff/nb2 using del/ins key lists sorted by row.
-int rebuild_dir(scc_ctx* c, Dyn* d, const int32_t* off, const int32_t* nb, int32_t* rdel, int32_t* rins,
-                int32_t* noff, int32_t* nb2, const unsigned long long* del, int64_t nd,
-                const unsigned long long* ins, int64_t ni) {
-  const int32_t n = c->n;
-  if (nd) LAUNCH(k_row_count, grid_for(nd), c->stream, del, nd, rdel, 1);
-  if (ni) LAUNCH(k_row_count, grid_for(ni), c->stream, ins, ni, rins, 1);
-  LAUNCH(k_new_deg, grid_for((int64_t)n + 1, 4096), c->stream, n, off, rdel, rins, d->deg);
-  size_t t = d->tmp_bytes;
-  int rc = hip_check(c, hipcub::DeviceScan::ExclusiveSum(d->tmp, t, d->deg, noff, n + 1, c->stream), "scan");
+int pull_dctr(scc_ctx* c, Dyn* d) {
+  int rc = hip_check(c, hipMemcpyAsync(d->h_dctr, d->dctr, 8ull * D_NUM, hipMemcpyDeviceToHost, c->stream),
+                     "update counters");
   if (rc) return rc;
-  LAUNCH(k_rebuild_old, grid_for(n, 4096), c->stream, n, off, nb, noff, nb2, rdel, rins, del, nd, ins, ni);
-  if (ni) LAUNCH(k_rebuild_ins, grid_for(ni), c->stream, off, nb, noff, nb2, del, nd, ins, ni);
-  if (nd) LAUNCH(k_row_zero, grid_for(nd), c->stream, del, nd, rdel, rins);
-  if (ni) LAUNCH(k_row_zero, grid_for(ni), c->stream, ins, ni, rdel, rins);
-  return hip_check(c, hipGetLastError(), "rebuild");
+  return hip_check(c, hipStreamSynchronize(c->stream), "update counters sync");
+}
+
+// (Re)build one direction in the gapped layout with fresh room for every row
+// and an overflow area of at least min_over entries.  Reads the current rows
+// (packed when *end is NULL), writes new begin / end / room ends / adj.
+int regap_dir(scc_ctx* c, Dyn* d, bool fwd, int64_t min_over) {
+  const int32_t n = c->n;
+  int32_t*& beg = fwd ? c->begin : c->rbegin;
+  int32_t*& en = fwd ? c->end : c->rend;
+  int32_t*& adj = fwd ? c->adj : c->radj;
+  int32_t*& cap = fwd ? d->cap_f : d->cap_r;
+  int64_t& size = fwd ? d->size_f : d->size_r;
+  const int32_t* cur_end = en ? en : beg + 1;
+  int32_t *nbeg = nullptr, *nend = nullptr, *ncap = nullptr, *nadj = nullptr;
+  int rc;
+  if ((rc = dev_alloc(c, (void**)&nbeg, 4ull * ((uint64_t)n + 1), "gapped begin")) ||
+      (rc = dev_alloc(c, (void**)&nend, 4ull * ((uint64_t)n + 1), "gapped end")) ||
+      (rc = dev_alloc(c, (void**)&ncap, 4ull * ((uint64_t)n + 1), "gapped rooms"))) {
+    dev_free(c, nbeg);
+    dev_free(c, nend);
+    dev_free(c, ncap);
+    return rc;
+  }
+  auto fail = [&](int r) {
+    dev_free(c, nbeg);
+    dev_free(c, nend);
+    dev_free(c, ncap);
+    dev_free(c, nadj);
+    return r;
+  };
+  // room sizes -> scan (in ncap as scratch) -> total
+  LAUNCH(k_room_sizes, grid_for((int64_t)n + 1, 4096), c->stream, n, beg, cur_end, ncap);
+  size_t t = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, t, ncap, nbeg, n + 1);
+  if ((rc = ensure_tmp(c, d, t + 256))) return fail(rc);
+  t = d->tmp_bytes;
+  if ((rc = hip_check(c, hipcub::DeviceScan::ExclusiveSum(d->tmp, t, ncap, nbeg, n + 1, c->stream), "room scan")))
+    return fail(rc);
+  int32_t total = 0;
+  if ((rc = hip_check(c, hipMemcpyAsync(&total, nbeg + n, 4, hipMemcpyDeviceToHost, c->stream), "room total")) ||
+      (rc = hip_check(c, hipStreamSynchronize(c->stream), "room total sync")))
+    return fail(rc);
+  const int64_t over = std::max<int64_t>(std::max<int64_t>((int64_t)total / 8, 1 << 20), min_over);
+  const int64_t nsize = (int64_t)total + over;
+  if (nsize >= INT32_MAX) return fail(set_error(c, SCC_ERR_LIMIT, "gapped layout would exceed int32 offsets"));
+  if ((rc = dev_alloc(c, (void**)&nadj, 4ull * (uint64_t)nsize, "gapped adjacency"))) return fail(rc);
+  LAUNCH(k_gather_rows, grid_for(n, 4096), c->stream, (const int32_t*)nullptr, (int64_t)n, beg, cur_end, adj,
+         nbeg, nadj);
+  LAUNCH(k_room_ends, grid_for(n, 4096), c->stream, n, beg, cur_end, nbeg, nend, ncap);
+  if ((rc = hip_check(c, hipGetLastError(), "regap"))) return fail(rc);
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "regap sync"))) return fail(rc);
+  dev_free(c, beg);
+  dev_free(c, en);
+  dev_free(c, adj);
+  dev_free(c, cap);
+  beg = nbeg;
+  en = nend;
+  cap = ncap;
+  adj = nadj;
+  size = nsize;
+  unsigned long long b = (unsigned long long)total;
+  if ((rc = hip_check(c, hipMemcpyAsync(d->dctr + (fwd ? D_BUMP_F : D_BUMP_R), &b, 8, hipMemcpyHostToDevice,
+                                        c->stream), "bump reset")))
+    return rc;
+  return hip_check(c, hipStreamSynchronize(c->stream), "bump reset sync");
+}
+
+// Apply one direction's sorted, filtered delete / insert lists to the gapped rows.
+int apply_dir(scc_ctx* c, Dyn* d, bool fwd, const unsigned long long* del, int64_t nd,
+              const unsigned long long* ins, int64_t ni) {
+  hipStream_t s = c->stream;
+  int rc;
+  // changed rows and per-row counts
+  LAUNCH(k_dctr_zero, 1, s, d->dctr, 1 << D_ROWS);
+  if (nd) LAUNCH(k_row_count, grid_for(nd), s, del, nd, d->rdel, 1);
+  if (ni) LAUNCH(k_row_count, grid_for(ni), s, ins, ni, d->rins, 1);
+  if (nd) LAUNCH(k_collect_rows, grid_for(nd), s, del, nd, d->rowmark, d->rows, d->dctr + D_ROWS);
+  if (ni) LAUNCH(k_collect_rows, grid_for(ni), s, ins, ni, d->rowmark, d->rows, d->dctr + D_ROWS);
+  if ((rc = pull_dctr(c, d))) return rc;
+  const int64_t nr = (int64_t)d->h_dctr[D_ROWS];
+  if (nr == 0) return SCC_OK;
+  for (int attempt = 0;; attempt++) {
+    LAUNCH(k_dctr_zero, 1, s, d->dctr, (1 << D_OVERFLOW) | (1 << D_MAXLEN) | (1 << D_STAGED));
+    LAUNCH(k_row_plan, grid_for(nr), s, d->rows, nr, fwd ? c->begin : c->rbegin, fwd ? c->end : c->rend,
+           fwd ? d->cap_f : d->cap_r, d->rdel, d->rins, d->rowmark, d->lens, d->newbeg, d->newcap, del, nd,
+           ins, ni, d->rng, d->dctr,
+           fwd ? (int)D_BUMP_F : (int)D_BUMP_R, (long long)(fwd ? d->size_f : d->size_r));
+    size_t t = d->tmp_bytes;
+    if ((rc = hip_check(c, hipcub::DeviceScan::ExclusiveSum(d->tmp, t, d->lens, d->tmpoff, (int)(nr + 1), s),
+                        "staging scan")))
+      return rc;
+    LAUNCH(k_store_total, 1, s, d->tmpoff + nr, d->dctr + D_STAGED);
+    if ((rc = pull_dctr(c, d))) return rc;
+    if (!d->h_dctr[D_OVERFLOW]) break;
+    if (attempt > 0) return set_error(c, SCC_ERR_LIMIT, "update: no room after rebuilding the layout");
+    // out of overflow room: rebuild this direction with fresh room, plan again
+    const int64_t staged = (int64_t)d->h_dctr[D_STAGED];
+    if ((rc = regap_dir(c, d, fwd, 2 * (staged + ni + 3 * nr) + 4096))) return rc;
+  }
+  c->maxdeg = std::max<int32_t>(c->maxdeg, (int32_t)d->h_dctr[D_MAXLEN]);
+  const int64_t staged = (int64_t)d->h_dctr[D_STAGED];
+  if (staged > d->stage_cap) {
+    dev_free(c, d->stage);
+    d->stage = nullptr;
+    const int64_t cap = staged + staged / 2 + 4096;
+    if ((rc = dev_alloc(c, (void**)&d->stage, 4ull * (uint64_t)cap, "staged rows"))) return rc;
+    d->stage_cap = cap;
+  }
+  int32_t* beg = fwd ? c->begin : c->rbegin;
+  int32_t* en = fwd ? c->end : c->rend;
+  int32_t* adj = fwd ? c->adj : c->radj;
+  (void)en;
+  const unsigned sg = grid_for(std::max<int64_t>(staged, 1), 2048);
+  if (staged) {
+    LAUNCH(k_stage_rows, sg, s, d->rows, (int32_t)nr, d->tmpoff, beg, adj, d->stage);
+    LAUNCH(k_merge_rows, sg, s, d->rows, (int32_t)nr, d->tmpoff, d->stage, d->newbeg, d->rng, del, ins, adj);
+  }
+  if (ni) LAUNCH(k_place_ins, grid_for(ni), s, ins, ni, d->rowmark, d->tmpoff, d->stage, d->newbeg, d->rng, del, adj);
+  LAUNCH(k_commit_rows, grid_for(nr), s, d->rows, nr, d->newbeg, d->newcap, d->rdel, d->rins, d->rowmark, beg, en,
+         fwd ? d->cap_f : d->cap_r);
+  return hip_check(c, hipGetLastError(), "apply rows");
 }
 
 double ms_since(std::chrono::steady_clock::time_point t0) {
@@ -451,17 +757,64 @@ void destroy_dynamic(scc_ctx* c) {
   Dyn* d = dyn_of(c);
   if (!d) return;
   void* ptrs[] = {d->k_in, d->k_tmp, d->del_f, d->del_r, d->ins_f, d->ins_r, d->ins_sorted,
-                  d->flags, d->src, d->dst, d->rdel_f, d->rins_f, d->rdel_r, d->rins_r,
-                  d->nb_f, d->nb_r, d->deg, d->aff, d->adj2, d->radj2, d->tmp};
+                  d->flags, d->src, d->dst, d->rdel, d->rins, d->rowmark, d->aff, d->cap_f, d->cap_r,
+                  d->rows, d->tmpoff, d->newbeg, d->newcap, d->lens, d->rng, d->stage, d->dctr, d->tmp};
   for (void* p : ptrs) dev_free(c, p);
   if (d->h_stage) (void)hipHostFree(d->h_stage);
+  if (d->h_dctr) (void)hipHostFree(d->h_dctr);
   delete d;
   c->dyn = nullptr;
+}
+
+// A new upload replaces the gapped layout: drop its room ends.
+void dynamic_graph_replaced(scc_ctx* c) {
+  Dyn* d = dyn_of(c);
+  if (!d) return;
+  dev_free(c, d->cap_f);
+  dev_free(c, d->cap_r);
+  d->cap_f = d->cap_r = nullptr;
+  d->size_f = d->size_r = 0;
+}
+
+// Copy a gapped graph out packed (rows in vertex order).
+int download_gapped(scc_ctx* c, int32_t* begin, int32_t* node_idx, int32_t* r_begin, int32_t* r_node_idx) {
+  Dyn* d = dyn_of(c);
+  const int32_t n = c->n;
+  int rc;
+  for (int dir = 0; dir < 2; dir++) {
+    int32_t* ob = dir ? r_begin : begin;
+    int32_t* oa = dir ? r_node_idx : node_idx;
+    if (!ob && !oa) continue;
+    const int64_t size = dir ? d->size_r : d->size_f;
+    std::vector<int32_t> hb(n + 1), he(n + 1), ha(size);
+    if ((rc = hip_check(c, hipMemcpyAsync(hb.data(), dir ? c->rbegin : c->begin, 4ull * n, hipMemcpyDeviceToHost,
+                                          c->stream), "download")) ||
+        (rc = hip_check(c, hipMemcpyAsync(he.data(), dir ? c->rend : c->end, 4ull * n, hipMemcpyDeviceToHost,
+                                          c->stream), "download")) ||
+        (rc = hip_check(c, hipMemcpyAsync(ha.data(), dir ? c->radj : c->adj, 4ull * size, hipMemcpyDeviceToHost,
+                                          c->stream), "download")) ||
+        (rc = hip_check(c, hipStreamSynchronize(c->stream), "download sync")))
+      return rc;
+    int64_t o = 0;
+    for (int32_t v = 0; v < n; v++) {
+      if (ob) ob[v] = (int32_t)o;
+      for (int32_t i = hb[v]; i < he[v]; i++) {
+        if (oa) oa[o] = ha[i];
+        o++;
+      }
+    }
+    if (ob) ob[n] = (int32_t)o;
+  }
+  return SCC_OK;
 }
 
 }  // namespace sccmi
 
 using namespace sccmi;
+
+extern "C" uint64_t scc_update_bytes_needed(int32_t n, int64_t m, int64_t m_max, int64_t max_batch) {
+  return update_bytes(n, m, m_max, max_batch);
+}
 
 extern "C" int scc_update(scc_ctx* c, const int32_t* ins_src, const int32_t* ins_dst, int64_t n_ins,
                           const int32_t* del_src, const int32_t* del_dst, int64_t n_del,
@@ -513,13 +866,14 @@ extern "C" int scc_update(scc_ctx* c, const int32_t* ins_src, const int32_t* ins
   if ((rc = upload_keys(ins_src, ins_dst, n_ins, true))) return rc;
   if ((rc = sort_keys(c, d, d->k_in, d->ins_sorted, n_ins))) return rc;
   if (n_ins) {
-    LAUNCH(k_flag_ins, grid_for(n_ins), s, d->ins_sorted, n_ins, c->begin, c->adj, d->flags);
+    LAUNCH(k_flag_ins, grid_for(n_ins), s, d->ins_sorted, n_ins, c->begin, row_end(c), c->adj, d->flags);
     if ((rc = select_flagged(c, d, d->ins_sorted, d->ins_f, n_ins, &ni))) return rc;
   }
   if ((rc = upload_keys(del_src, del_dst, n_del, false))) return rc;
   if ((rc = sort_keys(c, d, d->k_in, d->k_tmp, n_del))) return rc;
   if (n_del) {
-    LAUNCH(k_flag_del, grid_for(n_del), s, d->k_tmp, n_del, d->ins_sorted, n_ins, c->begin, c->adj, d->flags);
+    LAUNCH(k_flag_del, grid_for(n_del), s, d->k_tmp, n_del, d->ins_sorted, n_ins, c->begin, row_end(c), c->adj,
+           d->flags);
     if ((rc = select_flagged(c, d, d->k_tmp, d->del_f, n_del, &nd))) return rc;
   }
   // reverse-direction lists
@@ -543,31 +897,17 @@ extern "C" int scc_update(scc_ctx* c, const int32_t* ins_src, const int32_t* ins
     return SCC_OK;
   }
 
-  // ---- 2. rebuild CSR and reverse CSR ---------------------------------------
+  // ---- 2. apply to the rows (gapped layout, converted on first use) ----------
   auto ta = std::chrono::steady_clock::now();
-  const int64_t m_new = c->m + ni - nd;
-  if (m_new > d->cap2 || !d->adj2) {
-    dev_free(c, d->adj2);
-    dev_free(c, d->radj2);
-    d->adj2 = d->radj2 = nullptr;
-    const int64_t cap = m_new + m_new / 32 + 4096;
-    if ((rc = dev_alloc(c, (void**)&d->adj2, 4ull * cap, "spare adj")) ||
-        (rc = dev_alloc(c, (void**)&d->radj2, 4ull * cap, "spare radj")))
-      return rc;
-    d->cap2 = cap;
+  if (!c->end) {
+    if ((rc = regap_dir(c, d, true, 4 * d->cap_b)) || (rc = regap_dir(c, d, false, 4 * d->cap_b))) return rc;
   }
-  if ((rc = rebuild_dir(c, d, c->begin, c->adj, d->rdel_f, d->rins_f, d->nb_f, d->adj2, d->del_f, nd,
-                        d->ins_f, ni)) ||
-      (rc = rebuild_dir(c, d, c->rbegin, c->radj, d->rdel_r, d->rins_r, d->nb_r, d->radj2, d->del_r, nd,
-                        d->ins_r, ni)))
+  if ((rc = apply_dir(c, d, true, d->del_f, nd, d->ins_f, ni)) ||
+      (rc = apply_dir(c, d, false, d->del_r, nd, d->ins_r, ni)))
     return rc;
-  std::swap(c->begin, d->nb_f);
-  std::swap(c->rbegin, d->nb_r);
-  std::swap(c->adj, d->adj2);
-  std::swap(c->radj, d->radj2);
-  std::swap(c->m_cap, d->cap2);
-  c->m = m_new;
-  if ((rc = hip_check(c, hipStreamSynchronize(s), "rebuild sync"))) return rc;
+  c->m = c->m + ni - nd;
+  c->m_cap = d->size_f;
+  if ((rc = hip_check(c, hipStreamSynchronize(s), "apply sync"))) return rc;
   st.ms_apply = ms_since(ta);
 
   // ---- 3. affected region -----------------------------------------------------

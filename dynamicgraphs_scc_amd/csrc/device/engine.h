@@ -39,5 +39,7 @@ int do_canon(scc_ctx* c, int64_t* count);
 // optional subsystems (dynamic.hip, sched.hip)
 void destroy_dynamic(scc_ctx* c);
 void destroy_sched(scc_ctx* c);
+void dynamic_graph_replaced(scc_ctx* c);
+int download_gapped(scc_ctx* c, int32_t* begin, int32_t* node_idx, int32_t* r_begin, int32_t* r_node_idx);
 
 }  // namespace sccmi

@@ -82,7 +82,9 @@ void dev_free(scc_ctx* c, void* p) {
   }
 }
 
-GraphPtrs gp(const scc_ctx* c) { return GraphPtrs{c->begin, c->adj, c->rbegin, c->radj, c->n, c->maxdeg}; }
+GraphPtrs gp(const scc_ctx* c) {
+  return GraphPtrs{c->begin, row_end(c), c->adj, c->rbegin, rrow_end(c), c->radj, c->n, c->maxdeg};
+}
 
 // Copy the control block to the host and refresh the host copies of the live list.
 int pull_ctl(scc_ctx* c) {
@@ -130,7 +132,8 @@ uint64_t state_bytes(int32_t n) {
 }
 
 int free_graph(scc_ctx* c) {
-  int32_t** arrs[] = {&c->begin, &c->adj, &c->rbegin, &c->radj, &c->color, &c->label, &c->canon,
+  dynamic_graph_replaced(c);
+  int32_t** arrs[] = {&c->begin, &c->adj, &c->rbegin, &c->radj, &c->end, &c->rend, &c->color, &c->label, &c->canon,
                       &c->W, &c->live[0], &c->live[1], &c->q[0], &c->q[1], &c->cand[0],
                       &c->cand[1], &c->huge, &c->huge2, &c->rep, &c->scratch_i32};
   for (auto p : arrs) {
@@ -252,6 +255,7 @@ int do_trim2(scc_ctx* c, int64_t* removed) {
 int run_levels(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed) {
   const int32_t *off = fwd ? c->begin : c->rbegin, *nb = fwd ? c->adj : c->radj;
   const int32_t *roff = fwd ? c->rbegin : c->begin, *rnb = fwd ? c->radj : c->adj;
+  const int32_t *offe = fwd ? row_end(c) : rrow_end(c), *roffe = fwd ? rrow_end(c) : row_end(c);
   uint32_t* V = fwd ? c->vf : c->vb;
   const unsigned grid = grid_for(c->n_live);
   const int32_t max_chunks = std::max<int32_t>(1, (c->maxdeg + 2047) / 2048);
@@ -261,7 +265,7 @@ int run_levels(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed) {
     if ((rc = check_cancel(c))) return rc;
     if (timed) (void)hipEventRecord(c->ev[0], c->stream);
     for (int i = 0; i < batch; i++)
-      launch_level(c->stream, grid, c->dc, multi ? 1 : 0, off, nb, roff, rnb, c->color, V, c->q[0],
+      launch_level(c->stream, grid, c->dc, multi ? 1 : 0, off, offe, nb, roff, roffe, rnb, c->color, V, c->q[0],
                    c->q[1], c->cand[0], c->cand[1], c->huge, c->huge2, c->live[0], c->live[1],
                    max_chunks);
     if (timed) (void)hipEventRecord(c->ev[1], c->stream);
@@ -289,7 +293,7 @@ int traverse(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed) {
   uint32_t* V = fwd ? c->vf : c->vb;
   const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
   launch_ctl_init(c->stream, c->dc, 8, 0, 0, 0, avg / 14.0);
-  launch_seed_pivots(c->stream, grid_for(c->n_live), c->dc, c->n, off, c->color, c->pkey, V, c->q[0],
+  launch_seed_pivots(c->stream, grid_for(c->n_live), c->dc, c->n, off, fwd ? row_end(c) : rrow_end(c), c->color, c->pkey, V, c->q[0],
                      c->huge, c->live[0], c->live[1]);
   return run_levels(c, fwd, multi, levels, timed);
 }
@@ -300,7 +304,7 @@ int traverse_from(scc_ctx* c, bool fwd, bool multi, const int32_t* seeds,
   uint32_t* V = fwd ? c->vf : c->vb;
   const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
   launch_ctl_init(c->stream, c->dc, 8, 0, 0, 0, avg / 14.0);
-  launch_seed_list(c->stream, grid_for(c->n_live), c->dc, off, c->color, seeds, d_count, V, c->q[0]);
+  launch_seed_list(c->stream, grid_for(c->n_live), c->dc, off, fwd ? row_end(c) : rrow_end(c), c->color, seeds, d_count, V, c->q[0]);
   return run_levels(c, fwd, multi, levels, false);
 }
 
@@ -352,12 +356,12 @@ static int wcc_fixpoint(scc_ctx* c, bool seed) {
   int rc;
   const unsigned grid = grid_for(c->n_live);
   launch_ctl_init(c->stream, c->dc, 1 | 4, 0, 0, 0, 0.0);
-  launch_wcc_init(c->stream, grid, c->dc, c->begin, c->W, c->live[0], c->live[1]);
+  launch_wcc_init(c->stream, grid, c->dc, c->begin, row_end(c), c->W, c->live[0], c->live[1]);
   int batch = std::max(2, c->it_hint);
   for (int guard = 0;; guard++) {
     if ((rc = check_cancel(c))) return rc;
     for (int i = 0; i < batch; i++)
-      launch_wcc_step(c->stream, grid, c->dc, c->begin, c->adj, c->color, c->W, c->live[0], c->live[1]);
+      launch_wcc_step(c->stream, grid, c->dc, c->begin, row_end(c), c->adj, c->color, c->W, c->live[0], c->live[1]);
     if ((rc = pull_ctl(c))) return rc;
     if (c->h_dc->loop.done) break;
     batch = std::max(4, batch);
@@ -369,7 +373,7 @@ static int wcc_fixpoint(scc_ctx* c, bool seed) {
     launch_ctl_init(c->stream, c->dc, 8, 0, 0, 0, avg / 14.0);
     launch_fill_u32(c->stream, c->vf, 0u, ((int64_t)c->n + 31) / 32);
   }
-  launch_wcc_color(c->stream, grid, c->dc, c->color, c->W, seed ? 1 : 0, c->begin, c->vf, c->q[0],
+  launch_wcc_color(c->stream, grid, c->dc, c->color, c->W, seed ? 1 : 0, c->begin, row_end(c), c->vf, c->q[0],
                    c->live[0], c->live[1]);
   c->colors_split = true;
   return hip_check(c, hipGetLastError(), "wcc colour");
@@ -475,7 +479,7 @@ int scc_ctx_destroy(scc_ctx* c) {
   destroy_dynamic(c);
   destroy_sched(c);
   free_graph(c);
-  for (void* p : c->allocs) hipFree(p);
+  for (void* p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
   if (c->h_dc) (void)hipHostFree(c->h_dc);
   for (auto& e : c->ev)
@@ -489,6 +493,23 @@ const char* scc_last_error(const scc_ctx* c) { return c ? c->err.c_str() : "null
 uint64_t scc_ctx_bytes(const scc_ctx* c) { return c ? c->bytes : 0; }
 uint64_t scc_bytes_needed(int32_t n, int64_t m) {
   return 8ull * (uint64_t)(m > 0 ? m : 1) + 8ull * ((uint64_t)n + 1) + state_bytes(n) + sizeof(DevCtl);
+}
+int scc_device_memory(int device, uint64_t* free_bytes, uint64_t* total_bytes) {
+  if (!free_bytes || !total_bytes) return SCC_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return SCC_ERR_HIP;
+  size_t f = 0, t = 0;
+  if (hipMemGetInfo(&f, &t) != hipSuccess) return SCC_ERR_HIP;
+  *free_bytes = f;
+  *total_bytes = t;
+  return SCC_OK;
+}
+int scc_ctx_set_budget(scc_ctx* c, uint64_t hbm_budget_bytes) {
+  if (!c) return SCC_ERR_ARG;
+  if (hbm_budget_bytes && c->bytes > hbm_budget_bytes)
+    return set_error(c, SCC_ERR_BUDGET, "context already holds " + std::to_string(c->bytes) +
+                                            " B, more than the new budget");
+  c->budget = hbm_budget_bytes;
+  return SCC_OK;
 }
 void scc_ctx_cancel(scc_ctx* c) {
   if (c) c->cancel.store(1);
@@ -588,6 +609,7 @@ int scc_graph_download(scc_ctx* c, int32_t* begin, int32_t* node_idx, int32_t* r
   if (!c) return SCC_ERR_ARG;
   if (!c->begin) return set_error(c, SCC_ERR_STATE, "no graph");
   (void)hipSetDevice(c->device);
+  if (c->end) return download_gapped(c, begin, node_idx, r_begin, r_node_idx);
   int rc = 0;
   if (begin && (rc = hip_check(c, hipMemcpyAsync(begin, c->begin, 4ull * (c->n + 1), hipMemcpyDeviceToHost, c->stream), "download"))) return rc;
   if (r_begin && (rc = hip_check(c, hipMemcpyAsync(r_begin, c->rbegin, 4ull * (c->n + 1), hipMemcpyDeviceToHost, c->stream), "download"))) return rc;

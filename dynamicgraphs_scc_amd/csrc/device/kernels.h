@@ -9,8 +9,9 @@
 
 namespace sccmi {
 
+// Row v of the forward CSR is adj[begin[v] .. end[v]); the reverse CSR alike.
 struct GraphPtrs {
-  const int32_t *begin, *adj, *rbegin, *radj;
+  const int32_t *begin, *end, *adj, *rbegin, *rend, *radj;
   int32_t n;
   int32_t maxdeg;
 };
@@ -44,13 +45,15 @@ void launch_pivot_key(hipStream_t st, unsigned grid, const DevCtl* dc, GraphPtrs
 void launch_force_pivot(hipStream_t st, int32_t p, int32_t n, const int32_t* color,
                         unsigned long long* pkey);
 void launch_seed_pivots(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, const int32_t* off,
+                        const int32_t* offe,
                         const int32_t* color, const unsigned long long* pkey, uint32_t* V,
                         int32_t* q0, int32_t* h0, const int32_t* L0, const int32_t* L1);
-void launch_seed_list(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* off,
+void launch_seed_list(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* off, const int32_t* offe,
                       const int32_t* color, const int32_t* seeds, const unsigned long long* ns,
                       uint32_t* V, int32_t* q0);
 void launch_level(hipStream_t st, unsigned grid, DevCtl* dc, int multi, const int32_t* off,
-                  const int32_t* nb, const int32_t* roff, const int32_t* rnb, const int32_t* color,
+                  const int32_t* offe, const int32_t* nb, const int32_t* roff, const int32_t* roffe,
+                  const int32_t* rnb, const int32_t* color,
                   uint32_t* V, int32_t* q0, int32_t* q1, int32_t* c0, int32_t* c1, int32_t* h0,
                   int32_t* h1, const int32_t* L0, const int32_t* L1, int32_t max_chunks);
 void launch_fb_stats(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, const uint32_t* vf,
@@ -62,12 +65,14 @@ void launch_recolor(hipStream_t st, unsigned grid, const DevCtl* dc, int32_t n, 
                     const uint32_t* vf, const uint32_t* vb, const int32_t* rep, const int32_t* L0,
                     const int32_t* L1);
 void launch_wcc_init(hipStream_t st, unsigned grid, const DevCtl* dc, const int32_t* begin,
+                     const int32_t* bend,
                      int32_t* W, const int32_t* L0, const int32_t* L1);
 void launch_wcc_step(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* begin,
+                     const int32_t* bend,
                      const int32_t* adj, const int32_t* color, int32_t* W, const int32_t* L0,
                      const int32_t* L1);
 void launch_wcc_color(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, const int32_t* W,
-                      int seed, const int32_t* off, uint32_t* V, int32_t* q0, const int32_t* L0,
+                      int seed, const int32_t* off, const int32_t* offe, uint32_t* V, int32_t* q0, const int32_t* L0,
                       const int32_t* L1);
 void launch_extract(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, int32_t* label,
                     const uint32_t* V, int32_t* L0, int32_t* L1);

@@ -154,14 +154,14 @@ __global__ __launch_bounds__(BLOCK) void k_trim1_step(DevCtl* __restrict__ dc, G
     int32_t s = 0, e = 0;
     if (act) {
       s = G.begin[v];
-      e = G.begin[v + 1];
+      e = G.end[v];
     }
     const bool out_ok = row_any(act, v, c, s, e, G.adj, live);
     const bool act2 = act && out_ok;
     int32_t rs = 0, re = 0;
     if (act2) {
       rs = G.rbegin[v];
-      re = G.rbegin[v + 1];
+      re = G.rend[v];
     }
     const bool in_ok = row_any(act2, v, c, rs, re, G.radj, live);
     const bool trim = act && !in_ok;
@@ -219,11 +219,11 @@ __global__ __launch_bounds__(BLOCK) void k_compact(DevCtl* __restrict__ dc, cons
 // src/scc_trim2_new.cc:3-39: the unique same-colour neighbour other than v
 // (repeats skipped), or -1 when there are zero or several.  Rows are short for
 // every vertex trim2 can remove, so a lane scans alone and gives up at two.
-__device__ __forceinline__ int32_t unique_nbr(const int32_t* __restrict__ off,
+__device__ __forceinline__ int32_t unique_nbr(const int32_t* __restrict__ off, const int32_t* __restrict__ offe,
                                               const int32_t* __restrict__ nb,
                                               const int32_t* __restrict__ color, int32_t v, int32_t c) {
   int32_t the = -1;
-  const int32_t s = off[v], e = off[v + 1];
+  const int32_t s = off[v], e = offe[v];
   for (int32_t i = s; i < e; i++) {
     const int32_t k = nb[i];
     if (k == v || k == the || color[k] != c) continue;
@@ -246,11 +246,11 @@ __global__ __launch_bounds__(BLOCK) void k_trim2(DevCtl* __restrict__ dc, GraphP
     const int32_t v = list_in[i];
     const int32_t c = color[v];
     if (c == DONE) continue;
-    int32_t k = unique_nbr(G.begin, G.adj, color, v, c);
-    bool pair = k > v && unique_nbr(G.begin, G.adj, color, k, c) == v;
+    int32_t k = unique_nbr(G.begin, G.end, G.adj, color, v, c);
+    bool pair = k > v && unique_nbr(G.begin, G.end, G.adj, color, k, c) == v;
     if (!pair) {
-      k = unique_nbr(G.rbegin, G.radj, color, v, c);
-      pair = k > v && unique_nbr(G.rbegin, G.radj, color, k, c) == v;
+      k = unique_nbr(G.rbegin, G.rend, G.radj, color, v, c);
+      pair = k > v && unique_nbr(G.rbegin, G.rend, G.radj, color, k, c) == v;
     }
     if (pair) {
       label[v] = v;
@@ -308,8 +308,8 @@ __global__ __launch_bounds__(BLOCK) void k_pivot_key(const DevCtl* __restrict__ 
       if (c != DONE) {
         unsigned long long score = 0;
         if (policy == SCC_PIVOT_DEGREE_PRODUCT) {
-          const unsigned long long p = (unsigned long long)(G.begin[v + 1] - G.begin[v]) *
-                                       (unsigned long long)(G.rbegin[v + 1] - G.rbegin[v]);
+          const unsigned long long p = (unsigned long long)(G.end[v] - G.begin[v]) *
+                                       (unsigned long long)(G.rend[v] - G.rbegin[v]);
           score = p > 0x1FFFFFFFFull ? 0x1FFFFFFFFull : p;
         } else if (policy == SCC_PIVOT_LEARNED && scores) {
           score = scores[v];
@@ -366,7 +366,7 @@ __global__ void k_force_pivot(int32_t p, int32_t n, const int32_t* __restrict__ 
 // in V and enters the frontier q0 (dc->lv.nq).  off gives the traversal-
 // direction degree for the first switch decision.
 __global__ __launch_bounds__(BLOCK) void k_seed_pivots(DevCtl* __restrict__ dc, int32_t n,
-                                                       const int32_t* __restrict__ off,
+                                                       const int32_t* __restrict__ off, const int32_t* __restrict__ offe,
                                                        const int32_t* __restrict__ color,
                                                        const unsigned long long* __restrict__ pkey,
                                                        uint32_t* __restrict__ V, int32_t* __restrict__ q0,
@@ -386,13 +386,13 @@ __global__ __launch_bounds__(BLOCK) void k_seed_pivots(DevCtl* __restrict__ dc, 
       piv = c != DONE && pivot_of_key(pkey[slot_of(c, n)]) == v;
       if (piv) {
         atomicOr(&V[v >> 5], 1u << (v & 31));
-        edges += (unsigned long long)(off[v + 1] - off[v]);
+        edges += (unsigned long long)(offe[v] - off[v]);
         cnt++;
         dc->ctr[C_PIVOT] = (unsigned long long)v;  // the pivot, when there is one partition
       }
     }
     // rows of at least HUGE_ROW entries go straight to the chunked expansion
-    const bool hug = piv && (off[v + 1] - off[v]) >= HUGE_ROW;
+    const bool hug = piv && (offe[v] - off[v]) >= HUGE_ROW;
     wave_append(piv && !hug, v, q0, &dc->lv.nq);
     wave_append(hug, v, h0, &dc->lv.nh);
   }
@@ -402,7 +402,7 @@ __global__ __launch_bounds__(BLOCK) void k_seed_pivots(DevCtl* __restrict__ dc, 
 
 // Seed a search from an explicit vertex list (count at *ns): vertices not yet
 // in V and not of colour -2 enter the frontier once.
-__global__ __launch_bounds__(BLOCK) void k_seed_list(DevCtl* __restrict__ dc, const int32_t* __restrict__ off,
+__global__ __launch_bounds__(BLOCK) void k_seed_list(DevCtl* __restrict__ dc, const int32_t* __restrict__ off, const int32_t* __restrict__ offe,
                                                      const int32_t* __restrict__ color,
                                                      const int32_t* __restrict__ seeds,
                                                      const unsigned long long* __restrict__ ns,
@@ -419,7 +419,7 @@ __global__ __launch_bounds__(BLOCK) void k_seed_list(DevCtl* __restrict__ dc, co
         const uint32_t bit = 1u << (v & 31);
         take = !(atomicOr(&V[v >> 5], bit) & bit);
         if (take) {
-          edges += (unsigned long long)(off[v + 1] - off[v]);
+          edges += (unsigned long long)(offe[v] - off[v]);
           cnt++;
         }
       }
@@ -459,8 +459,8 @@ constexpr int TD_CHUNK = 2048;
 // search when a level admits nothing.  A switch to BU drops deferred rows:
 // their neighbours find them as parents.
 __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int multi,
-                                                 const int32_t* __restrict__ off, const int32_t* __restrict__ nb,
-                                                 const int32_t* __restrict__ roff, const int32_t* __restrict__ rnb,
+                                                 const int32_t* __restrict__ off, const int32_t* __restrict__ offe, const int32_t* __restrict__ nb,
+                                                 const int32_t* __restrict__ roff, const int32_t* __restrict__ roffe, const int32_t* __restrict__ rnb,
                                                  const int32_t* __restrict__ color, uint32_t* __restrict__ V,
                                                  int32_t* __restrict__ q0, int32_t* __restrict__ q1,
                                                  int32_t* __restrict__ c0, int32_t* __restrict__ c1,
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
         const int32_t u = h_in[g % nh];
         const int32_t ch = (int32_t)(g / nh);
         const int32_t s = off[u] + ch * TD_CHUNK;
-        const int32_t e = min(s + TD_CHUNK, off[u + 1]);
+        const int32_t e = min(s + TD_CHUNK, offe[u]);
         if (s >= e) continue;  // uniform across the workgroup
         const int32_t c = color[u];
         for (int32_t b = s; b < e; b += BLOCK) {
@@ -513,7 +513,7 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
           if (idx < e) {
             k = nb[idx];
             cl = try_claim(V, color, k, c);
-            if (cl) edges += (unsigned long long)(off[k + 1] - off[k]);
+            if (cl) edges += (unsigned long long)(offe[k] - off[k]);
           }
           QOut::push(cl, k, q_out, &lv->out_q);
         }
@@ -529,7 +529,7 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
       if (valid) {
         u = q_in[i];
         s = off[u];
-        e = off[u + 1];
+        e = offe[u];
         c = color[u];
       }
       const int32_t d = e - s;
@@ -544,7 +544,7 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
         if (j < lim) {
           k = nb[s + j];
           cl = try_claim(V, color, k, c);
-          if (cl) edges += (unsigned long long)(off[k + 1] - off[k]);
+          if (cl) edges += (unsigned long long)(offe[k] - off[k]);
         }
         QOut::push(cl, k, q_out, &lv->out_q);
       }
@@ -560,7 +560,7 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
           if (idx < he) {
             k = nb[idx];
             cl = try_claim(V, color, k, hc);
-            if (cl) edges += (unsigned long long)(off[k + 1] - off[k]);
+            if (cl) edges += (unsigned long long)(offe[k] - off[k]);
           }
           QOut::push(cl, k, q_out, &lv->out_q);
         }
@@ -590,14 +590,14 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
           act = c != DONE;
           if (act) {
             s = roff[v];
-            e = roff[v + 1];
+            e = roffe[v];
           }
         }
       }
       const bool found = row_any(act, v, c, s, e, rnb, reached);
       if (found) {
         atomicOr(&V[v >> 5], 1u << (v & 31));
-        edges += (unsigned long long)(off[v + 1] - off[v]);
+        edges += (unsigned long long)(offe[v] - off[v]);
       }
       QOut::push(found, v, q_out, &lv->out_q);
       COut::push(act && !found, v, c_out, &lv->out_c);
@@ -657,11 +657,11 @@ __global__ __launch_bounds__(BLOCK) void k_fb_stats(DevCtl* __restrict__ dc, Gra
     const bool inf = bit_test(vf, v), inb = bit_test(vb, v);
     if (inf) {
       f++;
-      ef += (unsigned long long)(G.begin[v + 1] - G.begin[v]);
+      ef += (unsigned long long)(G.end[v] - G.begin[v]);
     }
     if (inb) {
       b++;
-      eb += (unsigned long long)(G.rbegin[v + 1] - G.rbegin[v]);
+      eb += (unsigned long long)(G.rend[v] - G.rbegin[v]);
     }
     fb += (inf && inb) ? 1 : 0;
   }
@@ -747,13 +747,14 @@ __global__ __launch_bounds__(BLOCK) void k_recolor(const DevCtl* __restrict__ dc
 // src/scc_weak.cc:128-150: W[v] = v, tagged when out-degree >= 50.
 __global__ __launch_bounds__(BLOCK) void k_wcc_init(const DevCtl* __restrict__ dc,
                                                     const int32_t* __restrict__ begin,
+                                                    const int32_t* __restrict__ bend,
                                                     int32_t* __restrict__ W, const int32_t* __restrict__ L0,
                                                     const int32_t* __restrict__ L1) {
   const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
   const int64_t n_in = (int64_t)dc->live.n;
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_in; i += (int64_t)gridDim.x * BLOCK) {
     const int32_t v = list[i];
-    W[v] = (begin[v + 1] - begin[v] >= 50) ? (v | WCC_TAG) : v;
+    W[v] = (bend[v] - begin[v] >= 50) ? (v | WCC_TAG) : v;
   }
 }
 
@@ -765,6 +766,7 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_init(const DevCtl* __restrict__ d
 // launch is one iteration; the last workgroup ends the loop after an iteration
 // that changed nothing.
 __global__ __launch_bounds__(BLOCK) void k_wcc_step(DevCtl* __restrict__ dc, const int32_t* __restrict__ begin,
+                                                    const int32_t* __restrict__ bend,
                                                     const int32_t* __restrict__ adj,
                                                     const int32_t* __restrict__ color,
                                                     int32_t* __restrict__ W, const int32_t* __restrict__ L0,
@@ -785,7 +787,7 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_step(DevCtl* __restrict__ dc, con
       v = list[i];
       c = color[v];
       s = begin[v];
-      e = begin[v + 1];
+      e = bend[v];
       w0 = m = W[v];
     }
     const bool act = valid && c != DONE;
@@ -840,7 +842,7 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_step(DevCtl* __restrict__ dc, con
 // roots also seed a forward search (V, q0, dc->lv) for root-SCC extraction.
 __global__ __launch_bounds__(BLOCK) void k_wcc_color(DevCtl* __restrict__ dc, int32_t* __restrict__ color,
                                                      const int32_t* __restrict__ W, int seed,
-                                                     const int32_t* __restrict__ off,
+                                                     const int32_t* __restrict__ off, const int32_t* __restrict__ offe,
                                                      uint32_t* __restrict__ V, int32_t* __restrict__ q0,
                                                      const int32_t* __restrict__ L0,
                                                      const int32_t* __restrict__ L1) {
@@ -864,7 +866,7 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_color(DevCtl* __restrict__ dc, in
           roots++;
           if (seed) {
             atomicOr(&V[v >> 5], 1u << (v & 31));
-            edges += (unsigned long long)(off[v + 1] - off[v]);
+            edges += (unsigned long long)(offe[v] - off[v]);
           }
         }
       }
@@ -1011,20 +1013,22 @@ void launch_force_pivot(hipStream_t st, int32_t p, int32_t n, const int32_t* col
   LAUNCH(k_force_pivot, 1, st, p, n, color, pkey);
 }
 void launch_seed_pivots(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, const int32_t* off,
+                        const int32_t* offe,
                         const int32_t* color, const unsigned long long* pkey, uint32_t* V,
                         int32_t* q0, int32_t* h0, const int32_t* L0, const int32_t* L1) {
-  LAUNCH(k_seed_pivots, grid, st, dc, n, off, color, pkey, V, q0, h0, L0, L1);
+  LAUNCH(k_seed_pivots, grid, st, dc, n, off, offe, color, pkey, V, q0, h0, L0, L1);
 }
-void launch_seed_list(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* off,
+void launch_seed_list(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* off, const int32_t* offe,
                       const int32_t* color, const int32_t* seeds, const unsigned long long* ns,
                       uint32_t* V, int32_t* q0) {
-  LAUNCH(k_seed_list, grid, st, dc, off, color, seeds, ns, V, q0);
+  LAUNCH(k_seed_list, grid, st, dc, off, offe, color, seeds, ns, V, q0);
 }
 void launch_level(hipStream_t st, unsigned grid, DevCtl* dc, int multi, const int32_t* off,
-                  const int32_t* nb, const int32_t* roff, const int32_t* rnb, const int32_t* color,
+                  const int32_t* offe, const int32_t* nb, const int32_t* roff, const int32_t* roffe,
+                  const int32_t* rnb, const int32_t* color,
                   uint32_t* V, int32_t* q0, int32_t* q1, int32_t* c0, int32_t* c1, int32_t* h0,
                   int32_t* h1, const int32_t* L0, const int32_t* L1, int32_t max_chunks) {
-  LAUNCH(k_level, grid, st, dc, multi, off, nb, roff, rnb, color, V, q0, q1, c0, c1, h0, h1, L0, L1,
+  LAUNCH(k_level, grid, st, dc, multi, off, offe, nb, roff, roffe, rnb, color, V, q0, q1, c0, c1, h0, h1, L0, L1,
          max_chunks);
 }
 void launch_fb_stats(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, const uint32_t* vf,
@@ -1042,18 +1046,20 @@ void launch_recolor(hipStream_t st, unsigned grid, const DevCtl* dc, int32_t n, 
   LAUNCH(k_recolor, grid, st, dc, n, color, vf, vb, rep, L0, L1);
 }
 void launch_wcc_init(hipStream_t st, unsigned grid, const DevCtl* dc, const int32_t* begin,
+                     const int32_t* bend,
                      int32_t* W, const int32_t* L0, const int32_t* L1) {
-  LAUNCH(k_wcc_init, grid, st, dc, begin, W, L0, L1);
+  LAUNCH(k_wcc_init, grid, st, dc, begin, bend, W, L0, L1);
 }
 void launch_wcc_step(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* begin,
+                     const int32_t* bend,
                      const int32_t* adj, const int32_t* color, int32_t* W, const int32_t* L0,
                      const int32_t* L1) {
-  LAUNCH(k_wcc_step, grid, st, dc, begin, adj, color, W, L0, L1);
+  LAUNCH(k_wcc_step, grid, st, dc, begin, bend, adj, color, W, L0, L1);
 }
 void launch_wcc_color(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, const int32_t* W,
-                      int seed, const int32_t* off, uint32_t* V, int32_t* q0, const int32_t* L0,
+                      int seed, const int32_t* off, const int32_t* offe, uint32_t* V, int32_t* q0, const int32_t* L0,
                       const int32_t* L1) {
-  LAUNCH(k_wcc_color, grid, st, dc, color, W, seed, off, V, q0, L0, L1);
+  LAUNCH(k_wcc_color, grid, st, dc, color, W, seed, off, offe, V, q0, L0, L1);
 }
 void launch_extract(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, int32_t* label,
                     const uint32_t* V, int32_t* L0, int32_t* L1) {

@@ -707,6 +707,16 @@ sccgen_updates* sccgen_updates_create(const sccgen_graph* base, uint64_t seed, i
 
 int64_t sccgen_updates_peek(const sccgen_updates* u) { return u ? u->next_size() : -1; }
 
+int sccgen_updates_set_sizes(sccgen_updates* u, int64_t size_lo, int64_t size_hi) {
+  if (!u || size_lo < 1 || size_hi < size_lo) {
+    set_err("bad batch sizes");
+    return -1;
+  }
+  u->lo = size_lo;
+  u->hi = size_hi;
+  return 0;
+}
+
 int sccgen_updates_next(sccgen_updates* u, int64_t cap, int32_t* ins_src, int32_t* ins_dst,
                         int64_t* n_ins, int32_t* del_src, int32_t* del_dst, int64_t* n_del) {
   if (!u) return -1;
@@ -792,19 +802,29 @@ int64_t sccgen_updates_num_edges(const sccgen_updates* u) { return u ? u->live :
 
 sccgen_graph* sccgen_updates_graph(const sccgen_updates* u) {
   if (!u) return nullptr;
-  std::vector<int32_t> s, d;
-  s.reserve(u->live);
-  d.reserve(u->live);
-  for (int64_t i = 0; i < u->m_base; i++)
-    if (!u->base_dead[i]) {
-      s.push_back(u->base_src[i]);
-      d.push_back(u->adj[i]);
+  // live edges, compacted in parallel chunks (order kept)
+  const int64_t mb = u->m_base, mi = (int64_t)u->ins_s.size(), tot = mb + mi;
+  const int64_t chunk = 1 << 20, nch = (tot + chunk - 1) / chunk;
+  std::vector<int64_t> cnt(nch + 1, 0);
+  auto dead = [&](int64_t i) { return i < mb ? u->base_dead[i] != 0 : u->ins_dead[i - mb] != 0; };
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t c = 0; c < nch; c++) {
+    int64_t k = 0;
+    for (int64_t i = c * chunk; i < std::min(tot, (c + 1) * chunk); i++) k += !dead(i);
+    cnt[c + 1] = k;
+  }
+  for (int64_t c = 0; c < nch; c++) cnt[c + 1] += cnt[c];
+  std::vector<int32_t> s(cnt[nch]), d(cnt[nch]);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t c = 0; c < nch; c++) {
+    int64_t o = cnt[c];
+    for (int64_t i = c * chunk; i < std::min(tot, (c + 1) * chunk); i++) {
+      if (dead(i)) continue;
+      s[o] = i < mb ? u->base_src[i] : u->ins_s[i - mb];
+      d[o] = i < mb ? u->adj[i] : u->ins_d[i - mb];
+      o++;
     }
-  for (size_t j = 0; j < u->ins_s.size(); j++)
-    if (!u->ins_dead[j]) {
-      s.push_back(u->ins_s[j]);
-      d.push_back(u->ins_d[j]);
-    }
+  }
   return build_graph(u->n, (int64_t)s.size(), s.data(), d.data(), true);
 }
 

@@ -46,7 +46,8 @@ EXPORTS = [
     "scc_partitions_resolve",
     "scc_run_opts_default", "scc_run_method2", "scc_labels_download", "scc_colors_download",
     "scc_count", "scc_labels_device", "scc_update", "scc_ctx_set_kernel_timing",
-    "scc_ctx_synchronize",
+    "scc_ctx_synchronize", "scc_ctx_set_budget", "scc_update_bytes_needed",
+    "scc_device_memory",
 ]
 
 
@@ -129,6 +130,9 @@ def load_library():
         "scc_update": (C.c_int, [vp, vp, vp, i64, vp, vp, i64, C.POINTER(UpdateStats)]),
         "scc_ctx_set_kernel_timing": (C.c_int, [vp, C.c_int]),
         "scc_ctx_synchronize": (C.c_int, [vp]),
+        "scc_ctx_set_budget": (C.c_int, [vp, C.c_uint64]),
+        "scc_update_bytes_needed": (C.c_uint64, [i32, i64, i64, i64]),
+        "scc_device_memory": (C.c_int, [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -65,6 +65,8 @@ def _load():
     lib.sccgen_free.argtypes = [vp]
     lib.sccgen_updates_create.restype = vp
     lib.sccgen_updates_create.argtypes = [vp, C.c_uint64, C.c_int64, C.c_int64]
+    lib.sccgen_updates_set_sizes.restype = C.c_int
+    lib.sccgen_updates_set_sizes.argtypes = [vp, C.c_int64, C.c_int64]
     lib.sccgen_updates_peek.restype = C.c_int64
     lib.sccgen_updates_peek.argtypes = [vp]
     lib.sccgen_updates_next.restype = C.c_int
@@ -256,7 +258,14 @@ class UpdateStream:
                                            C.byref(nd))
         if rc != 0:
             raise RuntimeError(_err(self._lib))
-        return isrc[:ni.value], idst[:ni.value], dsrc[:nd.value], ddst[:nd.value]
+        return (isrc[:ni.value].copy(), idst[:ni.value].copy(), dsrc[:nd.value].copy(),
+                ddst[:nd.value].copy())
+
+    def set_sizes(self, lo: int, hi: int) -> None:
+        """Batch sizes of the following batches: log-uniform in [lo, hi]."""
+        if self._lib.sccgen_updates_set_sizes(self._h, int(lo), int(hi)) != 0:
+            raise RuntimeError(_err(self._lib))
+        self.hi = int(hi)
 
     def num_edges(self) -> int:
         return int(self._lib.sccgen_updates_num_edges(self._h))

@@ -111,6 +111,8 @@ sccgen_updates* sccgen_updates_create(const sccgen_graph* base, uint64_t seed,
                                       int64_t size_lo, int64_t size_hi);
 /* Size of the next batch (inserts + deletes) without drawing it. */
 int64_t sccgen_updates_peek(const sccgen_updates* u);
+/* Sizes of the following batches: log-uniform in [size_lo, size_hi].  0 on success. */
+int sccgen_updates_set_sizes(sccgen_updates* u, int64_t size_lo, int64_t size_hi);
 /* Draw the next batch: fills up to cap entries of each list; returns 0 and the
  * counts, or <0 if cap is too small. */
 int sccgen_updates_next(sccgen_updates* u, int64_t cap, int32_t* ins_src,

@@ -62,6 +62,11 @@ const char* scc_last_error(const scc_ctx* ctx);
 uint64_t scc_ctx_bytes(const scc_ctx* ctx);
 /* Device bytes a graph of n vertices and m edges needs in a context. */
 uint64_t scc_bytes_needed(int32_t n, int64_t m);
+/* Free and total device memory of a HIP device (hipMemGetInfo). */
+int scc_device_memory(int device, uint64_t* free_bytes, uint64_t* total_bytes);
+/* Change the context's HBM budget (0 = none).  Fails with SCC_ERR_BUDGET when
+ * the context already holds more than the new budget. */
+int scc_ctx_set_budget(scc_ctx* ctx, uint64_t hbm_budget_bytes);
 /* Ask a running call on this context to stop at its next phase boundary
  * (thread-safe).  The call returns SCC_ERR_CANCELLED. */
 void scc_ctx_cancel(scc_ctx* ctx);
@@ -200,6 +205,13 @@ typedef struct scc_update_stats {
 int scc_update(scc_ctx* ctx, const int32_t* ins_src, const int32_t* ins_dst, int64_t n_ins,
                const int32_t* del_src, const int32_t* del_dst, int64_t n_del,
                scc_update_stats* stats);
+
+/* Upper bound of the device bytes scc_update adds to a context that holds a
+ * graph of n vertices and m edges (scc_bytes_needed(n, m)), while the graph
+ * grows to at most m_max edges through batches of at most max_batch edges per
+ * direction: spare adjacency arrays, per-row counters, batch buffers.  (No
+ * reference counterpart: the reference rebuilds graphs on the host.) */
+uint64_t scc_update_bytes_needed(int32_t n, int64_t m, int64_t m_max, int64_t max_batch);
 
 /* ---- timing helpers (HIP events on the context's stream) ------------------ */
 

@@ -19,7 +19,8 @@ def declared(header):
     return sorted(set(n for n in names if n.startswith(("scc_", "sccgen_"))))
 
 
-@pytest.mark.parametrize("header,lib", [("sccmi.h", "libsccmi.so"), ("sccgen.h", "libsccgen.so")])
+@pytest.mark.parametrize("header,lib", [("sccmi.h", "libsccmi.so"), ("sccgen.h", "libsccgen.so"),
+                                        ("sccserve.h", "libsccserve.so")])
 def test_library_exports_every_declared_symbol(header, lib):
     names = declared(header)
     assert len(names) > 10
@@ -29,8 +30,9 @@ def test_library_exports_every_declared_symbol(header, lib):
 
 
 def test_python_binding_covers_the_header():
-    from dynamicgraphs_scc_amd import engine
+    from dynamicgraphs_scc_amd import engine, serve
     assert sorted(engine.EXPORTS) == declared("sccmi.h")
+    assert sorted(serve.EXPORTS) == declared("sccserve.h")
 
 
 def test_struct_layouts_match_header():
@@ -39,10 +41,13 @@ def test_struct_layouts_match_header():
     from dynamicgraphs_scc_amd import engine
     import subprocess
     import tempfile
+    from dynamicgraphs_scc_amd import serve
     src = r"""
 #include <stdio.h>
 #include "sccmi.h"
-int main(){printf("%zu %zu %zu\n", sizeof(scc_run_opts), sizeof(scc_run_stats), sizeof(scc_update_stats));}
+#include "sccserve.h"
+int main(){printf("%zu %zu %zu %zu %zu %zu\n", sizeof(scc_run_opts), sizeof(scc_run_stats),
+  sizeof(scc_update_stats), sizeof(scc_server_config), sizeof(scc_request_info), sizeof(scc_server_stats));}
 """
     with tempfile.TemporaryDirectory() as td:
         c = os.path.join(td, "s.c")
@@ -52,7 +57,8 @@ int main(){printf("%zu %zu %zu\n", sizeof(scc_run_opts), sizeof(scc_run_stats), 
         subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
         out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
     assert [int(x) for x in out] == [ctypes.sizeof(engine.RunOpts), ctypes.sizeof(engine.RunStats),
-                                     ctypes.sizeof(engine.UpdateStats)]
+                                     ctypes.sizeof(engine.UpdateStats), ctypes.sizeof(serve.ServerConfig),
+                                     ctypes.sizeof(serve.RequestInfo), ctypes.sizeof(serve.ServerStats)]
 
 
 def test_no_cpu_fallback_in_product_path():

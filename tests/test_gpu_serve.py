@@ -1,0 +1,166 @@
+"""GPU behaviour of the request server: every result in the mix equals the
+same request run alone (exact, integer labels), requests on different graphs
+overlap, admission follows the HBM budget, a faulted graph does not disturb
+the others, and latency statistics are reported."""
+import numpy as np
+import pytest
+
+import oracle
+from dynamicgraphs_scc_amd import graphs
+from dynamicgraphs_scc_amd.engine import SccEngine, SccError
+from dynamicgraphs_scc_amd.serve import SccServer
+
+pytestmark = pytest.mark.gpu
+
+
+def _workload(k, batches):
+    """k graphs of different kinds and sizes, each with its update batches."""
+    out = []
+    for i in range(k):
+        if i % 2 == 0:
+            g = graphs.rmat(11 + (i % 3), 8, seed=100 + i)
+        else:
+            g = graphs.planted(8000 + 2000 * i, seed=200 + i, n_medium=300)
+        us = graphs.UpdateStream(g, seed=300 + i, lo=50, hi=4000)
+        out.append((g, [us.next() for _ in range(batches)], us.graph()))
+        us.close()
+    return out
+
+
+def _alone(g, batches):
+    with SccEngine() as e:
+        e.upload(g)
+        e.run_method2()
+        res = [e.labels()]
+        for a, b, c, d in batches:
+            e.update(a, b, c, d)
+            res.append(e.labels())
+    return res
+
+
+def test_mix_equals_alone_and_overlaps():
+    work = _workload(6, 5)
+    alone = [_alone(g, bs) for g, bs, _ in work]
+    print("  alone done", flush=True)
+    with SccServer(slots=4) as srv:
+        gids, tickets = [], []
+        for g, _, _ in work:
+            gid, t = srv.load(g)
+            gids.append(gid)
+            tickets.append(srv.labels(gid, g.num_nodes))
+        per = [[t] for t in tickets]
+        # round-robin interleaving: every graph has work queued at once
+        for j in range(5):
+            for i, (g, bs, _) in enumerate(work):
+                a, b, c, d = bs[j]
+                srv.update(gids[i], a, b, c, d)
+                per[i].append(srv.labels(gids[i], g.num_nodes))
+        assert srv.drain(120_000)
+        for i in range(len(work)):
+            for j, (t, buf) in enumerate(per[i]):
+                assert srv.result(t)["status"] == 0
+                assert np.array_equal(buf, alone[i][j]), f"graph {i} step {j}"
+        st = srv.stats()
+        print("  stats", st, flush=True)
+        assert st["failed"] == 0 and st["updates_done"] == 30
+        assert st["max_concurrency"] >= 2
+        assert 0 < st["update_p50_ms"] <= st["update_p99_ms"] <= st["update_max_ms"]
+    # and the final labels are the recompute's
+    for i, (g, bs, final) in enumerate(work):
+        assert np.array_equal(alone[i][-1], oracle.scc_canonical(final, 8))
+
+
+def test_admission_waits_for_memory():
+    a = graphs.rmat(13, 8, seed=1)
+    b = graphs.rmat(13, 8, seed=2)
+    with SccServer(slots=2) as probe:
+        ga, ta = probe.load(a)
+        probe.result(ta)
+        one = probe.stats()["reserved_bytes"]
+    assert one > 0
+    # room for one graph only
+    with SccServer(slots=2, hbm_budget_bytes=int(one * 1.5)) as srv:
+        ga, ta = srv.load(a)
+        gb, tb = srv.load(b)
+        ra = srv.result(ta)
+        info = srv.wait(tb, 300)
+        assert info["status"] == 1, "the second load must wait for memory"
+        tu = srv.unload(ga)
+        ru = srv.result(tu)
+        rb = srv.result(tb)
+        assert rb["start_ms"] >= ru["end_ms"]
+        assert rb["num_sccs"] == len(np.unique(oracle.scc_canonical(b, 8)))
+        assert ra["num_sccs"] == len(np.unique(oracle.scc_canonical(a, 8)))
+        # a graph that can never fit is refused at submit
+        with pytest.raises(SccError) as ei:
+            srv.load(graphs.rmat(16, 16, seed=3))
+        assert ei.value.code == -4
+
+
+def test_fault_in_one_graph_leaves_others_running():
+    work = _workload(2, 3)
+    with SccServer(slots=2) as srv:
+        gids = [srv.load(g)[0] for g, _, _ in work]
+        assert srv.drain(60_000)
+        srv.inject_fault(gids[0], 2)
+        t_bad = srv.update(gids[0], *work[0][1][0])
+        t_ok = [srv.update(gids[1], *bs) for bs in work[1][1]]
+        assert srv.wait(t_bad)["status"] == -6
+        for t in t_ok:
+            assert srv.result(t)["status"] == 0
+        t2 = srv.update(gids[0], *work[0][1][1])
+        assert srv.wait(t2)["status"] == -5
+        tl, buf = srv.labels(gids[1], work[1][0].num_nodes)
+        srv.result(tl)
+        assert np.array_equal(buf, oracle.scc_canonical(work[1][2], 8))
+        srv.result(srv.unload(gids[0]))
+        with pytest.raises(SccError):
+            srv.update(gids[0], [0], [1])
+
+
+def test_cancel_queued_and_growth_beyond_headroom():
+    g = graphs.rmat(12, 4, seed=7)
+    with SccServer(slots=1, growth=0.0) as srv:
+        gid, t = srv.load(g)
+        rng = np.random.default_rng(0)
+        ins = rng.integers(0, g.num_nodes, (3000, 2)).astype(np.int32)
+        ins = ins[ins[:, 0] != ins[:, 1]]
+        t1 = srv.update(gid, ins[:, 0], ins[:, 1])
+        t2 = srv.update(gid, [0], [1])
+        srv.cancel(t2)
+        assert srv.wait(t2)["status"] == -7
+        r1 = srv.result(t1)
+        assert r1["inserted"] > 0
+        tl, buf = srv.labels(gid, g.num_nodes)
+        srv.result(tl)
+        src, dst = g.edges()
+        want = graphs.from_edges(g.num_nodes, np.concatenate([src, ins[:, 0]]),
+                                 np.concatenate([dst, ins[:, 1]]), dedup=True)
+        assert np.array_equal(buf, oracle.scc_canonical(want, 8))
+
+
+def test_poisson_stream_statistics():
+    g = graphs.rmat(14, 8, seed=11)
+    us = graphs.UpdateStream(g, seed=12, lo=100, hi=5000)
+    batches = [us.next() for _ in range(40)]
+    final = us.graph()
+    us.close()
+    rng = np.random.default_rng(13)
+    arrivals = np.cumsum(rng.exponential(2.0, len(batches)))  # mean 2 ms apart
+    with SccServer(slots=2) as srv:
+        gid, t = srv.load(g)
+        srv.result(t)
+        srv.reset_stats()
+        t0 = srv.now_ms()
+        for (a, b, c, d), at in zip(batches, arrivals):
+            while srv.now_ms() < t0 + at:
+                pass
+            srv.update(gid, a, b, c, d, arrival_ms=t0 + at)
+        assert srv.drain(120_000)
+        st = srv.stats()
+        print("  stats", st, flush=True)
+        assert st["updates_done"] == 40 and st["failed"] == 0
+        assert st["update_p50_ms"] <= st["update_p99_ms"] and st["updates_per_s"] > 0
+        tl, buf = srv.labels(gid, g.num_nodes)
+        srv.result(tl)
+    assert np.array_equal(buf, oracle.scc_canonical(final, 8))
