@@ -48,6 +48,10 @@ enum Ctr : int {
 // so the host queues launches in batches and synchronises only to learn that a
 // phase has finished.  Launches after `done` exit at once.
 
+// The live list a kernel reads (LIVE_IN) and the one it compacts into.
+#define LIVE_IN(dc, L0, L1) ((dc)->live.sel ? (L1) : (L0))
+#define LIVE_OUT(dc, L0, L1) ((dc)->live.sel ? (L0) : (L1))
+
 struct LiveCtl {  // the current vertex list: live[sel][0, n)
   int32_t sel;
   int32_t pad;
@@ -407,6 +411,11 @@ struct scc_ctx {
   uint32_t* sched_scores = nullptr;
   void* sched = nullptr;
   bool multi_colors = false;  // caller-set colours: the global step is a partition round
+  bool classes = false;       // live colours are WCC classes or finer (after WCC / a round)
+
+  // phase trace of the current run (scc_phase_trace)
+  std::vector<scc_phase_rec> trace;
+  int32_t round_idx = -1;
 
   std::vector<void*> allocs;
   std::vector<uint64_t> alloc_sizes;

@@ -751,7 +751,6 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
 
 }  // namespace
 
-void destroy_sched(scc_ctx* c) { (void)c; }
 
 void destroy_dynamic(scc_ctx* c) {
   Dyn* d = dyn_of(c);

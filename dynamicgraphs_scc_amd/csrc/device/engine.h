@@ -40,6 +40,10 @@ int do_canon(scc_ctx* c, int64_t* count);
 void destroy_dynamic(scc_ctx* c);
 void destroy_sched(scc_ctx* c);
 void dynamic_graph_replaced(scc_ctx* c);
+// learned scheduler (sched.hip)
+bool sched_ready(const scc_ctx* c);
+int sched_choose(scc_ctx* c, int* action);  // 0: forward-backward step, 1: colouring step
+int sched_pivots(scc_ctx* c);               // per-vertex pivot scores into c->sched_scores
 int download_gapped(scc_ctx* c, int32_t* begin, int32_t* node_idx, int32_t* r_begin, int32_t* r_node_idx);
 
 }  // namespace sccmi

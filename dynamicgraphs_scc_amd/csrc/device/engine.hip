@@ -100,6 +100,27 @@ int pull_ctl(scc_ctx* c) {
 
 int sync_ctr(scc_ctx* c) { return pull_ctl(c); }
 
+// One phase-trace record: construct at phase entry, finish() after the
+// phase's final synchronisation.
+struct PhaseRec {
+  scc_ctx* c;
+  int kind;
+  int64_t before;
+  std::chrono::steady_clock::time_point t0;
+  PhaseRec(scc_ctx* cc, int k) : c(cc), kind(k), before(cc->n_live), t0(std::chrono::steady_clock::now()) {}
+  void finish(int64_t aux) {
+    if (c->trace.size() >= 65536) return;
+    scc_phase_rec r;
+    r.kind = kind;
+    r.round = c->round_idx;
+    r.live_before = before;
+    r.live_after = c->n_live;
+    r.aux = aux;
+    r.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    c->trace.push_back(r);
+  }
+};
+
 int zero_ctr(scc_ctx* c) {
   launch_ctl_init(c->stream, c->dc, 1, 0, 0, 0, 0.0);
   return hip_check(c, hipGetLastError(), "counter reset");
@@ -194,6 +215,9 @@ int reset_state(scc_ctx* c) {
   c->labels_ready = false;
   c->multi_colors = false;
   c->colors_split = false;
+  c->classes = false;
+  c->trace.clear();
+  c->round_idx = -1;
   c->fresh = true;
   int rc = zero_ctr(c);
   if (rc) return rc;
@@ -220,6 +244,7 @@ int force_pivot(scc_ctx* c, int32_t pivot) {
 int do_trim1(scc_ctx* c, int64_t stop, int64_t* trimmed) {
   if (trimmed) *trimmed = 0;
   if (c->n_live == 0) return SCC_OK;
+  PhaseRec pr(c, SCC_PHASE_TRIM1);
   launch_ctl_init(c->stream, c->dc, 4, 0, 0, (long long)stop, 0.0);
   const unsigned grid = grid_for(c->n_live);
   int batch = 3, rc;
@@ -236,6 +261,7 @@ int do_trim1(scc_ctx* c, int64_t stop, int64_t* trimmed) {
     if (guard > (int64_t)c->n + 8) return set_error(c, SCC_ERR_LIMIT, "trim did not converge");
   }
   if (trimmed) *trimmed = (int64_t)c->h_dc->loop.total0;
+  pr.finish(c->h_dc->loop.iters);
   return SCC_OK;
 }
 
@@ -243,11 +269,13 @@ int do_trim2(scc_ctx* c, int64_t* removed) {
   c->fresh = false;
   if (removed) *removed = 0;
   if (c->n_live == 0) return SCC_OK;
+  PhaseRec pr(c, SCC_PHASE_TRIM2);
   int rc;
   if ((rc = zero_ctr(c))) return rc;
   launch_trim2(c->stream, grid_for(c->n_live), c->dc, gp(c), c->color, c->label, c->live[0], c->live[1]);
   if ((rc = pull_ctl(c))) return rc;
   if (removed) *removed = (int64_t)c->h_dc->ctr[C_TRIMMED];
+  pr.finish((int64_t)c->h_dc->ctr[C_TRIMMED]);
   return SCC_OK;
 }
 
@@ -316,8 +344,13 @@ int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int6
   int rc;
   if (scc) *scc = 0;
   if (c->n_live == 0) return SCC_OK;
+  PhaseRec pr(c, multi ? SCC_PHASE_FB_ROUND : SCC_PHASE_FB_GLOBAL);
+  int64_t lv = 0;
+  if (!levels) levels = &lv;
+  const int64_t lv0 = *levels;
   const unsigned grid = grid_for(c->n_live);
   if ((rc = zero_ctr(c))) return rc;
+  if (pivot_override < 0 && policy == SCC_PIVOT_LEARNED && sched_ready(c) && (rc = sched_pivots(c))) return rc;
   launch_reset_slots(c->stream, grid, c->dc, c->n, c->color, c->pkey, c->rep, c->live[0], c->live[1]);
   if (pivot_override >= 0) {
     if ((rc = force_pivot(c, pivot_override))) return rc;
@@ -346,6 +379,8 @@ int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int6
     sets[4] = (int64_t)h[C_FB];
   }
   c->colors_split = true;
+  if (multi) c->classes = true;
+  pr.finish(*levels - lv0);
   return SCC_OK;
 }
 
@@ -382,10 +417,13 @@ static int wcc_fixpoint(scc_ctx* c, bool seed) {
 int do_wcc(scc_ctx* c, int64_t* parts) {
   if (parts) *parts = 0;
   if (c->n_live == 0) return SCC_OK;
+  PhaseRec pr(c, SCC_PHASE_WCC);
   int rc;
   if ((rc = wcc_fixpoint(c, false))) return rc;
   if ((rc = pull_ctl(c))) return rc;
   if (parts) *parts = (int64_t)c->h_dc->ctr[C_ROOTS];
+  c->classes = true;
+  pr.finish(c->h_dc->loop.iters);
   return SCC_OK;
 }
 
@@ -394,25 +432,42 @@ int do_wcc(scc_ctx* c, int64_t* parts) {
 int do_color_round(scc_ctx* c, int64_t* scc) {
   if (scc) *scc = 0;
   if (c->n_live == 0) return SCC_OK;
+  PhaseRec pr(c, SCC_PHASE_COLOR_ROUND);
   int rc;
   if ((rc = wcc_fixpoint(c, true))) return rc;
+  const int64_t iters = c->h_dc->loop.iters;
   if ((rc = run_levels(c, true, true, nullptr, false))) return rc;
   launch_extract(c->stream, grid_for(c->n_live), c->dc, c->color, c->label, c->vf, c->live[0], c->live[1]);
   if ((rc = pull_ctl(c))) return rc;
   if (scc) *scc = (int64_t)c->h_dc->ctr[C_SCC];
+  c->classes = true;
+  pr.finish(iters);
   return SCC_OK;
 }
 
 int do_fb_parts(scc_ctx* c, int policy, int mode, bool trim_between, int64_t* rounds) {
   int64_t r = 0;
   int rc;
+  if (mode == SCC_PARTS_LEARNED && !sched_ready(c)) mode = SCC_PARTS_BOTH;
   while (c->n_live > 0) {
     if ((rc = check_cancel(c))) return rc;
+    c->round_idx = (int32_t)r;
     if (trim_between && r > 0) {
       if ((rc = do_trim1(c, 0, nullptr))) return rc;
       if (c->n_live == 0) break;
     }
-    if (mode == SCC_PARTS_COLOR) {
+    if (mode == SCC_PARTS_LEARNED) {
+      // one step per round, chosen by the scheduler
+      int action = 0;
+      if ((rc = sched_choose(c, &action))) return rc;
+      if (action == 0) {
+        if (!c->classes && (rc = do_wcc(c, nullptr))) return rc;
+        if ((rc = do_fb_round(c, SCC_PIVOT_LEARNED, true, -1, nullptr, nullptr, nullptr, false, nullptr)))
+          return rc;
+      } else if ((rc = do_color_round(c, nullptr))) {
+        return rc;
+      }
+    } else if (mode == SCC_PARTS_COLOR) {
       if ((rc = do_color_round(c, nullptr))) return rc;
     } else {
       if ((rc = do_fb_round(c, policy, true, -1, nullptr, nullptr, nullptr, false, nullptr))) return rc;
@@ -423,15 +478,18 @@ int do_fb_parts(scc_ctx* c, int policy, int mode, bool trim_between, int64_t* ro
     }
     r++;
   }
+  c->round_idx = -1;
   if (rounds) *rounds = r;
   return SCC_OK;
 }
 
 int do_canon(scc_ctx* c, int64_t* count) {
   int rc;
+  PhaseRec pr(c, SCC_PHASE_CANON);
   if ((rc = zero_ctr(c))) return rc;
   launch_canon(c->stream, c->dc, c->n, c->label, c->scratch_i32, c->canon);
   if ((rc = pull_ctl(c))) return rc;
+  pr.finish((int64_t)c->h_dc->ctr[C_COUNT]);
   if (count) *count = (int64_t)c->h_dc->ctr[C_COUNT];
   c->labels_ready = true;
   return SCC_OK;
@@ -688,7 +746,7 @@ int scc_wcc(scc_ctx* c, int64_t* parts) {
 
 int scc_partitions_resolve(scc_ctx* c, int policy, int mode, int trim_between, int64_t* rounds) {
   REQUIRE_GRAPH(c);
-  if (mode < SCC_PARTS_FB || mode > SCC_PARTS_BOTH) return set_error(c, SCC_ERR_ARG, "bad partition mode");
+  if (mode < SCC_PARTS_FB || mode > SCC_PARTS_LEARNED) return set_error(c, SCC_ERR_ARG, "bad partition mode");
   c->labels_ready = false;
   int rc = do_trim_compact(c);
   if (rc) return rc;
@@ -716,7 +774,7 @@ int scc_run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_o
   scc_run_opts opts;
   if (opts_in) opts = *opts_in;
   else scc_run_opts_default(&opts);
-  if (opts.partition_mode < SCC_PARTS_FB || opts.partition_mode > SCC_PARTS_BOTH)
+  if (opts.partition_mode < SCC_PARTS_FB || opts.partition_mode > SCC_PARTS_LEARNED)
     return set_error(c, SCC_ERR_ARG, "bad partition mode");
   scc_run_stats st;
   std::memset(&st, 0, sizeof(st));
@@ -773,7 +831,7 @@ int scc_run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_o
     st.ms_trim2 = elapsed(2, 3);
     if (c->n_live == 0) break;
     if ((rc = check_cancel(c))) break;
-    if (opts.use_wcc) {
+    if (opts.use_wcc && !(opts.partition_mode == SCC_PARTS_LEARNED && sched_ready(c))) {
       if ((rc = do_wcc(c, &st.wcc_parts))) break;
     }
     mark(4);

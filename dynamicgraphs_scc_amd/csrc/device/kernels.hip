@@ -16,8 +16,6 @@
 
 namespace sccmi {
 
-#define LIVE_IN(dc, L0, L1) ((dc)->live.sel ? (L1) : (L0))
-#define LIVE_OUT(dc, L0, L1) ((dc)->live.sel ? (L0) : (L1))
 
 // ---- row scans -----------------------------------------------------------------
 

@@ -31,6 +31,15 @@ STATUS_NAMES = {0: "OK", -1: "ARG", -2: "HIP", -3: "OOM", -4: "BUDGET", -5: "STA
 PARTS_FB = 0
 PARTS_COLOR = 1
 PARTS_BOTH = 2
+PARTS_LEARNED = 3
+
+SCHED_FEATURES = 16
+SCHED_HIDDEN = 64
+SCHED_HEADS = 2
+SCHED_NPARAMS = 5378
+
+PHASE_NAMES = {1: "trim1", 2: "fb_global", 3: "trim2", 4: "wcc", 5: "fb_round", 6: "color_round",
+               7: "canon"}
 
 # scc_pivot_policy
 PIVOT_DEGREE_PRODUCT = 0
@@ -47,7 +56,8 @@ EXPORTS = [
     "scc_run_opts_default", "scc_run_method2", "scc_labels_download", "scc_colors_download",
     "scc_count", "scc_labels_device", "scc_update", "scc_ctx_set_kernel_timing",
     "scc_ctx_synchronize", "scc_ctx_set_budget", "scc_update_bytes_needed",
-    "scc_device_memory",
+    "scc_device_memory", "scc_phase_trace", "scc_sched_set_weights", "scc_sched_get_weights",
+    "scc_sched_forward", "scc_sched_gradients", "scc_sched_train", "scc_sched_rollout",
 ]
 
 
@@ -77,6 +87,19 @@ class RunStats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad0"}
+
+
+class PhaseRec(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("round", C.c_int32), ("live_before", C.c_int64),
+                ("live_after", C.c_int64), ("aux", C.c_int64), ("ms", C.c_double)]
+
+
+class RolloutStats(C.Structure):
+    _fields_ = [("rounds", C.c_int64), ("agree", C.c_int64), ("num_sccs", C.c_int64),
+                ("ms_total", C.c_double), ("ms_best", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class UpdateStats(C.Structure):
@@ -133,6 +156,15 @@ def load_library():
         "scc_ctx_set_budget": (C.c_int, [vp, C.c_uint64]),
         "scc_update_bytes_needed": (C.c_uint64, [i32, i64, i64, i64]),
         "scc_device_memory": (C.c_int, [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+        "scc_phase_trace": (C.c_int, [vp, vp, i64, C.POINTER(i64)]),
+        "scc_sched_set_weights": (C.c_int, [vp, vp]),
+        "scc_sched_get_weights": (C.c_int, [vp, vp]),
+        "scc_sched_forward": (C.c_int, [vp, vp, i64, vp]),
+        "scc_sched_gradients": (C.c_int, [vp, vp, vp, vp, i64, C.POINTER(C.c_float), vp]),
+        "scc_sched_train": (C.c_int, [vp, vp, vp, vp, i64, C.c_int32, C.c_int32, C.c_float,
+                                      C.c_uint64, C.POINTER(C.c_float)]),
+        "scc_sched_rollout": (C.c_int, [vp, C.POINTER(RunOpts), C.c_uint64, C.c_float, C.c_int32,
+                                        vp, vp, vp, i64, C.POINTER(i64), C.POINTER(RolloutStats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -309,6 +341,68 @@ class SccEngine:
         self._check(self.lib.scc_update(self.h, _ptr(isrc), _ptr(idst), isrc.shape[0],
                                         _ptr(dsrc), _ptr(ddst), dsrc.shape[0], C.byref(st)))
         return st.as_dict()
+
+    # -- phase trace and learned scheduler
+    def phase_trace(self) -> list:
+        n = C.c_int64()
+        self._check(self.lib.scc_phase_trace(self.h, None, 0, C.byref(n)))
+        buf = (PhaseRec * max(1, n.value))()
+        self._check(self.lib.scc_phase_trace(self.h, buf, n.value, C.byref(n)))
+        return [{"phase": PHASE_NAMES.get(r.kind, r.kind), "round": r.round,
+                 "live_before": r.live_before, "live_after": r.live_after, "aux": r.aux, "ms": r.ms}
+                for r in buf[:n.value]]
+
+    def set_sched_weights(self, params) -> None:
+        if params is None:
+            self._check(self.lib.scc_sched_set_weights(self.h, None))
+            return
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.shape != (SCHED_NPARAMS,):
+            raise ValueError(f"expected {SCHED_NPARAMS} parameters")
+        self._check(self.lib.scc_sched_set_weights(self.h, p.ctypes.data))
+
+    def sched_weights(self) -> np.ndarray:
+        p = np.empty(SCHED_NPARAMS, np.float32)
+        self._check(self.lib.scc_sched_get_weights(self.h, p.ctypes.data))
+        return p
+
+    def sched_forward(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.empty((X.shape[0], SCHED_HEADS), np.float32)
+        self._check(self.lib.scc_sched_forward(self.h, X.ctypes.data, X.shape[0], out.ctypes.data))
+        return out
+
+    def sched_gradients(self, X, Y, M):
+        X, Y, M = (np.ascontiguousarray(a, dtype=np.float32) for a in (X, Y, M))
+        g = np.empty(SCHED_NPARAMS, np.float32)
+        loss = C.c_float()
+        self._check(self.lib.scc_sched_gradients(self.h, X.ctypes.data, Y.ctypes.data, M.ctypes.data,
+                                                 X.shape[0], C.byref(loss), g.ctypes.data))
+        return float(loss.value), g
+
+    def sched_train(self, X, Y, M, epochs=10, batch=1024, lr=1e-3, seed=0) -> float:
+        X, Y, M = (np.ascontiguousarray(a, dtype=np.float32) for a in (X, Y, M))
+        loss = C.c_float()
+        self._check(self.lib.scc_sched_train(self.h, X.ctypes.data, Y.ctypes.data, M.ctypes.data,
+                                             X.shape[0], int(epochs), int(batch), float(lr),
+                                             int(seed) & (2**64 - 1), C.byref(loss)))
+        return float(loss.value)
+
+    def sched_rollout(self, seed=0, explore=0.1, samples_per_round=256, cap=1 << 20,
+                      opts: RunOpts | None = None):
+        """(X, Y, M, stats) from one counterfactual decomposition of the uploaded graph."""
+        X = np.zeros((cap, SCHED_FEATURES), np.float32)
+        Y = np.zeros((cap, SCHED_HEADS), np.float32)
+        M = np.zeros((cap, SCHED_HEADS), np.float32)
+        n = C.c_int64()
+        st = RolloutStats()
+        o = opts if opts is not None else default_opts()
+        self._check(self.lib.scc_sched_rollout(self.h, C.byref(o), int(seed) & (2**64 - 1),
+                                               float(explore), int(samples_per_round), X.ctypes.data,
+                                               Y.ctypes.data, M.ctypes.data, cap, C.byref(n),
+                                               C.byref(st)))
+        k = n.value
+        return X[:k].copy(), Y[:k].copy(), M[:k].copy(), st.as_dict()
 
     # -- fault hooks (host-side failure tests)
     def cancel(self):

@@ -138,7 +138,11 @@ typedef struct scc_run_opts {
 typedef enum scc_partition_mode {
   SCC_PARTS_FB = 0,     /* one forward-backward step per partition per round (src/scc_fb_seq2.cc) */
   SCC_PARTS_COLOR = 1,  /* WCC colouring, then each class's root SCC by a forward search */
-  SCC_PARTS_BOTH = 2    /* a forward-backward step, then a colouring step, per round */
+  SCC_PARTS_BOTH = 2,   /* a forward-backward step, then a colouring step, per round */
+  SCC_PARTS_LEARNED = 3 /* each step chosen by the learned scheduler (scc_sched_*): a
+                           forward-backward step (after a WCC colouring when the live set is
+                           not yet split into classes) or a colouring step; the separate WCC
+                           phase becomes part of that choice.  Without weights: SCC_PARTS_BOTH */
 } scc_partition_mode;
 
 typedef struct scc_run_stats {
@@ -169,6 +173,85 @@ void scc_run_opts_default(scc_run_opts* o);
  * device.  Starts from the current state (call scc_state_reset first for a
  * from-scratch run).  stats may be NULL. */
 int scc_run_method2(scc_ctx* ctx, const scc_run_opts* opts, scc_run_stats* stats);
+
+/* ---- phase trace ------------------------------------------------------------
+ * Every phase call appends one record (host wall time, live-set sizes), the
+ * counts src/scc_main.cc:198-344 prints per phase.  Cleared by scc_state_reset
+ * (and by scc_graph_upload).  The learned scheduler's features come from it. */
+typedef enum scc_phase_kind {
+  SCC_PHASE_TRIM1 = 1,
+  SCC_PHASE_FB_GLOBAL = 2,
+  SCC_PHASE_TRIM2 = 3,
+  SCC_PHASE_WCC = 4,
+  SCC_PHASE_FB_ROUND = 5,
+  SCC_PHASE_COLOR_ROUND = 6,
+  SCC_PHASE_CANON = 7
+} scc_phase_kind;
+
+typedef struct scc_phase_rec {
+  int32_t kind;         /* scc_phase_kind */
+  int32_t round;        /* partition round index, -1 before the rounds */
+  int64_t live_before;  /* undecided vertices before the phase */
+  int64_t live_after;   /* and after */
+  int64_t aux;          /* trim sweeps, search levels, WCC iterations or SCCs found */
+  double ms;            /* host wall time of the phase (it ends synchronised) */
+} scc_phase_rec;
+
+/* Copy up to cap records of the current trace; *n = records available. */
+int scc_phase_trace(scc_ctx* ctx, scc_phase_rec* out, int64_t cap, int64_t* n);
+
+/* ---- learned scheduler -------------------------------------------------------
+ * (No reference counterpart: the reference's pivot is the first vertex of a
+ * colour, src/scc_color.cc:64-93, and its phase order is fixed,
+ * src/scc_main.cc:272-351.)  A 16 -> 64 -> 64 -> 2 MLP, BF16 matrix-core
+ * arithmetic with FP32 accumulation, evaluated per live vertex: head 0 scores
+ * the vertex as a pivot (estimated log-size of its SCC relative to its
+ * partition), head 1 votes for a colouring step over a forward-backward step.
+ * Parameters (FP32, row-major): W1[16][64] b1[64] W2[64][64] b2[64] W3[64][2] b3[2].
+ * Features (per sample, 16 floats): log2(1+outdeg)/16, log2(1+indeg)/16,
+ * log2(1+outdeg*indeg)/32, log2(1+|partition|)/24, |partition|/live,
+ * (outdeg-indeg)/(outdeg+indeg+1), log2(1+live)/24, live/n, classes flag,
+ * last trim / forward-backward / colouring resolved fractions, log2(1+ns per
+ * live vertex)/16 of the last forward-backward and colouring steps,
+ * round/16, (m/n)/64. */
+#define SCC_SCHED_FEATURES 16
+#define SCC_SCHED_HIDDEN 64
+#define SCC_SCHED_HEADS 2
+#define SCC_SCHED_NPARAMS 5378
+
+/* Install weights (NULL disables the scheduler: SCC_PIVOT_LEARNED and
+ * SCC_PARTS_LEARNED then fall back to the plain rules). */
+int scc_sched_set_weights(scc_ctx* ctx, const float* params);
+int scc_sched_get_weights(scc_ctx* ctx, float* params);
+/* out[n][2] = network(X[n][16]) on the device. */
+int scc_sched_forward(scc_ctx* ctx, const float* X, int64_t n, float* out);
+/* Mean loss over n samples and its gradient w.r.t. every parameter.  Loss per
+ * sample: M0 (out0 - Y0)^2 + M1 BCE(sigmoid(out1), Y1). */
+int scc_sched_gradients(scc_ctx* ctx, const float* X, const float* Y, const float* M, int64_t n,
+                        float* loss, float* grads);
+/* Adam training on the device from the installed weights; *last_loss = loss of
+ * the final mini-batch. */
+int scc_sched_train(scc_ctx* ctx, const float* X, const float* Y, const float* M, int64_t n,
+                    int32_t epochs, int32_t batch, float lr, uint64_t seed, float* last_loss);
+
+typedef struct scc_rollout_stats {
+  int64_t rounds;     /* partition rounds */
+  int64_t agree;      /* rounds where the installed network chose the measured better step */
+  int64_t num_sccs;
+  double ms_total;    /* wall time of the rollout (both steps measured each round) */
+  double ms_best;     /* estimated partition time had every round taken the better step */
+} scc_rollout_stats;
+
+/* Phase traces for training: a from-scratch decomposition in which every
+ * partition round runs both steps from the same state (timed), restores the
+ * state, records up to samples_per_round sampled live vertices (features in X,
+ * head-1 target Y1 = 1 when colouring resolved more per millisecond), then
+ * continues with the better step (or the installed network's choice), taking
+ * the other one with probability `explore`.  After the run each sample's
+ * head-0 target is log2(1+|SCC|)/log2(1+|partition|).  Labels are exact. */
+int scc_sched_rollout(scc_ctx* ctx, const scc_run_opts* opts, uint64_t seed, float explore,
+                      int32_t samples_per_round, float* X, float* Y, float* M, int64_t cap,
+                      int64_t* n, scc_rollout_stats* stats);
 
 /* ---- results -------------------------------------------------------------- */
 

@@ -1,0 +1,123 @@
+"""GPU parity of the learned scheduler's BF16-MFMA kernels against the CPU
+oracle (oracle/mlp.py: same bf16 rounding points, float64 sums), training on
+the device, counterfactual phase traces, and exact labels under the learned
+policy."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import mlp
+from dynamicgraphs_scc_amd import graphs
+from dynamicgraphs_scc_amd.engine import PARTS_LEARNED, PIVOT_LEARNED, SccEngine, SccError, default_opts
+
+pytestmark = pytest.mark.gpu
+
+# FP32 accumulation in another order than the oracle's float64 sums moves a
+# pre-activation by ~1e-6 relative; that can flip one bf16 rounding of a hidden
+# unit (a 2^-8 relative step), so outputs are compared to 2e-2 absolute +
+# relative, and the bulk to 1e-3.
+TOL = 2e-2
+
+
+def _data(n, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(0, 1, (n, 16)).astype(np.float32)
+    Y = np.stack([rng.uniform(0, 1, n), rng.uniform(0, 1, n) > 0.5], 1).astype(np.float32)
+    M = (rng.uniform(0, 1, (n, 2)) > 0.2).astype(np.float32)
+    return X, Y, M
+
+
+@pytest.mark.parametrize("n,seed", [(16, 0), (1000, 1), (4099, 2)])
+def test_forward_matches_oracle(n, seed):
+    p = mlp.init(seed)
+    p[mlp.NPARAMS - 2:] = [0.1, -0.2]
+    X, _, _ = _data(n, seed)
+    with SccEngine() as e:
+        e.set_sched_weights(p)
+        out = e.sched_forward(X)
+    ref, _ = mlp.forward(p, X, emulate_bf16=True)
+    err = np.abs(out - ref)
+    assert np.all(err <= TOL * (1 + np.abs(ref))), err.max()
+    assert np.median(err) < 1e-3
+
+
+@pytest.mark.parametrize("n,seed", [(64, 3), (3000, 4)])
+def test_gradients_match_oracle(n, seed):
+    p = mlp.init(seed)
+    X, Y, M = _data(n, seed)
+    with SccEngine() as e:
+        e.set_sched_weights(p)
+        loss, g = e.sched_gradients(X, Y, M)
+    lref, gref = mlp.loss_and_grads(p, X, Y, M, emulate_bf16=True)
+    assert abs(loss - lref) <= 1e-3 * (1 + abs(lref))
+    scale = np.abs(gref).max()
+    assert np.all(np.abs(g - gref) <= TOL * scale + 1e-7), np.abs(g - gref).max() / scale
+    assert np.corrcoef(g, gref)[0, 1] > 0.9995
+    # every parameter block is exercised (catches a lane-layout slip in one product)
+    o = 0
+    for name, shape in mlp.SIZES:
+        k = int(np.prod(shape))
+        blk, ref = g[o:o + k], gref[o:o + k]
+        assert np.abs(ref).max() > 0, name
+        assert np.linalg.norm(blk - ref) <= 0.02 * np.linalg.norm(ref) + 1e-7, name
+        o += k
+
+
+def test_training_on_device_reduces_loss():
+    rng = np.random.default_rng(5)
+    X = rng.uniform(0, 1, (8192, 16)).astype(np.float32)
+    Y = np.stack([0.5 * X[:, 0] + 0.25 * X[:, 3], (X[:, 1] > X[:, 2]).astype(np.float32)], 1)
+    M = np.ones_like(Y)
+    p0 = mlp.init(0)
+    l0, _ = mlp.loss_and_grads(p0, X, Y, M)
+    with SccEngine() as e:
+        e.set_sched_weights(p0)
+        e.sched_train(X, Y, M, epochs=15, batch=512, lr=3e-3, seed=1)
+        p1 = e.sched_weights()
+    l1, _ = mlp.loss_and_grads(p1, X, Y, M)
+    print(f"  loss {l0:.4f} -> {l1:.4f}", flush=True)
+    assert np.all(np.isfinite(p1)) and l1 < 0.3 * l0
+
+
+def test_phase_trace_records_method2():
+    g = graphs.planted(20000, seed=3, n_medium=400)
+    with SccEngine() as e:
+        e.upload(g)
+        e.run_method2()
+        tr = e.phase_trace()
+    kinds = [r["phase"] for r in tr]
+    assert kinds[0] == "trim1" and kinds[1] == "fb_global" and kinds[-1] == "canon"
+    assert all(r["live_after"] <= r["live_before"] for r in tr if r["phase"] != "canon")
+    assert tr[0]["live_before"] == g.num_nodes
+
+
+def test_rollout_samples_and_exact_labels():
+    g = graphs.planted(60000, seed=8, n_medium=1500)
+    want = oracle.scc_canonical(g, 8)
+    with SccEngine() as e:
+        e.upload(g)
+        X, Y, M, st = e.sched_rollout(seed=1, explore=0.2, samples_per_round=128)
+        assert np.array_equal(e.labels(), want)
+        assert st["rounds"] > 0 and X.shape[0] == Y.shape[0] > 0
+        assert np.all((Y[:, 0] >= 0) & (Y[:, 0] <= 1)) and np.all(M == 1)
+        assert set(np.unique(Y[:, 1])) <= {0.0, 1.0}
+        # trained briefly on its own traces, the learned policy still gives exact labels
+        e.set_sched_weights(mlp.init(2))
+        e.sched_train(X, Y, M, epochs=5, batch=256, lr=1e-3)
+        for mode_kw in ({"partition_mode": PARTS_LEARNED}, {"pivot_policy": PIVOT_LEARNED}):
+            e.reset()
+            e.run_method2(default_opts(**mode_kw))
+            assert np.array_equal(e.labels(), want), mode_kw
+
+
+def test_learned_policy_without_weights_falls_back():
+    g = graphs.planted(20000, seed=9, n_medium=300)
+    want = oracle.scc_canonical(g, 8)
+    with SccEngine() as e:
+        e.upload(g)
+        e.run_method2(default_opts(partition_mode=PARTS_LEARNED))
+        assert np.array_equal(e.labels(), want)
+        with pytest.raises(SccError):
+            e.sched_weights()
+        with pytest.raises(SccError):
+            e.set_sched_weights(np.full(5378, np.nan, np.float32))
