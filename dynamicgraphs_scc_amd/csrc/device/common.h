@@ -375,6 +375,7 @@ struct scc_ctx {
 
   // per-vertex state
   int32_t *color = nullptr, *label = nullptr, *canon = nullptr, *W = nullptr;
+  unsigned long long* cw = nullptr;  // WCC iterations: colour << 32 | label
   int32_t* live[2] = {nullptr, nullptr};
   int cur = 0;            // live[cur] is the current live list (host copy of dc->live)
   int64_t n_live = 0;     // its length (host copy, valid at phase boundaries)

@@ -149,11 +149,14 @@ uint64_t state_bytes(int32_t n) {
          + 8 * N * 4          // live[2] q[2] cand[2] huge[2]
          + 2 * words * 4      // vf vb
          + 8 * N + 12 * N     // pkey rep
-         + 4 * N;             // scratch
+         + 4 * N              // scratch
+         + 8 * N;             // packed WCC words
 }
 
 int free_graph(scc_ctx* c) {
   dynamic_graph_replaced(c);
+  dev_free(c, c->cw);
+  c->cw = nullptr;
   int32_t** arrs[] = {&c->begin, &c->adj, &c->rbegin, &c->radj, &c->end, &c->rend, &c->color, &c->label, &c->canon,
                       &c->W, &c->live[0], &c->live[1], &c->q[0], &c->q[1], &c->cand[0],
                       &c->cand[1], &c->huge, &c->huge2, &c->rep, &c->scratch_i32};
@@ -202,6 +205,7 @@ int alloc_graph(scc_ctx* c, int32_t n, int64_t m_cap) {
   ALLOC(c->pkey, 8 * N);
   ALLOC(c->rep, 12 * N);
   ALLOC(c->scratch_i32, 4 * N);
+  ALLOC(c->cw, 8 * N);
 #undef ALLOC
   c->n = n;
   c->m_cap = m_cap;
@@ -390,18 +394,24 @@ static int wcc_fixpoint(scc_ctx* c, bool seed) {
   c->fresh = false;
   int rc;
   const unsigned grid = grid_for(c->n_live);
-  launch_ctl_init(c->stream, c->dc, 1 | 4, 0, 0, 0, 0.0);
-  launch_wcc_init(c->stream, grid, c->dc, c->begin, row_end(c), c->W, c->live[0], c->live[1]);
+  const int64_t words = ((int64_t)c->n + 31) / 32;
+  launch_ctl_init(c->stream, c->dc, 1 | 4 | 8, 0, 0, 0, 0.0);
+  launch_fill_u32(c->stream, c->vf, 0u, words);
+  launch_fill_u32(c->stream, c->vb, 0u, words);
+  launch_cw_init(c->stream, c->n, c->begin, row_end(c), c->color, c->cw);
+  launch_wcc_pull(c->stream, grid, c->dc, c->begin, row_end(c), c->adj, c->cw, c->q[0], c->live[0], c->live[1]);
   int batch = std::max(2, c->it_hint);
   for (int guard = 0;; guard++) {
     if ((rc = check_cancel(c))) return rc;
     for (int i = 0; i < batch; i++)
-      launch_wcc_step(c->stream, grid, c->dc, c->begin, row_end(c), c->adj, c->color, c->W, c->live[0], c->live[1]);
+      launch_wcc_push(c->stream, grid, c->dc, c->rbegin, rrow_end(c), c->radj, c->cw, c->q[0], c->q[1], c->vf,
+                      c->vb);
     if ((rc = pull_ctl(c))) return rc;
     if (c->h_dc->loop.done) break;
     batch = std::max(4, batch);
     if (guard > 4 * (int64_t)c->n + 64) return set_error(c, SCC_ERR_LIMIT, "WCC did not converge");
   }
+  launch_wcc_store(c->stream, grid, c->dc, c->cw, c->W, c->live[0], c->live[1]);
   c->it_hint = std::min(64, c->h_dc->loop.iters + 1);
   if (seed) {
     const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
@@ -763,7 +773,7 @@ void scc_run_opts_default(scc_run_opts* o) {
   o->pivot_policy = SCC_PIVOT_DEGREE_PRODUCT;
   o->trim_stop = 0;
   o->use_trim2 = 1;
-  o->trim_between = 1;
+  o->trim_between = 0;
   o->use_wcc = 1;
   o->time_phases = 0;
   o->partition_mode = SCC_PARTS_BOTH;

@@ -64,13 +64,15 @@ void launch_classify(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, int32
 void launch_recolor(hipStream_t st, unsigned grid, const DevCtl* dc, int32_t n, int32_t* color,
                     const uint32_t* vf, const uint32_t* vb, const int32_t* rep, const int32_t* L0,
                     const int32_t* L1);
-void launch_wcc_init(hipStream_t st, unsigned grid, const DevCtl* dc, const int32_t* begin,
-                     const int32_t* bend,
-                     int32_t* W, const int32_t* L0, const int32_t* L1);
-void launch_wcc_step(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* begin,
-                     const int32_t* bend,
-                     const int32_t* adj, const int32_t* color, int32_t* W, const int32_t* L0,
-                     const int32_t* L1);
+void launch_cw_init(hipStream_t st, int32_t n, const int32_t* begin, const int32_t* bend, const int32_t* color,
+                    unsigned long long* cw);
+void launch_wcc_pull(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* begin, const int32_t* bend,
+                     const int32_t* adj, unsigned long long* cw, int32_t* q0, const int32_t* L0, const int32_t* L1);
+void launch_wcc_push(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* rbegin, const int32_t* rend,
+                     const int32_t* radj, unsigned long long* cw, int32_t* q0, int32_t* q1, uint32_t* B0,
+                     uint32_t* B1);
+void launch_wcc_store(hipStream_t st, unsigned grid, const DevCtl* dc, const unsigned long long* cw, int32_t* W,
+                      const int32_t* L0, const int32_t* L1);
 void launch_wcc_color(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, const int32_t* W,
                       int seed, const int32_t* off, const int32_t* offe, uint32_t* V, int32_t* q0, const int32_t* L0,
                       const int32_t* L1);

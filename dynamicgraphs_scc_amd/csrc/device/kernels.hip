@@ -742,96 +742,221 @@ __global__ __launch_bounds__(BLOCK) void k_recolor(const DevCtl* __restrict__ dc
 }
 
 // ---- WCC colouring --------------------------------------------------------------
-// src/scc_weak.cc:128-150: W[v] = v, tagged when out-degree >= 50.
-__global__ __launch_bounds__(BLOCK) void k_wcc_init(const DevCtl* __restrict__ dc,
-                                                    const int32_t* __restrict__ begin,
-                                                    const int32_t* __restrict__ bend,
-                                                    int32_t* __restrict__ W, const int32_t* __restrict__ L0,
-                                                    const int32_t* __restrict__ L1) {
-  const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
-  const int64_t n_in = (int64_t)dc->live.n;
-  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_in; i += (int64_t)gridDim.x * BLOCK) {
-    const int32_t v = list[i];
-    W[v] = (bend[v] - begin[v] >= 50) ? (v | WCC_TAG) : v;
+// ---- WCC labels, packed with the colour -------------------------------------------
+// cw[v] = colour << 32 | W: one 8-byte gather per neighbour tests the colour
+// and reads the label, and a 64-bit atomicMin on the packed word lowers the
+// label of a same-colour neighbour.  Vertices of colour -2 never match a live
+// colour (-1 or a member id).
+__device__ __forceinline__ unsigned long long cw_pack(int32_t c, int32_t w) {
+  return ((unsigned long long)(uint32_t)c << 32) | (uint32_t)w;
+}
+__device__ __forceinline__ uint32_t cw_hi(unsigned long long x) { return (uint32_t)(x >> 32); }
+__device__ __forceinline__ int32_t cw_lo(unsigned long long x) { return (int32_t)(uint32_t)x; }
+
+// src/scc_weak.cc:137-150 for every vertex: W = v, tagged when out-degree >= 50.
+__global__ __launch_bounds__(BLOCK) void k_cw_init(int32_t n, const int32_t* __restrict__ begin,
+                                                   const int32_t* __restrict__ bend,
+                                                   const int32_t* __restrict__ color,
+                                                   unsigned long long* __restrict__ cw) {
+  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < n; v += (int64_t)gridDim.x * BLOCK) {
+    const int32_t w = (bend[v] - begin[v] >= 50) ? ((int32_t)v | WCC_TAG) : (int32_t)v;
+    cw[v] = cw_pack(color[v], w);
   }
 }
 
-// src/scc_weak.cc:65-125 (propagate_color), both phases fused per vertex: pull
-// the minimum W over same-colour out-neighbours, then jump to the root's value
-// when the root is not self-rooted.  Each vertex writes only its own W and W
-// only decreases, so racing reads change the path to the fixpoint, not the
-// fixpoint (W[v] = min tag over v's same-colour forward-reachable set).  One
-// launch is one iteration; the last workgroup ends the loop after an iteration
-// that changed nothing.
-__global__ __launch_bounds__(BLOCK) void k_wcc_step(DevCtl* __restrict__ dc, const int32_t* __restrict__ begin,
+// First iteration (src/scc_weak.cc:65-125, both phases fused): every live
+// vertex pulls the minimum label of its same-colour out-neighbours, then jumps
+// to its root's label when the root is not self-rooted.  Vertices whose label
+// dropped form the frontier of the push iterations.
+__global__ __launch_bounds__(BLOCK) void k_wcc_pull(DevCtl* __restrict__ dc, const int32_t* __restrict__ begin,
                                                     const int32_t* __restrict__ bend,
                                                     const int32_t* __restrict__ adj,
-                                                    const int32_t* __restrict__ color,
-                                                    int32_t* __restrict__ W, const int32_t* __restrict__ L0,
+                                                    unsigned long long* __restrict__ cw,
+                                                    int32_t* __restrict__ q0, const int32_t* __restrict__ L0,
                                                     const int32_t* __restrict__ L1) {
-  __shared__ int s_done;
-  if (threadIdx.x == 0) s_done = dc->loop.done;
-  __syncthreads();
-  if (s_done) return;
   const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
   const int64_t n_in = (int64_t)dc->live.n;
   const int lane = lane_id();
-  unsigned long long changed = 0;
+  using Out = LdsOut<0>;
+  Out::init();
   for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
+    Out::maybe_flush(q0, &dc->lv.out_q, BLOCK);
     const int64_t i = base + threadIdx.x;
-    const bool valid = i < n_in;
-    int32_t v = 0, c = DONE, s = 0, e = 0, m = INT32_MAX, w0 = INT32_MAX;
-    if (valid) {
+    int32_t v = 0, s = 0, e = 0, m = INT32_MAX, w0 = INT32_MAX;
+    uint32_t hv = (uint32_t)DONE;
+    if (i < n_in) {
       v = list[i];
-      c = color[v];
+      const unsigned long long x = cw[v];
+      hv = cw_hi(x);
+      w0 = m = cw_lo(x);
       s = begin[v];
       e = bend[v];
-      w0 = m = W[v];
     }
-    const bool act = valid && c != DONE;
-    const int lim = act ? min(e - s, 64) : 0;
+    const bool act = hv != (uint32_t)DONE;
+    const int len = act ? e - s : 0;
+    const int lim = len < 32 ? len : 0;
     for (int j = 0; j < lim; j++) {
-      const int32_t k = adj[s + j];
-      if (color[k] == c) m = min(m, W[k]);
+      const unsigned long long y = cw[adj[s + j]];
+      if (cw_hi(y) == hv) m = min(m, cw_lo(y));
     }
-    uint64_t hm = __ballot(act && e - s > 64);
+    uint64_t hm = __ballot(len >= 32);
     while (hm) {
       const int L = __ffsll((unsigned long long)hm) - 1;
       hm &= hm - 1;
-      const int32_t hs = __shfl(s, L) + 64, he = __shfl(e, L), hc = __shfl(c, L);
+      const int32_t hs = __shfl(s, L), he = __shfl(e, L);
+      const uint32_t hh = __shfl(hv, L);
       int32_t hmin = INT32_MAX;
       for (int32_t b = hs; b < he; b += 64) {
         const int32_t idx = b + lane;
         if (idx < he) {
-          const int32_t k = adj[idx];
-          if (color[k] == hc) hmin = min(hmin, W[k]);
+          const unsigned long long y = cw[adj[idx]];
+          if (cw_hi(y) == hh) hmin = min(hmin, cw_lo(y));
         }
       }
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) hmin = min(hmin, __shfl_xor(hmin, o));
       if (lane == L) m = min(m, hmin);
     }
+    bool changed = false;
     if (act) {
       const int32_t r = m & WCC_MASK;
       if (r != v) {
-        const int32_t wr = W[r];
-        if ((wr & WCC_MASK) != r) m = min(m, wr);
+        const unsigned long long y = cw[r];
+        if (cw_hi(y) == hv && (cw_lo(y) & WCC_MASK) != r) m = min(m, cw_lo(y));
       }
       if (m < w0) {
-        W[v] = m;
-        changed++;
+        cw[v] = cw_pack((int32_t)hv, m);
+        changed = true;
+      }
+    }
+    Out::push(changed, v, q0, &dc->lv.out_q);
+  }
+  Out::flush(q0, &dc->lv.out_q);
+  if (last_block2(dc->done) && threadIdx.x == 0) {
+    LevelCtl& lv = dc->lv;
+    const unsigned long long nq = ld_acq(&lv.out_q);
+    lv.nq = nq;
+    lv.out_q = 0;
+    lv.q_sel = 0;
+    lv.levels = 1;
+    dc->loop.iters = 1;
+    dc->loop.total0 += nq;
+    dc->loop.done = nq == 0;
+    lv.blocks_done = 0;
+  }
+}
+
+// Later iterations: every vertex whose label dropped pushes it to its
+// same-colour in-neighbours (reverse rows) with a 64-bit atomicMin; a
+// neighbour whose label dropped joins the next frontier once (bitmap claim).
+// Ends when an iteration lowers nothing.  The fixpoint is the pull fixpoint:
+// W[v] = min tag over v's same-colour forward-reachable set.
+__global__ __launch_bounds__(BLOCK) void k_wcc_push(DevCtl* __restrict__ dc, const int32_t* __restrict__ rbegin,
+                                                    const int32_t* __restrict__ rend,
+                                                    const int32_t* __restrict__ radj,
+                                                    unsigned long long* __restrict__ cw,
+                                                    int32_t* __restrict__ q0, int32_t* __restrict__ q1,
+                                                    uint32_t* __restrict__ B0, uint32_t* __restrict__ B1) {
+  __shared__ int s_done, s_sel, s_t;
+  __shared__ unsigned long long s_nq;
+  if (threadIdx.x == 0) {
+    s_done = dc->loop.done;
+    s_sel = dc->lv.q_sel;
+    s_t = dc->lv.levels;
+    s_nq = dc->lv.nq;
+  }
+  __syncthreads();
+  if (s_done) return;
+  const int32_t* __restrict__ qin = s_sel ? q1 : q0;
+  int32_t* __restrict__ qout = s_sel ? q0 : q1;
+  uint32_t* __restrict__ Bin = ((s_t - 1) & 1) ? B1 : B0;
+  uint32_t* __restrict__ Bout = (s_t & 1) ? B1 : B0;
+  const int64_t nq = (int64_t)s_nq;
+  const int lane = lane_id();
+  unsigned long long lowered = 0;
+  using Out = LdsOut<0>;
+  Out::init();
+  // claim u for the next frontier when the push lowered its label
+  auto lower = [&](int32_t u, unsigned long long x) -> bool {
+    const unsigned long long y = cw[u];
+    if (cw_hi(y) != cw_hi(x) || cw_lo(x) >= cw_lo(y)) return false;
+    const unsigned long long old = atomicMin(&cw[u], x);
+    if (old <= x) return false;
+    const uint32_t bit = 1u << (u & 31);
+    return !(atomicOr(&Bout[u >> 5], bit) & bit);
+  };
+  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < nq; base += (int64_t)gridDim.x * BLOCK) {
+    Out::maybe_flush(qout, &dc->lv.out_q, BLOCK);
+    const int64_t i = base + threadIdx.x;
+    int32_t k = 0, s = 0, e = 0;
+    unsigned long long x = 0;
+    bool act = false;
+    if (i < nq) {
+      k = qin[i];
+      if (s_t > 1) atomicAnd(&Bin[k >> 5], ~(1u << (k & 31)));
+      x = cw[k];
+      s = rbegin[k];
+      e = rend[k];
+      act = cw_hi(x) != (uint32_t)DONE;
+    }
+    const int len = act ? e - s : 0;
+    const int lim = len < 32 ? len : 0;
+    const int wl = wave_max(lim);
+    for (int j = 0; j < wl; j++) {
+      bool add = false;
+      int32_t u = 0;
+      if (j < lim) {
+        u = radj[s + j];
+        add = lower(u, x);
+      }
+      lowered += add;
+      Out::push(add, u, qout, &dc->lv.out_q);
+    }
+    uint64_t hm = __ballot(len >= 32);
+    while (hm) {
+      const int L = __ffsll((unsigned long long)hm) - 1;
+      hm &= hm - 1;
+      const int32_t hs = __shfl(s, L), he = __shfl(e, L);
+      const unsigned long long hx = __shfl(x, L);
+      for (int32_t b = hs; b < he; b += 64) {
+        const int32_t idx = b + lane;
+        bool add = false;
+        int32_t u = 0;
+        if (idx < he) {
+          u = radj[idx];
+          add = lower(u, hx);
+        }
+        lowered += add;
+        Out::push(add, u, qout, &dc->lv.out_q);
       }
     }
   }
-  block_count(changed, &dc->loop.acc0);
+  Out::flush(qout, &dc->lv.out_q);
+  (void)lowered;
   if (last_block2(dc->done) && threadIdx.x == 0) {
-    LoopCtl& l = dc->loop;
-    const unsigned long long ch = ld_acq(&l.acc0);
-    l.total0 += ch;
-    l.iters++;
-    l.done = ch == 0;
-    l.acc0 = 0;
-    l.blocks_done = 0;
+    LevelCtl& lv = dc->lv;
+    const unsigned long long nn = ld_acq(&lv.out_q);
+    lv.nq = nn;
+    lv.out_q = 0;
+    lv.q_sel ^= 1;
+    lv.levels++;
+    dc->loop.iters++;
+    dc->loop.total0 += nn;
+    dc->loop.done = nn == 0;
+    lv.blocks_done = 0;
+  }
+}
+
+// W of the live vertices from the packed words.
+__global__ __launch_bounds__(BLOCK) void k_wcc_store(const DevCtl* __restrict__ dc,
+                                                     const unsigned long long* __restrict__ cw,
+                                                     int32_t* __restrict__ W, const int32_t* __restrict__ L0,
+                                                     const int32_t* __restrict__ L1) {
+  const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
+  const int64_t n_in = (int64_t)dc->live.n;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_in; i += (int64_t)gridDim.x * BLOCK) {
+    const int32_t v = list[i];
+    W[v] = cw_lo(cw[v]);
   }
 }
 
@@ -1043,16 +1168,22 @@ void launch_recolor(hipStream_t st, unsigned grid, const DevCtl* dc, int32_t n, 
                     const int32_t* L1) {
   LAUNCH(k_recolor, grid, st, dc, n, color, vf, vb, rep, L0, L1);
 }
-void launch_wcc_init(hipStream_t st, unsigned grid, const DevCtl* dc, const int32_t* begin,
-                     const int32_t* bend,
-                     int32_t* W, const int32_t* L0, const int32_t* L1) {
-  LAUNCH(k_wcc_init, grid, st, dc, begin, bend, W, L0, L1);
+void launch_cw_init(hipStream_t st, int32_t n, const int32_t* begin, const int32_t* bend, const int32_t* color,
+                    unsigned long long* cw) {
+  if (n > 0) LAUNCH(k_cw_init, grid_for(n, 4096), st, n, begin, bend, color, cw);
 }
-void launch_wcc_step(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* begin,
-                     const int32_t* bend,
-                     const int32_t* adj, const int32_t* color, int32_t* W, const int32_t* L0,
-                     const int32_t* L1) {
-  LAUNCH(k_wcc_step, grid, st, dc, begin, bend, adj, color, W, L0, L1);
+void launch_wcc_pull(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* begin, const int32_t* bend,
+                     const int32_t* adj, unsigned long long* cw, int32_t* q0, const int32_t* L0, const int32_t* L1) {
+  LAUNCH(k_wcc_pull, grid, st, dc, begin, bend, adj, cw, q0, L0, L1);
+}
+void launch_wcc_push(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* rbegin, const int32_t* rend,
+                     const int32_t* radj, unsigned long long* cw, int32_t* q0, int32_t* q1, uint32_t* B0,
+                     uint32_t* B1) {
+  LAUNCH(k_wcc_push, grid, st, dc, rbegin, rend, radj, cw, q0, q1, B0, B1);
+}
+void launch_wcc_store(hipStream_t st, unsigned grid, const DevCtl* dc, const unsigned long long* cw, int32_t* W,
+                      const int32_t* L0, const int32_t* L1) {
+  LAUNCH(k_wcc_store, grid, st, dc, cw, W, L0, L1);
 }
 void launch_wcc_color(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, const int32_t* W,
                       int seed, const int32_t* off, const int32_t* offe, uint32_t* V, int32_t* q0, const int32_t* L0,

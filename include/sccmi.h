@@ -164,8 +164,9 @@ typedef struct scc_run_stats {
   int64_t fb_f, fb_b, fb_ef, fb_eb, fb_fb;
 } scc_run_stats;
 
-/* Default options: degree-product pivot, trim_stop 0, trim2 on, trim between
- * rounds on, WCC on, no phase timing, SCC_PARTS_BOTH. */
+/* Default options: degree-product pivot, trim_stop 0, trim2 on, no trim between
+ * partition rounds (measured faster on planted graphs), WCC on, no phase
+ * timing, SCC_PARTS_BOTH. */
 void scc_run_opts_default(scc_run_opts* o);
 
 /* do_baseline_global_wcc_fb (src/scc_main.cc:272-351): trim1 -> global FB ->
