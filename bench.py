@@ -95,6 +95,59 @@ def cpu_baseline(g, cores: int, want_ref: bool = True):
     return res
 
 
+def learned_leg(args, eng, g, ms_plain_rmat):
+    """Plain vs learned scheduler: R-MAT-22 (the engine's graph, same step as
+    `value`) and the planted-SCC graph of 8,388,608 vertices (seed 3)."""
+    import numpy as np
+    from dynamicgraphs_scc_amd import graphs
+    from dynamicgraphs_scc_amd.engine import PARTS_BOTH, PARTS_LEARNED, SccEngine, default_opts
+    from dynamicgraphs_scc_amd.sched import load_weights
+
+    w = load_weights()
+    if w is None:
+        return None
+
+    def timed(e, opts, reps):
+        ts = []
+        for _ in range(reps):
+            e.reset()
+            e.synchronize()
+            t0 = time.perf_counter()
+            e.run_method2(opts)
+            e.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return float(np.median(ts))
+
+    out = {}
+    lab_plain = eng.labels()
+    eng.set_sched_weights(w)
+    lo = default_opts(partition_mode=PARTS_LEARNED)
+    timed(eng, lo, 2)
+    ms_l = timed(eng, lo, max(5, args.steps))
+    same = bool(np.array_equal(lab_plain, eng.labels()))
+    eng.set_sched_weights(None)
+    out["rmat22"] = {"ms_plain": round(ms_plain_rmat, 4), "ms_learned": round(ms_l, 4),
+                     "speedup": round(ms_plain_rmat / ms_l, 3), "labels_equal": same,
+                     "learned_gteps": round(g.num_edges / (ms_l / 1e3) / 1e9, 4)}
+    gp = graphs.planted(8_388_608, seed=3)
+    with SccEngine(device=0) as e:
+        e.upload(gp)
+        po = default_opts(partition_mode=PARTS_BOTH)
+        timed(e, po, 1)
+        ms_p = timed(e, po, 5)
+        lp = e.labels()
+        e.set_sched_weights(w)
+        timed(e, lo, 1)
+        ms_l = timed(e, lo, 5)
+        same = bool(np.array_equal(lp, e.labels()))
+    out["planted8m"] = {"n": gp.num_nodes, "m": gp.num_edges, "ms_plain": round(ms_p, 3),
+                        "ms_learned": round(ms_l, 3), "speedup": round(ms_p / ms_l, 3),
+                        "labels_equal": same,
+                        "gteps_plain": round(gp.num_edges / (ms_p / 1e3) / 1e9, 4),
+                        "gteps_learned": round(gp.num_edges / (ms_l / 1e3) / 1e9, 4)}
+    return out
+
+
 def _sleep_until(srv, t_ms):
     while True:
         d = t_ms - srv.now_ms()
@@ -199,6 +252,7 @@ def main():
                     help="first run a plain device copy of this many MiB (reference point for "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE runs)")
     ap.add_argument("--no-dyn", action="store_true", help="skip the dynamic serving leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-scheduler leg")
     ap.add_argument("--dyn-scale", type=int, default=23)
     ap.add_argument("--dyn-batches", type=int, default=2000)
     ap.add_argument("--dyn-check-every", type=int, default=100)
@@ -279,6 +333,11 @@ def main():
                                            "ms_fb_parts", "ms_canon")}
     log(f"step {ms_step:.3f} ms  GTEPS {gteps:.2f}  phases {phases}  fb_kernels {t_fb_ms:.3f} ms")
 
+    learned = None
+    if not args.no_learned:
+        t = time.time()
+        learned = learned_leg(args, eng, g, ms_step)
+        log(f"learned leg in {time.time() - t:.1f}s: {learned}")
     eng.close()
     cpu = None
     if not args.no_cpu:
@@ -305,7 +364,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "phases_ms": phases,
-        "learned_gteps": None,
+        "learned_gteps": learned["rmat22"]["learned_gteps"] if learned else None,
+        "learned": learned,
         "dynamic": dyn,
     }
     print(json.dumps(out), flush=True)

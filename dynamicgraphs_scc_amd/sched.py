@@ -1,0 +1,168 @@
+"""Training and evaluation of the learned pivot / phase scheduler.
+
+The network and its kernels live in csrc/device/sched.hip (16 features ->
+64 -> 64 -> 2 heads, BF16 matrix-core arithmetic, FP32 accumulation).  This
+module drives them:
+
+* traces: counterfactual decompositions of synthetic graphs
+  (SccEngine.sched_rollout) -- each partition round runs both the
+  forward-backward step and the colouring step from the same state and
+  records which resolved more vertices per millisecond, and every sampled
+  vertex's SCC size relative to its partition;
+* training: Adam on the device (SccEngine.sched_train), from seeded He
+  initial weights;
+* evaluation: wall time of the plain method (degree-product pivots, a
+  forward-backward step then a colouring step per round) against the learned
+  policy on held-out graphs, labels compared for equality.
+
+The trained weights ship as sched_weights.npy next to this file (FP32, 5378
+values, loaded with allow_pickle=False).  `python -m dynamicgraphs_scc_amd.sched
+--out DIR` regenerates them on a GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import graphs
+from .engine import (PARTS_BOTH, PARTS_LEARNED, SCHED_FEATURES, SCHED_HEADS, SCHED_HIDDEN, SCHED_NPARAMS,
+                     SccEngine, default_opts)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+WEIGHTS = os.path.join(HERE, "sched_weights.npy")
+
+
+def load_weights(path: str = WEIGHTS):
+    """The shipped weights, or None when the file is absent."""
+    if not os.path.exists(path):
+        return None
+    w = np.load(path, allow_pickle=False)
+    if w.shape != (SCHED_NPARAMS,) or w.dtype != np.float32 or not np.all(np.isfinite(w)):
+        raise ValueError(f"{path}: not a scheduler weight vector")
+    return w
+
+
+def initial_weights(seed: int = 0) -> np.ndarray:
+    """He-normal weights, zero biases, in the kernel's parameter order."""
+    rng = np.random.default_rng(seed)
+    nf, nh, no = SCHED_FEATURES, SCHED_HIDDEN, SCHED_HEADS
+    parts = [rng.normal(0, np.sqrt(2 / nf), nf * nh), np.zeros(nh),
+             rng.normal(0, np.sqrt(2 / nh), nh * nh), np.zeros(nh),
+             rng.normal(0, np.sqrt(1 / nh), nh * no), np.zeros(no)]
+    return np.concatenate(parts).astype(np.float32)
+
+
+def training_graphs(seed: int = 100):
+    """Synthetic graphs whose decompositions need partition rounds: planted SCC
+    graphs over a range of sizes and medium-SCC counts.  (R-MAT graphs leave no
+    partition rounds after the global step, trims and WCC, so they give no
+    samples.)"""
+    specs = []
+    rng = np.random.default_rng(seed)
+    for i in range(10):
+        n = int(rng.integers(100_000, 1_500_000))
+        med = int(n * rng.uniform(0.005, 0.04))
+        specs.append((f"planted_n{n}_m{med}_s{seed + i}",
+                      lambda n=n, med=med, s=seed + i: graphs.planted(n, seed=s, n_medium=med)))
+    return specs
+
+
+def held_out_graphs():
+    return [("planted8m_s3", lambda: graphs.planted(8_388_608, seed=3)),
+            ("planted1m_s4", lambda: graphs.planted(1_000_000, seed=4, n_medium=25_000)),
+            ("rmat22_ef16_s22", lambda: graphs.rmat(22, 16, seed=22)),
+            ("rmat20_ef3_s7", lambda: graphs.rmat(20, 3, seed=7))]
+
+
+def generate(specs, rollouts: int = 2, explore: float = 0.25, samples_per_round: int = 512, log=print):
+    Xs, Ys, Ms = [], [], []
+    with SccEngine() as e:
+        for name, make in specs:
+            g = make()
+            e.upload(g)
+            for r in range(rollouts):
+                X, Y, M, st = e.sched_rollout(seed=1000 * r + 7, explore=explore,
+                                              samples_per_round=samples_per_round)
+                Xs.append(X)
+                Ys.append(Y)
+                Ms.append(M)
+                log(f"  trace {name} #{r}: {X.shape[0]} samples, {st['rounds']} rounds, "
+                    f"colour better in {Y[:, 1].mean() if len(Y) else 0:.2f}")
+    return np.concatenate(Xs), np.concatenate(Ys), np.concatenate(Ms)
+
+
+def train(X, Y, M, epochs: int = 40, batch: int = 1024, lr: float = 2e-3, seed: int = 0):
+    with SccEngine() as e:
+        e.set_sched_weights(initial_weights(seed))
+        loss = e.sched_train(X, Y, M, epochs=epochs, batch=batch, lr=lr, seed=seed)
+        return e.sched_weights(), loss
+
+
+def time_runs(e, opts, reps: int):
+    ts = []
+    for _ in range(reps):
+        e.reset()
+        t = time.perf_counter()
+        e.run_method2(opts)
+        e.synchronize()
+        ts.append((time.perf_counter() - t) * 1e3)
+    return float(np.median(ts))
+
+
+def evaluate(weights, specs, reps: int = 5, log=print):
+    """Median wall ms of plain and learned runs; labels must be equal."""
+    out = []
+    with SccEngine() as e:
+        for name, make in specs:
+            g = make()
+            e.upload(g)
+            plain_opts = default_opts(partition_mode=PARTS_BOTH)
+            e.set_sched_weights(None)
+            time_runs(e, plain_opts, 1)
+            t_plain = time_runs(e, plain_opts, reps)
+            lab_plain = e.labels()
+            e.set_sched_weights(weights)
+            learned_opts = default_opts(partition_mode=PARTS_LEARNED)
+            time_runs(e, learned_opts, 1)
+            t_learned = time_runs(e, learned_opts, reps)
+            same = bool(np.array_equal(lab_plain, e.labels()))
+            e.set_sched_weights(None)
+            rec = {"graph": name, "n": g.num_nodes, "m": g.num_edges, "ms_plain": round(t_plain, 3),
+                   "ms_learned": round(t_learned, 3), "speedup": round(t_plain / t_learned, 3),
+                   "labels_equal": same}
+            log(f"  eval {rec}")
+            out.append(rec)
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--out", default=".", help="directory for sched_weights.npy and sched_report.json")
+    ap.add_argument("--rollouts", type=int, default=2)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args(argv)
+
+    def log(*a):
+        print(*a, flush=True)
+
+    t = time.time()
+    X, Y, M = generate(training_graphs(), rollouts=args.rollouts, log=log)
+    log(f"traces: {X.shape[0]} samples in {time.time() - t:.1f}s")
+    w, loss = train(X, Y, M, epochs=args.epochs, seed=args.seed)
+    log(f"trained: last batch loss {loss:.4f}")
+    os.makedirs(args.out, exist_ok=True)
+    np.save(os.path.join(args.out, "sched_weights.npy"), w, allow_pickle=False)
+    ev = evaluate(w, held_out_graphs(), log=log)
+    with open(os.path.join(args.out, "sched_report.json"), "w") as f:
+        json.dump({"samples": int(X.shape[0]), "loss": loss, "eval": ev}, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -121,3 +121,35 @@ def test_learned_policy_without_weights_falls_back():
             e.sched_weights()
         with pytest.raises(SccError):
             e.set_sched_weights(np.full(5378, np.nan, np.float32))
+
+
+def test_shipped_weights_gain_over_plain():
+    """The trained network against the method without it, on a held-out planted
+    graph: equal labels, and no slower (measured gain is reported)."""
+    import time
+    from dynamicgraphs_scc_amd.engine import PARTS_BOTH
+    from dynamicgraphs_scc_amd.sched import load_weights
+    w = load_weights()
+    assert w is not None
+    g = graphs.planted(1_000_000, seed=4, n_medium=25_000)
+
+    def timed(e, opts):
+        ts = []
+        for _ in range(6):
+            e.reset()
+            e.synchronize()
+            t0 = time.perf_counter()
+            e.run_method2(opts)
+            e.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts[1:]))
+
+    with SccEngine() as e:
+        e.upload(g)
+        tp = timed(e, default_opts(partition_mode=PARTS_BOTH))
+        lp = e.labels()
+        e.set_sched_weights(w)
+        tl = timed(e, default_opts(partition_mode=PARTS_LEARNED))
+        assert np.array_equal(lp, e.labels())
+    print(f"  plain {tp * 1e3:.2f} ms, learned {tl * 1e3:.2f} ms, speedup {tp / tl:.3f}", flush=True)
+    assert tl <= 1.05 * tp
