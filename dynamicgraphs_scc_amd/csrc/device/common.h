@@ -16,8 +16,9 @@ constexpr int32_t DONE = -2;   // colour of a vertex whose SCC is known
 constexpr int32_t UNSET = -1;  // initial colour
 constexpr int BLOCK = 256;     // threads per workgroup (4 wave64s)
 constexpr int WAVES = BLOCK / 64;
-constexpr int LIGHT = 16;      // row entries a lane scans alone before the wave helps
-constexpr int HUGE_ROW = 8192; // rows at least this long are expanded by many blocks
+constexpr int LIGHT = 16;      // default: row entries a lane scans alone before the wave helps
+constexpr int HUGE_ROW = 8192; // default: rows at least this long are expanded by many blocks
+
 constexpr int32_t WCC_TAG = 0x20000000, WCC_MASK = 0x1FFFFFFF;  // src/scc_weak.cc:57-62
 
 // Counter slots of DevCtl::ctr (unsigned long long).
@@ -85,7 +86,18 @@ struct LevelCtl {  // one reachability search (all partitions at once)
   double bu_edges_per_cand;    // Beamer's switch: BU when m_f > n_cand * this
 };
 
+// Launch-shape knobs the autotuner sets (scc_tuning); kernels read them here.
+struct Tune {
+  int32_t light;     // lane-alone prefix of a row scan
+  int32_t huge_row;  // rows split over the grid in top-down levels
+  float td_beta;     // bottom-up -> top-down when the frontier < live / td_beta
+  float bu_alpha;    // top-down -> bottom-up when frontier edges > candidates' edges / bu_alpha
+  int32_t grid_cap;  // workgroups per sweep launch
+  int32_t pad;
+};
+
 struct DevCtl {
+  Tune tune;
   LiveCtl live;
   LoopCtl loop;
   LevelCtl lv;
@@ -376,6 +388,12 @@ struct scc_ctx {
   // per-vertex state
   int32_t *color = nullptr, *label = nullptr, *canon = nullptr, *W = nullptr;
   unsigned long long* cw = nullptr;  // WCC iterations: colour << 32 | label
+  sccmi::Tune tune = {sccmi::LIGHT, sccmi::HUGE_ROW, 24.f, 14.f, 2048, 0};  // host copy of dc->tune
+
+  // batched graphs: vertex ranges [boff[g], boff[g+1]) (scc_batch_set)
+  int32_t batch_k = 0;
+  int32_t* d_boff = nullptr;
+  std::vector<int32_t> h_boff;
   int32_t* live[2] = {nullptr, nullptr};
   int cur = 0;            // live[cur] is the current live list (host copy of dc->live)
   int64_t n_live = 0;     // its length (host copy, valid at phase boundaries)

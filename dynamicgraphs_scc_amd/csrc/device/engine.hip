@@ -157,6 +157,10 @@ int free_graph(scc_ctx* c) {
   dynamic_graph_replaced(c);
   dev_free(c, c->cw);
   c->cw = nullptr;
+  dev_free(c, c->d_boff);
+  c->d_boff = nullptr;
+  c->batch_k = 0;
+  c->h_boff.clear();
   int32_t** arrs[] = {&c->begin, &c->adj, &c->rbegin, &c->radj, &c->end, &c->rend, &c->color, &c->label, &c->canon,
                       &c->W, &c->live[0], &c->live[1], &c->q[0], &c->q[1], &c->cand[0],
                       &c->cand[1], &c->huge, &c->huge2, &c->rep, &c->scratch_i32};
@@ -250,7 +254,7 @@ int do_trim1(scc_ctx* c, int64_t stop, int64_t* trimmed) {
   if (c->n_live == 0) return SCC_OK;
   PhaseRec pr(c, SCC_PHASE_TRIM1);
   launch_ctl_init(c->stream, c->dc, 4, 0, 0, (long long)stop, 0.0);
-  const unsigned grid = grid_for(c->n_live);
+  const unsigned grid = grid_for(c->n_live, (unsigned)c->tune.grid_cap);
   int batch = 3, rc;
   for (int guard = 0;; guard++) {
     if ((rc = check_cancel(c))) return rc;
@@ -289,7 +293,7 @@ int run_levels(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed) {
   const int32_t *roff = fwd ? c->rbegin : c->begin, *rnb = fwd ? c->radj : c->adj;
   const int32_t *offe = fwd ? row_end(c) : rrow_end(c), *roffe = fwd ? rrow_end(c) : row_end(c);
   uint32_t* V = fwd ? c->vf : c->vb;
-  const unsigned grid = grid_for(c->n_live);
+  const unsigned grid = grid_for(c->n_live, (unsigned)c->tune.grid_cap);
   const int32_t max_chunks = std::max<int32_t>(1, (c->maxdeg + 2047) / 2048);
   int& hint = c->lv_hint[fwd ? 0 : 1];
   int batch = std::max(2, hint), rc;
@@ -324,7 +328,7 @@ int traverse(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed) {
   const int32_t* off = fwd ? c->begin : c->rbegin;
   uint32_t* V = fwd ? c->vf : c->vb;
   const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
-  launch_ctl_init(c->stream, c->dc, 8, 0, 0, 0, avg / 14.0);
+  launch_ctl_init(c->stream, c->dc, 8, 0, 0, 0, avg / c->tune.bu_alpha);
   launch_seed_pivots(c->stream, grid_for(c->n_live), c->dc, c->n, off, fwd ? row_end(c) : rrow_end(c), c->color, c->pkey, V, c->q[0],
                      c->huge, c->live[0], c->live[1]);
   return run_levels(c, fwd, multi, levels, timed);
@@ -335,7 +339,7 @@ int traverse_from(scc_ctx* c, bool fwd, bool multi, const int32_t* seeds,
   const int32_t* off = fwd ? c->begin : c->rbegin;
   uint32_t* V = fwd ? c->vf : c->vb;
   const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
-  launch_ctl_init(c->stream, c->dc, 8, 0, 0, 0, avg / 14.0);
+  launch_ctl_init(c->stream, c->dc, 8, 0, 0, 0, avg / c->tune.bu_alpha);
   launch_seed_list(c->stream, grid_for(c->n_live), c->dc, off, fwd ? row_end(c) : rrow_end(c), c->color, seeds, d_count, V, c->q[0]);
   return run_levels(c, fwd, multi, levels, false);
 }
@@ -393,7 +397,7 @@ int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int6
 static int wcc_fixpoint(scc_ctx* c, bool seed) {
   c->fresh = false;
   int rc;
-  const unsigned grid = grid_for(c->n_live);
+  const unsigned grid = grid_for(c->n_live, (unsigned)c->tune.grid_cap);
   const int64_t words = ((int64_t)c->n + 31) / 32;
   launch_ctl_init(c->stream, c->dc, 1 | 4 | 8, 0, 0, 0, 0.0);
   launch_fill_u32(c->stream, c->vf, 0u, words);
@@ -415,7 +419,7 @@ static int wcc_fixpoint(scc_ctx* c, bool seed) {
   c->it_hint = std::min(64, c->h_dc->loop.iters + 1);
   if (seed) {
     const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
-    launch_ctl_init(c->stream, c->dc, 8, 0, 0, 0, avg / 14.0);
+    launch_ctl_init(c->stream, c->dc, 8, 0, 0, 0, avg / c->tune.bu_alpha);
     launch_fill_u32(c->stream, c->vf, 0u, ((int64_t)c->n + 31) / 32);
   }
   launch_wcc_color(c->stream, grid, c->dc, c->color, c->W, seed ? 1 : 0, c->begin, row_end(c), c->vf, c->q[0],
@@ -531,7 +535,7 @@ int scc_ctx_create(int device, uint64_t hbm_budget_bytes, scc_ctx** out) {
   c->h_ctr = c->h_dc->ctr;
   std::memset(c->h_dc, 0, sizeof(DevCtl));
   for (auto& e : c->ev) (void)hipEventCreate(&e);
-  launch_ctl_init(c->stream, c->dc, 1 | 2 | 4 | 8, 0, 0, 0, 0.0);
+  launch_ctl_init(c->stream, c->dc, 1 | 2 | 4 | 8 | 16, 0, 0, 0, 0.0);
   if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "init"))) {
     scc_ctx_destroy(c);
     return rc;

@@ -25,20 +25,20 @@ namespace sccmi {
 // return false.  Must be reached by all lanes of the wave.
 template <class Pred>
 __device__ __forceinline__ bool row_any(bool act, int32_t v, int32_t c, int32_t s, int32_t e,
-                                        const int32_t* __restrict__ nb, Pred pred) {
+                                        const int32_t* __restrict__ nb, Pred pred, int light) {
   bool found = false;
-  const int lim = act ? min(e - s, LIGHT) : 0;
+  const int lim = act ? min(e - s, light) : 0;
   const int mx = wave_max(lim);
   for (int j = 0; j < mx; j++) {
     if (j < lim && !found) found = pred(nb[s + j], v, c);
     if (__ballot((j + 1 < lim) && !found) == 0) break;
   }
-  uint64_t hm = __ballot(act && !found && (e - s > LIGHT));
+  uint64_t hm = __ballot(act && !found && (e - s > light));
   const int lane = lane_id();
   while (hm) {
     const int L = __ffsll((unsigned long long)hm) - 1;
     hm &= hm - 1;
-    const int32_t hs = __shfl(s, L) + LIGHT, he = __shfl(e, L);
+    const int32_t hs = __shfl(s, L) + light, he = __shfl(e, L);
     const int32_t hv = __shfl(v, L), hc = __shfl(c, L);
     bool f = false;
     for (int32_t b = hs; b < he; b += 256) {
@@ -97,6 +97,13 @@ __global__ void k_ctl_init(DevCtl* dc, int what, int32_t live_sel, unsigned long
     l.stop = stop;
     l.acc0 = l.acc1 = l.total0 = 0;
   }
+  if (what & 16) {
+    dc->tune.light = LIGHT;
+    dc->tune.huge_row = HUGE_ROW;
+    dc->tune.td_beta = 24.f;
+    dc->tune.bu_alpha = 14.f;
+    dc->tune.grid_cap = 2048;
+  }
   if (what & 8) {
     LevelCtl& v = dc->lv;
     v.done = 0;
@@ -154,14 +161,14 @@ __global__ __launch_bounds__(BLOCK) void k_trim1_step(DevCtl* __restrict__ dc, G
       s = G.begin[v];
       e = G.end[v];
     }
-    const bool out_ok = row_any(act, v, c, s, e, G.adj, live);
+    const bool out_ok = row_any(act, v, c, s, e, G.adj, live, dc->tune.light);
     const bool act2 = act && out_ok;
     int32_t rs = 0, re = 0;
     if (act2) {
       rs = G.rbegin[v];
       re = G.rend[v];
     }
-    const bool in_ok = row_any(act2, v, c, rs, re, G.radj, live);
+    const bool in_ok = row_any(act2, v, c, rs, re, G.radj, live, dc->tune.light);
     const bool trim = act && !in_ok;
     if (trim) {
       label[v] = v;
@@ -390,7 +397,7 @@ __global__ __launch_bounds__(BLOCK) void k_seed_pivots(DevCtl* __restrict__ dc, 
       }
     }
     // rows of at least HUGE_ROW entries go straight to the chunked expansion
-    const bool hug = piv && (offe[v] - off[v]) >= HUGE_ROW;
+    const bool hug = piv && (offe[v] - off[v]) >= dc->tune.huge_row;
     wave_append(piv && !hug, v, q0, &dc->lv.nq);
     wave_append(hug, v, h0, &dc->lv.nh);
   }
@@ -465,10 +472,12 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
                                                  int32_t* __restrict__ h0, int32_t* __restrict__ h1,
                                                  const int32_t* __restrict__ L0, const int32_t* __restrict__ L1,
                                                  int32_t max_chunks) {
-  __shared__ int s_done, s_mode, s_q, s_c, s_h;
+  __shared__ int s_done, s_mode, s_q, s_c, s_h, s_light, s_huge;
   __shared__ unsigned long long s_nq, s_nh, s_nc;
   LevelCtl* lv = &dc->lv;
   if (threadIdx.x == 0) {
+    s_light = dc->tune.light;
+    s_huge = dc->tune.huge_row;
     s_done = lv->done;
     s_mode = lv->mode;
     s_q = lv->q_sel;
@@ -531,10 +540,10 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
         c = color[u];
       }
       const int32_t d = e - s;
-      const bool hug = valid && d >= HUGE_ROW;
+      const bool hug = valid && d >= s_huge;
       wave_append(hug, u, h_out, &lv->out_h);
       const bool act = valid && !hug;
-      const int lim = act ? min(d, LIGHT) : 0;
+      const int lim = act ? min(d, s_light) : 0;
       const int mx = wave_max(lim);
       for (int j = 0; j < mx; j++) {
         int32_t k = 0;
@@ -546,11 +555,11 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
         }
         QOut::push(cl, k, q_out, &lv->out_q);
       }
-      uint64_t hm = __ballot(act && d > LIGHT);
+      uint64_t hm = __ballot(act && d > s_light);
       while (hm) {
         const int L = __ffsll((unsigned long long)hm) - 1;
         hm &= hm - 1;
-        const int32_t hs = __shfl(s, L) + LIGHT, he = __shfl(e, L), hc = __shfl(c, L);
+        const int32_t hs = __shfl(s, L) + s_light, he = __shfl(e, L), hc = __shfl(c, L);
         for (int32_t b = hs; b < he; b += 64) {
           const int32_t idx = b + lane;
           int32_t k = 0;
@@ -592,7 +601,7 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
           }
         }
       }
-      const bool found = row_any(act, v, c, s, e, rnb, reached);
+      const bool found = row_any(act, v, c, s, e, rnb, reached, s_light);
       if (found) {
         atomicOr(&V[v >> 5], 1u << (v & 31));
         edges += (unsigned long long)(offe[v] - off[v]);
@@ -632,7 +641,7 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
         lv->mode = MODE_BU;
         lv->nh = 0;
       }
-    } else if (oq < lv->prev_nq && oq * 24 < nlive) {
+    } else if (oq < lv->prev_nq && (double)oq * dc->tune.td_beta < (double)nlive) {
       lv->mode = MODE_TD;
     }
     lv->out_q = lv->out_c = lv->out_h = lv->out_edges = 0;

@@ -58,6 +58,8 @@ EXPORTS = [
     "scc_ctx_synchronize", "scc_ctx_set_budget", "scc_update_bytes_needed",
     "scc_device_memory", "scc_phase_trace", "scc_sched_set_weights", "scc_sched_get_weights",
     "scc_sched_forward", "scc_sched_gradients", "scc_sched_train", "scc_sched_rollout",
+    "scc_batch_set", "scc_batch_run", "scc_batch_counts", "scc_ctx_get_tuning", "scc_ctx_set_tuning",
+    "scc_autotune",
 ]
 
 
@@ -100,6 +102,14 @@ class RolloutStats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Tuning(C.Structure):
+    _fields_ = [("light_row", C.c_int32), ("huge_row", C.c_int32), ("grid_cap", C.c_int32),
+                ("pad", C.c_int32), ("bu_alpha", C.c_float), ("td_beta", C.c_float)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
 class UpdateStats(C.Structure):
@@ -157,6 +167,13 @@ def load_library():
         "scc_update_bytes_needed": (C.c_uint64, [i32, i64, i64, i64]),
         "scc_device_memory": (C.c_int, [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "scc_phase_trace": (C.c_int, [vp, vp, i64, C.POINTER(i64)]),
+        "scc_batch_set": (C.c_int, [vp, i32, vp]),
+        "scc_batch_run": (C.c_int, [vp, C.POINTER(RunOpts), C.POINTER(RunStats)]),
+        "scc_batch_counts": (C.c_int, [vp, vp]),
+        "scc_ctx_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
+        "scc_ctx_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
+        "scc_autotune": (C.c_int, [vp, C.POINTER(RunOpts), i32, C.POINTER(Tuning),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "scc_sched_set_weights": (C.c_int, [vp, vp]),
         "scc_sched_get_weights": (C.c_int, [vp, vp]),
         "scc_sched_forward": (C.c_int, [vp, vp, i64, vp]),
@@ -341,6 +358,50 @@ class SccEngine:
         self._check(self.lib.scc_update(self.h, _ptr(isrc), _ptr(idst), isrc.shape[0],
                                         _ptr(dsrc), _ptr(ddst), dsrc.shape[0], C.byref(st)))
         return st.as_dict()
+
+    # -- batched graphs and tuning
+    def batch_set(self, offsets) -> None:
+        if offsets is None:
+            self._check(self.lib.scc_batch_set(self.h, 0, None))
+            return
+        o = np.ascontiguousarray(offsets, dtype=np.int32)
+        self._check(self.lib.scc_batch_set(self.h, int(o.shape[0] - 1), o.ctypes.data))
+        self._batch_k = int(o.shape[0] - 1)
+
+    def batch_run(self, opts: RunOpts | None = None, **kw) -> dict:
+        o = opts if opts is not None else default_opts(**kw)
+        st = RunStats()
+        self._check(self.lib.scc_batch_run(self.h, C.byref(o), C.byref(st)))
+        return st.as_dict()
+
+    def batch_counts(self) -> np.ndarray:
+        out = np.empty(self._batch_k, np.int64)
+        self._check(self.lib.scc_batch_counts(self.h, out.ctypes.data))
+        return out
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        self._check(self.lib.scc_ctx_get_tuning(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **kw) -> None:
+        """Set knobs (light_row, huge_row, grid_cap, bu_alpha, td_beta); none = defaults."""
+        if not kw:
+            self._check(self.lib.scc_ctx_set_tuning(self.h, None))
+            return
+        t = Tuning()
+        self._check(self.lib.scc_ctx_get_tuning(self.h, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, v)
+        self._check(self.lib.scc_ctx_set_tuning(self.h, C.byref(t)))
+
+    def autotune(self, reps: int = 3, opts: RunOpts | None = None) -> dict:
+        t = Tuning()
+        b, d = C.c_double(), C.c_double()
+        o = opts if opts is not None else default_opts()
+        self._check(self.lib.scc_autotune(self.h, C.byref(o), int(reps), C.byref(t), C.byref(b),
+                                          C.byref(d)))
+        return {"tuning": t.as_dict(), "best_ms": b.value, "default_ms": d.value}
 
     # -- phase trace and learned scheduler
     def phase_trace(self) -> list:

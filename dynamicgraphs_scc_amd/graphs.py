@@ -220,6 +220,50 @@ def planted(n: int, seed: int = 3, giant_frac: float = 0.4, n_medium: int = 200_
     return (g, comp) if with_components else g
 
 
+def disjoint_union(gs) -> tuple:
+    """(union Graph, vertex offsets) of independent graphs: graph g keeps its
+    rows, its ids shifted by offsets[g]; no edge joins two graphs."""
+    ns = np.array([g.num_nodes for g in gs], np.int64)
+    ms = np.array([g.num_edges for g in gs], np.int64)
+    voff = np.concatenate([[0], np.cumsum(ns)])
+    eoff = np.concatenate([[0], np.cumsum(ms)])
+    if voff[-1] >= 2**31 or eoff[-1] >= 2**31:
+        raise ValueError("union exceeds int32 ids or offsets")
+    begin = np.empty(voff[-1] + 1, np.int32)
+    rbegin = np.empty(voff[-1] + 1, np.int32)
+    adj = np.empty(eoff[-1], np.int32)
+    radj = np.empty(eoff[-1], np.int32)
+    for i, g in enumerate(gs):
+        v0, v1, e0, e1 = voff[i], voff[i + 1], eoff[i], eoff[i + 1]
+        begin[v0:v1] = g.begin[:-1] + e0
+        rbegin[v0:v1] = g.r_begin[:-1] + e0
+        adj[e0:e1] = g.node_idx + v0
+        radj[e0:e1] = g.r_node_idx + v0
+    begin[-1] = rbegin[-1] = eoff[-1]
+    return Graph(begin, adj, rbegin, radj), voff.astype(np.int32)
+
+
+def batched_config(k: int = 512, seed: int = 4000, n_lo: int = 1_000, n_hi: int = 1_000_000):
+    """The batched workload: k graphs with vertex counts log-uniform in
+    [n_lo, n_hi]; even g is R-MAT (edge factor uniform in [8, 32], (a,b,c,d) =
+    (0.57, 0.19, 0.19, 0.05), 2^ceil(log2 N) ids generated and only edges
+    between the first N kept, ids permuted, duplicates and self-loops removed),
+    odd g is planted-SCC (40 % giant, 0.024 N medium SCCs of power-law sizes in
+    [3, 1000], singletons and 2-cycles, 4 inter-component edges per vertex).
+    Graph g is seeded with seed + g.  Returns the list of graphs."""
+    out = []
+    for g in range(k):
+        rng = np.random.default_rng(seed + g)
+        n = int(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi))))
+        if g % 2 == 0:
+            ef = float(rng.uniform(8, 32))
+            scale = max(1, int(np.ceil(np.log2(n))))
+            out.append(rmat(scale, ef, seed=seed + g, n_keep=n))
+        else:
+            out.append(planted(n, seed=seed + g, n_medium=max(1, int(round(0.024 * n)))))
+    return out
+
+
 def canonicalize(labels) -> np.ndarray:
     """Min-id canonical labels."""
     lib = _load()

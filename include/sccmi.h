@@ -297,6 +297,42 @@ int scc_update(scc_ctx* ctx, const int32_t* ins_src, const int32_t* ins_dst, int
  * reference counterpart: the reference rebuilds graphs on the host.) */
 uint64_t scc_update_bytes_needed(int32_t n, int64_t m, int64_t m_max, int64_t max_batch);
 
+/* ---- batched graphs -----------------------------------------------------------
+ * (No reference counterpart: the reference decomposes one graph per process.)
+ * Many independent graphs decomposed by the same launches: upload their
+ * disjoint union (graph g owns vertex ids [offsets[g], offsets[g+1]), no edge
+ * leaves its graph), name the boundaries, then run.  Each graph starts as its
+ * own partition, so the first forward-backward step takes one pivot per graph.
+ * Canonical labels are union ids (graph-local id = label - offsets[g]). */
+/* Set (num_graphs > 0) or clear (0) the batch; fails with SCC_ERR_ARG when an
+ * edge crosses between graphs.  A new upload clears it. */
+int scc_batch_set(scc_ctx* ctx, int32_t num_graphs, const int32_t* offsets);
+/* Fresh decomposition of every graph of the batch (method 2 from per-graph partitions). */
+int scc_batch_run(scc_ctx* ctx, const scc_run_opts* opts, scc_run_stats* stats);
+/* SCC count of each graph after a run (num_graphs entries). */
+int scc_batch_counts(scc_ctx* ctx, int64_t* counts);
+
+/* ---- launch-shape tuning --------------------------------------------------------
+ * (No reference counterpart: generalises the fixed switch points of
+ * gm_graph/inc/gm_bfs_template.h:424, :841-842.) */
+typedef struct scc_tuning {
+  int32_t light_row; /* a lane scans the first light_row entries of a row alone, the wave the rest (16) */
+  int32_t huge_row;  /* top-down frontier rows at least this long are split over the grid (8192) */
+  int32_t grid_cap;  /* workgroups per sweep launch (2048) */
+  int32_t pad;
+  float bu_alpha;    /* top-down -> bottom-up when frontier edges > candidates' edges / bu_alpha (14) */
+  float td_beta;     /* bottom-up -> top-down when frontier < live / td_beta and shrinking (24) */
+} scc_tuning;
+int scc_ctx_get_tuning(scc_ctx* ctx, scc_tuning* t);
+/* NULL restores the defaults. */
+int scc_ctx_set_tuning(scc_ctx* ctx, const scc_tuning* t);
+/* Coordinate descent over the knobs, timing full decompositions of the uploaded
+ * graph (or batch): median of reps runs per candidate; a change is kept when it
+ * gains more than 2 %.  Installs the best tuning, leaves the labels of a run
+ * with it, and reports the best and the default median wall times (ms). */
+int scc_autotune(scc_ctx* ctx, const scc_run_opts* opts, int32_t reps, scc_tuning* best,
+                 double* best_ms, double* default_ms);
+
 /* ---- timing helpers (HIP events on the context's stream) ------------------ */
 
 /* Record per-launch durations of the global-FB traversal kernels during the

@@ -46,8 +46,9 @@ def test_struct_layouts_match_header():
 #include <stdio.h>
 #include "sccmi.h"
 #include "sccserve.h"
-int main(){printf("%zu %zu %zu %zu %zu %zu\n", sizeof(scc_run_opts), sizeof(scc_run_stats),
-  sizeof(scc_update_stats), sizeof(scc_server_config), sizeof(scc_request_info), sizeof(scc_server_stats));}
+int main(){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(scc_run_opts), sizeof(scc_run_stats),
+  sizeof(scc_update_stats), sizeof(scc_server_config), sizeof(scc_request_info), sizeof(scc_server_stats),
+  sizeof(scc_phase_rec), sizeof(scc_rollout_stats), sizeof(scc_tuning));}
 """
     with tempfile.TemporaryDirectory() as td:
         c = os.path.join(td, "s.c")
@@ -58,7 +59,9 @@ int main(){printf("%zu %zu %zu %zu %zu %zu\n", sizeof(scc_run_opts), sizeof(scc_
         out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
     assert [int(x) for x in out] == [ctypes.sizeof(engine.RunOpts), ctypes.sizeof(engine.RunStats),
                                      ctypes.sizeof(engine.UpdateStats), ctypes.sizeof(serve.ServerConfig),
-                                     ctypes.sizeof(serve.RequestInfo), ctypes.sizeof(serve.ServerStats)]
+                                     ctypes.sizeof(serve.RequestInfo), ctypes.sizeof(serve.ServerStats),
+                                     ctypes.sizeof(engine.PhaseRec), ctypes.sizeof(engine.RolloutStats),
+                                     ctypes.sizeof(engine.Tuning)]
 
 
 def test_no_cpu_fallback_in_product_path():

@@ -186,3 +186,19 @@ def test_gm_binary_reader_rejects_malformed(tmp_path, case):
     open(p, "wb").write(data)
     with pytest.raises(RuntimeError):
         graphs.read_gm_binary(p)
+
+
+def test_disjoint_union_and_batched_config():
+    gs = graphs.batched_config(k=6, n_lo=200, n_hi=3000)
+    assert [g.num_nodes for g in gs] == [g.num_nodes for g in graphs.batched_config(k=6, n_lo=200, n_hi=3000)]
+    u, off = graphs.disjoint_union(gs)
+    assert off[0] == 0 and off[-1] == u.num_nodes and u.num_edges == sum(g.num_edges for g in gs)
+    for i, g in enumerate(gs):
+        src, dst = g.edges()
+        rows = slice(off[i], off[i + 1] + 1)
+        b = u.begin[rows] - u.begin[off[i]]
+        assert np.array_equal(b, g.begin)
+        assert np.array_equal(u.node_idx[u.begin[off[i]]:u.begin[off[i + 1]]], g.node_idx + off[i])
+        assert np.array_equal(u.r_begin[rows] - u.r_begin[off[i]], g.r_begin)
+    # R-MAT members keep exactly N vertices; planted members are planted partitions
+    assert all(gs[i].num_nodes >= 200 for i in range(6))
