@@ -148,6 +148,47 @@ def learned_leg(args, eng, g, ms_plain_rmat):
     return out
 
 
+def batched_leg(args):
+    """The batched workload (512 graphs, vertex counts log-uniform in [1e3, 1e6],
+    half R-MAT, half planted) as one disjoint union decomposed by shared
+    launches: default launch shape, then after the autotuner."""
+    import numpy as np
+    from dynamicgraphs_scc_amd import graphs
+    from dynamicgraphs_scc_amd.engine import SccEngine
+
+    t = time.time()
+    gs = graphs.batched_config()
+    u, off = graphs.disjoint_union(gs)
+    del gs
+    log(f"batched: {len(off) - 1} graphs, n={u.num_nodes} m={u.num_edges} in {time.time() - t:.1f}s")
+
+    def timed(e, reps):
+        ts = []
+        for _ in range(reps):
+            e.synchronize()
+            t0 = time.perf_counter()
+            e.batch_run()
+            e.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return float(np.median(ts))
+
+    with SccEngine(device=0) as e:
+        e.upload(u)
+        e.batch_set(off)
+        timed(e, 1)
+        ms_default = timed(e, 5)
+        tune = e.autotune(reps=2)
+        timed(e, 1)
+        ms_tuned = timed(e, 5)
+        counts = e.batch_counts()
+    return {"workload": "512 graphs, N log-uniform [1e3,1e6], even R-MAT EF U[8,32], odd planted, seeds 4000+g",
+            "graphs": int(len(off) - 1), "n": int(u.num_nodes), "m": int(u.num_edges),
+            "num_sccs": int(counts.sum()),
+            "ms_default": round(ms_default, 3), "gteps_default": round(u.num_edges / ms_default / 1e6, 4),
+            "ms_tuned": round(ms_tuned, 3), "gteps_tuned": round(u.num_edges / ms_tuned / 1e6, 4),
+            "tuning": tune["tuning"]}
+
+
 def _sleep_until(srv, t_ms):
     while True:
         d = t_ms - srv.now_ms()
@@ -253,6 +294,7 @@ def main():
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE runs)")
     ap.add_argument("--no-dyn", action="store_true", help="skip the dynamic serving leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-scheduler leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-graphs leg")
     ap.add_argument("--dyn-scale", type=int, default=23)
     ap.add_argument("--dyn-batches", type=int, default=2000)
     ap.add_argument("--dyn-check-every", type=int, default=100)
@@ -346,6 +388,12 @@ def main():
         cpu = cpu_baseline(g, cores, want_ref=not args.no_ref)
         log(f"cpu baseline {cpu['kind']}: {cpu['value']:.3f} GTEPS in {time.time() - t:.1f}s")
 
+    batched = None
+    if not args.no_batched:
+        t = time.time()
+        batched = batched_leg(args)
+        log(f"batched leg in {time.time() - t:.1f}s: {batched}")
+
     dyn = None
     if not args.no_dyn:
         t = time.time()
@@ -366,6 +414,7 @@ def main():
         "phases_ms": phases,
         "learned_gteps": learned["rmat22"]["learned_gteps"] if learned else None,
         "learned": learned,
+        "batched": batched,
         "dynamic": dyn,
     }
     print(json.dumps(out), flush=True)

@@ -92,7 +92,8 @@ int push_tuning(scc_ctx* c) {
 }
 
 bool tuning_ok(const scc_tuning* t) {
-  return t->light_row >= 1 && t->light_row <= 1024 && t->huge_row >= 256 && t->huge_row <= (1 << 30) &&
+  const int l = t->light_row;
+  return (l == 4 || l == 8 || l == 16 || l == 32 || l == 64) && t->huge_row >= 256 && t->huge_row <= (1 << 30) &&
          t->grid_cap >= 16 && t->grid_cap <= 65536 && t->bu_alpha > 0.f && t->bu_alpha <= 1e6f &&
          t->td_beta > 0.f && t->td_beta <= 1e6f;
 }

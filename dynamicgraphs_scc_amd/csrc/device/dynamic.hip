@@ -4,15 +4,21 @@
 // condensation built by src/common_main.h:180-314) and decremental
 // (src/scc_decremental.cc:7-24 with src/common_main.h:343-393) paths with one
 // exact call.  Let G' = G - D + I and C(v) the canonical SCC labels of G.
-//   X_d = union of old SCCs that lost an internal edge (only these can split;
-//         the reference's read_file2 marks the same SCCs, :361-370);
-//   S/T = heads/tails of inserted edges u->v with C(u) != C(v) or C(u) in X_d;
-//   X   = X_d  u  (FW_G'(S) n BW_G'(T)).
-// Every SCC of G' outside X is an SCC of G (a cycle through it must use an
-// inserted S/T edge), and X is a union of SCCs of G', so recomputing the SCCs
-// of G'[X] with the static pipeline and keeping C elsewhere gives the SCCs of G'.
-// BW(T) is searched only inside FW(S) (a path from w in FW(S) to T stays in
-// FW(S)), the same restriction the reference's backward search uses.
+// First P = FW_G'(p) n BW_G'(p) for the max in x out vertex p: one
+// forward-backward step over G'.  On graphs with a giant SCC, P is the new
+// giant (old giant minus what split off, plus every SCC merged into it), and
+// the rest of the work stays outside it.  Then, in G' - P,
+//   X_d = union of old SCCs that lost an internal edge, minus P (only these can
+//         split; the reference's read_file2 marks the same SCCs, :361-370);
+//   S/T = heads/tails of inserted edges u->v outside P with C(u) != C(v) or
+//         C(u) in X_d;
+//   X   = X_d  u  (FW(S) n BW(T)), both searches confined to G' - P.
+// Every SCC of G' outside P and X is an SCC of G (a cycle joining old SCCs must
+// use an inserted S/T edge, and one through P would put its vertices in P), and
+// X is a union of SCCs of G', so recomputing the SCCs of G'[X] with the static
+// pipeline, P's label from the first step, and C elsewhere gives the SCCs of
+// G'.  BW(T) is searched only inside FW(S) (a path from w in FW(S) to T stays
+// in FW(S)), the same restriction the reference's backward search uses.
 //
 // Graph storage under updates ("gapped" CSR, per direction): row u occupies
 // adj[begin[u] .. end[u]) inside its own room adj[begin[u] .. cap[u]); rooms
@@ -76,6 +82,7 @@ struct Dyn {
 };
 
 constexpr unsigned long long BAD_KEY = ~0ull;
+constexpr int32_t OUTSIDE = -3;  // region search: outside FW(S)
 
 __device__ __forceinline__ int32_t key_hi(unsigned long long k) { return (int32_t)(k >> 32); }
 __device__ __forceinline__ int32_t key_lo(unsigned long long k) { return (int32_t)(k & 0xFFFFFFFFull); }
@@ -404,10 +411,11 @@ __global__ __launch_bounds__(BLOCK) void k_mark_aff(const unsigned long long* __
   }
 }
 
-// Seeds S (heads) and T (tails) of inserts that can merge SCCs.
+// Seeds S (heads) and T (tails) of inserts that can merge SCCs outside P.
 __global__ __launch_bounds__(BLOCK) void k_ins_seeds(const unsigned long long* __restrict__ ins, int64_t ni,
                                                      const int32_t* __restrict__ canon,
                                                      const int32_t* __restrict__ aff,
+                                                     const int32_t* __restrict__ color,
                                                      int32_t* __restrict__ S, int32_t* __restrict__ T,
                                                      unsigned long long* __restrict__ ctr) {
   for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < ni; base += (int64_t)gridDim.x * BLOCK) {
@@ -418,14 +426,15 @@ __global__ __launch_bounds__(BLOCK) void k_ins_seeds(const unsigned long long* _
       u = key_hi(ins[i]);
       v = key_lo(ins[i]);
       const int32_t cu = canon[u];
-      use = cu != canon[v] || aff[cu] != 0;
+      use = color[u] != DONE && color[v] != DONE && (cu != canon[v] || aff[cu] != 0);
     }
     wave_append(use, v, S, &ctr[C_Q_OUT]);
     wave_append(use, u, T, &ctr[C_AUX_T]);
   }
 }
 
-// colour = 0 inside FW(S), -2 outside; the live list becomes FW(S) in live[sel].
+// Outside P: colour 0 inside FW(S), OUTSIDE elsewhere; the live list becomes
+// FW(S) in live[sel].
 __global__ __launch_bounds__(BLOCK) void k_restrict_fw(DevCtl* __restrict__ dc, int32_t n,
                                                        const uint32_t* __restrict__ vf,
                                                        int32_t* __restrict__ color, int32_t* __restrict__ live,
@@ -436,9 +445,9 @@ __global__ __launch_bounds__(BLOCK) void k_restrict_fw(DevCtl* __restrict__ dc, 
     Out::maybe_flush(live, &dc->ctr[C_LIST_OUT], BLOCK);
     const int64_t v = base + threadIdx.x;
     bool in = false;
-    if (v < n) {
+    if (v < n && color[v] != DONE) {
       in = bit_test(vf, (int32_t)v);
-      color[v] = in ? 0 : DONE;
+      color[v] = in ? 0 : OUTSIDE;
     }
     Out::push(in, (int32_t)v, live, &dc->ctr[C_LIST_OUT]);
   }
@@ -451,8 +460,8 @@ __global__ __launch_bounds__(BLOCK) void k_restrict_fw(DevCtl* __restrict__ dc, 
   }
 }
 
-// Region X: recompute state for members, keep the canonical label elsewhere;
-// the live list becomes X in live[sel].
+// Region X outside P: recompute state for members, keep the canonical label
+// elsewhere; the live list becomes X in live[sel].
 __global__ __launch_bounds__(BLOCK) void k_region(DevCtl* __restrict__ dc, int32_t n,
                                                   const int32_t* __restrict__ canon,
                                                   const int32_t* __restrict__ aff,
@@ -466,7 +475,7 @@ __global__ __launch_bounds__(BLOCK) void k_region(DevCtl* __restrict__ dc, int32
     Out::maybe_flush(live, &dc->ctr[C_LIST_OUT], BLOCK);
     const int64_t v = base + threadIdx.x;
     bool in = false;
-    if (v < n) {
+    if (v < n && color[v] != DONE) {  // P keeps its new label
       const int32_t cv = canon[v];
       in = aff[cv] != 0 || (have_fb && bit_test(vf, (int32_t)v) && bit_test(vb, (int32_t)v));
       color[v] = in ? UNSET : DONE;
@@ -481,6 +490,16 @@ __global__ __launch_bounds__(BLOCK) void k_region(DevCtl* __restrict__ dc, int32
     dc->ctr[C_LIST_OUT] = 0;
     dc->blocks_done = 0;
   }
+}
+
+// Live vertices (outside P) back to one domain.
+__global__ __launch_bounds__(BLOCK) void k_unset_live(const DevCtl* __restrict__ dc, int32_t* __restrict__ color,
+                                                      const int32_t* __restrict__ L0,
+                                                      const int32_t* __restrict__ L1) {
+  const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
+  const int64_t n_in = (int64_t)dc->live.n;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_in; i += (int64_t)gridDim.x * BLOCK)
+    color[list[i]] = UNSET;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_clear_aff(const unsigned long long* __restrict__ del, int64_t nd,
@@ -909,23 +928,35 @@ extern "C" int scc_update(scc_ctx* c, const int32_t* ins_src, const int32_t* ins
   if ((rc = hip_check(c, hipStreamSynchronize(s), "apply sync"))) return rc;
   st.ms_apply = ms_since(ta);
 
-  // ---- 3. affected region -----------------------------------------------------
+  // ---- 3. affected region ------------------------------------------------------
   auto tr = std::chrono::steady_clock::now();
   if (nd) LAUNCH(k_mark_aff, grid_for(nd), s, d->del_f, nd, c->canon, d->aff);
+  // P = SCC of the max in x out vertex of G', by one forward-backward step over
+  // all of G'.  On graphs with a giant SCC this is the new giant, merges into it
+  // included, and everything left to do lies outside it.
+  launch_fill_i32(s, c->color, UNSET, c->n);
+  launch_iota(s, c->live[0], c->n);
+  if ((rc = set_live(c, 0, c->n))) return rc;
+  c->colors_split = false;
+  c->multi_colors = false;
+  c->classes = false;
+  c->fresh = false;
+  int64_t psize = 0;
+  if ((rc = do_fb_round(c, SCC_PIVOT_DEGREE_PRODUCT, false, -1, &psize, nullptr, nullptr, false, nullptr)))
+    return rc;
+  st.giant = psize;
+  LAUNCH(k_unset_live, grid_for(std::max<int64_t>(1, c->n_live)), s, c->dc, c->color, c->live[0], c->live[1]);
   if ((rc = zero_ctr(c))) return rc;
   // S (heads) -> scratch_i32, T (tails) -> W; both idle until the recompute
   int32_t* S = c->scratch_i32;
   int32_t* T = c->W;
-  if (ni) LAUNCH(k_ins_seeds, grid_for(ni), s, d->ins_f, ni, c->canon, d->aff, S, T, c->ctr);
+  if (ni) LAUNCH(k_ins_seeds, grid_for(ni), s, d->ins_f, ni, c->canon, d->aff, c->color, S, T, c->ctr);
   if ((rc = pull_ctl(c))) return rc;
   const int64_t ns = (int64_t)c->h_ctr[C_Q_OUT];
   bool have_fb = false;
   const int64_t words = ((int64_t)c->n + 31) / 32;
-  if (ns > 0) {
-    // FW(S) over all of G'
-    launch_fill_i32(s, c->color, UNSET, c->n);
-    launch_iota(s, c->live[0], c->n);
-    if ((rc = set_live(c, 0, c->n))) return rc;
+  if (ns > 0 && c->n_live > 0) {
+    // FW(S) within G' - P
     launch_fill_u32(s, c->vf, 0u, words);
     launch_fill_u32(s, c->vb, 0u, words);
     if ((rc = traverse_from(c, true, false, S, c->ctr + C_Q_OUT, nullptr))) return rc;

@@ -303,7 +303,7 @@ int run_levels(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed) {
     for (int i = 0; i < batch; i++)
       launch_level(c->stream, grid, c->dc, multi ? 1 : 0, off, offe, nb, roff, roffe, rnb, c->color, V, c->q[0],
                    c->q[1], c->cand[0], c->cand[1], c->huge, c->huge2, c->live[0], c->live[1],
-                   max_chunks);
+                   max_chunks, c->tune.light);
     if (timed) (void)hipEventRecord(c->ev[1], c->stream);
     if ((rc = pull_ctl(c))) return rc;
     if (timed) {

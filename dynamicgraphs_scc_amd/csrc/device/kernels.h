@@ -55,7 +55,7 @@ void launch_level(hipStream_t st, unsigned grid, DevCtl* dc, int multi, const in
                   const int32_t* offe, const int32_t* nb, const int32_t* roff, const int32_t* roffe,
                   const int32_t* rnb, const int32_t* color,
                   uint32_t* V, int32_t* q0, int32_t* q1, int32_t* c0, int32_t* c1, int32_t* h0,
-                  int32_t* h1, const int32_t* L0, const int32_t* L1, int32_t max_chunks);
+                  int32_t* h1, const int32_t* L0, const int32_t* L1, int32_t max_chunks, int light);
 void launch_fb_stats(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, const uint32_t* vf,
                      const uint32_t* vb, const int32_t* L0, const int32_t* L1);
 void launch_classify(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, int32_t* color,

@@ -161,14 +161,14 @@ __global__ __launch_bounds__(BLOCK) void k_trim1_step(DevCtl* __restrict__ dc, G
       s = G.begin[v];
       e = G.end[v];
     }
-    const bool out_ok = row_any(act, v, c, s, e, G.adj, live, dc->tune.light);
+    const bool out_ok = row_any(act, v, c, s, e, G.adj, live, LIGHT);
     const bool act2 = act && out_ok;
     int32_t rs = 0, re = 0;
     if (act2) {
       rs = G.rbegin[v];
       re = G.rend[v];
     }
-    const bool in_ok = row_any(act2, v, c, rs, re, G.radj, live, dc->tune.light);
+    const bool in_ok = row_any(act2, v, c, rs, re, G.radj, live, LIGHT);
     const bool trim = act && !in_ok;
     if (trim) {
       label[v] = v;
@@ -463,6 +463,7 @@ constexpr int TD_CHUNK = 2048;
 // edges / 14, back to TD when the frontier shrinks below n/24) and ends the
 // search when a level admits nothing.  A switch to BU drops deferred rows:
 // their neighbours find them as parents.
+template <int s_light>
 __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int multi,
                                                  const int32_t* __restrict__ off, const int32_t* __restrict__ offe, const int32_t* __restrict__ nb,
                                                  const int32_t* __restrict__ roff, const int32_t* __restrict__ roffe, const int32_t* __restrict__ rnb,
@@ -472,11 +473,10 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
                                                  int32_t* __restrict__ h0, int32_t* __restrict__ h1,
                                                  const int32_t* __restrict__ L0, const int32_t* __restrict__ L1,
                                                  int32_t max_chunks) {
-  __shared__ int s_done, s_mode, s_q, s_c, s_h, s_light, s_huge;
+  __shared__ int s_done, s_mode, s_q, s_c, s_h, s_huge;
   __shared__ unsigned long long s_nq, s_nh, s_nc;
   LevelCtl* lv = &dc->lv;
   if (threadIdx.x == 0) {
-    s_light = dc->tune.light;
     s_huge = dc->tune.huge_row;
     s_done = lv->done;
     s_mode = lv->mode;
@@ -1159,9 +1159,18 @@ void launch_level(hipStream_t st, unsigned grid, DevCtl* dc, int multi, const in
                   const int32_t* offe, const int32_t* nb, const int32_t* roff, const int32_t* roffe,
                   const int32_t* rnb, const int32_t* color,
                   uint32_t* V, int32_t* q0, int32_t* q1, int32_t* c0, int32_t* c1, int32_t* h0,
-                  int32_t* h1, const int32_t* L0, const int32_t* L1, int32_t max_chunks) {
-  LAUNCH(k_level, grid, st, dc, multi, off, offe, nb, roff, roffe, rnb, color, V, q0, q1, c0, c1, h0, h1, L0, L1,
-         max_chunks);
+                  int32_t* h1, const int32_t* L0, const int32_t* L1, int32_t max_chunks, int light) {
+#define LEVEL(L)                                                                                         \
+  LAUNCH(k_level<L>, grid, st, dc, multi, off, offe, nb, roff, roffe, rnb, color, V, q0, q1, c0, c1, h0, h1, L0, \
+         L1, max_chunks)
+  switch (light) {
+    case 4: LEVEL(4); break;
+    case 8: LEVEL(8); break;
+    case 32: LEVEL(32); break;
+    case 64: LEVEL(64); break;
+    default: LEVEL(16); break;
+  }
+#undef LEVEL
 }
 void launch_fb_stats(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, const uint32_t* vf,
                      const uint32_t* vb, const int32_t* L0, const int32_t* L1) {

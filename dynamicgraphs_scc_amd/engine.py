@@ -115,7 +115,7 @@ class Tuning(C.Structure):
 class UpdateStats(C.Structure):
     _fields_ = [("inserted", C.c_int64), ("deleted", C.c_int64), ("affected", C.c_int64),
                 ("num_sccs", C.c_int64), ("ms_total", C.c_double), ("ms_apply", C.c_double),
-                ("ms_region", C.c_double), ("ms_recompute", C.c_double)]
+                ("ms_region", C.c_double), ("ms_recompute", C.c_double), ("giant", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

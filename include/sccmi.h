@@ -277,6 +277,7 @@ typedef struct scc_update_stats {
   double ms_apply;         /* CSR rebuild */
   double ms_region;        /* affected-region search */
   double ms_recompute;     /* SCC recompute inside the region */
+  int64_t giant;           /* size of P, the SCC of the max in x out vertex, found first */
 } scc_update_stats;
 
 /* Apply one batch of edge inserts and deletes to the device graph and bring
@@ -316,7 +317,8 @@ int scc_batch_counts(scc_ctx* ctx, int64_t* counts);
  * (No reference counterpart: generalises the fixed switch points of
  * gm_graph/inc/gm_bfs_template.h:424, :841-842.) */
 typedef struct scc_tuning {
-  int32_t light_row; /* a lane scans the first light_row entries of a row alone, the wave the rest (16) */
+  int32_t light_row; /* search levels: a lane scans the first light_row entries of a row alone, the
+                        wave the rest; one of 4, 8, 16, 32, 64 (16) */
   int32_t huge_row;  /* top-down frontier rows at least this long are split over the grid (8192) */
   int32_t grid_cap;  /* workgroups per sweep launch (2048) */
   int32_t pad;

@@ -75,7 +75,7 @@ def test_batch_rejects_cross_edges_and_bad_offsets():
         assert np.array_equal(e.labels(), oracle.scc_canonical(u, 8))
 
 
-@pytest.mark.parametrize("knobs", [dict(light_row=1, huge_row=256, grid_cap=16, bu_alpha=0.5, td_beta=1.0),
+@pytest.mark.parametrize("knobs", [dict(light_row=4, huge_row=256, grid_cap=16, bu_alpha=0.5, td_beta=1.0),
                                    dict(light_row=64, huge_row=65536, grid_cap=4096, bu_alpha=1000.0,
                                         td_beta=1000.0),
                                    dict(light_row=4, huge_row=2048, grid_cap=512, bu_alpha=4.0, td_beta=8.0)])
