@@ -105,6 +105,8 @@ struct DevCtl {
   int32_t pad;
   unsigned long long ctr[C_NUM];
   uint32_t done[9 * 16];  // last_block2 slots, 64 B apart
+  // per level of the last reachability search: mode, input size, admitted, their edges
+  unsigned long long lvlog[64][4];
 };
 
 // ---- wave64 primitives -----------------------------------------------------
@@ -384,6 +386,10 @@ struct scc_ctx {
   // leave these NULL; the update engine keeps rows with spare room behind them
   // (see dynamic.hip) and sets them.
   int32_t *end = nullptr, *rend = nullptr;
+  // Traversal copies: the same rows, each ordered by decreasing degree of the
+  // neighbour (ties by id), so scans that stop at the first live / reached
+  // neighbour meet hubs first.  NULL when absent (after updates).
+  int32_t *tadj = nullptr, *tradj = nullptr;
 
   // per-vertex state
   int32_t *color = nullptr, *label = nullptr, *canon = nullptr, *W = nullptr;
@@ -444,4 +450,6 @@ namespace sccmi {
 // Row-end arrays of the forward and reverse CSR.
 inline const int32_t* row_end(const scc_ctx* c) { return c->end ? c->end : c->begin + 1; }
 inline const int32_t* rrow_end(const scc_ctx* c) { return c->rend ? c->rend : c->rbegin + 1; }
+inline const int32_t* tr_adj(const scc_ctx* c) { return c->tadj ? c->tadj : c->adj; }
+inline const int32_t* tr_radj(const scc_ctx* c) { return c->tradj ? c->tradj : c->radj; }
 }  // namespace sccmi

@@ -863,6 +863,7 @@ extern "C" int scc_update(scc_ctx* c, const int32_t* ins_src, const int32_t* ins
   std::memset(&st, 0, sizeof(st));
   int rc;
   if ((rc = ensure_dyn(c, std::max(n_ins, n_del)))) return rc;
+  drop_order(c);  // the traversal copies describe the old rows
   Dyn* d = dyn_of(c);
   hipStream_t s = c->stream;
 

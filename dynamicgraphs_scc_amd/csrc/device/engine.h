@@ -19,6 +19,8 @@ int set_live(scc_ctx* c, int sel, int64_t n);
 int check_cancel(scc_ctx* c);
 int reset_state(scc_ctx* c);
 int alloc_graph(scc_ctx* c, int32_t n, int64_t m_cap);
+int build_order(scc_ctx* c);  // traversal copies of the rows, hubs first (order.hip)
+void drop_order(scc_ctx* c);
 
 // phases (operate on the live list of the control block)
 int do_trim_compact(scc_ctx* c);  // drop colour -2 entries from the live list

@@ -83,7 +83,7 @@ void dev_free(scc_ctx* c, void* p) {
 }
 
 GraphPtrs gp(const scc_ctx* c) {
-  return GraphPtrs{c->begin, row_end(c), c->adj, c->rbegin, rrow_end(c), c->radj, c->n, c->maxdeg};
+  return GraphPtrs{c->begin, row_end(c), tr_adj(c), c->rbegin, rrow_end(c), tr_radj(c), c->n, c->maxdeg};
 }
 
 // Copy the control block to the host and refresh the host copies of the live list.
@@ -152,11 +152,16 @@ uint64_t state_bytes(int32_t n) {
          + 4 * N              // scratch
          + 8 * N;             // packed WCC words
 }
+// Traversal copies (8 m) and, while they are built, sort keys and temp (~20 m).
+uint64_t order_bytes(int64_t m) {
+  return 28ull * (uint64_t)(m > 0 ? m : 1) + (1ull << 20);
+}
 
 int free_graph(scc_ctx* c) {
   dynamic_graph_replaced(c);
   dev_free(c, c->cw);
   c->cw = nullptr;
+  drop_order(c);
   dev_free(c, c->d_boff);
   c->d_boff = nullptr;
   c->batch_k = 0;
@@ -289,8 +294,8 @@ int do_trim2(scc_ctx* c, int64_t* removed) {
 
 // Run the level launches of one search whose frontier has been seeded.
 int run_levels(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed) {
-  const int32_t *off = fwd ? c->begin : c->rbegin, *nb = fwd ? c->adj : c->radj;
-  const int32_t *roff = fwd ? c->rbegin : c->begin, *rnb = fwd ? c->radj : c->adj;
+  const int32_t *off = fwd ? c->begin : c->rbegin, *nb = fwd ? tr_adj(c) : tr_radj(c);
+  const int32_t *roff = fwd ? c->rbegin : c->begin, *rnb = fwd ? tr_radj(c) : tr_adj(c);
   const int32_t *offe = fwd ? row_end(c) : rrow_end(c), *roffe = fwd ? rrow_end(c) : row_end(c);
   uint32_t* V = fwd ? c->vf : c->vb;
   const unsigned grid = grid_for(c->n_live, (unsigned)c->tune.grid_cap);
@@ -564,7 +569,8 @@ int scc_ctx_destroy(scc_ctx* c) {
 const char* scc_last_error(const scc_ctx* c) { return c ? c->err.c_str() : "null context"; }
 uint64_t scc_ctx_bytes(const scc_ctx* c) { return c ? c->bytes : 0; }
 uint64_t scc_bytes_needed(int32_t n, int64_t m) {
-  return 8ull * (uint64_t)(m > 0 ? m : 1) + 8ull * ((uint64_t)n + 1) + state_bytes(n) + sizeof(DevCtl);
+  return 8ull * (uint64_t)(m > 0 ? m : 1) + 8ull * ((uint64_t)n + 1) + state_bytes(n) + order_bytes(m) +
+         4ull * ((uint64_t)n + 1) + sizeof(DevCtl);
 }
 int scc_device_memory(int device, uint64_t* free_bytes, uint64_t* total_bytes) {
   if (!free_bytes || !total_bytes) return SCC_ERR_ARG;
@@ -635,6 +641,7 @@ int scc_graph_upload(scc_ctx* c, int32_t n, int64_t m, const int32_t* begin, con
       (m > 0 && (rc = hip_check(c, hipMemcpyAsync(c->adj, node_idx, 4ull * m, hipMemcpyHostToDevice, c->stream), "upload"))) ||
       (m > 0 && (rc = hip_check(c, hipMemcpyAsync(c->radj, r_node_idx, 4ull * m, hipMemcpyHostToDevice, c->stream), "upload"))))
     return rc;
+  if ((rc = build_order(c))) return rc;
   if ((rc = reset_state(c))) return rc;
   c->ran = false;
   return hip_check(c, hipStreamSynchronize(c->stream), "upload sync");
@@ -664,6 +671,7 @@ int scc_graph_upload_device(scc_ctx* c, int32_t n, int64_t m, const int32_t* d_b
       return rc;
     c->maxdeg = max_degree(n, b.data(), rb.data());
   }
+  if ((rc = build_order(c))) return rc;
   if ((rc = reset_state(c))) return rc;
   c->ran = false;
   return hip_check(c, hipStreamSynchronize(c->stream), "upload sync");
@@ -910,6 +918,18 @@ int scc_labels_device(scc_ctx* c, const int32_t** d_labels) {
 int scc_ctx_set_kernel_timing(scc_ctx* c, int enable) {
   if (!c) return SCC_ERR_ARG;
   c->kernel_timing = enable != 0;
+  return SCC_OK;
+}
+
+int scc_level_log(scc_ctx* c, int64_t* out, int32_t* levels) {
+  if (!c || !out || !levels) return SCC_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  int rc;
+  if ((rc = pull_ctl(c))) return rc;
+  const int32_t k = std::min(64, c->h_dc->lv.levels);
+  for (int32_t i = 0; i < k; i++)
+    for (int j = 0; j < 4; j++) out[4 * i + j] = (int64_t)c->h_dc->lvlog[i][j];
+  *levels = k;
   return SCC_OK;
 }
 

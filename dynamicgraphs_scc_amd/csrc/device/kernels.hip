@@ -592,20 +592,17 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
       bool act = false;
       if (valid) {
         v = c_in[i];
-        if (!bit_test(V, v)) {
-          c = color[v];
-          act = c != DONE;
-          if (act) {
-            s = roff[v];
-            e = roffe[v];
-          }
-        }
+        // independent loads: visited bit, colour (several partitions only), row bounds
+        const bool vis = bit_test(V, v);
+        c = multi ? color[v] : UNSET;
+        s = roff[v];
+        e = roffe[v];
+        act = !vis && c != DONE;
       }
       const bool found = row_any(act, v, c, s, e, rnb, reached, s_light);
-      if (found) {
-        atomicOr(&V[v >> 5], 1u << (v & 31));
-        edges += (unsigned long long)(offe[v] - off[v]);
-      }
+      // (no edge count: after a bottom-up level only the admitted count decides
+      // the next direction)
+      if (found) atomicOr(&V[v >> 5], 1u << (v & 31));
       QOut::push(found, v, q_out, &lv->out_q);
       COut::push(act && !found, v, c_out, &lv->out_c);
     }
@@ -617,6 +614,14 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
     const unsigned long long oq = ld_acq(&lv->out_q), oc = ld_acq(&lv->out_c);
     const unsigned long long oh = ld_acq(&lv->out_h), oe = ld_acq(&lv->out_edges);
     const unsigned long long nlive = dc->live.n;
+    if (lv->levels < 64) {
+      unsigned long long* lg = dc->lvlog[lv->levels];
+      lg[0] = (unsigned long long)s_mode;
+      lg[1] = s_mode == MODE_TD ? (unsigned long long)s_nq + ((unsigned long long)s_nh << 32)
+                                : (s_c < 0 ? nlive : (unsigned long long)s_nc);
+      lg[2] = oq;
+      lg[3] = oe;
+    }
     lv->visited += oq;
     lv->levels++;
     if (s_mode == MODE_TD) {

@@ -59,7 +59,7 @@ EXPORTS = [
     "scc_device_memory", "scc_phase_trace", "scc_sched_set_weights", "scc_sched_get_weights",
     "scc_sched_forward", "scc_sched_gradients", "scc_sched_train", "scc_sched_rollout",
     "scc_batch_set", "scc_batch_run", "scc_batch_counts", "scc_ctx_get_tuning", "scc_ctx_set_tuning",
-    "scc_autotune",
+    "scc_autotune", "scc_level_log",
 ]
 
 
@@ -168,6 +168,7 @@ def load_library():
         "scc_device_memory": (C.c_int, [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "scc_phase_trace": (C.c_int, [vp, vp, i64, C.POINTER(i64)]),
         "scc_batch_set": (C.c_int, [vp, i32, vp]),
+        "scc_level_log": (C.c_int, [vp, vp, C.POINTER(i32)]),
         "scc_batch_run": (C.c_int, [vp, C.POINTER(RunOpts), C.POINTER(RunStats)]),
         "scc_batch_counts": (C.c_int, [vp, vp]),
         "scc_ctx_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
@@ -358,6 +359,15 @@ class SccEngine:
         self._check(self.lib.scc_update(self.h, _ptr(isrc), _ptr(idst), isrc.shape[0],
                                         _ptr(dsrc), _ptr(ddst), dsrc.shape[0], C.byref(st)))
         return st.as_dict()
+
+    def level_log(self) -> list:
+        """Per level of the last reachability search: mode, input, admitted, edges."""
+        out = np.zeros(256, np.int64)
+        k = C.c_int32()
+        self._check(self.lib.scc_level_log(self.h, out.ctypes.data, C.byref(k)))
+        return [{"mode": "BU" if out[4 * i] else "TD", "input": int(out[4 * i + 1] & 0xFFFFFFFF),
+                 "split_rows": int(out[4 * i + 1] >> 32) if out[4 * i] == 0 else 0,
+                 "admitted": int(out[4 * i + 2]), "edges": int(out[4 * i + 3])} for i in range(k.value)]
 
     # -- batched graphs and tuning
     def batch_set(self, offsets) -> None:

@@ -335,6 +335,12 @@ int scc_ctx_set_tuning(scc_ctx* ctx, const scc_tuning* t);
 int scc_autotune(scc_ctx* ctx, const scc_run_opts* opts, int32_t reps, scc_tuning* best,
                  double* best_ms, double* default_ms);
 
+/* Per-level record of the last reachability search the context ran (up to 64
+ * levels): out[4*i..] = {mode (0 top-down, 1 bottom-up), input size (top-down:
+ * frontier + (split rows << 32); bottom-up: candidates), vertices admitted,
+ * out-degree sum of the admitted}.  *levels = levels recorded.  For analysis. */
+int scc_level_log(scc_ctx* ctx, int64_t* out, int32_t* levels);
+
 /* ---- timing helpers (HIP events on the context's stream) ------------------ */
 
 /* Record per-launch durations of the global-FB traversal kernels during the
