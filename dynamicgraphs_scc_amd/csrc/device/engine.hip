@@ -783,7 +783,7 @@ void scc_run_opts_default(scc_run_opts* o) {
   if (!o) return;
   std::memset(o, 0, sizeof(*o));
   o->pivot_policy = SCC_PIVOT_DEGREE_PRODUCT;
-  o->trim_stop = 0;
+  o->trim_stop = -2;
   o->use_trim2 = 1;
   o->trim_between = 0;
   o->use_wcc = 1;

@@ -184,7 +184,12 @@ __global__ __launch_bounds__(BLOCK) void k_trim1_step(DevCtl* __restrict__ dc, G
     const unsigned long long t = ld_acq(&l.acc0), out = ld_acq(&l.acc1);
     l.total0 += t;
     l.iters++;
-    l.done = ((long long)t <= l.stop) || out == 0;
+    // stop >= 0: a sweep trimming at most `stop` vertices ends the loop (the
+    // reference's TRIM_STOP); stop < 0: one trimming at most -stop percent of
+    // the vertices it examined does
+    const bool small = l.stop >= 0 ? (long long)t <= l.stop
+                                   : t * 100ull <= (unsigned long long)(-l.stop) * (t + out);
+    l.done = small || out == 0;
     l.acc0 = l.acc1 = 0;
     l.blocks_done = 0;
     dc->live.sel ^= 1;
@@ -443,6 +448,23 @@ __global__ __launch_bounds__(BLOCK) void k_seed_list(DevCtl* __restrict__ dc, co
 // :123-127; during a search colours do not change, so "same colour" is the
 // partition).  Visited state is a bitmap, claimed with atomicOr.
 
+// Set bit v of V for every lane with flag, one atomicOr per distinct word in
+// the wave (lanes hold nearby ids, so a wave touches few words).  All lanes.
+__device__ __forceinline__ void set_bits_wave(bool flag, int32_t v, uint32_t* __restrict__ V) {
+  uint64_t m = __ballot(flag);
+  const int lane = lane_id();
+  while (m) {
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    const int32_t w = __shfl(v >> 5, leader);
+    const bool mine = flag && (v >> 5) == w;
+    uint32_t bits = mine ? (1u << (v & 31)) : 0u;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) bits |= __shfl_xor(bits, o);
+    if (lane == leader) atomicOr(&V[w], bits);
+    m &= ~__ballot(mine);
+  }
+}
+
 __device__ __forceinline__ bool try_claim(uint32_t* __restrict__ V, const int32_t* __restrict__ color,
                                           int32_t k, int32_t c) {
   const uint32_t bit = 1u << (k & 31);
@@ -583,6 +605,8 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
       if (!((V[k >> 5] >> (k & 31)) & 1u)) return false;
       return !multi || color[k] == pc;
     };
+    __shared__ int32_t s_qv[BLOCK], s_qs[BLOCK], s_qe[BLOCK], s_qc[BLOCK], s_qhit[BLOCK];
+    __shared__ int s_bq;
     for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < ncin; base += (int64_t)gridDim.x * BLOCK) {
       QOut::maybe_flush(q_out, &lv->out_q, BLOCK);
       COut::maybe_flush(c_out, &lv->out_c, BLOCK);
@@ -599,10 +623,52 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
         e = roffe[v];
         act = !vis && c != DONE;
       }
-      const bool found = row_any(act, v, c, s, e, rnb, reached, s_light);
+      // phase 1: each lane tests the first s_light entries of its row
+      bool found = false;
+      {
+        const int lim = act ? min(e - s, s_light) : 0;
+        const int mx = wave_max(lim);
+        for (int j = 0; j < mx; j++) {
+          if (j < lim && !found) found = reached(rnb[s + j], v, c);
+          if (__ballot((j + 1 < lim) && !found) == 0) break;
+        }
+      }
+      // phase 2: rows still open go to a workgroup queue and are scanned by
+      // 16-lane groups, so one long row no longer holds up its wave's other lanes
+      if (threadIdx.x == 0) s_bq = 0;
+      __syncthreads();
+      int slot = -1;
+      if (act && !found && e - s > s_light) {
+        slot = atomicAdd(&s_bq, 1);
+        s_qv[slot] = v;
+        s_qs[slot] = s + s_light;
+        s_qe[slot] = e;
+        s_qc[slot] = c;
+        s_qhit[slot] = 0;
+      }
+      __syncthreads();
+      {
+        const int nbq = s_bq;
+        const int gl = threadIdx.x & 15, gsh = lane & 48;
+        for (int j = threadIdx.x >> 4; j < nbq; j += BLOCK / 16) {
+          const int32_t rs = s_qs[j], re = s_qe[j], rv = s_qv[j], rc = s_qc[j];
+          bool hit = false;
+          for (int32_t b = rs; b < re; b += 16) {
+            const int32_t idx = b + gl;
+            const bool h = idx < re && reached(rnb[idx], rv, rc);
+            if ((__ballot(h) >> gsh) & 0xFFFFull) {
+              hit = true;
+              break;
+            }
+          }
+          if (gl == 0 && hit) s_qhit[j] = 1;
+        }
+      }
+      __syncthreads();
+      if (slot >= 0) found = s_qhit[slot] != 0;
       // (no edge count: after a bottom-up level only the admitted count decides
       // the next direction)
-      if (found) atomicOr(&V[v >> 5], 1u << (v & 31));
+      set_bits_wave(found, v, V);
       QOut::push(found, v, q_out, &lv->out_q);
       COut::push(act && !found, v, c_out, &lv->out_c);
     }

@@ -125,7 +125,9 @@ int scc_partitions_resolve(scc_ctx* ctx, int policy, int mode, int trim_between,
 
 typedef struct scc_run_opts {
   int32_t pivot_policy;    /* scc_pivot_policy for every FB step */
-  int32_t trim_stop;       /* trim1 stop threshold (0 = fixpoint; reference CPU uses 100) */
+  int32_t trim_stop;       /* trim1 ends after a sweep trimming <= trim_stop vertices (0 = fixpoint;
+                              the reference CPU uses 100), or, when negative, <= -trim_stop
+                              percent of the vertices the sweep examined */
   int32_t use_trim2;       /* reference USE_TRIM2 (src/scc_main.cc:46) */
   int32_t trim_between;    /* trim1 between partition FB rounds */
   int32_t use_wcc;         /* WCC colouring before the partition rounds */
@@ -164,7 +166,7 @@ typedef struct scc_run_stats {
   int64_t fb_f, fb_b, fb_ef, fb_eb, fb_fb;
 } scc_run_stats;
 
-/* Default options: degree-product pivot, trim_stop 0, trim2 on, no trim between
+/* Default options: degree-product pivot, trim_stop -2 (2 %), trim2 on, no trim between
  * partition rounds (measured faster on planted graphs), WCC on, no phase
  * timing, SCC_PARTS_BOTH. */
 void scc_run_opts_default(scc_run_opts* o);
