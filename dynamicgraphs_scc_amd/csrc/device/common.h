@@ -18,6 +18,8 @@ constexpr int BLOCK = 256;     // threads per workgroup (4 wave64s)
 constexpr int WAVES = BLOCK / 64;
 constexpr int LIGHT = 16;      // default: row entries a lane scans alone before the wave helps
 constexpr int HUGE_ROW = 8192; // default: rows at least this long are expanded by many blocks
+constexpr int GRID_CAP = 1024; // default: workgroups of a sweep launch (4 per CU; more only adds
+                               // per-workgroup set-up and flushes to the short levels)
 
 constexpr int32_t WCC_TAG = 0x20000000, WCC_MASK = 0x1FFFFFFF;  // src/scc_weak.cc:57-62
 
@@ -107,6 +109,19 @@ struct DevCtl {
   uint32_t done[9 * 16];  // last_block2 slots, 64 B apart
   // per level of the last reachability search: mode, input size, admitted, their edges
   unsigned long long lvlog[64][4];
+  // the backward search when both directions run in the same launches
+  LevelCtl lv2;
+  uint32_t done2[9 * 16];
+  unsigned long long lvlog2[64][4];
+};
+
+// The arrays one reachability direction reads and writes: rows to expand
+// (off/offe/nb), the opposite rows a bottom-up level scans (roff/roffe/rnb), the
+// visited bitmap, and ping-pong frontier, candidate and deferred-row lists.
+struct LevelDir {
+  const int32_t *off, *offe, *nb, *roff, *roffe, *rnb;
+  uint32_t* V;
+  int32_t *q0, *q1, *c0, *c1, *h0, *h1;
 };
 
 // ---- wave64 primitives -----------------------------------------------------
@@ -263,12 +278,13 @@ struct LdsOut {
 // (blockIdx % 8) count on their own slot, the last of each slot on the top
 // slot, so no address sees more than grid/8 atomics.  Counters reset
 // themselves.  True in every thread of the workgroup that finishes last.
-__device__ __forceinline__ bool last_block2(uint32_t* ctr) {
+// bid/nb: the workgroup's index and count within its share of the grid (a
+// launch that runs two searches gives each half of it).
+__device__ __forceinline__ bool last_block2(uint32_t* ctr, unsigned bid, unsigned nb) {
   __shared__ int s_last;
   __syncthreads();
   if (threadIdx.x == 0) {
-    const unsigned nb = gridDim.x;
-    const unsigned x = blockIdx.x & 7u;
+    const unsigned x = bid & 7u;
     const unsigned in_slot = (nb > x) ? (nb - x + 7u) / 8u : 0u;
     int last = 0;
     const unsigned prev = atomicAdd(&ctr[(1 + x) * 16], 1u);
@@ -286,6 +302,7 @@ __device__ __forceinline__ bool last_block2(uint32_t* ctr) {
   __syncthreads();
   return s_last;
 }
+__device__ __forceinline__ bool last_block2(uint32_t* ctr) { return last_block2(ctr, blockIdx.x, gridDim.x); }
 
 // atomicMax of `val` into target[slot] for every lane with act, after the lanes
 // of the wave that share a slot have reduced among themselves (one atomic per
@@ -394,7 +411,7 @@ struct scc_ctx {
   // per-vertex state
   int32_t *color = nullptr, *label = nullptr, *canon = nullptr, *W = nullptr;
   unsigned long long* cw = nullptr;  // WCC iterations: colour << 32 | label
-  sccmi::Tune tune = {sccmi::LIGHT, sccmi::HUGE_ROW, 24.f, 14.f, 2048, 0};  // host copy of dc->tune
+  sccmi::Tune tune = {sccmi::LIGHT, sccmi::HUGE_ROW, 24.f, 14.f, sccmi::GRID_CAP, 0};  // host copy of dc->tune
 
   // batched graphs: vertex ranges [boff[g], boff[g+1]) (scc_batch_set)
   int32_t batch_k = 0;
@@ -406,6 +423,11 @@ struct scc_ctx {
   int32_t* q[2] = {nullptr, nullptr};
   int32_t* cand[2] = {nullptr, nullptr};
   int32_t *huge = nullptr, *huge2 = nullptr;  // deferred huge-row frontier vertices (ping-pong)
+  // the backward search's lists when both directions run together
+  int32_t* bq[2] = {nullptr, nullptr};
+  int32_t* bcand[2] = {nullptr, nullptr};
+  int32_t* bhuge[2] = {nullptr, nullptr};
+  int32_t last_dirs = 1;                  // last search: 1 forward, -1 backward, 2 both (level log)
   uint32_t *vf = nullptr, *vb = nullptr;  // reach bitmaps
   unsigned long long* pkey = nullptr;     // per-slot pivot key (n+1)
   int32_t* rep = nullptr;                 // per-slot new-partition representatives 3*(n+1)
@@ -418,7 +440,7 @@ struct scc_ctx {
   int32_t update_policy = 0;              // pivot policy of dynamic recomputes
   bool colors_split = false;              // colours other than -1/-2 exist (after FB/WCC/set)
   bool fresh = false;                     // state just reset: all colours -1, nothing done
-  int32_t lv_hint[2] = {8, 8};            // level launches per search batch (fw, bw)
+  int32_t lv_hint[3] = {8, 8, 8};         // level launches per search batch (fw, bw, both)
   int32_t it_hint = 4;                    // WCC iterations per batch
   bool labels_ready = false;              // canon[] valid for the current graph
   bool ran = false;

@@ -5,6 +5,7 @@
 //   begin[n+1] adj[m] rbegin[n+1] radj[m]            graph (CSR + reverse CSR)
 //   color[n] label[n] canon[n] W[n]                  per-vertex state
 //   live[2][n] q[2][n] cand[2][n] huge[2][n]         ping-pong vertex lists
+//   bq[2][n] bcand[2][n] bhuge[2][n]                 the backward search's lists
 //   vf[n/32] vb[n/32]                                forward / backward reach bitmaps
 //   pkey[n+1] (u64), rep[3(n+1)], scratch[n+1]       per-partition slots
 //   DevCtl                                           device-resident loop control
@@ -147,6 +148,7 @@ uint64_t state_bytes(int32_t n) {
   const uint64_t words = ((uint64_t)n + 31) / 32 + 1;
   return 4 * N * 4            // color label canon W
          + 8 * N * 4          // live[2] q[2] cand[2] huge[2]
+         + 6 * N * 4          // bq[2] bcand[2] bhuge[2]
          + 2 * words * 4      // vf vb
          + 8 * N + 12 * N     // pkey rep
          + 4 * N              // scratch
@@ -168,7 +170,8 @@ int free_graph(scc_ctx* c) {
   c->h_boff.clear();
   int32_t** arrs[] = {&c->begin, &c->adj, &c->rbegin, &c->radj, &c->end, &c->rend, &c->color, &c->label, &c->canon,
                       &c->W, &c->live[0], &c->live[1], &c->q[0], &c->q[1], &c->cand[0],
-                      &c->cand[1], &c->huge, &c->huge2, &c->rep, &c->scratch_i32};
+                      &c->cand[1], &c->huge, &c->huge2, &c->bq[0], &c->bq[1], &c->bcand[0], &c->bcand[1],
+                      &c->bhuge[0], &c->bhuge[1], &c->rep, &c->scratch_i32};
   for (auto p : arrs) {
     dev_free(c, *p);
     *p = nullptr;
@@ -209,6 +212,12 @@ int alloc_graph(scc_ctx* c, int32_t n, int64_t m_cap) {
   ALLOC(c->cand[1], 4 * N);
   ALLOC(c->huge, 4 * N);
   ALLOC(c->huge2, 4 * N);
+  ALLOC(c->bq[0], 4 * N);
+  ALLOC(c->bq[1], 4 * N);
+  ALLOC(c->bcand[0], 4 * N);
+  ALLOC(c->bcand[1], 4 * N);
+  ALLOC(c->bhuge[0], 4 * N);
+  ALLOC(c->bhuge[1], 4 * N);
   ALLOC(c->vf, 4 * words);
   ALLOC(c->vb, 4 * words);
   ALLOC(c->pkey, 8 * N);
@@ -292,23 +301,47 @@ int do_trim2(scc_ctx* c, int64_t* removed) {
   return SCC_OK;
 }
 
-// Run the level launches of one search whose frontier has been seeded.
-int run_levels(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed) {
-  const int32_t *off = fwd ? c->begin : c->rbegin, *nb = fwd ? tr_adj(c) : tr_radj(c);
-  const int32_t *roff = fwd ? c->rbegin : c->begin, *rnb = fwd ? tr_radj(c) : tr_adj(c);
-  const int32_t *offe = fwd ? row_end(c) : rrow_end(c), *roffe = fwd ? rrow_end(c) : row_end(c);
-  uint32_t* V = fwd ? c->vf : c->vb;
+// The arrays of the forward (fwd) or backward search; with own_lists the
+// backward search uses its own lists so that it can run alongside the forward one.
+static LevelDir level_dir(scc_ctx* c, bool fwd, bool own_lists) {
+  LevelDir d;
+  d.off = fwd ? c->begin : c->rbegin;
+  d.offe = fwd ? row_end(c) : rrow_end(c);
+  d.nb = fwd ? tr_adj(c) : tr_radj(c);
+  d.roff = fwd ? c->rbegin : c->begin;
+  d.roffe = fwd ? rrow_end(c) : row_end(c);
+  d.rnb = fwd ? tr_radj(c) : tr_adj(c);
+  d.V = fwd ? c->vf : c->vb;
+  const bool b = !fwd && own_lists;
+  d.q0 = b ? c->bq[0] : c->q[0];
+  d.q1 = b ? c->bq[1] : c->q[1];
+  d.c0 = b ? c->bcand[0] : c->cand[0];
+  d.c1 = b ? c->bcand[1] : c->cand[1];
+  d.h0 = b ? c->bhuge[0] : c->huge;
+  d.h1 = b ? c->bhuge[1] : c->huge2;
+  return d;
+}
+
+// Queue level launches in batches until the search (both == false: the one of
+// dc->lv; both: the two of dc->lv and dc->lv2) has finished.  hint holds the
+// launches the last such search needed.
+static int level_loop(scc_ctx* c, bool both, bool fwd, bool multi, int64_t* levels, bool timed) {
   const unsigned grid = grid_for(c->n_live, (unsigned)c->tune.grid_cap);
   const int32_t max_chunks = std::max<int32_t>(1, (c->maxdeg + 2047) / 2048);
-  int& hint = c->lv_hint[fwd ? 0 : 1];
+  const LevelDir F = level_dir(c, fwd || both, both), B = level_dir(c, false, true);
+  int& hint = c->lv_hint[both ? 2 : (fwd ? 0 : 1)];
   int batch = std::max(2, hint), rc;
   for (int guard = 0;; guard++) {
     if ((rc = check_cancel(c))) return rc;
     if (timed) (void)hipEventRecord(c->ev[0], c->stream);
-    for (int i = 0; i < batch; i++)
-      launch_level(c->stream, grid, c->dc, multi ? 1 : 0, off, offe, nb, roff, roffe, rnb, c->color, V, c->q[0],
-                   c->q[1], c->cand[0], c->cand[1], c->huge, c->huge2, c->live[0], c->live[1],
-                   max_chunks, c->tune.light);
+    for (int i = 0; i < batch; i++) {
+      if (both)
+        launch_level2(c->stream, 2 * grid, c->dc, multi ? 1 : 0, F, B, c->color, c->live[0], c->live[1],
+                      max_chunks, c->tune.light);
+      else
+        launch_level(c->stream, grid, c->dc, multi ? 1 : 0, F, c->color, c->live[0], c->live[1], max_chunks,
+                     c->tune.light);
+    }
     if (timed) (void)hipEventRecord(c->ev[1], c->stream);
     if ((rc = pull_ctl(c))) return rc;
     if (timed) {
@@ -317,26 +350,32 @@ int run_levels(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed) {
       c->kernel_ms += ms;
       c->kernel_launches += batch;
     }
-    if (c->h_dc->lv.done) break;
+    if (c->h_dc->lv.done && (!both || c->h_dc->lv2.done)) break;
     batch = std::max(4, batch);
     if (guard > 4 * (int64_t)c->n + 64) return set_error(c, SCC_ERR_LIMIT, "search did not converge");
   }
-  const int used = c->h_dc->lv.levels;
-  if (levels) *levels += used;
+  const int used = both ? std::max(c->h_dc->lv.levels, c->h_dc->lv2.levels) : c->h_dc->lv.levels;
+  if (levels) *levels += c->h_dc->lv.levels + (both ? c->h_dc->lv2.levels : 0);
+  c->last_dirs = both ? 2 : (fwd ? 1 : -1);
   // next time, queue one more launch than this search needed (its last level
   // admits nothing)
   hint = std::min(64, used + 1);
   return SCC_OK;
 }
 
-int traverse(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed) {
-  const int32_t* off = fwd ? c->begin : c->rbegin;
-  uint32_t* V = fwd ? c->vf : c->vb;
+// Run the level launches of one search whose frontier has been seeded.
+int run_levels(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed) {
+  return level_loop(c, false, fwd, multi, levels, timed);
+}
+
+// The forward and the backward search from the partitions' pivots, advancing
+// together (one launch runs a level of each).
+int traverse_both(scc_ctx* c, bool multi, int64_t* levels, bool timed) {
   const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
-  launch_ctl_init(c->stream, c->dc, 8, 0, 0, 0, avg / c->tune.bu_alpha);
-  launch_seed_pivots(c->stream, grid_for(c->n_live), c->dc, c->n, off, fwd ? row_end(c) : rrow_end(c), c->color, c->pkey, V, c->q[0],
-                     c->huge, c->live[0], c->live[1]);
-  return run_levels(c, fwd, multi, levels, timed);
+  launch_ctl_init(c->stream, c->dc, 8 | 32, 0, 0, 0, avg / c->tune.bu_alpha);
+  launch_seed_pivots2(c->stream, grid_for(c->n_live), c->dc, c->n, level_dir(c, true, true),
+                      level_dir(c, false, true), c->color, c->pkey, c->live[0], c->live[1]);
+  return level_loop(c, true, true, multi, levels, timed);
 }
 
 int traverse_from(scc_ctx* c, bool fwd, bool multi, const int32_t* seeds,
@@ -374,8 +413,7 @@ int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int6
   const int64_t words = ((int64_t)c->n + 31) / 32;
   launch_fill_u32(c->stream, c->vf, 0u, words);
   launch_fill_u32(c->stream, c->vb, 0u, words);
-  if ((rc = traverse(c, true, multi, levels, timed))) return rc;
-  if ((rc = traverse(c, false, multi, levels, timed))) return rc;
+  if ((rc = traverse_both(c, multi, levels, timed))) return rc;
   if (sets) launch_fb_stats(c->stream, grid, c->dc, gp(c), c->vf, c->vb, c->live[0], c->live[1]);
   launch_classify(c->stream, grid, c->dc, c->n, c->color, c->label, c->vf, c->vb, c->pkey, c->rep,
                   c->live[0], c->live[1]);
@@ -929,7 +967,17 @@ int scc_level_log(scc_ctx* c, int64_t* out, int32_t* levels) {
   const int32_t k = std::min(64, c->h_dc->lv.levels);
   for (int32_t i = 0; i < k; i++)
     for (int j = 0; j < 4; j++) out[4 * i + j] = (int64_t)c->h_dc->lvlog[i][j];
-  *levels = k;
+  if (c->last_dirs == -1)
+    for (int32_t i = 0; i < k; i++) out[4 * i] |= 256;  // a backward search alone
+  int32_t k2 = 0;
+  if (c->last_dirs == 2) {
+    k2 = std::min(64 - k, c->h_dc->lv2.levels);
+    for (int32_t i = 0; i < k2; i++) {
+      for (int j = 0; j < 4; j++) out[4 * (k + i) + j] = (int64_t)c->h_dc->lvlog2[i][j];
+      out[4 * (k + i)] |= 256;  // backward search
+    }
+  }
+  *levels = k + k2;
   return SCC_OK;
 }
 

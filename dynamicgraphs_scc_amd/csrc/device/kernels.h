@@ -27,7 +27,8 @@ inline unsigned grid_for(int64_t n, unsigned cap = 2048) {
 void launch_fill_i32(hipStream_t st, int32_t* p, int32_t v, int64_t n);
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, int64_t n);
 void launch_iota(hipStream_t st, int32_t* p, int32_t n);
-// what: bit 0 counters, 1 live list, 2 loop control, 3 level control
+// what: bit 0 counters, 1 live list, 2 loop control, 3 level control, 4 tuning
+// defaults, 5 the second level control (backward search run alongside)
 void launch_ctl_init(hipStream_t st, DevCtl* dc, int what, int32_t live_sel, unsigned long long live_n,
                      long long stop, double bu_ratio);
 void launch_trim1_step(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, int32_t* color,
@@ -44,18 +45,18 @@ void launch_pivot_key(hipStream_t st, unsigned grid, const DevCtl* dc, GraphPtrs
                       unsigned long long* pkey, const int32_t* L0, const int32_t* L1);
 void launch_force_pivot(hipStream_t st, int32_t p, int32_t n, const int32_t* color,
                         unsigned long long* pkey);
-void launch_seed_pivots(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, const int32_t* off,
-                        const int32_t* offe,
-                        const int32_t* color, const unsigned long long* pkey, uint32_t* V,
-                        int32_t* q0, int32_t* h0, const int32_t* L0, const int32_t* L1);
 void launch_seed_list(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* off, const int32_t* offe,
                       const int32_t* color, const int32_t* seeds, const unsigned long long* ns,
                       uint32_t* V, int32_t* q0);
-void launch_level(hipStream_t st, unsigned grid, DevCtl* dc, int multi, const int32_t* off,
-                  const int32_t* offe, const int32_t* nb, const int32_t* roff, const int32_t* roffe,
-                  const int32_t* rnb, const int32_t* color,
-                  uint32_t* V, int32_t* q0, int32_t* q1, int32_t* c0, int32_t* c1, int32_t* h0,
-                  int32_t* h1, const int32_t* L0, const int32_t* L1, int32_t max_chunks, int light);
+void launch_seed_pivots2(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, const LevelDir& F,
+                         const LevelDir& B, const int32_t* color, const unsigned long long* pkey,
+                         const int32_t* L0, const int32_t* L1);
+void launch_level(hipStream_t st, unsigned grid, DevCtl* dc, int multi, const LevelDir& D,
+                  const int32_t* color, const int32_t* L0, const int32_t* L1, int32_t max_chunks, int light);
+// both searches of a step, one level each; grid = workgroups per search x 2,
+// rounded up to a multiple of 16
+void launch_level2(hipStream_t st, unsigned grid, DevCtl* dc, int multi, const LevelDir& F, const LevelDir& B,
+                   const int32_t* color, const int32_t* L0, const int32_t* L1, int32_t max_chunks, int light);
 void launch_fb_stats(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, const uint32_t* vf,
                      const uint32_t* vb, const int32_t* L0, const int32_t* L1);
 void launch_classify(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, int32_t* color,

@@ -77,13 +77,27 @@ __global__ __launch_bounds__(BLOCK) void k_iota(int32_t* __restrict__ p, int32_t
 
 // Control-block set-up (one thread): the counters, the live list, the loop
 // control and the level control.  what: bit 0 counters, 1 live, 2 loop, 3 level.
+__device__ void init_level(LevelCtl& v, double bu_ratio) {
+  v.done = 0;
+  v.mode = MODE_TD;
+  v.q_sel = 0;
+  v.c_sel = -1;
+  v.h_sel = 0;
+  v.levels = 0;
+  v.blocks_done = 0;
+  v.nq = v.nh = v.nc = v.mf = v.prev_nq = 0;
+  v.out_q = v.out_c = v.out_h = v.out_edges = 0;
+  v.visited = 0;
+  v.bu_edges_per_cand = bu_ratio;
+}
+
 __global__ void k_ctl_init(DevCtl* dc, int what, int32_t live_sel, unsigned long long live_n,
                            long long stop, double bu_ratio) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (what & 1) {
     for (int i = 0; i < C_NUM; i++) dc->ctr[i] = 0;
     dc->blocks_done = 0;
-    for (int i = 0; i < 9 * 16; i++) dc->done[i] = 0;
+    for (int i = 0; i < 9 * 16; i++) dc->done[i] = dc->done2[i] = 0;
   }
   if (what & 2) {
     dc->live.sel = live_sel;
@@ -102,22 +116,10 @@ __global__ void k_ctl_init(DevCtl* dc, int what, int32_t live_sel, unsigned long
     dc->tune.huge_row = HUGE_ROW;
     dc->tune.td_beta = 24.f;
     dc->tune.bu_alpha = 14.f;
-    dc->tune.grid_cap = 2048;
+    dc->tune.grid_cap = GRID_CAP;
   }
-  if (what & 8) {
-    LevelCtl& v = dc->lv;
-    v.done = 0;
-    v.mode = MODE_TD;
-    v.q_sel = 0;
-    v.c_sel = -1;
-    v.h_sel = 0;
-    v.levels = 0;
-    v.blocks_done = 0;
-    v.nq = v.nh = v.nc = v.mf = v.prev_nq = 0;
-    v.out_q = v.out_c = v.out_h = v.out_edges = 0;
-    v.visited = 0;
-    v.bu_edges_per_cand = bu_ratio;
-  }
+  if (what & 8) init_level(dc->lv, bu_ratio);
+  if (what & 32) init_level(dc->lv2, bu_ratio);
 }
 
 // ---- trim1 -------------------------------------------------------------------
@@ -372,42 +374,49 @@ __global__ void k_force_pivot(int32_t p, int32_t n, const int32_t* __restrict__ 
   }
 }
 
-// Seed a search: every live vertex that is its partition's pivot marks itself
-// in V and enters the frontier q0 (dc->lv.nq).  off gives the traversal-
-// direction degree for the first switch decision.
-__global__ __launch_bounds__(BLOCK) void k_seed_pivots(DevCtl* __restrict__ dc, int32_t n,
-                                                       const int32_t* __restrict__ off, const int32_t* __restrict__ offe,
-                                                       const int32_t* __restrict__ color,
-                                                       const unsigned long long* __restrict__ pkey,
-                                                       uint32_t* __restrict__ V, int32_t* __restrict__ q0,
-                                                       int32_t* __restrict__ h0,
-                                                       const int32_t* __restrict__ L0,
-                                                       const int32_t* __restrict__ L1) {
+// Seeds of the two searches of a step, which run together: each live vertex
+// that is its partition's pivot marks itself in both visited bitmaps and enters
+// both frontiers (or, for a row of at least huge_row entries, the chunked
+// expansion of that direction's first level).  The summed row lengths give the
+// first direction-switch decision.
+__global__ __launch_bounds__(BLOCK) void k_seed_pivots2(DevCtl* __restrict__ dc, int32_t n, LevelDir F, LevelDir B,
+                                                        const int32_t* __restrict__ color,
+                                                        const unsigned long long* __restrict__ pkey,
+                                                        const int32_t* __restrict__ L0,
+                                                        const int32_t* __restrict__ L1) {
   const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
   const int64_t n_in = (int64_t)dc->live.n;
-  unsigned long long edges = 0, cnt = 0;
+  const int32_t huge = dc->tune.huge_row;
+  unsigned long long ef = 0, eb = 0, cnt = 0;
   for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
     const int64_t i = base + threadIdx.x;
     bool piv = false;
-    int32_t v = 0;
+    int32_t v = 0, df = 0, db = 0;
     if (i < n_in) {
       v = list[i];
       const int32_t c = color[v];
       piv = c != DONE && pivot_of_key(pkey[slot_of(c, n)]) == v;
       if (piv) {
-        atomicOr(&V[v >> 5], 1u << (v & 31));
-        edges += (unsigned long long)(offe[v] - off[v]);
+        const uint32_t bit = 1u << (v & 31);
+        atomicOr(&F.V[v >> 5], bit);
+        atomicOr(&B.V[v >> 5], bit);
+        df = F.offe[v] - F.off[v];
+        db = B.offe[v] - B.off[v];
+        ef += (unsigned long long)df;
+        eb += (unsigned long long)db;
         cnt++;
         dc->ctr[C_PIVOT] = (unsigned long long)v;  // the pivot, when there is one partition
       }
     }
-    // rows of at least HUGE_ROW entries go straight to the chunked expansion
-    const bool hug = piv && (offe[v] - off[v]) >= dc->tune.huge_row;
-    wave_append(piv && !hug, v, q0, &dc->lv.nq);
-    wave_append(hug, v, h0, &dc->lv.nh);
+    wave_append(piv && df < huge, v, F.q0, &dc->lv.nq);
+    wave_append(piv && df >= huge, v, F.h0, &dc->lv.nh);
+    wave_append(piv && db < huge, v, B.q0, &dc->lv2.nq);
+    wave_append(piv && db >= huge, v, B.h0, &dc->lv2.nh);
   }
-  block_count(edges, &dc->lv.mf);
+  block_count(ef, &dc->lv.mf);
+  block_count(eb, &dc->lv2.mf);
   block_count(cnt, &dc->lv.visited);
+  block_count(cnt, &dc->lv2.visited);
 }
 
 // Seed a search from an explicit vertex list (count at *ns): vertices not yet
@@ -485,19 +494,29 @@ constexpr int TD_CHUNK = 2048;
 // edges / 14, back to TD when the frontier shrinks below n/24) and ends the
 // search when a level admits nothing.  A switch to BU drops deferred rows:
 // their neighbours find them as parents.
+// One workgroup's part of a level: workgroup bid of nblk working on the
+// search whose control is lv / done / lvlog and whose arrays are D.
 template <int s_light>
-__global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int multi,
-                                                 const int32_t* __restrict__ off, const int32_t* __restrict__ offe, const int32_t* __restrict__ nb,
-                                                 const int32_t* __restrict__ roff, const int32_t* __restrict__ roffe, const int32_t* __restrict__ rnb,
-                                                 const int32_t* __restrict__ color, uint32_t* __restrict__ V,
-                                                 int32_t* __restrict__ q0, int32_t* __restrict__ q1,
-                                                 int32_t* __restrict__ c0, int32_t* __restrict__ c1,
-                                                 int32_t* __restrict__ h0, int32_t* __restrict__ h1,
-                                                 const int32_t* __restrict__ L0, const int32_t* __restrict__ L1,
-                                                 int32_t max_chunks) {
+__device__ __forceinline__ void level_part(DevCtl* __restrict__ dc, LevelCtl* __restrict__ lv,
+                                           uint32_t* __restrict__ done, unsigned long long (*lvlog)[4],
+                                           int multi, const LevelDir& D, const int32_t* __restrict__ color,
+                                           const int32_t* __restrict__ L0, const int32_t* __restrict__ L1,
+                                           int32_t max_chunks, unsigned bid, unsigned nblk) {
+  const int32_t* __restrict__ off = D.off;
+  const int32_t* __restrict__ offe = D.offe;
+  const int32_t* __restrict__ nb = D.nb;
+  const int32_t* __restrict__ roff = D.roff;
+  const int32_t* __restrict__ roffe = D.roffe;
+  const int32_t* __restrict__ rnb = D.rnb;
+  uint32_t* __restrict__ V = D.V;
+  int32_t* __restrict__ q0 = D.q0;
+  int32_t* __restrict__ q1 = D.q1;
+  int32_t* __restrict__ c0 = D.c0;
+  int32_t* __restrict__ c1 = D.c1;
+  int32_t* __restrict__ h0 = D.h0;
+  int32_t* __restrict__ h1 = D.h1;
   __shared__ int s_done, s_mode, s_q, s_c, s_h, s_huge;
   __shared__ unsigned long long s_nq, s_nh, s_nc;
-  LevelCtl* lv = &dc->lv;
   if (threadIdx.x == 0) {
     s_huge = dc->tune.huge_row;
     s_done = lv->done;
@@ -527,7 +546,7 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
     const int64_t nh = (int64_t)s_nh;
     if (nh > 0) {
       const int64_t total = nh * max_chunks;
-      for (int64_t g = blockIdx.x; g < total; g += gridDim.x) {
+      for (int64_t g = bid; g < total; g += nblk) {
         QOut::maybe_flush(q_out, &lv->out_q, BLOCK);
         const int32_t u = h_in[g % nh];
         const int32_t ch = (int32_t)(g / nh);
@@ -550,7 +569,7 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
     }
     // (b) the frontier
     const int64_t nq = (int64_t)s_nq;
-    for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < nq; base += (int64_t)gridDim.x * BLOCK) {
+    for (int64_t base = (int64_t)bid * BLOCK; base < nq; base += (int64_t)nblk * BLOCK) {
       QOut::maybe_flush(q_out, &lv->out_q, BLOCK);
       const int64_t i = base + threadIdx.x;
       const bool valid = i < nq;
@@ -607,7 +626,7 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
     };
     __shared__ int32_t s_qv[BLOCK], s_qs[BLOCK], s_qe[BLOCK], s_qc[BLOCK], s_qhit[BLOCK];
     __shared__ int s_bq;
-    for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < ncin; base += (int64_t)gridDim.x * BLOCK) {
+    for (int64_t base = (int64_t)bid * BLOCK; base < ncin; base += (int64_t)nblk * BLOCK) {
       QOut::maybe_flush(q_out, &lv->out_q, BLOCK);
       COut::maybe_flush(c_out, &lv->out_c, BLOCK);
       const int64_t i = base + threadIdx.x;
@@ -676,12 +695,12 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
   }
   QOut::flush(q_out_g, &lv->out_q);
   block_count(edges, &lv->out_edges);
-  if (last_block2(dc->done) && threadIdx.x == 0) {
+  if (last_block2(done, bid, nblk) && threadIdx.x == 0) {
     const unsigned long long oq = ld_acq(&lv->out_q), oc = ld_acq(&lv->out_c);
     const unsigned long long oh = ld_acq(&lv->out_h), oe = ld_acq(&lv->out_edges);
     const unsigned long long nlive = dc->live.n;
     if (lv->levels < 64) {
-      unsigned long long* lg = dc->lvlog[lv->levels];
+      unsigned long long* lg = lvlog[lv->levels];
       lg[0] = (unsigned long long)s_mode;
       lg[1] = s_mode == MODE_TD ? (unsigned long long)s_nq + ((unsigned long long)s_nh << 32)
                                 : (s_c < 0 ? nlive : (unsigned long long)s_nc);
@@ -718,6 +737,30 @@ __global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int mu
     lv->out_q = lv->out_c = lv->out_h = lv->out_edges = 0;
     lv->blocks_done = 0;
   }
+}
+
+template <int s_light>
+__global__ __launch_bounds__(BLOCK) void k_level(DevCtl* __restrict__ dc, int multi, LevelDir D,
+                                                 const int32_t* __restrict__ color,
+                                                 const int32_t* __restrict__ L0, const int32_t* __restrict__ L1,
+                                                 int32_t max_chunks) {
+  level_part<s_light>(dc, &dc->lv, dc->done, dc->lvlog, multi, D, color, L0, L1, max_chunks, blockIdx.x,
+                      gridDim.x);
+}
+
+// The forward and the backward search of a step advance one level each in the
+// same launch: workgroups alternate between them in runs of eight (so each XCD
+// serves both) and each search completes on its own counters.  The grid is a
+// multiple of 16.  A finished search's workgroups return at once.
+template <int s_light>
+__global__ __launch_bounds__(BLOCK) void k_level2(DevCtl* __restrict__ dc, int multi, LevelDir F, LevelDir B,
+                                                  const int32_t* __restrict__ color,
+                                                  const int32_t* __restrict__ L0, const int32_t* __restrict__ L1,
+                                                  int32_t max_chunks) {
+  const bool bw = (blockIdx.x >> 3) & 1u;
+  const unsigned bid = ((blockIdx.x >> 4) << 3) | (blockIdx.x & 7u);
+  level_part<s_light>(dc, bw ? &dc->lv2 : &dc->lv, bw ? dc->done2 : dc->done, bw ? dc->lvlog2 : dc->lvlog, multi,
+                      bw ? B : F, color, L0, L1, max_chunks, bid, gridDim.x >> 1);
 }
 
 // Sizes of a step's reach sets for the traffic model: |F|, |B|, E_F (out-degrees
@@ -1215,25 +1258,32 @@ void launch_force_pivot(hipStream_t st, int32_t p, int32_t n, const int32_t* col
                         unsigned long long* pkey) {
   LAUNCH(k_force_pivot, 1, st, p, n, color, pkey);
 }
-void launch_seed_pivots(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, const int32_t* off,
-                        const int32_t* offe,
-                        const int32_t* color, const unsigned long long* pkey, uint32_t* V,
-                        int32_t* q0, int32_t* h0, const int32_t* L0, const int32_t* L1) {
-  LAUNCH(k_seed_pivots, grid, st, dc, n, off, offe, color, pkey, V, q0, h0, L0, L1);
-}
 void launch_seed_list(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* off, const int32_t* offe,
                       const int32_t* color, const int32_t* seeds, const unsigned long long* ns,
                       uint32_t* V, int32_t* q0) {
   LAUNCH(k_seed_list, grid, st, dc, off, offe, color, seeds, ns, V, q0);
 }
-void launch_level(hipStream_t st, unsigned grid, DevCtl* dc, int multi, const int32_t* off,
-                  const int32_t* offe, const int32_t* nb, const int32_t* roff, const int32_t* roffe,
-                  const int32_t* rnb, const int32_t* color,
-                  uint32_t* V, int32_t* q0, int32_t* q1, int32_t* c0, int32_t* c1, int32_t* h0,
-                  int32_t* h1, const int32_t* L0, const int32_t* L1, int32_t max_chunks, int light) {
-#define LEVEL(L)                                                                                         \
-  LAUNCH(k_level<L>, grid, st, dc, multi, off, offe, nb, roff, roffe, rnb, color, V, q0, q1, c0, c1, h0, h1, L0, \
-         L1, max_chunks)
+void launch_seed_pivots2(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, const LevelDir& F,
+                         const LevelDir& B, const int32_t* color, const unsigned long long* pkey,
+                         const int32_t* L0, const int32_t* L1) {
+  LAUNCH(k_seed_pivots2, grid, st, dc, n, F, B, color, pkey, L0, L1);
+}
+void launch_level(hipStream_t st, unsigned grid, DevCtl* dc, int multi, const LevelDir& D,
+                  const int32_t* color, const int32_t* L0, const int32_t* L1, int32_t max_chunks, int light) {
+#define LEVEL(L) LAUNCH(k_level<L>, grid, st, dc, multi, D, color, L0, L1, max_chunks)
+  switch (light) {
+    case 4: LEVEL(4); break;
+    case 8: LEVEL(8); break;
+    case 32: LEVEL(32); break;
+    case 64: LEVEL(64); break;
+    default: LEVEL(16); break;
+  }
+#undef LEVEL
+}
+void launch_level2(hipStream_t st, unsigned grid, DevCtl* dc, int multi, const LevelDir& F, const LevelDir& B,
+                   const int32_t* color, const int32_t* L0, const int32_t* L1, int32_t max_chunks, int light) {
+  grid = (grid + 15u) & ~15u;
+#define LEVEL(L) LAUNCH(k_level2<L>, grid, st, dc, multi, F, B, color, L0, L1, max_chunks)
   switch (light) {
     case 4: LEVEL(4); break;
     case 8: LEVEL(8); break;

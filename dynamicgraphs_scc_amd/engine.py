@@ -361,12 +361,14 @@ class SccEngine:
         return st.as_dict()
 
     def level_log(self) -> list:
-        """Per level of the last reachability search: mode, input, admitted, edges."""
+        """Per level of the last reachability search (both searches of a forward-
+        backward step, forward first): direction, mode, input, admitted, edges."""
         out = np.zeros(256, np.int64)
         k = C.c_int32()
         self._check(self.lib.scc_level_log(self.h, out.ctypes.data, C.byref(k)))
-        return [{"mode": "BU" if out[4 * i] else "TD", "input": int(out[4 * i + 1] & 0xFFFFFFFF),
-                 "split_rows": int(out[4 * i + 1] >> 32) if out[4 * i] == 0 else 0,
+        return [{"mode": "BU" if out[4 * i] & 1 else "TD", "dir": "B" if out[4 * i] & 256 else "F",
+                 "input": int(out[4 * i + 1] & 0xFFFFFFFF),
+                 "split_rows": int(out[4 * i + 1] >> 32) if not out[4 * i] & 1 else 0,
                  "admitted": int(out[4 * i + 2]), "edges": int(out[4 * i + 3])} for i in range(k.value)]
 
     # -- batched graphs and tuning

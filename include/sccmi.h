@@ -322,7 +322,7 @@ typedef struct scc_tuning {
   int32_t light_row; /* search levels: a lane scans the first light_row entries of a row alone, the
                         wave the rest; one of 4, 8, 16, 32, 64 (16) */
   int32_t huge_row;  /* top-down frontier rows at least this long are split over the grid (8192) */
-  int32_t grid_cap;  /* workgroups per sweep launch (2048) */
+  int32_t grid_cap;  /* workgroups per sweep launch (1024) */
   int32_t pad;
   float bu_alpha;    /* top-down -> bottom-up when frontier edges > candidates' edges / bu_alpha (14) */
   float td_beta;     /* bottom-up -> top-down when frontier < live / td_beta and shrinking (24) */
@@ -338,9 +338,11 @@ int scc_autotune(scc_ctx* ctx, const scc_run_opts* opts, int32_t reps, scc_tunin
                  double* best_ms, double* default_ms);
 
 /* Per-level record of the last reachability search the context ran (up to 64
- * levels): out[4*i..] = {mode (0 top-down, 1 bottom-up), input size (top-down:
+ * levels; a forward-backward step runs both searches together and records the
+ * forward levels, then the backward ones): out[4*i..] = {mode (bit 0: 0
+ * top-down, 1 bottom-up; bit 8: backward search), input size (top-down:
  * frontier + (split rows << 32); bottom-up: candidates), vertices admitted,
- * out-degree sum of the admitted}.  *levels = levels recorded.  For analysis. */
+ * row-length sum of the admitted}.  *levels = levels recorded.  For analysis. */
 int scc_level_log(scc_ctx* ctx, int64_t* out, int32_t* levels);
 
 /* ---- timing helpers (HIP events on the context's stream) ------------------ */
