@@ -201,6 +201,21 @@ __device__ __forceinline__ void block_count(unsigned long long x, unsigned long 
   __syncthreads();
 }
 
+// block_count for totals that no completion count depends on: the atomic's
+// result is not waited for.
+__device__ __forceinline__ void block_count_nowait(unsigned long long x, unsigned long long* count) {
+  __shared__ unsigned long long s_sum[WAVES];
+  x = wave_sum_u64(x);
+  if (lane_id() == 0) s_sum[threadIdx.x >> 6] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+#pragma unroll
+    for (int i = 0; i < WAVES; i++) t += s_sum[i];
+    if (t) atomicAdd(count, t);
+  }
+}
+
 // LDS-staged list output.  Lanes push entries into a CAP-entry workgroup buffer
 // (reserved with LDS atomics); the buffer goes to the global list with one
 // global atomic per flush.  A wave whose reservation does not fit writes its

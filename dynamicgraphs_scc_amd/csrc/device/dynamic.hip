@@ -958,8 +958,7 @@ extern "C" int scc_update(scc_ctx* c, const int32_t* ins_src, const int32_t* ins
   const int64_t words = ((int64_t)c->n + 31) / 32;
   if (ns > 0 && c->n_live > 0) {
     // FW(S) within G' - P
-    launch_fill_u32(s, c->vf, 0u, words);
-    launch_fill_u32(s, c->vb, 0u, words);
+    launch_fill_u32(s, c->vf, 0u, 2 * words);  // vf and vb (adjacent)
     if ((rc = traverse_from(c, true, false, S, c->ctr + C_Q_OUT, nullptr))) return rc;
     // BW(T) inside FW(S)
     LAUNCH(k_restrict_fw, grid_for(c->n), s, c->dc, c->n, c->vf, c->color, c->live[1], 1);

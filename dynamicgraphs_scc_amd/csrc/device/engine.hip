@@ -176,8 +176,7 @@ int free_graph(scc_ctx* c) {
     dev_free(c, *p);
     *p = nullptr;
   }
-  dev_free(c, c->vf);
-  dev_free(c, c->vb);
+  dev_free(c, c->vf);  // vb shares this allocation
   dev_free(c, c->pkey);
   c->vf = c->vb = nullptr;
   c->pkey = nullptr;
@@ -218,8 +217,8 @@ int alloc_graph(scc_ctx* c, int32_t n, int64_t m_cap) {
   ALLOC(c->bcand[1], 4 * N);
   ALLOC(c->bhuge[0], 4 * N);
   ALLOC(c->bhuge[1], 4 * N);
-  ALLOC(c->vf, 4 * words);
-  ALLOC(c->vb, 4 * words);
+  ALLOC(c->vf, 8 * words);  // vf then vb, adjacent so that one fill clears both
+  c->vb = c->vf + ((uint64_t)n + 31) / 32;
   ALLOC(c->pkey, 8 * N);
   ALLOC(c->rep, 12 * N);
   ALLOC(c->scratch_i32, 4 * N);
@@ -403,7 +402,8 @@ int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int6
   const unsigned grid = grid_for(c->n_live);
   if ((rc = zero_ctr(c))) return rc;
   if (pivot_override < 0 && policy == SCC_PIVOT_LEARNED && sched_ready(c) && (rc = sched_pivots(c))) return rc;
-  launch_reset_slots(c->stream, grid, c->dc, c->n, c->color, c->pkey, c->rep, c->live[0], c->live[1]);
+  launch_reset_slots(c->stream, grid, c->dc, c->n, c->color, c->pkey, c->rep, c->live[0], c->live[1],
+                     multi ? 0 : 1);
   if (pivot_override >= 0) {
     if ((rc = force_pivot(c, pivot_override))) return rc;
   } else {
@@ -411,8 +411,7 @@ int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int6
                      c->live[0], c->live[1]);
   }
   const int64_t words = ((int64_t)c->n + 31) / 32;
-  launch_fill_u32(c->stream, c->vf, 0u, words);
-  launch_fill_u32(c->stream, c->vb, 0u, words);
+  launch_fill_u32(c->stream, c->vf, 0u, 2 * words);
   if ((rc = traverse_both(c, multi, levels, timed))) return rc;
   if (sets) launch_fb_stats(c->stream, grid, c->dc, gp(c), c->vf, c->vb, c->live[0], c->live[1]);
   launch_classify(c->stream, grid, c->dc, c->n, c->color, c->label, c->vf, c->vb, c->pkey, c->rep,
@@ -443,8 +442,7 @@ static int wcc_fixpoint(scc_ctx* c, bool seed) {
   const unsigned grid = grid_for(c->n_live, (unsigned)c->tune.grid_cap);
   const int64_t words = ((int64_t)c->n + 31) / 32;
   launch_ctl_init(c->stream, c->dc, 1 | 4 | 8, 0, 0, 0, 0.0);
-  launch_fill_u32(c->stream, c->vf, 0u, words);
-  launch_fill_u32(c->stream, c->vb, 0u, words);
+  launch_fill_u32(c->stream, c->vf, 0u, 2 * words);
   launch_cw_init(c->stream, c->n, c->begin, row_end(c), c->color, c->cw);
   launch_wcc_pull(c->stream, grid, c->dc, c->begin, row_end(c), c->adj, c->cw, c->q[0], c->live[0], c->live[1]);
   int batch = std::max(2, c->it_hint);

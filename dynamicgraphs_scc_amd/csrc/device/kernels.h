@@ -39,7 +39,7 @@ void launch_trim2(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, int32_
                   int32_t* label, const int32_t* L0, const int32_t* L1);
 void launch_reset_slots(hipStream_t st, unsigned grid, const DevCtl* dc, int32_t n,
                         const int32_t* color, unsigned long long* pkey, int32_t* rep,
-                        const int32_t* L0, const int32_t* L1);
+                        const int32_t* L0, const int32_t* L1, int single);
 void launch_pivot_key(hipStream_t st, unsigned grid, const DevCtl* dc, GraphPtrs G,
                       const int32_t* color, int policy, const uint32_t* scores,
                       unsigned long long* pkey, const int32_t* L0, const int32_t* L1);

@@ -143,11 +143,7 @@ __global__ __launch_bounds__(BLOCK) void k_trim1_step(DevCtl* __restrict__ dc, G
   const int32_t* __restrict__ list_in = LIVE_IN(dc, L0, L1);
   int32_t* __restrict__ list_out = LIVE_OUT(dc, L0, L1);
   const int64_t n_in = (int64_t)dc->live.n;
-  // fresh: every vertex still has colour -1 and none is done, so any neighbour
-  // other than v itself is live (a Jacobi sweep; the fixpoint is the same)
-  auto live = [color, fresh](int32_t k, int32_t pv, int32_t pc) {
-    return k != pv && (fresh || color[k] == pc);
-  };
+  auto live = [color](int32_t k, int32_t pv, int32_t pc) { return k != pv && color[k] == pc; };
   unsigned long long trimmed = 0;
   using Out = LdsOut<0>;
   Out::init();
@@ -155,22 +151,36 @@ __global__ __launch_bounds__(BLOCK) void k_trim1_step(DevCtl* __restrict__ dc, G
     Out::maybe_flush(list_out, &dc->loop.acc1, BLOCK);
     const int64_t i = base + threadIdx.x;
     const bool valid = i < n_in;
-    const int32_t v = valid ? list_in[i] : 0;
-    const int32_t c = valid ? color[v] : DONE;
+    int32_t v = 0, c = DONE;
+    bool out_ok, in_ok;
+    if (fresh) {
+      // fresh state: the live list is 0..n-1, every colour is -1 and none is
+      // done, so only empty rows decide this sweep (a row holding nothing but
+      // a self-loop survives it and falls to the next sweep, which reads
+      // colours; trimming later is always safe, trimming early never happens)
+      v = (int32_t)i;
+      c = UNSET;
+      out_ok = valid && G.end[v] > G.begin[v];
+      in_ok = out_ok && G.rend[v] > G.rbegin[v];
+    } else {
+      v = valid ? list_in[i] : 0;
+      c = valid ? color[v] : DONE;
+      const bool act = valid && c != DONE;
+      int32_t s = 0, e = 0;
+      if (act) {
+        s = G.begin[v];
+        e = G.end[v];
+      }
+      out_ok = row_any(act, v, c, s, e, G.adj, live, LIGHT);
+      const bool act2 = act && out_ok;
+      int32_t rs = 0, re = 0;
+      if (act2) {
+        rs = G.rbegin[v];
+        re = G.rend[v];
+      }
+      in_ok = row_any(act2, v, c, rs, re, G.radj, live, LIGHT);
+    }
     const bool act = valid && c != DONE;
-    int32_t s = 0, e = 0;
-    if (act) {
-      s = G.begin[v];
-      e = G.end[v];
-    }
-    const bool out_ok = row_any(act, v, c, s, e, G.adj, live, LIGHT);
-    const bool act2 = act && out_ok;
-    int32_t rs = 0, re = 0;
-    if (act2) {
-      rs = G.rbegin[v];
-      re = G.rend[v];
-    }
-    const bool in_ok = row_any(act2, v, c, rs, re, G.radj, live, LIGHT);
     const bool trim = act && !in_ok;
     if (trim) {
       label[v] = v;
@@ -282,7 +292,14 @@ __global__ __launch_bounds__(BLOCK) void k_reset_slots(const DevCtl* __restrict_
                                                        unsigned long long* __restrict__ pkey,
                                                        int32_t* __restrict__ rep,
                                                        const int32_t* __restrict__ L0,
-                                                       const int32_t* __restrict__ L1) {
+                                                       const int32_t* __restrict__ L1, int single) {
+  if (single) {  // every live vertex has colour -1: only slot n is in use
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      pkey[n] = 0ull;
+      rep[3 * (int64_t)n + 0] = rep[3 * (int64_t)n + 1] = rep[3 * (int64_t)n + 2] = INT32_MAX;
+    }
+    return;
+  }
   const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
   const int64_t n_in = (int64_t)dc->live.n;
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_in; i += (int64_t)gridDim.x * BLOCK) {
@@ -1212,7 +1229,7 @@ __global__ __launch_bounds__(BLOCK) void k_canon_apply(DevCtl* __restrict__ dc, 
     out[v] = c;
     self += (c == (int32_t)v) ? 1 : 0;
   }
-  block_count(self, &dc->ctr[C_COUNT]);
+  block_count_nowait(self, &dc->ctr[C_COUNT]);
 }
 
 // ---- host launchers -------------------------------------------------------------
@@ -1246,8 +1263,8 @@ void launch_trim2(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, int32_
 }
 void launch_reset_slots(hipStream_t st, unsigned grid, const DevCtl* dc, int32_t n,
                         const int32_t* color, unsigned long long* pkey, int32_t* rep,
-                        const int32_t* L0, const int32_t* L1) {
-  LAUNCH(k_reset_slots, grid, st, dc, n, color, pkey, rep, L0, L1);
+                        const int32_t* L0, const int32_t* L1, int single) {
+  LAUNCH(k_reset_slots, single ? 1u : grid, st, dc, n, color, pkey, rep, L0, L1, single);
 }
 void launch_pivot_key(hipStream_t st, unsigned grid, const DevCtl* dc, GraphPtrs G,
                       const int32_t* color, int policy, const uint32_t* scores,
@@ -1338,7 +1355,7 @@ void launch_canon(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label, i
   if (n <= 0) return;
   LAUNCH(k_iota, grid_for(n, 4096), st, mn, n);
   LAUNCH(k_canon_min, grid_for(n), st, n, label, mn);
-  LAUNCH(k_canon_apply, grid_for(n, 4096), st, dc, n, label, mn, out);
+  LAUNCH(k_canon_apply, grid_for(n, 1024), st, dc, n, label, mn, out);
 }
 
 #undef LAUNCH
