@@ -364,7 +364,7 @@ def main():
             traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_level (global forward-backward reachability levels, both directions)",
+                "kernel": "k_level2 (one level of the forward and one of the backward search of the global step per launch)",
                 "algorithmic_bytes": int(b_fb),
                 "algorithmic_bytes_per_launch": int(b_fb / max(1, statistics.median(launches))),
                 "avg_launch_us": round(1e3 * t_fb_ms / max(1, statistics.median(launches)), 3),
