@@ -517,7 +517,7 @@ int do_fb_parts(scc_ctx* c, int policy, int mode, bool trim_between, int64_t* ro
       if ((rc = sched_choose(c, &action))) return rc;
       if (action == 0) {
         if (!c->classes && (rc = do_wcc(c, nullptr))) return rc;
-        if ((rc = do_fb_round(c, SCC_PIVOT_LEARNED, true, -1, nullptr, nullptr, nullptr, false, nullptr)))
+        if ((rc = do_fb_round(c, policy, true, -1, nullptr, nullptr, nullptr, false, nullptr)))
           return rc;
       } else if ((rc = do_color_round(c, nullptr))) {
         return rc;

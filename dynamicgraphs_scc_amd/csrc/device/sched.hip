@@ -6,12 +6,13 @@
 // (src/scc_main.cc:272-351).  Here each live vertex gets 16 features (its
 // degrees, its partition's size, and round-level counts from the phase trace:
 // live fraction, what the last trim / forward-backward / colouring steps
-// resolved and cost).  Head 0 estimates how large the vertex's SCC is relative
-// to its partition (the partition's pivot is its best-scoring vertex); head 1
-// is the logit of "a colouring step resolves more per millisecond than a
-// forward-backward step here", averaged over the live set to choose the
-// next partition step.  Labels stay exact whatever the network says: it only
-// orders work.
+// resolved and cost, and which kind of partition step came last).  Head 0
+// estimates how large the vertex's SCC is relative to its partition (the
+// partition's pivot is its best-scoring vertex); head 1 is the logit of "taking
+// a colouring step now, then alternating, finishes sooner than taking a
+// forward-backward step now, then alternating", averaged over the live set to
+// choose the next partition step.  Labels stay exact whatever the network
+// says: it only orders work.
 //
 // Network layout (FP32 master copy, row-major): W1[16][64] b1[64] W2[64][64]
 // b2[64] W3[64][2] b3[2].  One wave evaluates 16 samples at a time in
@@ -572,9 +573,13 @@ RoundFeat round_features(const scc_ctx* c) {
   RoundFeat rf;
   for (auto& x : rf.f) x = 0.f;
   const double nl = (double)c->n_live, n = std::max<double>(1.0, (double)c->n);
-  double trim = 0, fb = 0, col = 0, fbms = 0, colms = 0;
-  bool got_trim = false, got_fb = false, got_col = false;
+  double trim = 0, fb = 0, col = 0, fbms = 0, colms = 0, last = 0.5;
+  bool got_trim = false, got_fb = false, got_col = false, got_last = false;
   for (auto it = c->trace.rbegin(); it != c->trace.rend(); ++it) {
+    if (!got_last && (it->kind == SCC_PHASE_FB_ROUND || it->kind == SCC_PHASE_COLOR_ROUND)) {
+      last = it->kind == SCC_PHASE_COLOR_ROUND ? 1.0 : 0.0;
+      got_last = true;
+    }
     const double before = std::max<double>(1.0, (double)it->live_before);
     const double res = (double)(it->live_before - it->live_after) / before;
     if (!got_trim && it->kind == SCC_PHASE_TRIM1) {
@@ -598,7 +603,7 @@ RoundFeat round_features(const scc_ctx* c) {
   rf.f[11] = (float)col;
   rf.f[12] = (float)(std::log2(1.0 + fbms) / 16.0);
   rf.f[13] = (float)(std::log2(1.0 + colms) / 16.0);
-  rf.f[14] = (float)std::max(0, (int)c->round_idx) / 16.f;
+  rf.f[14] = (float)last;
   rf.f[15] = (float)((double)c->m / n / 64.0);
   return rf;
 }
@@ -642,6 +647,7 @@ struct Snap {
   int64_t n_live = 0;
   bool colors_split = false, fresh = false, multi_colors = false, classes = false;
   size_t trace_len = 0;
+  int32_t lv_hint[3] = {0, 0, 0}, it_hint = 0;  // launch-batch hints: both branches start alike
 };
 
 int take_snap(scc_ctx* c, Snap& sn) {
@@ -661,6 +667,8 @@ int take_snap(scc_ctx* c, Snap& sn) {
   sn.multi_colors = c->multi_colors;
   sn.classes = c->classes;
   sn.trace_len = c->trace.size();
+  for (int i = 0; i < 3; i++) sn.lv_hint[i] = c->lv_hint[i];
+  sn.it_hint = c->it_hint;
   return SCC_OK;
 }
 
@@ -676,6 +684,8 @@ int restore_snap(scc_ctx* c, const Snap& sn) {
   c->multi_colors = sn.multi_colors;
   c->classes = sn.classes;
   c->trace.resize(sn.trace_len);
+  for (int i = 0; i < 3; i++) c->lv_hint[i] = sn.lv_hint[i];
+  c->it_hint = sn.it_hint;
   return hip_check(c, hipStreamSynchronize(c->stream), "restore sync");
 }
 
@@ -943,19 +953,24 @@ int scc_sched_rollout(scc_ctx* c, const scc_run_opts* opts_in, uint64_t seed, fl
       }
       rows += k;
     }
-    // counterfactual: both steps from the same state
+    // counterfactual: from the same state, each step followed by alternating
+    // steps (the other kind next) until nothing is live; the better step is
+    // the one whose decomposition finishes first
     if ((rc = take_snap(c, sn))) return rc;
-    double ms[2], res[2];
+    double total[2];
     for (int a = 0; a < 2; a++) {
-      const int64_t before = c->n_live;
       auto t0 = std::chrono::steady_clock::now();
-      if ((rc = run_action(c, a, false))) return rc;
-      ms[a] = ms_since(t0);
-      res[a] = (double)(before - c->n_live);
+      int step = a;
+      for (int64_t guard = 0; c->n_live > 0; guard++) {
+        if ((rc = check_cancel(c))) return rc;
+        if (guard > 4 * (int64_t)c->n + 64) return set_error(c, SCC_ERR_LIMIT, "rollout did not converge");
+        if ((rc = run_action(c, step, false))) return rc;
+        step = 1 - step;
+      }
+      total[a] = ms_since(t0);
       if ((rc = restore_snap(c, sn))) return rc;
     }
-    const double eff0 = res[0] / std::max(1e-3, ms[0]), eff1 = res[1] / std::max(1e-3, ms[1]);
-    const int better = eff1 > eff0 ? 1 : 0;
+    const int better = total[1] < total[0] ? 1 : 0;
     for (int64_t i = base_row; i < rows; i++) {
       Y[i * NO + 1] = (float)better;
       M[i * NO + 1] = 1.f;
@@ -968,8 +983,8 @@ int scc_sched_rollout(scc_ctx* c, const scc_run_opts* opts_in, uint64_t seed, fl
       st.agree += la == better;
     }
     if (U(rng) < explore) act = 1 - act;
+    if (st.rounds == 0) st.ms_best = std::min(total[0], total[1]);
     st.rounds++;
-    st.ms_best += std::min(ms[0] / std::max(1.0, res[0]), ms[1] / std::max(1.0, res[1])) * (double)sn.n_live;
     if ((rc = run_action(c, act, learned))) return rc;
     r++;
   }

@@ -5,10 +5,10 @@ The network and its kernels live in csrc/device/sched.hip (16 features ->
 module drives them:
 
 * traces: counterfactual decompositions of synthetic graphs
-  (SccEngine.sched_rollout) -- each partition round runs both the
-  forward-backward step and the colouring step from the same state and
-  records which resolved more vertices per millisecond, and every sampled
-  vertex's SCC size relative to its partition;
+  (SccEngine.sched_rollout) -- at each partition round, the forward-backward
+  step and the colouring step are each taken from the same state and followed
+  by alternating steps to the end; the trace records which finished sooner,
+  and every sampled vertex's SCC size relative to its partition;
 * training: Adam on the device (SccEngine.sched_train), from seeded He
   initial weights;
 * evaluation: wall time of the plain method (degree-product pivots, a

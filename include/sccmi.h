@@ -144,7 +144,9 @@ typedef enum scc_partition_mode {
   SCC_PARTS_LEARNED = 3 /* each step chosen by the learned scheduler (scc_sched_*): a
                            forward-backward step (after a WCC colouring when the live set is
                            not yet split into classes) or a colouring step; the separate WCC
-                           phase becomes part of that choice.  Without weights: SCC_PARTS_BOTH */
+                           phase becomes part of that choice.  The forward-backward steps take
+                           pivots by pivot_policy (SCC_PIVOT_LEARNED for the network's
+                           scores).  Without weights: SCC_PARTS_BOTH */
 } scc_partition_mode;
 
 typedef struct scc_run_stats {
@@ -215,8 +217,9 @@ int scc_phase_trace(scc_ctx* ctx, scc_phase_rec* out, int64_t cap, int64_t* n);
  * log2(1+outdeg*indeg)/32, log2(1+|partition|)/24, |partition|/live,
  * (outdeg-indeg)/(outdeg+indeg+1), log2(1+live)/24, live/n, classes flag,
  * last trim / forward-backward / colouring resolved fractions, log2(1+ns per
- * live vertex)/16 of the last forward-backward and colouring steps,
- * round/16, (m/n)/64. */
+ * live vertex)/16 of the last forward-backward and colouring steps, the kind
+ * of the last partition step (0 forward-backward, 1 colouring, 0.5 none yet),
+ * (m/n)/64. */
 #define SCC_SCHED_FEATURES 16
 #define SCC_SCHED_HIDDEN 64
 #define SCC_SCHED_HEADS 2
@@ -241,17 +244,19 @@ typedef struct scc_rollout_stats {
   int64_t rounds;     /* partition rounds */
   int64_t agree;      /* rounds where the installed network chose the measured better step */
   int64_t num_sccs;
-  double ms_total;    /* wall time of the rollout (both steps measured each round) */
-  double ms_best;     /* estimated partition time had every round taken the better step */
+  double ms_total;    /* wall time of the rollout (both continuations measured each round) */
+  double ms_best;     /* partition time from the first round, the faster of the two alternations */
 } scc_rollout_stats;
 
-/* Phase traces for training: a from-scratch decomposition in which every
- * partition round runs both steps from the same state (timed), restores the
- * state, records up to samples_per_round sampled live vertices (features in X,
- * head-1 target Y1 = 1 when colouring resolved more per millisecond), then
- * continues with the better step (or the installed network's choice), taking
- * the other one with probability `explore`.  After the run each sample's
- * head-0 target is log2(1+|SCC|)/log2(1+|partition|).  Labels are exact. */
+/* Phase traces for training: a from-scratch decomposition in which, at every
+ * partition round, each of the two steps is taken from the same state and
+ * followed by alternating steps until nothing is live (timed), the state is
+ * restored, up to samples_per_round sampled live vertices are recorded
+ * (features in X, head-1 target Y1 = 1 when the colouring-first continuation
+ * finished sooner), and the run goes on with the better step (or the installed
+ * network's choice), taking the other one with probability `explore`.  After
+ * the run each sample's head-0 target is log2(1+|SCC|)/log2(1+|partition|).
+ * Labels are exact. */
 int scc_sched_rollout(scc_ctx* ctx, const scc_run_opts* opts, uint64_t seed, float explore,
                       int32_t samples_per_round, float* X, float* Y, float* M, int64_t cap,
                       int64_t* n, scc_rollout_stats* stats);

@@ -104,7 +104,8 @@ def test_rollout_samples_and_exact_labels():
         # trained briefly on its own traces, the learned policy still gives exact labels
         e.set_sched_weights(mlp.init(2))
         e.sched_train(X, Y, M, epochs=5, batch=256, lr=1e-3)
-        for mode_kw in ({"partition_mode": PARTS_LEARNED}, {"pivot_policy": PIVOT_LEARNED}):
+        for mode_kw in ({"partition_mode": PARTS_LEARNED}, {"pivot_policy": PIVOT_LEARNED},
+                        {"partition_mode": PARTS_LEARNED, "pivot_policy": PIVOT_LEARNED}):
             e.reset()
             e.run_method2(default_opts(**mode_kw))
             assert np.array_equal(e.labels(), want), mode_kw
