@@ -449,8 +449,8 @@ static int wcc_fixpoint(scc_ctx* c, bool seed) {
   for (int guard = 0;; guard++) {
     if ((rc = check_cancel(c))) return rc;
     for (int i = 0; i < batch; i++)
-      launch_wcc_push(c->stream, grid, c->dc, c->rbegin, rrow_end(c), c->radj, c->cw, c->q[0], c->q[1], c->vf,
-                      c->vb);
+      launch_wcc_push(c->stream, grid, c->dc, c->begin, row_end(c), c->adj, c->rbegin, rrow_end(c), c->radj, c->cw,
+                      c->q[0], c->q[1], c->vf, c->vb, c->live[0], c->live[1]);
     if ((rc = pull_ctl(c))) return rc;
     if (c->h_dc->loop.done) break;
     batch = std::max(4, batch);

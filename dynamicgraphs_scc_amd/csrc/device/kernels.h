@@ -69,9 +69,12 @@ void launch_cw_init(hipStream_t st, int32_t n, const int32_t* begin, const int32
                     unsigned long long* cw);
 void launch_wcc_pull(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* begin, const int32_t* bend,
                      const int32_t* adj, unsigned long long* cw, int32_t* q0, const int32_t* L0, const int32_t* L1);
-void launch_wcc_push(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* rbegin, const int32_t* rend,
-                     const int32_t* radj, unsigned long long* cw, int32_t* q0, int32_t* q1, uint32_t* B0,
-                     uint32_t* B1);
+// later WCC iterations: pull sweeps over the live list while the frontier is
+// large, frontier pushes over the reverse rows after
+void launch_wcc_push(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* begin, const int32_t* bend,
+                     const int32_t* adj, const int32_t* rbegin, const int32_t* rend, const int32_t* radj,
+                     unsigned long long* cw, int32_t* q0, int32_t* q1, uint32_t* B0, uint32_t* B1,
+                     const int32_t* L0, const int32_t* L1);
 void launch_wcc_store(hipStream_t st, unsigned grid, const DevCtl* dc, const unsigned long long* cw, int32_t* W,
                       const int32_t* L0, const int32_t* L1);
 void launch_wcc_color(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, const int32_t* W,

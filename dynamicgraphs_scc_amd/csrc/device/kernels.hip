@@ -904,23 +904,21 @@ __global__ __launch_bounds__(BLOCK) void k_cw_init(int32_t n, const int32_t* __r
   }
 }
 
-// First iteration (src/scc_weak.cc:65-125, both phases fused): every live
+// One pull sweep (src/scc_weak.cc:65-125, both phases fused): every live
 // vertex pulls the minimum label of its same-colour out-neighbours, then jumps
 // to its root's label when the root is not self-rooted.  Vertices whose label
-// dropped form the frontier of the push iterations.
-__global__ __launch_bounds__(BLOCK) void k_wcc_pull(DevCtl* __restrict__ dc, const int32_t* __restrict__ begin,
-                                                    const int32_t* __restrict__ bend,
-                                                    const int32_t* __restrict__ adj,
-                                                    unsigned long long* __restrict__ cw,
-                                                    int32_t* __restrict__ q0, const int32_t* __restrict__ L0,
-                                                    const int32_t* __restrict__ L1) {
-  const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
-  const int64_t n_in = (int64_t)dc->live.n;
+// dropped are appended to qout (count at *qcount).  All threads of the
+// workgroup; workgroup bid of nblk.
+__device__ __forceinline__ void wcc_pull_sweep(const int32_t* __restrict__ list, int64_t n_in,
+                                               const int32_t* __restrict__ begin,
+                                               const int32_t* __restrict__ bend,
+                                               const int32_t* __restrict__ adj,
+                                               unsigned long long* __restrict__ cw,
+                                               int32_t* __restrict__ qout, unsigned long long* qcount) {
   const int lane = lane_id();
   using Out = LdsOut<0>;
-  Out::init();
   for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
-    Out::maybe_flush(q0, &dc->lv.out_q, BLOCK);
+    Out::maybe_flush(qout, qcount, BLOCK);
     const int64_t i = base + threadIdx.x;
     int32_t v = 0, s = 0, e = 0, m = INT32_MAX, w0 = INT32_MAX;
     uint32_t hv = (uint32_t)DONE;
@@ -964,14 +962,26 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_pull(DevCtl* __restrict__ dc, con
         const unsigned long long y = cw[r];
         if (cw_hi(y) == hv && (cw_lo(y) & WCC_MASK) != r) m = min(m, cw_lo(y));
       }
-      if (m < w0) {
+      if (m < w0) {  // only v's own thread writes cw[v] in a pull sweep
         cw[v] = cw_pack((int32_t)hv, m);
         changed = true;
       }
     }
-    Out::push(changed, v, q0, &dc->lv.out_q);
+    Out::push(changed, v, qout, qcount);
   }
-  Out::flush(q0, &dc->lv.out_q);
+  Out::flush(qout, qcount);
+}
+
+// First iteration: a pull sweep over the live list; the vertices whose label
+// dropped form the frontier of the later iterations.
+__global__ __launch_bounds__(BLOCK) void k_wcc_pull(DevCtl* __restrict__ dc, const int32_t* __restrict__ begin,
+                                                    const int32_t* __restrict__ bend,
+                                                    const int32_t* __restrict__ adj,
+                                                    unsigned long long* __restrict__ cw,
+                                                    int32_t* __restrict__ q0, const int32_t* __restrict__ L0,
+                                                    const int32_t* __restrict__ L1) {
+  LdsOut<0>::init();
+  wcc_pull_sweep(LIVE_IN(dc, L0, L1), (int64_t)dc->live.n, begin, bend, adj, cw, q0, &dc->lv.out_q);
   if (last_block2(dc->done) && threadIdx.x == 0) {
     LevelCtl& lv = dc->lv;
     const unsigned long long nq = ld_acq(&lv.out_q);
@@ -979,6 +989,7 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_pull(DevCtl* __restrict__ dc, con
     lv.out_q = 0;
     lv.q_sel = 0;
     lv.levels = 1;
+    lv.mode = 1;  // the last iteration pulled (claimed no frontier bits)
     dc->loop.iters = 1;
     dc->loop.total0 += nq;
     dc->loop.done = nq == 0;
@@ -986,23 +997,33 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_pull(DevCtl* __restrict__ dc, con
   }
 }
 
-// Later iterations: every vertex whose label dropped pushes it to its
-// same-colour in-neighbours (reverse rows) with a 64-bit atomicMin; a
-// neighbour whose label dropped joins the next frontier once (bitmap claim).
-// Ends when an iteration lowers nothing.  The fixpoint is the pull fixpoint:
-// W[v] = min tag over v's same-colour forward-reachable set.
-__global__ __launch_bounds__(BLOCK) void k_wcc_push(DevCtl* __restrict__ dc, const int32_t* __restrict__ rbegin,
+// Later iterations.  While the frontier (vertices whose label dropped) holds
+// more than 1/WCC_DENSE of the live set, an iteration is another pull sweep
+// over the whole live list; below that, every frontier vertex pushes its
+// label to its same-colour in-neighbours (reverse rows) with a 64-bit
+// atomicMin, and a neighbour whose label dropped joins the next frontier once
+// (claimed in the bitmap of this iteration's parity; each iteration clears
+// the claims of its own input frontier, so a bitmap is clean when it is
+// reused).  Ends when an iteration lowers nothing.  The fixpoint is the pull
+// fixpoint: W[v] = min tag over v's same-colour forward-reachable set.
+constexpr int WCC_DENSE = 2;
+__global__ __launch_bounds__(BLOCK) void k_wcc_push(DevCtl* __restrict__ dc, const int32_t* __restrict__ begin,
+                                                    const int32_t* __restrict__ bend,
+                                                    const int32_t* __restrict__ adj,
+                                                    const int32_t* __restrict__ rbegin,
                                                     const int32_t* __restrict__ rend,
                                                     const int32_t* __restrict__ radj,
                                                     unsigned long long* __restrict__ cw,
                                                     int32_t* __restrict__ q0, int32_t* __restrict__ q1,
-                                                    uint32_t* __restrict__ B0, uint32_t* __restrict__ B1) {
-  __shared__ int s_done, s_sel, s_t;
+                                                    uint32_t* __restrict__ B0, uint32_t* __restrict__ B1,
+                                                    const int32_t* __restrict__ L0, const int32_t* __restrict__ L1) {
+  __shared__ int s_done, s_sel, s_t, s_pulled;
   __shared__ unsigned long long s_nq;
   if (threadIdx.x == 0) {
     s_done = dc->loop.done;
     s_sel = dc->lv.q_sel;
     s_t = dc->lv.levels;
+    s_pulled = dc->lv.mode;
     s_nq = dc->lv.nq;
   }
   __syncthreads();
@@ -1012,67 +1033,75 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_push(DevCtl* __restrict__ dc, con
   uint32_t* __restrict__ Bin = ((s_t - 1) & 1) ? B1 : B0;
   uint32_t* __restrict__ Bout = (s_t & 1) ? B1 : B0;
   const int64_t nq = (int64_t)s_nq;
+  const int64_t n_live = (int64_t)dc->live.n;
+  const bool dense = nq * WCC_DENSE > n_live;
   const int lane = lane_id();
-  unsigned long long lowered = 0;
   using Out = LdsOut<0>;
   Out::init();
-  // claim u for the next frontier when the push lowered its label
-  auto lower = [&](int32_t u, unsigned long long x) -> bool {
-    const unsigned long long y = cw[u];
-    if (cw_hi(y) != cw_hi(x) || cw_lo(x) >= cw_lo(y)) return false;
-    const unsigned long long old = atomicMin(&cw[u], x);
-    if (old <= x) return false;
-    const uint32_t bit = 1u << (u & 31);
-    return !(atomicOr(&Bout[u >> 5], bit) & bit);
-  };
-  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < nq; base += (int64_t)gridDim.x * BLOCK) {
-    Out::maybe_flush(qout, &dc->lv.out_q, BLOCK);
-    const int64_t i = base + threadIdx.x;
-    int32_t k = 0, s = 0, e = 0;
-    unsigned long long x = 0;
-    bool act = false;
-    if (i < nq) {
-      k = qin[i];
-      if (s_t > 1) atomicAnd(&Bin[k >> 5], ~(1u << (k & 31)));
-      x = cw[k];
-      s = rbegin[k];
-      e = rend[k];
-      act = cw_hi(x) != (uint32_t)DONE;
-    }
-    const int len = act ? e - s : 0;
-    const int lim = len < 32 ? len : 0;
-    const int wl = wave_max(lim);
-    for (int j = 0; j < wl; j++) {
-      bool add = false;
-      int32_t u = 0;
-      if (j < lim) {
-        u = radj[s + j];
-        add = lower(u, x);
-      }
-      lowered += add;
-      Out::push(add, u, qout, &dc->lv.out_q);
-    }
-    uint64_t hm = __ballot(len >= 32);
-    while (hm) {
-      const int L = __ffsll((unsigned long long)hm) - 1;
-      hm &= hm - 1;
-      const int32_t hs = __shfl(s, L), he = __shfl(e, L);
-      const unsigned long long hx = __shfl(x, L);
-      for (int32_t b = hs; b < he; b += 64) {
-        const int32_t idx = b + lane;
-        bool add = false;
-        int32_t u = 0;
-        if (idx < he) {
-          u = radj[idx];
-          add = lower(u, hx);
-        }
-        lowered += add;
-        Out::push(add, u, qout, &dc->lv.out_q);
-      }
+  if (!s_pulled) {
+    // the input frontier's claims were made by the previous (push) iteration
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < nq; i += (int64_t)gridDim.x * BLOCK) {
+      const int32_t k = qin[i];
+      atomicAnd(&Bin[k >> 5], ~(1u << (k & 31)));
     }
   }
-  Out::flush(qout, &dc->lv.out_q);
-  (void)lowered;
+  if (dense) {
+    wcc_pull_sweep(LIVE_IN(dc, L0, L1), n_live, begin, bend, adj, cw, qout, &dc->lv.out_q);
+  } else {
+    // claim u for the next frontier when the push lowered its label
+    auto lower = [&](int32_t u, unsigned long long x) -> bool {
+      const unsigned long long y = cw[u];
+      if (cw_hi(y) != cw_hi(x) || cw_lo(x) >= cw_lo(y)) return false;
+      const unsigned long long old = atomicMin(&cw[u], x);
+      if (old <= x) return false;
+      const uint32_t bit = 1u << (u & 31);
+      return !(atomicOr(&Bout[u >> 5], bit) & bit);
+    };
+    for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < nq; base += (int64_t)gridDim.x * BLOCK) {
+      Out::maybe_flush(qout, &dc->lv.out_q, BLOCK);
+      const int64_t i = base + threadIdx.x;
+      int32_t k = 0, s = 0, e = 0;
+      unsigned long long x = 0;
+      bool act = false;
+      if (i < nq) {
+        k = qin[i];
+        x = cw[k];
+        s = rbegin[k];
+        e = rend[k];
+        act = cw_hi(x) != (uint32_t)DONE;
+      }
+      const int len = act ? e - s : 0;
+      const int lim = len < 32 ? len : 0;
+      const int wl = wave_max(lim);
+      for (int j = 0; j < wl; j++) {
+        bool add = false;
+        int32_t u = 0;
+        if (j < lim) {
+          u = radj[s + j];
+          add = lower(u, x);
+        }
+        Out::push(add, u, qout, &dc->lv.out_q);
+      }
+      uint64_t hm = __ballot(len >= 32);
+      while (hm) {
+        const int L = __ffsll((unsigned long long)hm) - 1;
+        hm &= hm - 1;
+        const int32_t hs = __shfl(s, L), he = __shfl(e, L);
+        const unsigned long long hx = __shfl(x, L);
+        for (int32_t b = hs; b < he; b += 64) {
+          const int32_t idx = b + lane;
+          bool add = false;
+          int32_t u = 0;
+          if (idx < he) {
+            u = radj[idx];
+            add = lower(u, hx);
+          }
+          Out::push(add, u, qout, &dc->lv.out_q);
+        }
+      }
+    }
+    Out::flush(qout, &dc->lv.out_q);
+  }
   if (last_block2(dc->done) && threadIdx.x == 0) {
     LevelCtl& lv = dc->lv;
     const unsigned long long nn = ld_acq(&lv.out_q);
@@ -1080,6 +1109,7 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_push(DevCtl* __restrict__ dc, con
     lv.out_q = 0;
     lv.q_sel ^= 1;
     lv.levels++;
+    lv.mode = dense ? 1 : 0;
     dc->loop.iters++;
     dc->loop.total0 += nn;
     dc->loop.done = nn == 0;
@@ -1332,10 +1362,11 @@ void launch_wcc_pull(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* b
                      const int32_t* adj, unsigned long long* cw, int32_t* q0, const int32_t* L0, const int32_t* L1) {
   LAUNCH(k_wcc_pull, grid, st, dc, begin, bend, adj, cw, q0, L0, L1);
 }
-void launch_wcc_push(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* rbegin, const int32_t* rend,
-                     const int32_t* radj, unsigned long long* cw, int32_t* q0, int32_t* q1, uint32_t* B0,
-                     uint32_t* B1) {
-  LAUNCH(k_wcc_push, grid, st, dc, rbegin, rend, radj, cw, q0, q1, B0, B1);
+void launch_wcc_push(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* begin, const int32_t* bend,
+                     const int32_t* adj, const int32_t* rbegin, const int32_t* rend, const int32_t* radj,
+                     unsigned long long* cw, int32_t* q0, int32_t* q1, uint32_t* B0, uint32_t* B1,
+                     const int32_t* L0, const int32_t* L1) {
+  LAUNCH(k_wcc_push, grid, st, dc, begin, bend, adj, rbegin, rend, radj, cw, q0, q1, B0, B1, L0, L1);
 }
 void launch_wcc_store(hipStream_t st, unsigned grid, const DevCtl* dc, const unsigned long long* cw, int32_t* W,
                       const int32_t* L0, const int32_t* L1) {
