@@ -1,5 +1,6 @@
 // Internal interface shared by the engine's translation units.
 #pragma once
+#include <functional>
 #include <string>
 
 #include "common.h"
@@ -28,7 +29,7 @@ int do_trim1(scc_ctx* c, int64_t stop, int64_t* trimmed);
 int do_trim2(scc_ctx* c, int64_t* removed);
 int force_pivot(scc_ctx* c, int32_t pivot);
 int run_levels(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed);
-int traverse_both(scc_ctx* c, bool multi, int64_t* levels, bool timed);
+int traverse_both(scc_ctx* c, bool multi, int64_t* levels, bool timed, const std::function<void()>& post);
 int traverse_from(scc_ctx* c, bool fwd, bool multi, const int32_t* seeds,
                   const unsigned long long* d_count, int64_t* levels);
 int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int64_t* scc,

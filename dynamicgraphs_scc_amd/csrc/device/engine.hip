@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <functional>
 #include <string>
 
 #include "common.h"
@@ -323,8 +324,11 @@ static LevelDir level_dir(scc_ctx* c, bool fwd, bool own_lists) {
 
 // Queue level launches in batches until the search (both == false: the one of
 // dc->lv; both: the two of dc->lv and dc->lv2) has finished.  hint holds the
-// launches the last such search needed.
-static int level_loop(scc_ctx* c, bool both, bool fwd, bool multi, int64_t* levels, bool timed) {
+// launches the last such search needed.  post, when given, queues kernels
+// behind every batch that do nothing until the searches have ended, so the
+// batch that ends them also runs what follows them before the host reads back.
+static int level_loop(scc_ctx* c, bool both, bool fwd, bool multi, int64_t* levels, bool timed,
+                      const std::function<void()>& post = nullptr) {
   const unsigned grid = grid_for(c->n_live, (unsigned)c->tune.grid_cap);
   const int32_t max_chunks = std::max<int32_t>(1, (c->maxdeg + 2047) / 2048);
   const LevelDir F = level_dir(c, fwd || both, both), B = level_dir(c, false, true);
@@ -342,6 +346,7 @@ static int level_loop(scc_ctx* c, bool both, bool fwd, bool multi, int64_t* leve
                      c->tune.light);
     }
     if (timed) (void)hipEventRecord(c->ev[1], c->stream);
+    if (post) post();
     if ((rc = pull_ctl(c))) return rc;
     if (timed) {
       float ms = 0;
@@ -368,13 +373,13 @@ int run_levels(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed) {
 }
 
 // The forward and the backward search from the partitions' pivots, advancing
-// together (one launch runs a level of each).
-int traverse_both(scc_ctx* c, bool multi, int64_t* levels, bool timed) {
+// together (one launch runs a level of each); post as in level_loop.
+int traverse_both(scc_ctx* c, bool multi, int64_t* levels, bool timed, const std::function<void()>& post) {
   const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
   launch_ctl_init(c->stream, c->dc, 8 | 32, 0, 0, 0, avg / c->tune.bu_alpha);
   launch_seed_pivots2(c->stream, grid_for(c->n_live), c->dc, c->n, level_dir(c, true, true),
                       level_dir(c, false, true), c->color, c->pkey, c->live[0], c->live[1]);
-  return level_loop(c, true, true, multi, levels, timed);
+  return level_loop(c, true, true, multi, levels, timed, post);
 }
 
 int traverse_from(scc_ctx* c, bool fwd, bool multi, const int32_t* seeds,
@@ -412,12 +417,15 @@ int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int6
   }
   const int64_t words = ((int64_t)c->n + 31) / 32;
   launch_fill_u32(c->stream, c->vf, 0u, 2 * words);
-  if ((rc = traverse_both(c, multi, levels, timed))) return rc;
-  if (sets) launch_fb_stats(c->stream, grid, c->dc, gp(c), c->vf, c->vb, c->live[0], c->live[1]);
-  launch_classify(c->stream, grid, c->dc, c->n, c->color, c->label, c->vf, c->vb, c->pkey, c->rep,
-                  c->live[0], c->live[1]);
-  launch_recolor(c->stream, grid, c->dc, c->n, c->color, c->vf, c->vb, c->rep, c->live[0], c->live[1]);
-  if ((rc = pull_ctl(c))) return rc;
+  // reach-set sizes (when asked), FW n BW -> done, the rest -> new partitions:
+  // queued behind the level batches, they run once both searches have ended
+  auto post = [&]() {
+    if (sets) launch_fb_stats(c->stream, grid, c->dc, gp(c), c->vf, c->vb, c->live[0], c->live[1]);
+    launch_classify(c->stream, grid, c->dc, c->n, c->color, c->label, c->vf, c->vb, c->pkey, c->rep,
+                    c->live[0], c->live[1]);
+    launch_recolor(c->stream, grid, c->dc, c->n, c->color, c->vf, c->vb, c->rep, c->live[0], c->live[1]);
+  };
+  if ((rc = traverse_both(c, multi, levels, timed, post))) return rc;
   const unsigned long long* h = c->h_dc->ctr;
   if (scc) *scc = (int64_t)h[C_SCC];
   if (pivot_out) *pivot_out = multi ? -1 : (int32_t)h[C_PIVOT];

@@ -780,6 +780,14 @@ __global__ __launch_bounds__(BLOCK) void k_level2(DevCtl* __restrict__ dc, int m
                       bw ? B : F, color, L0, L1, max_chunks, bid, gridDim.x >> 1);
 }
 
+// The kernels after a forward-backward step's searches are queued right behind
+// each batch of level launches, before the host has seen whether the searches
+// ended; they do nothing until both have (every workgroup reads the same
+// flags: the level launches before them in the stream have finished).
+__device__ __forceinline__ bool searches_open(const DevCtl* __restrict__ dc) {
+  return !(dc->lv.done && dc->lv2.done);
+}
+
 // Sizes of a step's reach sets for the traffic model: |F|, |B|, E_F (out-degrees
 // over F), E_B (in-degrees over B), |F n B|.
 __global__ __launch_bounds__(BLOCK) void k_fb_stats(DevCtl* __restrict__ dc, GraphPtrs G,
@@ -787,6 +795,7 @@ __global__ __launch_bounds__(BLOCK) void k_fb_stats(DevCtl* __restrict__ dc, Gra
                                                     const uint32_t* __restrict__ vb,
                                                     const int32_t* __restrict__ L0,
                                                     const int32_t* __restrict__ L1) {
+  if (searches_open(dc)) return;
   const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
   const int64_t n_in = (int64_t)dc->live.n;
   unsigned long long f = 0, b = 0, ef = 0, eb = 0, fb = 0;
@@ -822,6 +831,7 @@ __global__ __launch_bounds__(BLOCK) void k_classify(DevCtl* __restrict__ dc, int
                                                     const unsigned long long* __restrict__ pkey,
                                                     int32_t* __restrict__ rep, int32_t* __restrict__ L0,
                                                     int32_t* __restrict__ L1) {
+  if (searches_open(dc)) return;
   const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
   int32_t* __restrict__ list_out = LIVE_OUT(dc, L0, L1);
   const int64_t n_in = (int64_t)dc->live.n;
@@ -870,6 +880,7 @@ __global__ __launch_bounds__(BLOCK) void k_recolor(const DevCtl* __restrict__ dc
                                                    const int32_t* __restrict__ rep,
                                                    const int32_t* __restrict__ L0,
                                                    const int32_t* __restrict__ L1) {
+  if (searches_open(dc)) return;
   const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
   const int64_t n_in = (int64_t)dc->live.n;
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_in; i += (int64_t)gridDim.x * BLOCK) {
