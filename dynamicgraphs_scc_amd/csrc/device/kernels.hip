@@ -1233,10 +1233,46 @@ __global__ __launch_bounds__(BLOCK) void k_extract(DevCtl* __restrict__ dc, int3
 // mn[l] = min vertex labelled l.  Most labels are shared by one vertex; the
 // forward-backward pivots label whole SCCs, so lanes of a wave that share a
 // label first reduce among themselves and one lane issues the atomic (test
-// first: the slot usually already holds a smaller id).
+// first: the slot usually already holds a smaller id).  A label that many
+// workgroups meet in many lanes (a giant SCC's) would make one address hot:
+// each workgroup takes the most frequent shared label of its first 64
+// entries as its hot label and handles it in LDS, with one global access.
+
+// The most frequent label held by at least two of entries
+// [blockIdx.x * BLOCK, + 64) that are not self-labelled, or -1.  All threads.
+__device__ int32_t block_hot_label(int32_t n, const int32_t* __restrict__ label) {
+  __shared__ int32_t s_hot;
+  if (threadIdx.x < 64) {  // wave 0
+    const int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const int32_t l = v < n ? label[v] : -1;
+    bool pend = (uint32_t)l < (uint32_t)n && l != (int32_t)v;
+    int32_t best = -1;
+    int bestc = 1;
+    uint64_t m = __ballot(pend);
+    while (m) {
+      const int32_t L = __shfl(l, __ffsll((unsigned long long)m) - 1);
+      const bool same = pend && l == L;
+      const int cnt = __popcll(__ballot(same));
+      if (cnt > bestc) {
+        bestc = cnt;
+        best = L;
+      }
+      if (same) pend = false;
+      m = __ballot(pend);
+    }
+    if (threadIdx.x == 0) s_hot = best;
+  }
+  __syncthreads();
+  return s_hot;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_canon_min(int32_t n, const int32_t* __restrict__ label,
                                                      int32_t* __restrict__ mn) {
   const int lane = lane_id();
+  const int32_t hot = block_hot_label(n, label);
+  __shared__ int32_t s_hmin;
+  if (threadIdx.x == 0) s_hmin = INT32_MAX;
+  __syncthreads();
   for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n; base += (int64_t)gridDim.x * BLOCK) {
     const int64_t v = base + threadIdx.x;
     int32_t l = -1;
@@ -1244,6 +1280,14 @@ __global__ __launch_bounds__(BLOCK) void k_canon_min(int32_t n, const int32_t* _
     // mn starts as the identity (a label is always a vertex labelled with
     // itself), so self-labelled vertices have nothing to contribute
     bool pending = (uint32_t)l < (uint32_t)n && l != (int32_t)v;
+    if (hot >= 0) {
+      const bool h = pending && l == hot;
+      int32_t x = h ? (int32_t)v : INT32_MAX;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o));
+      if (lane == 0 && x < INT32_MAX) atomicMin(&s_hmin, x);
+      if (h) pending = false;
+    }
     uint64_t act = __ballot(pending);
     while (act) {
       const int leader = __ffsll((unsigned long long)act) - 1;
@@ -1257,16 +1301,23 @@ __global__ __launch_bounds__(BLOCK) void k_canon_min(int32_t n, const int32_t* _
       act = __ballot(pending);
     }
   }
+  __syncthreads();
+  if (threadIdx.x == 0 && s_hmin < INT32_MAX && s_hmin < mn[hot]) atomicMin(&mn[hot], s_hmin);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_canon_apply(DevCtl* __restrict__ dc, int32_t n,
                                                        const int32_t* __restrict__ label,
                                                        const int32_t* __restrict__ mn,
                                                        int32_t* __restrict__ out) {
+  const int32_t hot = block_hot_label(n, label);
+  __shared__ int32_t s_hv;
+  if (threadIdx.x == 0) s_hv = hot >= 0 ? mn[hot] : -1;
+  __syncthreads();
+  const int32_t hv = s_hv;
   unsigned long long self = 0;
   for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < n; v += (int64_t)gridDim.x * BLOCK) {
     const int32_t l = label[v];
-    const int32_t c = ((uint32_t)l < (uint32_t)n) ? mn[l] : -1;
+    const int32_t c = (l == hot) ? hv : (((uint32_t)l < (uint32_t)n) ? mn[l] : -1);
     out[v] = c;
     self += (c == (int32_t)v) ? 1 : 0;
   }
