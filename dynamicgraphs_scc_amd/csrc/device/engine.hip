@@ -827,7 +827,10 @@ void scc_run_opts_default(scc_run_opts* o) {
   if (!o) return;
   std::memset(o, 0, sizeof(*o));
   o->pivot_policy = SCC_PIVOT_DEGREE_PRODUCT;
-  o->trim_stop = -2;
+  // a sweep after the first gathers neighbour colours; it pays only when it
+  // removes most of what it examines (what it would remove is otherwise left
+  // to the searches, as unreached candidates, and to the trim after them)
+  o->trim_stop = -70;
   o->use_trim2 = 1;
   o->trim_between = 0;
   o->use_wcc = 1;

@@ -168,7 +168,7 @@ typedef struct scc_run_stats {
   int64_t fb_f, fb_b, fb_ef, fb_eb, fb_fb;
 } scc_run_stats;
 
-/* Default options: degree-product pivot, trim_stop -2 (2 %), trim2 on, no trim between
+/* Default options: degree-product pivot, trim_stop -70 (70 %), trim2 on, no trim between
  * partition rounds (measured faster on planted graphs), WCC on, no phase
  * timing, SCC_PARTS_BOTH. */
 void scc_run_opts_default(scc_run_opts* o);

@@ -31,7 +31,8 @@ def test_golden_fixture(engine, name):
 @pytest.mark.parametrize("policy", [PIVOT_DEGREE_PRODUCT, PIVOT_MIN_ID])
 @pytest.mark.parametrize("flags", [dict(), dict(use_trim2=0), dict(use_wcc=0),
                                    dict(trim_between=1), dict(trim_stop=100),
-                                   dict(trim_stop=0), dict(trim_stop=-50)])
+                                   dict(trim_stop=0), dict(trim_stop=-2), dict(trim_stop=-50),
+                                   dict(trim_stop=-100)])
 def test_pipeline_options_same_labels(engine, policy, flags):
     g = graphs.rmat(13, 6, seed=31)
     want = oracle.scc_canonical(g, 8)
