@@ -253,7 +253,7 @@ int do_trim_compact(scc_ctx* c) {
   if (c->n_live == 0) return SCC_OK;
   int rc;
   if ((rc = zero_ctr(c))) return rc;
-  launch_compact(c->stream, grid_for(c->n_live), c->dc, c->color, c->live[0], c->live[1]);
+  launch_compact(c->stream, grid_for(c->n_live, (unsigned)c->tune.grid_cap), c->dc, c->color, c->live[0], c->live[1]);
   return pull_ctl(c);
 }
 
@@ -294,7 +294,7 @@ int do_trim2(scc_ctx* c, int64_t* removed) {
   PhaseRec pr(c, SCC_PHASE_TRIM2);
   int rc;
   if ((rc = zero_ctr(c))) return rc;
-  launch_trim2(c->stream, grid_for(c->n_live), c->dc, gp(c), c->color, c->label, c->live[0], c->live[1]);
+  launch_trim2(c->stream, grid_for(c->n_live, (unsigned)c->tune.grid_cap), c->dc, gp(c), c->color, c->label, c->live[0], c->live[1]);
   if ((rc = pull_ctl(c))) return rc;
   if (removed) *removed = (int64_t)c->h_dc->ctr[C_TRIMMED];
   pr.finish((int64_t)c->h_dc->ctr[C_TRIMMED]);
@@ -377,7 +377,7 @@ int run_levels(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed) {
 int traverse_both(scc_ctx* c, bool multi, int64_t* levels, bool timed, const std::function<void()>& post) {
   const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
   launch_ctl_init(c->stream, c->dc, 8 | 32, 0, 0, 0, avg / c->tune.bu_alpha);
-  launch_seed_pivots2(c->stream, grid_for(c->n_live), c->dc, c->n, level_dir(c, true, true),
+  launch_seed_pivots2(c->stream, grid_for(c->n_live, (unsigned)c->tune.grid_cap), c->dc, c->n, level_dir(c, true, true),
                       level_dir(c, false, true), c->color, c->pkey, c->live[0], c->live[1]);
   return level_loop(c, true, true, multi, levels, timed, post);
 }
@@ -388,7 +388,7 @@ int traverse_from(scc_ctx* c, bool fwd, bool multi, const int32_t* seeds,
   uint32_t* V = fwd ? c->vf : c->vb;
   const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
   launch_ctl_init(c->stream, c->dc, 8, 0, 0, 0, avg / c->tune.bu_alpha);
-  launch_seed_list(c->stream, grid_for(c->n_live), c->dc, off, fwd ? row_end(c) : rrow_end(c), c->color, seeds, d_count, V, c->q[0]);
+  launch_seed_list(c->stream, grid_for(c->n_live, (unsigned)c->tune.grid_cap), c->dc, off, fwd ? row_end(c) : rrow_end(c), c->color, seeds, d_count, V, c->q[0]);
   return run_levels(c, fwd, multi, levels, false);
 }
 
@@ -404,7 +404,7 @@ int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int6
   int64_t lv = 0;
   if (!levels) levels = &lv;
   const int64_t lv0 = *levels;
-  const unsigned grid = grid_for(c->n_live);
+  const unsigned grid = grid_for(c->n_live, (unsigned)c->tune.grid_cap);
   if ((rc = zero_ctr(c))) return rc;
   if (pivot_override < 0 && policy == SCC_PIVOT_LEARNED && sched_ready(c) && (rc = sched_pivots(c))) return rc;
   launch_reset_slots(c->stream, grid, c->dc, c->n, c->color, c->pkey, c->rep, c->live[0], c->live[1],
@@ -500,7 +500,7 @@ int do_color_round(scc_ctx* c, int64_t* scc) {
   if ((rc = wcc_fixpoint(c, true))) return rc;
   const int64_t iters = c->h_dc->loop.iters;
   if ((rc = run_levels(c, true, true, nullptr, false))) return rc;
-  launch_extract(c->stream, grid_for(c->n_live), c->dc, c->color, c->label, c->vf, c->live[0], c->live[1]);
+  launch_extract(c->stream, grid_for(c->n_live, (unsigned)c->tune.grid_cap), c->dc, c->color, c->label, c->vf, c->live[0], c->live[1]);
   if ((rc = pull_ctl(c))) return rc;
   if (scc) *scc = (int64_t)c->h_dc->ctr[C_SCC];
   c->classes = true;
