@@ -457,6 +457,7 @@ struct scc_ctx {
   bool fresh = false;                     // state just reset: all colours -1, nothing done
   int32_t lv_hint[3] = {8, 8, 8};         // level launches per search batch (fw, bw, both)
   int32_t it_hint = 4;                    // WCC iterations per batch
+  int32_t trim_hint[2] = {3, 3};          // trim sweeps per batch (absolute stop, relative stop)
   bool labels_ready = false;              // canon[] valid for the current graph
   bool ran = false;
 

@@ -269,7 +269,10 @@ int do_trim1(scc_ctx* c, int64_t stop, int64_t* trimmed) {
   PhaseRec pr(c, SCC_PHASE_TRIM1);
   launch_ctl_init(c->stream, c->dc, 4, 0, 0, (long long)stop, 0.0);
   const unsigned grid = grid_for(c->n_live, (unsigned)c->tune.grid_cap);
-  int batch = 3, rc;
+  // queue as many sweeps as the last loop of this kind needed (a sweep that
+  // finds the loop ended returns at once)
+  int32_t& hint = c->trim_hint[stop < 0 ? 1 : 0];
+  int batch = std::max(1, hint), rc;
   for (int guard = 0;; guard++) {
     if ((rc = check_cancel(c))) return rc;
     for (int i = 0; i < batch; i++) {
@@ -282,6 +285,7 @@ int do_trim1(scc_ctx* c, int64_t stop, int64_t* trimmed) {
     batch = 4;
     if (guard > (int64_t)c->n + 8) return set_error(c, SCC_ERR_LIMIT, "trim did not converge");
   }
+  hint = std::min(64, c->h_dc->loop.iters);
   if (trimmed) *trimmed = (int64_t)c->h_dc->loop.total0;
   pr.finish(c->h_dc->loop.iters);
   return SCC_OK;

@@ -835,13 +835,25 @@ __global__ __launch_bounds__(BLOCK) void k_classify(DevCtl* __restrict__ dc, int
   const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
   int32_t* __restrict__ list_out = LIVE_OUT(dc, L0, L1);
   const int64_t n_in = (int64_t)dc->live.n;
+  // the partition of the workgroup's first entry (in the global step, every
+  // entry's): its three representatives are reduced in LDS and written with
+  // one global atomic each, instead of every wave hitting the same addresses
+  __shared__ int32_t s_hot, s_hmin[3];
+  if (threadIdx.x == 0) {
+    const int64_t i0 = (int64_t)blockIdx.x * BLOCK;
+    const int32_t c0 = i0 < n_in ? color[list[i0]] : DONE;
+    s_hot = c0 != DONE ? slot_of(c0, n) : -1;
+  }
+  if (threadIdx.x < 3) s_hmin[threadIdx.x] = INT32_MAX;
+  __syncthreads();
+  const int32_t hot = s_hot;
   unsigned long long nscc = 0;
   using Out = LdsOut<0>;
   Out::init();
   for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
     Out::maybe_flush(list_out, &dc->ctr[C_LIST_OUT], BLOCK);
     const int64_t i = base + threadIdx.x;
-    bool keep = false;
+    bool keep = false, hk = false;
     int64_t rslot = 0;
     int32_t v = 0;
     if (i < n_in) {
@@ -855,15 +867,23 @@ __global__ __launch_bounds__(BLOCK) void k_classify(DevCtl* __restrict__ dc, int
           color[v] = DONE;
           nscc++;
         } else {
-          rslot = 3 * (int64_t)s + (f ? 0 : (b ? 1 : 2));
+          const int t = f ? 0 : (b ? 1 : 2);
+          rslot = 3 * (int64_t)s + t;
           keep = true;
+          hk = s == hot;
+          if (hk) atomicMin(&s_hmin[t], v);
         }
       }
     }
-    grouped_min_i32(keep, rslot, v, rep);
+    grouped_min_i32(keep && !hk, rslot, v, rep);
     Out::push(keep, v, list_out, &dc->ctr[C_LIST_OUT]);
   }
   Out::flush(list_out, &dc->ctr[C_LIST_OUT]);
+  if (threadIdx.x < 3 && hot >= 0) {
+    const int32_t x = s_hmin[threadIdx.x];
+    int32_t* r = &rep[3 * (int64_t)hot + threadIdx.x];
+    if (x < *r) atomicMin(r, x);
+  }
   block_count(nscc, &dc->ctr[C_SCC]);
   if (last_block2(dc->done) && threadIdx.x == 0) {
     dc->live.n = ld_acq(&dc->ctr[C_LIST_OUT]);
@@ -1314,13 +1334,26 @@ __global__ __launch_bounds__(BLOCK) void k_canon_apply(DevCtl* __restrict__ dc, 
   if (threadIdx.x == 0) s_hv = hot >= 0 ? mn[hot] : -1;
   __syncthreads();
   const int32_t hv = s_hv;
+  auto canon = [&](int32_t l) { return (l == hot) ? hv : (((uint32_t)l < (uint32_t)n) ? mn[l] : -1); };
   unsigned long long self = 0;
-  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < n; v += (int64_t)gridDim.x * BLOCK) {
-    const int32_t l = label[v];
-    const int32_t c = (l == hot) ? hv : (((uint32_t)l < (uint32_t)n) ? mn[l] : -1);
-    out[v] = c;
-    self += (c == (int32_t)v) ? 1 : 0;
+  // four labels per lane (16-byte loads and stores; the four gathers are
+  // independent), then the tail
+  const int64_t n4 = (int64_t)n / 4;
+  const int4* __restrict__ label4 = reinterpret_cast<const int4*>(label);
+  int4* __restrict__ out4 = reinterpret_cast<int4*>(out);
+  for (int64_t q = (int64_t)blockIdx.x * BLOCK + threadIdx.x; q < n4; q += (int64_t)gridDim.x * BLOCK) {
+    const int4 l = label4[q];
+    const int4 c = make_int4(canon(l.x), canon(l.y), canon(l.z), canon(l.w));
+    out4[q] = c;
+    const int32_t v = (int32_t)(4 * q);
+    self += (c.x == v) + (c.y == v + 1) + (c.z == v + 2) + (c.w == v + 3);
   }
+  if (blockIdx.x == 0)
+    for (int64_t v = 4 * n4 + threadIdx.x; v < n; v += BLOCK) {
+      const int32_t c = canon(label[v]);
+      out[v] = c;
+      self += (c == (int32_t)v) ? 1 : 0;
+    }
   block_count_nowait(self, &dc->ctr[C_COUNT]);
 }
 
@@ -1448,7 +1481,7 @@ void launch_canon(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label, i
   if (n <= 0) return;
   LAUNCH(k_iota, grid_for(n, 4096), st, mn, n);
   LAUNCH(k_canon_min, grid_for(n), st, n, label, mn);
-  LAUNCH(k_canon_apply, grid_for(n, 1024), st, dc, n, label, mn, out);
+  LAUNCH(k_canon_apply, grid_for((n + 3) / 4, 1024), st, dc, n, label, mn, out);
 }
 
 #undef LAUNCH

@@ -647,7 +647,7 @@ struct Snap {
   int64_t n_live = 0;
   bool colors_split = false, fresh = false, multi_colors = false, classes = false;
   size_t trace_len = 0;
-  int32_t lv_hint[3] = {0, 0, 0}, it_hint = 0;  // launch-batch hints: both branches start alike
+  int32_t lv_hint[3] = {0, 0, 0}, it_hint = 0, trim_hint[2] = {0, 0};  // launch-batch hints: both branches start alike
 };
 
 int take_snap(scc_ctx* c, Snap& sn) {
@@ -669,6 +669,7 @@ int take_snap(scc_ctx* c, Snap& sn) {
   sn.trace_len = c->trace.size();
   for (int i = 0; i < 3; i++) sn.lv_hint[i] = c->lv_hint[i];
   sn.it_hint = c->it_hint;
+  for (int i = 0; i < 2; i++) sn.trim_hint[i] = c->trim_hint[i];
   return SCC_OK;
 }
 
@@ -686,6 +687,7 @@ int restore_snap(scc_ctx* c, const Snap& sn) {
   c->trace.resize(sn.trace_len);
   for (int i = 0; i < 3; i++) c->lv_hint[i] = sn.lv_hint[i];
   c->it_hint = sn.it_hint;
+  for (int i = 0; i < 2; i++) c->trim_hint[i] = sn.trim_hint[i];
   return hip_check(c, hipStreamSynchronize(c->stream), "restore sync");
 }
 
