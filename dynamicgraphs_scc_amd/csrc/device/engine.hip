@@ -410,14 +410,17 @@ int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int6
   const int64_t lv0 = *levels;
   const unsigned grid = grid_for(c->n_live, (unsigned)c->tune.grid_cap);
   if ((rc = zero_ctr(c))) return rc;
-  if (pivot_override < 0 && policy == SCC_PIVOT_LEARNED && sched_ready(c) && (rc = sched_pivots(c))) return rc;
+  // learned pivots need the network (else the plain rule); sched_pivots
+  // writes this graph's scores for every live vertex first
+  if (policy == SCC_PIVOT_LEARNED && !sched_ready(c)) policy = SCC_PIVOT_DEGREE_PRODUCT;
+  if (pivot_override < 0 && policy == SCC_PIVOT_LEARNED && (rc = sched_pivots(c))) return rc;
   launch_reset_slots(c->stream, grid, c->dc, c->n, c->color, c->pkey, c->rep, c->live[0], c->live[1],
                      multi ? 0 : 1);
   if (pivot_override >= 0) {
     if ((rc = force_pivot(c, pivot_override))) return rc;
   } else {
-    launch_pivot_key(c->stream, grid, c->dc, gp(c), c->color, policy, c->sched_scores, c->pkey,
-                     c->live[0], c->live[1]);
+    launch_pivot_key(c->stream, grid, c->dc, gp(c), c->color, policy,
+                     policy == SCC_PIVOT_LEARNED ? c->sched_scores : nullptr, c->pkey, c->live[0], c->live[1]);
   }
   const int64_t words = ((int64_t)c->n + 31) / 32;
   launch_fill_u32(c->stream, c->vf, 0u, 2 * words);

@@ -420,6 +420,74 @@ __global__ __launch_bounds__(BLOCK) void k_part_clear(const DevCtl* __restrict__
     pcount[slot_of(color[list[i]], n)] = 0;
 }
 
+// The phase vote needs the partition sizes of a sample of live vertices only.
+// Sampled entries list[e * stride] (e < m, m <= VOTE_SAMPLES) name their
+// partition slots; one pass over the live list counts the members of those
+// slots in an LDS table per workgroup (open addressing), flushed with one
+// atomic per slot and workgroup.
+constexpr int VOTE_SAMPLES = 1024;
+constexpr int VOTE_HASH = 2048;  // LDS buckets, a power of two >= 2 * VOTE_SAMPLES
+
+__device__ __forceinline__ int vote_bucket(int32_t slot) {
+  return (int)(((uint32_t)slot * 2654435761u) >> 21) & (VOTE_HASH - 1);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_sample_slots(const DevCtl* __restrict__ dc, int32_t n,
+                                                        const int32_t* __restrict__ color,
+                                                        const int32_t* __restrict__ L0, const int32_t* __restrict__ L1,
+                                                        int32_t stride, int32_t m, int32_t* __restrict__ sslots) {
+  const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
+  const int e = blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= m) return;
+  const int32_t c = color[list[(int64_t)e * stride]];
+  sslots[e] = c != DONE ? slot_of(c, n) : -1;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_part_count_sampled(const DevCtl* __restrict__ dc, int32_t n,
+                                                              const int32_t* __restrict__ color,
+                                                              const int32_t* __restrict__ L0,
+                                                              const int32_t* __restrict__ L1,
+                                                              const int32_t* __restrict__ sslots, int32_t m,
+                                                              int32_t* __restrict__ pcount) {
+  __shared__ int32_t hk[VOTE_HASH], hc[VOTE_HASH];
+  for (int i = threadIdx.x; i < VOTE_HASH; i += BLOCK) {
+    hk[i] = -1;
+    hc[i] = 0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < m; e += BLOCK) {
+    const int32_t slot = sslots[e];
+    if (slot < 0) continue;
+    for (int h = vote_bucket(slot);; h = (h + 1) & (VOTE_HASH - 1)) {
+      const int32_t old = atomicCAS(&hk[h], -1, slot);
+      if (old == -1 || old == slot) break;
+    }
+  }
+  __syncthreads();
+  const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
+  const int64_t n_in = (int64_t)dc->live.n;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_in; i += (int64_t)gridDim.x * BLOCK) {
+    const int32_t c = color[list[i]];
+    if (c == DONE) continue;
+    const int32_t slot = slot_of(c, n);
+    for (int h = vote_bucket(slot); hk[h] != -1; h = (h + 1) & (VOTE_HASH - 1)) {
+      if (hk[h] == slot) {
+        atomicAdd(&hc[h], 1);
+        break;
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < VOTE_HASH; i += BLOCK)
+    if (hc[i] > 0) atomicAdd(&pcount[hk[i]], hc[i]);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_part_clear_sampled(const int32_t* __restrict__ sslots, int32_t m,
+                                                              int32_t* __restrict__ pcount) {
+  const int e = blockIdx.x * BLOCK + threadIdx.x;
+  if (e < m && sslots[e] >= 0) pcount[sslots[e]] = 0;
+}
+
 // Inference over the live list: pivot scores for every live vertex and the sum
 // of sigmoid(head 1) (phase vote) into acc[0], count into acc[1].
 __global__ __launch_bounds__(BLOCK) void k_sched_infer(const DevCtl* __restrict__ dc, GraphPtrs G,
@@ -498,6 +566,7 @@ struct Sched {
   float* acc = nullptr;     // inference vote accumulators (2) ...
   float* h_acc = nullptr;   // pinned mirror
   int32_t* pcount = nullptr;  // partition sizes (n + 1)
+  int32_t* sslots = nullptr;  // partition slots of the vote's samples (VOTE_SAMPLES)
   int32_t pcount_n = -1;
   int64_t adam_t = 0;
   // dataset buffers for training / evaluation
@@ -524,7 +593,8 @@ int ensure_sched(scc_ctx* c) {
         (rc = dev_alloc(c, (void**)&s->G, 4ull * (NPARAMS + 1), "scheduler gradients")) ||
         (rc = dev_alloc(c, (void**)&s->m1, 4ull * NPARAMS, "scheduler moments")) ||
         (rc = dev_alloc(c, (void**)&s->m2, 4ull * NPARAMS, "scheduler moments")) ||
-        (rc = dev_alloc(c, (void**)&s->acc, 64, "scheduler vote")))
+        (rc = dev_alloc(c, (void**)&s->acc, 64, "scheduler vote")) ||
+        (rc = dev_alloc(c, (void**)&s->sslots, 4ull * VOTE_SAMPLES, "vote sample slots")))
       return rc;
     if ((rc = hip_check(c, hipHostMalloc((void**)&s->h_acc, 64, 0), "pinned vote"))) return rc;
     if ((rc = hip_check(c, hipMemsetAsync(s->m1, 0, 4ull * NPARAMS, c->stream), "moments")) ||
@@ -534,17 +604,21 @@ int ensure_sched(scc_ctx* c) {
   return SCC_OK;
 }
 
+// Per-vertex buffers of the current graph: partition sizes and pivot scores,
+// both (re)sized together whenever the vertex count changes.
 int ensure_pcount(scc_ctx* c, Sched* s) {
-  if (s->pcount_n == c->n && s->pcount) return SCC_OK;
+  if (s->pcount_n == c->n && s->pcount && c->sched_scores) return SCC_OK;
   dev_free(c, s->pcount);
+  dev_free(c, c->sched_scores);
   s->pcount = nullptr;
-  int rc = dev_alloc(c, (void**)&s->pcount, 4ull * ((uint64_t)c->n + 1), "partition sizes");
-  if (rc) return rc;
+  c->sched_scores = nullptr;
+  s->pcount_n = -1;
+  int rc;
+  if ((rc = dev_alloc(c, (void**)&s->pcount, 4ull * ((uint64_t)c->n + 1), "partition sizes")) ||
+      (rc = dev_alloc(c, (void**)&c->sched_scores, 4ull * ((uint64_t)c->n + 1), "pivot scores")))
+    return rc;
   launch_fill_i32(c->stream, s->pcount, 0, c->n + 1);
   s->pcount_n = c->n;
-  if (!c->sched_scores) {
-    if ((rc = dev_alloc(c, (void**)&c->sched_scores, 4ull * ((uint64_t)c->n + 1), "pivot scores"))) return rc;
-  }
   return SCC_OK;
 }
 
@@ -625,15 +699,32 @@ int clear_partitions(scc_ctx* c, Sched* s) {
 // vertices when scores are wanted).  Returns the mean phase vote.
 int infer(scc_ctx* c, Sched* s, bool scores, double* vote) {
   int rc;
-  if ((rc = count_partitions(c, s))) return rc;
-  const RoundFeat rf = round_features(c);
-  const int32_t stride = scores ? 1 : (int32_t)std::max<int64_t>(1, c->n_live / 65536);
-  (void)hipMemsetAsync(s->acc, 0, 8, c->stream);
+  // scores: every live vertex, with every partition's size; the vote alone:
+  // at most VOTE_SAMPLES evenly spaced entries, with their partitions' sizes
+  const int32_t stride =
+      scores ? 1 : (int32_t)std::max<int64_t>(1, (c->n_live + VOTE_SAMPLES - 1) / VOTE_SAMPLES);
   const int64_t m = (c->n_live + stride - 1) / stride;
+  if (scores) {
+    if ((rc = count_partitions(c, s))) return rc;
+  } else {
+    if ((rc = ensure_pcount(c, s))) return rc;
+    const unsigned sg = (unsigned)std::max<int64_t>(1, (m + BLOCK - 1) / BLOCK);
+    LAUNCH(k_sample_slots, sg, c->stream, c->dc, c->n, c->color, c->live[0], c->live[1], stride, (int32_t)m,
+           s->sslots);
+    LAUNCH(k_part_count_sampled, grid_for(c->n_live, (unsigned)c->tune.grid_cap), c->stream, c->dc, c->n, c->color,
+           c->live[0], c->live[1], s->sslots, (int32_t)m, s->pcount);
+  }
+  const RoundFeat rf = round_features(c);
+  (void)hipMemsetAsync(s->acc, 0, 8, c->stream);
   const unsigned grid = grid_for(std::max<int64_t>(1, (m + 15) / 16) * 64, 2048);
   LAUNCH(k_sched_infer, grid, c->stream, c->dc, gp(c), c->color, s->pcount, rf, s->P,
          scores ? c->sched_scores : nullptr, s->acc, c->live[0], c->live[1], stride);
-  if ((rc = clear_partitions(c, s))) return rc;
+  if (scores) {
+    if ((rc = clear_partitions(c, s))) return rc;
+  } else {
+    LAUNCH(k_part_clear_sampled, (unsigned)std::max<int64_t>(1, (m + BLOCK - 1) / BLOCK), c->stream, s->sslots,
+           (int32_t)m, s->pcount);
+  }
   if ((rc = hip_check(c, hipMemcpyAsync(s->h_acc, s->acc, 8, hipMemcpyDeviceToHost, c->stream), "vote")) ||
       (rc = hip_check(c, hipStreamSynchronize(c->stream), "vote sync")))
     return rc;
@@ -714,12 +805,23 @@ bool sched_ready(const scc_ctx* c) {
   return s && s->have_weights;
 }
 
+// The network overrides the plain rule (alternate, starting with a
+// forward-backward step) only when its mean vote is far from even.
+constexpr double SCHED_MARGIN = 0.25;
+
 int sched_choose(scc_ctx* c, int* action) {
   Sched* s = sched_of(c);
   double vote = 0;
   int rc = infer(c, s, false, &vote);
   if (rc) return rc;
-  *action = vote > 0.5 ? 1 : 0;
+  int plain = 0;
+  for (auto it = c->trace.rbegin(); it != c->trace.rend(); ++it) {
+    if (it->kind == SCC_PHASE_FB_ROUND || it->kind == SCC_PHASE_COLOR_ROUND) {
+      plain = it->kind == SCC_PHASE_FB_ROUND ? 1 : 0;
+      break;
+    }
+  }
+  *action = vote > 0.5 + SCHED_MARGIN ? 1 : (vote < 0.5 - SCHED_MARGIN ? 0 : plain);
   return SCC_OK;
 }
 
@@ -732,7 +834,7 @@ int sched_pivots(scc_ctx* c) {
 void destroy_sched(scc_ctx* c) {
   Sched* s = sched_of(c);
   if (!s) return;
-  void* ptrs[] = {s->P, s->G, s->m1, s->m2, s->acc, s->pcount, s->dX, s->dY, s->dM, s->dO, s->didx,
+  void* ptrs[] = {s->P, s->G, s->m1, s->m2, s->acc, s->sslots, s->pcount, s->dX, s->dY, s->dM, s->dO, s->didx,
                   s->dverts, s->dsx, c->sched_scores};
   for (void* p : ptrs) dev_free(c, p);
   c->sched_scores = nullptr;
@@ -955,12 +1057,15 @@ int scc_sched_rollout(scc_ctx* c, const scc_run_opts* opts_in, uint64_t seed, fl
       }
       rows += k;
     }
-    // counterfactual: from the same state, each step followed by alternating
-    // steps (the other kind next) until nothing is live; the better step is
-    // the one whose decomposition finishes first
+    // counterfactual: from the same state, each step followed by the
+    // continuation policy until nothing is live (the installed network's
+    // choices, else alternating steps: the other kind next); the better step
+    // is the one whose decomposition finishes first
     if ((rc = take_snap(c, sn))) return rc;
     double total[2];
-    for (int a = 0; a < 2; a++) {
+    const int first = U(rng) < 0.5f ? 0 : 1;  // random order: neither branch always runs first
+    for (int k = 0; k < 2; k++) {
+      const int a = k == 0 ? first : 1 - first;
       auto t0 = std::chrono::steady_clock::now();
       int step = a;
       for (int64_t guard = 0; c->n_live > 0; guard++) {
@@ -968,6 +1073,7 @@ int scc_sched_rollout(scc_ctx* c, const scc_run_opts* opts_in, uint64_t seed, fl
         if (guard > 4 * (int64_t)c->n + 64) return set_error(c, SCC_ERR_LIMIT, "rollout did not converge");
         if ((rc = run_action(c, step, false))) return rc;
         step = 1 - step;
+        if (learned && c->n_live > 0 && (rc = sched_choose(c, &step))) return rc;
       }
       total[a] = ms_since(t0);
       if ((rc = restore_snap(c, sn))) return rc;

@@ -7,10 +7,13 @@ module drives them:
 * traces: counterfactual decompositions of synthetic graphs
   (SccEngine.sched_rollout) -- at each partition round, the forward-backward
   step and the colouring step are each taken from the same state and followed
-  by alternating steps to the end; the trace records which finished sooner,
-  and every sampled vertex's SCC size relative to its partition;
+  by the continuation policy to the end; the trace records which finished
+  sooner, and every sampled vertex's SCC size relative to its partition;
 * training: Adam on the device (SccEngine.sched_train), from seeded He
-  initial weights;
+  initial weights; the first round of traces continues with alternating
+  steps, and further rounds (--iterations) continue with the network trained
+  so far (policy iteration: its one-step improvements are then measured
+  against its own later choices);
 * evaluation: wall time of the plain method (degree-product pivots, a
   forward-backward step then a colouring step per round) against the learned
   policy on held-out graphs, labels compared for equality.
@@ -79,12 +82,14 @@ def held_out_graphs():
             ("rmat20_ef3_s7", lambda: graphs.rmat(20, 3, seed=7))]
 
 
-def generate(specs, rollouts: int = 2, explore: float = 0.25, samples_per_round: int = 512, log=print):
+def generate(specs, rollouts: int = 2, explore: float = 0.25, samples_per_round: int = 512, log=print,
+             weights=None):
     Xs, Ys, Ms = [], [], []
     with SccEngine() as e:
         for name, make in specs:
             g = make()
             e.upload(g)
+            e.set_sched_weights(weights)
             for r in range(rollouts):
                 X, Y, M, st = e.sched_rollout(seed=1000 * r + 7, explore=explore,
                                               samples_per_round=samples_per_round)
@@ -96,9 +101,9 @@ def generate(specs, rollouts: int = 2, explore: float = 0.25, samples_per_round:
     return np.concatenate(Xs), np.concatenate(Ys), np.concatenate(Ms)
 
 
-def train(X, Y, M, epochs: int = 40, batch: int = 1024, lr: float = 2e-3, seed: int = 0):
+def train(X, Y, M, epochs: int = 40, batch: int = 1024, lr: float = 2e-3, seed: int = 0, start=None):
     with SccEngine() as e:
-        e.set_sched_weights(initial_weights(seed))
+        e.set_sched_weights(initial_weights(seed) if start is None else start)
         loss = e.sched_train(X, Y, M, epochs=epochs, batch=batch, lr=lr, seed=seed)
         return e.sched_weights(), loss
 
@@ -146,16 +151,21 @@ def main(argv=None):
     ap.add_argument("--rollouts", type=int, default=2)
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--iterations", type=int, default=1,
+                    help="rounds of traces + training (policy iteration beyond 1)")
     args = ap.parse_args(argv)
 
     def log(*a):
         print(*a, flush=True)
 
-    t = time.time()
-    X, Y, M = generate(training_graphs(), rollouts=args.rollouts, log=log)
-    log(f"traces: {X.shape[0]} samples in {time.time() - t:.1f}s")
-    w, loss = train(X, Y, M, epochs=args.epochs, seed=args.seed)
-    log(f"trained: last batch loss {loss:.4f}")
+    w = None
+    for it in range(args.iterations):
+        # each round learns from traces of its own continuation policy only
+        t = time.time()
+        X, Y, M = generate(training_graphs(100 + 10 * it), rollouts=args.rollouts, log=log, weights=w)
+        log(f"iteration {it}: {X.shape[0]} samples in {time.time() - t:.1f}s")
+        w, loss = train(X, Y, M, epochs=args.epochs, seed=args.seed, start=w)
+        log(f"iteration {it}: last batch loss {loss:.4f}")
     os.makedirs(args.out, exist_ok=True)
     np.save(os.path.join(args.out, "sched_weights.npy"), w, allow_pickle=False)
     ev = evaluate(w, held_out_graphs(), log=log)

@@ -250,7 +250,8 @@ typedef struct scc_rollout_stats {
 
 /* Phase traces for training: a from-scratch decomposition in which, at every
  * partition round, each of the two steps is taken from the same state and
- * followed by alternating steps until nothing is live (timed), the state is
+ * followed by the continuation policy until nothing is live (timed; the
+ * installed network's choices, else alternating steps), the state is
  * restored, up to samples_per_round sampled live vertices are recorded
  * (features in X, head-1 target Y1 = 1 when the colouring-first continuation
  * finished sooner), and the run goes on with the better step (or the installed

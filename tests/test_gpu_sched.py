@@ -111,12 +111,32 @@ def test_rollout_samples_and_exact_labels():
             assert np.array_equal(e.labels(), want), mode_kw
 
 
+def test_learned_pivots_follow_a_larger_upload():
+    """Pivot scores are per vertex of the current graph: after a larger graph
+    replaces a smaller one, learned pivots must score all of its vertices."""
+    from dynamicgraphs_scc_amd.sched import load_weights
+    with SccEngine() as e:
+        e.set_sched_weights(load_weights())
+        for n, seed in ((20000, 5), (90000, 6)):
+            g = graphs.planted(n, seed=seed, n_medium=n // 60)
+            want = oracle.scc_canonical(g, 8)
+            e.upload(g)
+            for kw in ({"pivot_policy": PIVOT_LEARNED}, {"pivot_policy": PIVOT_LEARNED,
+                                                         "partition_mode": PARTS_LEARNED}):
+                e.reset()
+                e.run_method2(default_opts(**kw))
+                assert np.array_equal(e.labels(), want), (n, kw)
+
+
 def test_learned_policy_without_weights_falls_back():
     g = graphs.planted(20000, seed=9, n_medium=300)
     want = oracle.scc_canonical(g, 8)
     with SccEngine() as e:
         e.upload(g)
         e.run_method2(default_opts(partition_mode=PARTS_LEARNED))
+        assert np.array_equal(e.labels(), want)
+        e.reset()
+        e.run_method2(default_opts(pivot_policy=PIVOT_LEARNED))
         assert np.array_equal(e.labels(), want)
         with pytest.raises(SccError):
             e.sched_weights()
