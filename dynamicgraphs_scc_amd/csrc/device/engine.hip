@@ -231,9 +231,7 @@ int alloc_graph(scc_ctx* c, int32_t n, int64_t m_cap) {
 }
 
 int reset_state(scc_ctx* c) {
-  launch_fill_i32(c->stream, c->color, UNSET, c->n);
-  launch_fill_i32(c->stream, c->label, -1, c->n);
-  launch_iota(c->stream, c->live[0], c->n);
+  launch_reset(c->stream, c->dc, c->n, c->color, c->label, c->live[0]);
   c->labels_ready = false;
   c->multi_colors = false;
   c->colors_split = false;
@@ -241,9 +239,9 @@ int reset_state(scc_ctx* c) {
   c->trace.clear();
   c->round_idx = -1;
   c->fresh = true;
-  int rc = zero_ctr(c);
-  if (rc) return rc;
-  return set_live(c, 0, c->n);
+  c->cur = 0;
+  c->n_live = c->n;
+  return hip_check(c, hipGetLastError(), "reset");
 }
 
 // ---- phases -----------------------------------------------------------------------
@@ -379,8 +377,7 @@ int run_levels(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed) {
 // The forward and the backward search from the partitions' pivots, advancing
 // together (one launch runs a level of each); post as in level_loop.
 int traverse_both(scc_ctx* c, bool multi, int64_t* levels, bool timed, const std::function<void()>& post) {
-  const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
-  launch_ctl_init(c->stream, c->dc, 8 | 32, 0, 0, 0, avg / c->tune.bu_alpha);
+  // (both search controls were set by the step's set-up launch)
   launch_seed_pivots2(c->stream, grid_for(c->n_live, (unsigned)c->tune.grid_cap), c->dc, c->n, level_dir(c, true, true),
                       level_dir(c, false, true), c->color, c->pkey, c->live[0], c->live[1]);
   return level_loop(c, true, true, multi, levels, timed, post);
@@ -409,21 +406,26 @@ int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int6
   if (!levels) levels = &lv;
   const int64_t lv0 = *levels;
   const unsigned grid = grid_for(c->n_live, (unsigned)c->tune.grid_cap);
-  if ((rc = zero_ctr(c))) return rc;
   // learned pivots need the network (else the plain rule); sched_pivots
   // writes this graph's scores for every live vertex first
   if (policy == SCC_PIVOT_LEARNED && !sched_ready(c)) policy = SCC_PIVOT_DEGREE_PRODUCT;
   if (pivot_override < 0 && policy == SCC_PIVOT_LEARNED && (rc = sched_pivots(c))) return rc;
-  launch_reset_slots(c->stream, grid, c->dc, c->n, c->color, c->pkey, c->rep, c->live[0], c->live[1],
-                     multi ? 0 : 1);
+  // counters, both search controls and (one partition) its slot; several
+  // partitions reset their slots from the live list
+  const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
+  launch_fb_setup(c->stream, c->dc, c->n, c->pkey, c->rep, multi ? 0 : 1, avg / c->tune.bu_alpha);
+  if (multi)
+    launch_reset_slots(c->stream, grid, c->dc, c->n, c->color, c->pkey, c->rep, c->live[0], c->live[1], 0);
+  const int64_t words = ((int64_t)c->n + 31) / 32;
   if (pivot_override >= 0) {
     if ((rc = force_pivot(c, pivot_override))) return rc;
+    launch_fill_u32(c->stream, c->vf, 0u, 2 * words);
   } else {
+    // (also clears vf and vb)
     launch_pivot_key(c->stream, grid, c->dc, gp(c), c->color, policy,
-                     policy == SCC_PIVOT_LEARNED ? c->sched_scores : nullptr, c->pkey, c->live[0], c->live[1]);
+                     policy == SCC_PIVOT_LEARNED ? c->sched_scores : nullptr, c->pkey, c->live[0], c->live[1],
+                     c->vf, 2 * words);
   }
-  const int64_t words = ((int64_t)c->n + 31) / 32;
-  launch_fill_u32(c->stream, c->vf, 0u, 2 * words);
   // reach-set sizes (when asked), FW n BW -> done, the rest -> new partitions:
   // queued behind the level batches, they run once both searches have ended
   auto post = [&]() {
@@ -556,7 +558,6 @@ int do_fb_parts(scc_ctx* c, int policy, int mode, bool trim_between, int64_t* ro
 int do_canon(scc_ctx* c, int64_t* count) {
   int rc;
   PhaseRec pr(c, SCC_PHASE_CANON);
-  if ((rc = zero_ctr(c))) return rc;
   launch_canon(c->stream, c->dc, c->n, c->label, c->scratch_i32, c->canon);
   if ((rc = pull_ctl(c))) return rc;
   pr.finish((int64_t)c->h_dc->ctr[C_COUNT]);

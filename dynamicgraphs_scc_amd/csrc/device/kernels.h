@@ -40,9 +40,17 @@ void launch_trim2(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, int32_
 void launch_reset_slots(hipStream_t st, unsigned grid, const DevCtl* dc, int32_t n,
                         const int32_t* color, unsigned long long* pkey, int32_t* rep,
                         const int32_t* L0, const int32_t* L1, int single);
+// per-partition pivot keys; also zeroes zbits[0, zwords) (the step's reach bitmaps)
 void launch_pivot_key(hipStream_t st, unsigned grid, const DevCtl* dc, GraphPtrs G,
                       const int32_t* color, int policy, const uint32_t* scores,
-                      unsigned long long* pkey, const int32_t* L0, const int32_t* L1);
+                      unsigned long long* pkey, const int32_t* L0, const int32_t* L1, uint32_t* zbits,
+                      int64_t zwords);
+// fresh state: colours -1, labels -1, live list 0..n-1, counters zeroed, live = all
+void launch_reset(hipStream_t st, DevCtl* dc, int32_t n, int32_t* color, int32_t* label, int32_t* live0);
+// forward-backward step set-up: counters zeroed, both search controls, and with
+// single the slot of colour -1 reset
+void launch_fb_setup(hipStream_t st, DevCtl* dc, int32_t n, unsigned long long* pkey, int32_t* rep, int single,
+                     double bu_ratio);
 void launch_force_pivot(hipStream_t st, int32_t p, int32_t n, const int32_t* color,
                         unsigned long long* pkey);
 void launch_seed_list(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* off, const int32_t* offe,
@@ -82,6 +90,8 @@ void launch_wcc_color(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color,
                       const int32_t* L1);
 void launch_extract(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, int32_t* label,
                     const uint32_t* V, int32_t* L0, int32_t* L1);
+// canonical labels into out (mn: n ints of scratch); zeroes the counters first
+// and leaves the SCC count in dc->ctr[C_COUNT]
 void launch_canon(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label, int32_t* mn,
                   int32_t* out);
 

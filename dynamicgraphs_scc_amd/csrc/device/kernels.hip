@@ -91,9 +91,10 @@ __device__ void init_level(LevelCtl& v, double bu_ratio) {
   v.bu_edges_per_cand = bu_ratio;
 }
 
-__global__ void k_ctl_init(DevCtl* dc, int what, int32_t live_sel, unsigned long long live_n,
-                           long long stop, double bu_ratio) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// what: bit 0 counters, 1 live list, 2 loop control, 3 and 5 the two level
+// controls, 4 tuning defaults.  One thread.
+__device__ void ctl_init_body(DevCtl* dc, int what, int32_t live_sel, unsigned long long live_n, long long stop,
+                              double bu_ratio) {
   if (what & 1) {
     for (int i = 0; i < C_NUM; i++) dc->ctr[i] = 0;
     dc->blocks_done = 0;
@@ -120,6 +121,36 @@ __global__ void k_ctl_init(DevCtl* dc, int what, int32_t live_sel, unsigned long
   }
   if (what & 8) init_level(dc->lv, bu_ratio);
   if (what & 32) init_level(dc->lv2, bu_ratio);
+}
+
+__global__ void k_ctl_init(DevCtl* dc, int what, int32_t live_sel, unsigned long long live_n,
+                           long long stop, double bu_ratio) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) ctl_init_body(dc, what, live_sel, live_n, stop, bu_ratio);
+}
+
+// Fresh decomposition state in one launch: colours -1, labels -1, the live
+// list 0..n-1, zeroed counters.
+__global__ __launch_bounds__(BLOCK) void k_reset(DevCtl* __restrict__ dc, int32_t n, int32_t* __restrict__ color,
+                                                 int32_t* __restrict__ label, int32_t* __restrict__ live0) {
+  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < n; v += (int64_t)gridDim.x * BLOCK) {
+    color[v] = UNSET;
+    label[v] = -1;
+    live0[v] = (int32_t)v;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl_init_body(dc, 1 | 2, 0, (unsigned long long)n, 0, 0.0);
+}
+
+// Set-up of a forward-backward step in one launch: zeroed counters, both
+// search controls, and (single: every live vertex has colour -1) the one
+// partition slot in use.
+__global__ void k_fb_setup(DevCtl* __restrict__ dc, int32_t n, unsigned long long* __restrict__ pkey,
+                           int32_t* __restrict__ rep, int single, double bu_ratio) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  ctl_init_body(dc, 1 | 8 | 32, 0, 0, 0, bu_ratio);
+  if (single) {
+    pkey[n] = 0ull;
+    rep[3 * (int64_t)n + 0] = rep[3 * (int64_t)n + 1] = rep[3 * (int64_t)n + 2] = INT32_MAX;
+  }
 }
 
 // ---- trim1 -------------------------------------------------------------------
@@ -323,7 +354,11 @@ __global__ __launch_bounds__(BLOCK) void k_pivot_key(const DevCtl* __restrict__ 
                                                      const uint32_t* __restrict__ scores,
                                                      unsigned long long* __restrict__ pkey,
                                                      const int32_t* __restrict__ L0,
-                                                     const int32_t* __restrict__ L1) {
+                                                     const int32_t* __restrict__ L1,
+                                                     uint32_t* __restrict__ zbits, int64_t zwords) {
+  // the step's reach bitmaps start empty (nothing here reads them)
+  for (int64_t w = (int64_t)blockIdx.x * BLOCK + threadIdx.x; w < zwords; w += (int64_t)gridDim.x * BLOCK)
+    zbits[w] = 0u;
   const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
   const int64_t n_in = (int64_t)dc->live.n;
   for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
@@ -1286,6 +1321,13 @@ __device__ int32_t block_hot_label(int32_t n, const int32_t* __restrict__ label)
   return s_hot;
 }
 
+// mn = identity (a label is always a vertex labelled with itself), counters zeroed.
+__global__ __launch_bounds__(BLOCK) void k_canon_init(DevCtl* __restrict__ dc, int32_t* __restrict__ mn, int32_t n) {
+  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < n; v += (int64_t)gridDim.x * BLOCK)
+    mn[v] = (int32_t)v;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl_init_body(dc, 1, 0, 0, 0, 0.0);
+}
+
 __global__ __launch_bounds__(BLOCK) void k_canon_min(int32_t n, const int32_t* __restrict__ label,
                                                      int32_t* __restrict__ mn) {
   const int lane = lane_id();
@@ -1393,8 +1435,16 @@ void launch_reset_slots(hipStream_t st, unsigned grid, const DevCtl* dc, int32_t
 }
 void launch_pivot_key(hipStream_t st, unsigned grid, const DevCtl* dc, GraphPtrs G,
                       const int32_t* color, int policy, const uint32_t* scores,
-                      unsigned long long* pkey, const int32_t* L0, const int32_t* L1) {
-  LAUNCH(k_pivot_key, grid, st, dc, G, color, policy, scores, pkey, L0, L1);
+                      unsigned long long* pkey, const int32_t* L0, const int32_t* L1, uint32_t* zbits,
+                      int64_t zwords) {
+  LAUNCH(k_pivot_key, grid, st, dc, G, color, policy, scores, pkey, L0, L1, zbits, zwords);
+}
+void launch_reset(hipStream_t st, DevCtl* dc, int32_t n, int32_t* color, int32_t* label, int32_t* live0) {
+  LAUNCH(k_reset, grid_for(n > 0 ? n : 1, 4096), st, dc, n, color, label, live0);
+}
+void launch_fb_setup(hipStream_t st, DevCtl* dc, int32_t n, unsigned long long* pkey, int32_t* rep, int single,
+                     double bu_ratio) {
+  hipLaunchKernelGGL(k_fb_setup, dim3(1), dim3(64), 0, st, dc, n, pkey, rep, single, bu_ratio);
 }
 void launch_force_pivot(hipStream_t st, int32_t p, int32_t n, const int32_t* color,
                         unsigned long long* pkey) {
@@ -1478,8 +1528,11 @@ void launch_extract(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, i
 }
 void launch_canon(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label, int32_t* mn,
                   int32_t* out) {
-  if (n <= 0) return;
-  LAUNCH(k_iota, grid_for(n, 4096), st, mn, n);
+  if (n <= 0) {
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, dc, 1, 0, 0ull, 0ll, 0.0);
+    return;
+  }
+  LAUNCH(k_canon_init, grid_for(n, 4096), st, dc, mn, n);
   LAUNCH(k_canon_min, grid_for(n), st, n, label, mn);
   LAUNCH(k_canon_apply, grid_for((n + 3) / 4, 1024), st, dc, n, label, mn, out);
 }
