@@ -65,7 +65,7 @@ struct LoopCtl {  // trim sweeps and WCC iterations
   int32_t done;
   int32_t iters;
   uint32_t blocks_done;
-  int32_t pad;
+  int32_t epoch;  // trim loop the state belongs to (a sweep of a new loop starts it afresh)
   long long stop;               // trim: stop when a sweep trims <= stop
   unsigned long long acc0, acc1;  // per-launch accumulators
   unsigned long long total0;      // sum of acc0 over launches
@@ -458,6 +458,7 @@ struct scc_ctx {
   int32_t lv_hint[3] = {8, 8, 8};         // level launches per search batch (fw, bw, both)
   int32_t it_hint = 4;                    // WCC iterations per batch
   int32_t trim_hint[2] = {3, 3};          // trim sweeps per batch (absolute stop, relative stop)
+  int32_t trim_epoch = 0;                 // number of the last trim loop queued
   bool labels_ready = false;              // canon[] valid for the current graph
   bool ran = false;
 

@@ -14,7 +14,6 @@ int dev_alloc(scc_ctx* c, void** p, uint64_t bytes, const char* what);
 void dev_free(scc_ctx* c, void* p);
 GraphPtrs gp(const scc_ctx* c);
 int pull_ctl(scc_ctx* c);   // control block -> host (syncs the stream)
-int sync_ctr(scc_ctx* c);   // same
 int zero_ctr(scc_ctx* c);   // zero DevCtl::ctr (queued)
 int set_live(scc_ctx* c, int sel, int64_t n);
 int check_cancel(scc_ctx* c);

@@ -100,7 +100,6 @@ int pull_ctl(scc_ctx* c) {
   return SCC_OK;
 }
 
-int sync_ctr(scc_ctx* c) { return pull_ctl(c); }
 
 // One phase-trace record: construct at phase entry, finish() after the
 // phase's final synchronisation.
@@ -265,7 +264,7 @@ int do_trim1(scc_ctx* c, int64_t stop, int64_t* trimmed) {
   if (trimmed) *trimmed = 0;
   if (c->n_live == 0) return SCC_OK;
   PhaseRec pr(c, SCC_PHASE_TRIM1);
-  launch_ctl_init(c->stream, c->dc, 4, 0, 0, (long long)stop, 0.0);
+  const int32_t epoch = ++c->trim_epoch;
   const unsigned grid = grid_for(c->n_live, (unsigned)c->tune.grid_cap);
   // queue as many sweeps as the last loop of this kind needed (a sweep that
   // finds the loop ended returns at once)
@@ -275,7 +274,7 @@ int do_trim1(scc_ctx* c, int64_t stop, int64_t* trimmed) {
     if ((rc = check_cancel(c))) return rc;
     for (int i = 0; i < batch; i++) {
       launch_trim1_step(c->stream, grid, c->dc, gp(c), c->color, c->label, c->live[0], c->live[1],
-                        c->fresh ? 1 : 0);
+                        c->fresh ? 1 : 0, (long long)stop, epoch);
       c->fresh = false;
     }
     if ((rc = pull_ctl(c))) return rc;

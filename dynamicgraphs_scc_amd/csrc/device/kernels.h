@@ -31,8 +31,9 @@ void launch_iota(hipStream_t st, int32_t* p, int32_t n);
 // defaults, 5 the second level control (backward search run alongside)
 void launch_ctl_init(hipStream_t st, DevCtl* dc, int what, int32_t live_sel, unsigned long long live_n,
                      long long stop, double bu_ratio);
+// one sweep of trim loop number epoch (a new number starts a loop)
 void launch_trim1_step(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, int32_t* color,
-                       int32_t* label, int32_t* L0, int32_t* L1, int fresh);
+                       int32_t* label, int32_t* L0, int32_t* L1, int fresh, long long stop, int32_t epoch);
 void launch_compact(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* color, int32_t* L0,
                     int32_t* L1);
 void launch_trim2(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, int32_t* color,

@@ -110,6 +110,7 @@ __device__ void ctl_init_body(DevCtl* dc, int what, int32_t live_sel, unsigned l
     l.iters = 0;
     l.blocks_done = 0;
     l.stop = stop;
+    l.epoch = 0;  // trim loops number themselves from 1
     l.acc0 = l.acc1 = l.total0 = 0;
   }
   if (what & 16) {
@@ -161,14 +162,17 @@ __global__ void k_fb_setup(DevCtl* __restrict__ dc, int32_t n, unsigned long lon
 // monotone, so the fixpoint is the same (the unique maximal set where every
 // vertex keeps a same-colour in- and out-neighbour).  One launch is one sweep;
 // the last workgroup ends the loop when a sweep trims <= stop
-// (src/scc_trim1.cc:195-209 repeat_global_trim1).
+// (src/scc_trim1.cc:195-209 repeat_global_trim1).  epoch numbers the loop:
+// loop state of another epoch is stale, so the first sweep of a loop needs no
+// separate initialisation (the per-sweep accumulators are left zero by every
+// sweep's last workgroup).
 __global__ __launch_bounds__(BLOCK) void k_trim1_step(DevCtl* __restrict__ dc, GraphPtrs G,
                                                       int32_t* __restrict__ color,
                                                       int32_t* __restrict__ label,
                                                       int32_t* __restrict__ L0, int32_t* __restrict__ L1,
-                                                      int fresh) {
+                                                      int fresh, long long stop, int32_t epoch) {
   __shared__ int s_done;
-  if (threadIdx.x == 0) s_done = dc->loop.done;
+  if (threadIdx.x == 0) s_done = dc->loop.epoch == epoch ? dc->loop.done : 0;
   __syncthreads();
   if (s_done) return;
   const int32_t* __restrict__ list_in = LIVE_IN(dc, L0, L1);
@@ -225,13 +229,19 @@ __global__ __launch_bounds__(BLOCK) void k_trim1_step(DevCtl* __restrict__ dc, G
   if (last_block2(dc->done) && threadIdx.x == 0) {
     LoopCtl& l = dc->loop;
     const unsigned long long t = ld_acq(&l.acc0), out = ld_acq(&l.acc1);
+    if (l.epoch != epoch) {  // the loop's first sweep
+      l.epoch = epoch;
+      l.total0 = 0;
+      l.iters = 0;
+      l.stop = stop;
+    }
     l.total0 += t;
     l.iters++;
     // stop >= 0: a sweep trimming at most `stop` vertices ends the loop (the
     // reference's TRIM_STOP); stop < 0: one trimming at most -stop percent of
     // the vertices it examined does
-    const bool small = l.stop >= 0 ? (long long)t <= l.stop
-                                   : t * 100ull <= (unsigned long long)(-l.stop) * (t + out);
+    const bool small = stop >= 0 ? (long long)t <= stop
+                                 : t * 100ull <= (unsigned long long)(-stop) * (t + out);
     l.done = small || out == 0;
     l.acc0 = l.acc1 = 0;
     l.blocks_done = 0;
@@ -1417,8 +1427,8 @@ void launch_ctl_init(hipStream_t st, DevCtl* dc, int what, int32_t live_sel, uns
   hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, dc, what, live_sel, live_n, stop, bu_ratio);
 }
 void launch_trim1_step(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, int32_t* color,
-                       int32_t* label, int32_t* L0, int32_t* L1, int fresh) {
-  LAUNCH(k_trim1_step, grid, st, dc, G, color, label, L0, L1, fresh);
+                       int32_t* label, int32_t* L0, int32_t* L1, int fresh, long long stop, int32_t epoch) {
+  LAUNCH(k_trim1_step, grid, st, dc, G, color, label, L0, L1, fresh, stop, epoch);
 }
 void launch_compact(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* color, int32_t* L0,
                     int32_t* L1) {
