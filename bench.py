@@ -162,31 +162,48 @@ def batched_leg(args):
     del gs
     log(f"batched: {len(off) - 1} graphs, n={u.num_nodes} m={u.num_edges} in {time.time() - t:.1f}s")
 
-    def timed(e, reps):
+    from dynamicgraphs_scc_amd.engine import PARTS_LEARNED, default_opts
+    from dynamicgraphs_scc_amd.sched import load_weights
+
+    def timed(e, reps, **kw):
         ts = []
         for _ in range(reps):
             e.synchronize()
             t0 = time.perf_counter()
-            e.batch_run()
+            e.batch_run(default_opts(**kw))
             e.synchronize()
             ts.append((time.perf_counter() - t0) * 1e3)
         return float(np.median(ts))
 
+    weights = load_weights()
     with SccEngine(device=0) as e:
         e.upload(u)
         e.batch_set(off)
         timed(e, 1)
         ms_default = timed(e, 5)
+        counts = e.batch_counts()
+        lab_plain = e.labels()
+        ms_learned = same = None
+        if weights is not None:
+            e.set_sched_weights(weights)
+            timed(e, 1, partition_mode=PARTS_LEARNED)
+            ms_learned = timed(e, 5, partition_mode=PARTS_LEARNED)
+            same = bool(np.array_equal(lab_plain, e.labels()))
+            e.set_sched_weights(None)
+        del lab_plain
         tune = e.autotune(reps=2)
         timed(e, 1)
         ms_tuned = timed(e, 5)
-        counts = e.batch_counts()
-    return {"workload": "512 graphs, N log-uniform [1e3,1e6], even R-MAT EF U[8,32], odd planted, seeds 4000+g",
-            "graphs": int(len(off) - 1), "n": int(u.num_nodes), "m": int(u.num_edges),
-            "num_sccs": int(counts.sum()),
-            "ms_default": round(ms_default, 3), "gteps_default": round(u.num_edges / ms_default / 1e6, 4),
-            "ms_tuned": round(ms_tuned, 3), "gteps_tuned": round(u.num_edges / ms_tuned / 1e6, 4),
-            "tuning": tune["tuning"]}
+    out = {"workload": "512 graphs, N log-uniform [1e3,1e6], even R-MAT EF U[8,32], odd planted, seeds 4000+g",
+           "graphs": int(len(off) - 1), "n": int(u.num_nodes), "m": int(u.num_edges),
+           "num_sccs": int(counts.sum()),
+           "ms_default": round(ms_default, 3), "gteps_default": round(u.num_edges / ms_default / 1e6, 4),
+           "ms_tuned": round(ms_tuned, 3), "gteps_tuned": round(u.num_edges / ms_tuned / 1e6, 4),
+           "tuning": tune["tuning"]}
+    if ms_learned is not None:
+        out.update({"ms_learned": round(ms_learned, 3), "gteps_learned": round(u.num_edges / ms_learned / 1e6, 4),
+                    "learned_speedup": round(ms_default / ms_learned, 3), "learned_labels_equal": same})
+    return out
 
 
 def _sleep_until(srv, t_ms):
