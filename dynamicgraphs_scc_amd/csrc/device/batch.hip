@@ -72,17 +72,6 @@ __global__ __launch_bounds__(BLOCK) void k_batch_counts(int32_t n, const int32_t
 
 #define LAUNCH(kern, grid, st, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, st, __VA_ARGS__)
 
-// Fresh state with the batch colours (or the plain reset without a batch).
-int start_run(scc_ctx* c) {
-  int rc;
-  if ((rc = reset_state(c))) return rc;
-  if (c->batch_k <= 0) return SCC_OK;
-  LAUNCH(k_batch_colors, grid_for(c->n, 4096), c->stream, c->n, c->d_boff, c->batch_k, c->color);
-  c->multi_colors = true;
-  c->colors_split = true;
-  c->fresh = false;
-  return hip_check(c, hipGetLastError(), "batch colours");
-}
 
 int push_tuning(scc_ctx* c) {
   int rc = hip_check(c, hipMemcpyAsync(&c->dc->tune, &c->tune, sizeof(Tune), hipMemcpyHostToDevice, c->stream),
@@ -119,6 +108,18 @@ Tune from_api(const scc_tuning* t) {
 }
 
 }  // namespace
+
+// Fresh state with the batch colours (or the plain reset without a batch).
+int start_run(scc_ctx* c) {
+  int rc;
+  if ((rc = reset_state(c))) return rc;
+  if (c->batch_k <= 0) return SCC_OK;
+  LAUNCH(k_batch_colors, grid_for(c->n, 4096), c->stream, c->n, c->d_boff, c->batch_k, c->color);
+  c->multi_colors = true;
+  c->colors_split = true;
+  c->fresh = false;
+  return hip_check(c, hipGetLastError(), "batch colours");
+}
 
 }  // namespace sccmi
 

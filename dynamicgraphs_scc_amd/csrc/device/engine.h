@@ -18,6 +18,7 @@ int zero_ctr(scc_ctx* c);   // zero DevCtl::ctr (queued)
 int set_live(scc_ctx* c, int sel, int64_t n);
 int check_cancel(scc_ctx* c);
 int reset_state(scc_ctx* c);
+int start_run(scc_ctx* c);  // reset_state, then the batch's per-graph colours when a batch is set
 int alloc_graph(scc_ctx* c, int32_t n, int64_t m_cap);
 int build_order(scc_ctx* c);  // traversal copies of the rows, hubs first (order.hip)
 void drop_order(scc_ctx* c);

@@ -997,10 +997,12 @@ int scc_sched_rollout(scc_ctx* c, const scc_run_opts* opts_in, uint64_t seed, fl
   scc_rollout_stats st;
   std::memset(&st, 0, sizeof(st));
   auto t_all = std::chrono::steady_clock::now();
-  if ((rc = reset_state(c))) return rc;
+  // a batch of graphs starts from its per-graph partitions (scc_batch_run)
+  if ((rc = start_run(c))) return rc;
   // method-2 prefix: trim1 -> global FB -> trim1 / trim2 / trim1
   if ((rc = do_trim1(c, opts.trim_stop, nullptr))) return rc;
-  if (c->n_live && (rc = do_fb_round(c, opts.pivot_policy, false, -1, nullptr, nullptr, nullptr, false, nullptr)))
+  if (c->n_live &&
+      (rc = do_fb_round(c, opts.pivot_policy, c->colors_split, -1, nullptr, nullptr, nullptr, false, nullptr)))
     return rc;
   if ((rc = do_trim1(c, opts.trim_stop, nullptr))) return rc;
   if (opts.use_trim2 && c->n_live) {

@@ -60,11 +60,18 @@ def initial_weights(seed: int = 0) -> np.ndarray:
     return np.concatenate(parts).astype(np.float32)
 
 
+def batch_of(k: int, seed: int, n_hi: int):
+    """A disjoint union of the batched workload's kind (half R-MAT, half
+    planted) with its vertex offsets, for batch_set."""
+    u, off = graphs.disjoint_union(graphs.batched_config(k=k, seed=seed, n_hi=n_hi))
+    return u, off
+
+
 def training_graphs(seed: int = 100):
     """Synthetic graphs whose decompositions need partition rounds: planted SCC
-    graphs over a range of sizes and medium-SCC counts.  (R-MAT graphs leave no
-    partition rounds after the global step, trims and WCC, so they give no
-    samples.)"""
+    graphs over a range of sizes and medium-SCC counts, and two small batches of
+    the batched workload's kind.  (R-MAT graphs alone leave no partition rounds
+    after the global step, trims and WCC, so they give no samples.)"""
     specs = []
     rng = np.random.default_rng(seed)
     for i in range(10):
@@ -72,11 +79,15 @@ def training_graphs(seed: int = 100):
         med = int(n * rng.uniform(0.005, 0.04))
         specs.append((f"planted_n{n}_m{med}_s{seed + i}",
                       lambda n=n, med=med, s=seed + i: graphs.planted(n, seed=s, n_medium=med)))
+    for i in range(2):
+        specs.append((f"batch48_s{5000 + 100 * seed + 50 * i}",
+                      lambda s=5000 + 100 * seed + 50 * i: batch_of(48, s, 200_000)))
     return specs
 
 
 def held_out_graphs():
-    return [("planted8m_s3", lambda: graphs.planted(8_388_608, seed=3)),
+    return [("batch128_s4000", lambda: batch_of(128, 4000, 1_000_000)),
+            ("planted8m_s3", lambda: graphs.planted(8_388_608, seed=3)),
             ("planted1m_s4", lambda: graphs.planted(1_000_000, seed=4, n_medium=25_000)),
             ("rmat22_ef16_s22", lambda: graphs.rmat(22, 16, seed=22)),
             ("rmat20_ef3_s7", lambda: graphs.rmat(20, 3, seed=7))]
@@ -87,8 +98,10 @@ def generate(specs, rollouts: int = 2, explore: float = 0.25, samples_per_round:
     Xs, Ys, Ms = [], [], []
     with SccEngine() as e:
         for name, make in specs:
-            g = make()
+            g, off = _graph(make())
             e.upload(g)
+            if off is not None:
+                e.batch_set(off)
             e.set_sched_weights(weights)
             for r in range(rollouts):
                 X, Y, M, st = e.sched_rollout(seed=1000 * r + 7, explore=explore,
@@ -108,12 +121,21 @@ def train(X, Y, M, epochs: int = 40, batch: int = 1024, lr: float = 2e-3, seed: 
         return e.sched_weights(), loss
 
 
-def time_runs(e, opts, reps: int):
+def _graph(x):
+    """(graph, batch offsets or None) from a spec's result."""
+    return x if isinstance(x, tuple) else (x, None)
+
+
+def time_runs(e, opts, reps: int, batch: bool = False):
     ts = []
     for _ in range(reps):
         e.reset()
+        e.synchronize()
         t = time.perf_counter()
-        e.run_method2(opts)
+        if batch:
+            e.batch_run(opts)
+        else:
+            e.run_method2(opts)
         e.synchronize()
         ts.append((time.perf_counter() - t) * 1e3)
     return float(np.median(ts))
@@ -124,17 +146,20 @@ def evaluate(weights, specs, reps: int = 5, log=print):
     out = []
     with SccEngine() as e:
         for name, make in specs:
-            g = make()
+            g, off = _graph(make())
             e.upload(g)
+            if off is not None:
+                e.batch_set(off)
+            b = off is not None
             plain_opts = default_opts(partition_mode=PARTS_BOTH)
             e.set_sched_weights(None)
-            time_runs(e, plain_opts, 1)
-            t_plain = time_runs(e, plain_opts, reps)
+            time_runs(e, plain_opts, 1, b)
+            t_plain = time_runs(e, plain_opts, reps, b)
             lab_plain = e.labels()
             e.set_sched_weights(weights)
             learned_opts = default_opts(partition_mode=PARTS_LEARNED)
-            time_runs(e, learned_opts, 1)
-            t_learned = time_runs(e, learned_opts, reps)
+            time_runs(e, learned_opts, 1, b)
+            t_learned = time_runs(e, learned_opts, reps, b)
             same = bool(np.array_equal(lab_plain, e.labels()))
             e.set_sched_weights(None)
             rec = {"graph": name, "n": g.num_nodes, "m": g.num_edges, "ms_plain": round(t_plain, 3),
