@@ -3,10 +3,13 @@
 //
 // Data layout in HBM (per context, n vertices, m edges, all int32):
 //   begin[n+1] adj[m] rbegin[n+1] radj[m]            graph (CSR + reverse CSR)
+//   tadj[m] tradj[m]                                 traversal copies, hubs first (order.hip)
+//   end[n] rend[n]                                   row ends once updates made rows gapped
 //   color[n] label[n] canon[n] W[n]                  per-vertex state
+//   cw[n] (u64)                                      WCC words: colour << 32 | label
 //   live[2][n] q[2][n] cand[2][n] huge[2][n]         ping-pong vertex lists
 //   bq[2][n] bcand[2][n] bhuge[2][n]                 the backward search's lists
-//   vf[n/32] vb[n/32]                                forward / backward reach bitmaps
+//   vf[n/32] vb[n/32]                                forward / backward reach bitmaps (adjacent)
 //   pkey[n+1] (u64), rep[3(n+1)], scratch[n+1]       per-partition slots
 //   DevCtl                                           device-resident loop control
 // A partition is identified by its colour; colours are -1 (the initial
