@@ -206,7 +206,7 @@ int scc_ctx_set_tuning(scc_ctx* c, const scc_tuning* t) {
   if (!c) return SCC_ERR_ARG;
   (void)hipSetDevice(c->device);
   if (!t) {
-    c->tune = Tune{LIGHT, HUGE_ROW, 24.f, 14.f, GRID_CAP, 0};
+    c->tune = Tune{LEVEL_LIGHT, HUGE_ROW, 24.f, 14.f, GRID_CAP, 0};
     return push_tuning(c);
   }
   if (!tuning_ok(t)) return set_error(c, SCC_ERR_ARG, "tuning value out of range");

@@ -16,9 +16,11 @@ constexpr int32_t DONE = -2;   // colour of a vertex whose SCC is known
 constexpr int32_t UNSET = -1;  // initial colour
 constexpr int BLOCK = 256;     // threads per workgroup (4 wave64s)
 constexpr int WAVES = BLOCK / 64;
-constexpr int LIGHT = 16;      // default: row entries a lane scans alone before the wave helps
+constexpr int LIGHT = 16;      // trim row scans: row entries a lane scans alone before the wave helps
+constexpr int LEVEL_LIGHT = 32; // default of the same for search levels (tunable; planted graphs'
+                               // rows of ~10-30 entries are then scanned by their lane alone)
 constexpr int HUGE_ROW = 8192; // default: rows at least this long are expanded by many blocks
-constexpr int GRID_CAP = 1024; // default: workgroups of a sweep launch (4 per CU; more only adds
+constexpr int GRID_CAP = 640;  // default: workgroups of a sweep launch (2.5 per CU; more only adds
                                // per-workgroup set-up and flushes to the short levels)
 
 constexpr int32_t WCC_TAG = 0x20000000, WCC_MASK = 0x1FFFFFFF;  // src/scc_weak.cc:57-62
@@ -426,7 +428,7 @@ struct scc_ctx {
   // per-vertex state
   int32_t *color = nullptr, *label = nullptr, *canon = nullptr, *W = nullptr;
   unsigned long long* cw = nullptr;  // WCC iterations: colour << 32 | label
-  sccmi::Tune tune = {sccmi::LIGHT, sccmi::HUGE_ROW, 24.f, 14.f, sccmi::GRID_CAP, 0};  // host copy of dc->tune
+  sccmi::Tune tune = {sccmi::LEVEL_LIGHT, sccmi::HUGE_ROW, 24.f, 14.f, sccmi::GRID_CAP, 0};  // host copy of dc->tune
 
   // batched graphs: vertex ranges [boff[g], boff[g+1]) (scc_batch_set)
   int32_t batch_k = 0;

@@ -114,7 +114,7 @@ __device__ void ctl_init_body(DevCtl* dc, int what, int32_t live_sel, unsigned l
     l.acc0 = l.acc1 = l.total0 = 0;
   }
   if (what & 16) {
-    dc->tune.light = LIGHT;
+    dc->tune.light = LEVEL_LIGHT;
     dc->tune.huge_row = HUGE_ROW;
     dc->tune.td_beta = 24.f;
     dc->tune.bu_alpha = 14.f;

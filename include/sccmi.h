@@ -326,9 +326,9 @@ int scc_batch_counts(scc_ctx* ctx, int64_t* counts);
  * gm_graph/inc/gm_bfs_template.h:424, :841-842.) */
 typedef struct scc_tuning {
   int32_t light_row; /* search levels: a lane scans the first light_row entries of a row alone, the
-                        wave the rest; one of 4, 8, 16, 32, 64 (16) */
+                        wave the rest; one of 4, 8, 16, 32, 64 (32) */
   int32_t huge_row;  /* top-down frontier rows at least this long are split over the grid (8192) */
-  int32_t grid_cap;  /* workgroups per sweep launch (1024) */
+  int32_t grid_cap;  /* workgroups per sweep launch (640) */
   int32_t pad;
   float bu_alpha;    /* top-down -> bottom-up when frontier edges > candidates' edges / bu_alpha (14) */
   float td_beta;     /* bottom-up -> top-down when frontier < live / td_beta and shrinking (24) */

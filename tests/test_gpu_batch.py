@@ -90,7 +90,7 @@ def test_labels_exact_under_tuning(knobs):
             e.run_method2()
             assert np.array_equal(e.labels(), oracle.scc_canonical(h, 8))
         e.set_tuning()
-        assert e.tuning()["light_row"] == 16
+        assert e.tuning()["light_row"] == 32 and e.tuning()["grid_cap"] == 640
 
 
 def test_autotune_picks_a_valid_tuning():
