@@ -380,7 +380,8 @@ int run_levels(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed) {
 // together (one launch runs a level of each); post as in level_loop.
 int traverse_both(scc_ctx* c, bool multi, int64_t* levels, bool timed, const std::function<void()>& post) {
   // (both search controls were set by the step's set-up launch)
-  launch_seed_pivots2(c->stream, grid_for(c->n_live, (unsigned)c->tune.grid_cap), c->dc, c->n, level_dir(c, true, true),
+  // (a streaming pass with no completion count: more workgroups hide its latency)
+  launch_seed_pivots2(c->stream, grid_for(c->n_live, 2048u), c->dc, c->n, level_dir(c, true, true),
                       level_dir(c, false, true), c->color, c->pkey, c->live[0], c->live[1]);
   return level_loop(c, true, true, multi, levels, timed, post);
 }
@@ -424,7 +425,8 @@ int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int6
     launch_fill_u32(c->stream, c->vf, 0u, 2 * words);
   } else {
     // (also clears vf and vb)
-    launch_pivot_key(c->stream, grid, c->dc, gp(c), c->color, policy,
+    // (a streaming pass with no completion count: more workgroups hide its latency)
+    launch_pivot_key(c->stream, grid_for(c->n_live, 2048u), c->dc, gp(c), c->color, policy,
                      policy == SCC_PIVOT_LEARNED ? c->sched_scores : nullptr, c->pkey, c->live[0], c->live[1],
                      c->vf, 2 * words);
   }
