@@ -29,9 +29,20 @@ __device__ __forceinline__ bool row_any(bool act, int32_t v, int32_t c, int32_t 
   bool found = false;
   const int lim = act ? min(e - s, light) : 0;
   const int mx = wave_max(lim);
-  for (int j = 0; j < mx; j++) {
-    if (j < lim && !found) found = pred(nb[s + j], v, c);
-    if (__ballot((j + 1 < lim) && !found) == 0) break;
+  // the first entry alone (it usually decides), then four entries per step
+  // with their loads all in flight before any is tested
+  if (lim > 0) found = pred(nb[s], v, c);
+  if (__ballot(lim > 1 && !found) != 0) {
+    for (int j = 1; j < mx; j += 4) {
+      int32_t k[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) k[t] = (!found && j + t < lim) ? nb[s + j + t] : -1;
+      bool hit[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) hit[t] = k[t] >= 0 && pred(k[t], v, c);
+      found = found || hit[0] || hit[1] || hit[2] || hit[3];
+      if (__ballot((j + 4 < lim) && !found) == 0) break;
+    }
   }
   uint64_t hm = __ballot(act && !found && (e - s > light));
   const int lane = lane_id();
@@ -544,6 +555,53 @@ __device__ __forceinline__ bool try_claim(uint32_t* __restrict__ V, const int32_
   return !(atomicOr(&V[k >> 5], bit) & bit);
 }
 
+// try_claim for four neighbours at once (k[t] < 0: none).  Each round of
+// loads is issued for all four before any result is used: visited words, then
+// the colours of the unvisited, then the atomics; a lane then waits on memory
+// three times per four neighbours instead of three times per neighbour.
+__device__ __forceinline__ void claim4(uint32_t* __restrict__ V, const int32_t* __restrict__ color,
+                                       const int32_t (&k)[4], int32_t c, bool (&cl)[4]) {
+  uint32_t w[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) w[t] = k[t] >= 0 ? V[k[t] >> 5] : 0xFFFFFFFFu;
+  bool open[4];
+  int32_t ck[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    open[t] = k[t] >= 0 && !((w[t] >> (k[t] & 31)) & 1u);
+    ck[t] = open[t] ? color[k[t]] : DONE;
+  }
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    cl[t] = false;
+    if (open[t] && ck[t] == c) {
+      const uint32_t bit = 1u << (k[t] & 31);
+      cl[t] = !(atomicOr(&V[k[t] >> 5], bit) & bit);
+    }
+  }
+}
+
+// Whether any of four candidates parents k[t] (k[t] < 0: none) is visited
+// and, with several partitions, of colour c; the loads of all four are issued
+// together.
+__device__ __forceinline__ bool reached4(const uint32_t* __restrict__ V, const int32_t* __restrict__ color,
+                                         int multi, const int32_t (&k)[4], int32_t c) {
+  uint32_t w[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) w[t] = k[t] >= 0 ? V[k[t] >> 5] : 0u;
+  bool hit[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) hit[t] = k[t] >= 0 && ((w[t] >> (k[t] & 31)) & 1u);
+  if (multi) {
+    int32_t ck[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) ck[t] = hit[t] ? color[k[t]] : DONE;
+#pragma unroll
+    for (int t = 0; t < 4; t++) hit[t] = hit[t] && ck[t] == c;
+  }
+  return hit[0] || hit[1] || hit[2] || hit[3];
+}
+
 constexpr int TD_CHUNK = 2048;
 
 // One level.  Top-down: expand the frontier (rows of at least HUGE_ROW entries
@@ -616,16 +674,20 @@ __device__ __forceinline__ void level_part(DevCtl* __restrict__ dc, LevelCtl* __
         const int32_t e = min(s + TD_CHUNK, offe[u]);
         if (s >= e) continue;  // uniform across the workgroup
         const int32_t c = color[u];
-        for (int32_t b = s; b < e; b += BLOCK) {
-          const int32_t idx = b + threadIdx.x;
-          int32_t k = 0;
-          bool cl = false;
-          if (idx < e) {
-            k = nb[idx];
-            cl = try_claim(V, color, k, c);
-            if (cl) edges += (unsigned long long)(offe[k] - off[k]);
+        for (int32_t b = s; b < e; b += 4 * BLOCK) {
+          int32_t k[4];
+          bool cl[4];
+#pragma unroll
+          for (int t = 0; t < 4; t++) {
+            const int32_t idx = b + t * BLOCK + threadIdx.x;
+            k[t] = idx < e ? nb[idx] : -1;
           }
-          QOut::push(cl, k, q_out, &lv->out_q);
+          claim4(V, color, k, c, cl);
+#pragma unroll
+          for (int t = 0; t < 4; t++)
+            if (cl[t]) edges += (unsigned long long)(offe[k[t]] - off[k[t]]);
+#pragma unroll
+          for (int t = 0; t < 4; t++) QOut::push(cl[t], k[t], q_out, &lv->out_q);
         }
       }
     }
@@ -648,31 +710,37 @@ __device__ __forceinline__ void level_part(DevCtl* __restrict__ dc, LevelCtl* __
       const bool act = valid && !hug;
       const int lim = act ? min(d, s_light) : 0;
       const int mx = wave_max(lim);
-      for (int j = 0; j < mx; j++) {
-        int32_t k = 0;
-        bool cl = false;
-        if (j < lim) {
-          k = nb[s + j];
-          cl = try_claim(V, color, k, c);
-          if (cl) edges += (unsigned long long)(offe[k] - off[k]);
-        }
-        QOut::push(cl, k, q_out, &lv->out_q);
+      for (int j = 0; j < mx; j += 4) {
+        int32_t k[4];
+        bool cl[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) k[t] = j + t < lim ? nb[s + j + t] : -1;
+        claim4(V, color, k, c, cl);
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+          if (cl[t]) edges += (unsigned long long)(offe[k[t]] - off[k[t]]);
+#pragma unroll
+        for (int t = 0; t < 4; t++) QOut::push(cl[t], k[t], q_out, &lv->out_q);
       }
       uint64_t hm = __ballot(act && d > s_light);
       while (hm) {
         const int L = __ffsll((unsigned long long)hm) - 1;
         hm &= hm - 1;
         const int32_t hs = __shfl(s, L) + s_light, he = __shfl(e, L), hc = __shfl(c, L);
-        for (int32_t b = hs; b < he; b += 64) {
-          const int32_t idx = b + lane;
-          int32_t k = 0;
-          bool cl = false;
-          if (idx < he) {
-            k = nb[idx];
-            cl = try_claim(V, color, k, hc);
-            if (cl) edges += (unsigned long long)(offe[k] - off[k]);
+        for (int32_t b = hs; b < he; b += 256) {
+          int32_t k[4];
+          bool cl[4];
+#pragma unroll
+          for (int t = 0; t < 4; t++) {
+            const int32_t idx = b + t * 64 + lane;
+            k[t] = idx < he ? nb[idx] : -1;
           }
-          QOut::push(cl, k, q_out, &lv->out_q);
+          claim4(V, color, k, hc, cl);
+#pragma unroll
+          for (int t = 0; t < 4; t++)
+            if (cl[t]) edges += (unsigned long long)(offe[k[t]] - off[k[t]]);
+#pragma unroll
+          for (int t = 0; t < 4; t++) QOut::push(cl[t], k[t], q_out, &lv->out_q);
         }
       }
     }
@@ -681,12 +749,7 @@ __device__ __forceinline__ void level_part(DevCtl* __restrict__ dc, LevelCtl* __
     const int64_t ncin = (s_c < 0) ? (int64_t)dc->live.n : (int64_t)s_nc;
     int32_t* __restrict__ c_out = (s_c == 0) ? c1 : c0;
     int32_t* __restrict__ q_out = s_q ? q0 : q1;
-    auto reached = [V, color, multi](int32_t k, int32_t pv, int32_t pc) {
-      (void)pv;
-      if (!((V[k >> 5] >> (k & 31)) & 1u)) return false;
-      return !multi || color[k] == pc;
-    };
-    __shared__ int32_t s_qv[BLOCK], s_qs[BLOCK], s_qe[BLOCK], s_qc[BLOCK], s_qhit[BLOCK];
+    __shared__ int32_t s_qs[BLOCK], s_qe[BLOCK], s_qc[BLOCK], s_qhit[BLOCK];
     __shared__ int s_bq;
     for (int64_t base = (int64_t)bid * BLOCK; base < ncin; base += (int64_t)nblk * BLOCK) {
       QOut::maybe_flush(q_out, &lv->out_q, BLOCK);
@@ -709,9 +772,20 @@ __device__ __forceinline__ void level_part(DevCtl* __restrict__ dc, LevelCtl* __
       {
         const int lim = act ? min(e - s, s_light) : 0;
         const int mx = wave_max(lim);
-        for (int j = 0; j < mx; j++) {
-          if (j < lim && !found) found = reached(rnb[s + j], v, c);
-          if (__ballot((j + 1 < lim) && !found) == 0) break;
+        // the first entry alone (hub-first rows put a likely parent there),
+        // then four per step with their loads issued together
+        if (lim > 0) {
+          const int32_t k0[4] = {rnb[s], -1, -1, -1};
+          found = reached4(V, color, multi, k0, c);
+        }
+        if (__ballot(lim > 1 && !found) != 0) {
+          for (int j = 1; j < mx; j += 4) {
+            int32_t k[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) k[t] = (!found && j + t < lim) ? rnb[s + j + t] : -1;
+            found = found || reached4(V, color, multi, k, c);
+            if (__ballot((j + 4 < lim) && !found) == 0) break;
+          }
         }
       }
       // phase 2: rows still open go to a workgroup queue and are scanned by
@@ -721,7 +795,6 @@ __device__ __forceinline__ void level_part(DevCtl* __restrict__ dc, LevelCtl* __
       int slot = -1;
       if (act && !found && e - s > s_light) {
         slot = atomicAdd(&s_bq, 1);
-        s_qv[slot] = v;
         s_qs[slot] = s + s_light;
         s_qe[slot] = e;
         s_qc[slot] = c;
@@ -732,11 +805,16 @@ __device__ __forceinline__ void level_part(DevCtl* __restrict__ dc, LevelCtl* __
         const int nbq = s_bq;
         const int gl = threadIdx.x & 15, gsh = lane & 48;
         for (int j = threadIdx.x >> 4; j < nbq; j += BLOCK / 16) {
-          const int32_t rs = s_qs[j], re = s_qe[j], rv = s_qv[j], rc = s_qc[j];
+          const int32_t rs = s_qs[j], re = s_qe[j], rc = s_qc[j];
           bool hit = false;
-          for (int32_t b = rs; b < re; b += 16) {
-            const int32_t idx = b + gl;
-            const bool h = idx < re && reached(rnb[idx], rv, rc);
+          for (int32_t b = rs; b < re; b += 64) {
+            int32_t k[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+              const int32_t idx = b + t * 16 + gl;
+              k[t] = idx < re ? rnb[idx] : -1;
+            }
+            const bool h = reached4(V, color, multi, k, rc);
             if ((__ballot(h) >> gsh) & 0xFFFFull) {
               hit = true;
               break;
@@ -1009,9 +1087,17 @@ __device__ __forceinline__ void wcc_pull_sweep(const int32_t* __restrict__ list,
     const bool act = hv != (uint32_t)DONE;
     const int len = act ? e - s : 0;
     const int lim = len < 32 ? len : 0;
-    for (int j = 0; j < lim; j++) {
-      const unsigned long long y = cw[adj[s + j]];
-      if (cw_hi(y) == hv) m = min(m, cw_lo(y));
+    // four neighbours per step, their loads issued together
+    for (int j = 0; j < lim; j += 4) {
+      int32_t a[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) a[t] = j + t < lim ? adj[s + j + t] : -1;
+      unsigned long long y[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) y[t] = a[t] >= 0 ? cw[a[t]] : 0ull;
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+        if (a[t] >= 0 && cw_hi(y[t]) == hv) m = min(m, cw_lo(y[t]));
     }
     uint64_t hm = __ballot(len >= 32);
     while (hm) {
@@ -1020,12 +1106,19 @@ __device__ __forceinline__ void wcc_pull_sweep(const int32_t* __restrict__ list,
       const int32_t hs = __shfl(s, L), he = __shfl(e, L);
       const uint32_t hh = __shfl(hv, L);
       int32_t hmin = INT32_MAX;
-      for (int32_t b = hs; b < he; b += 64) {
-        const int32_t idx = b + lane;
-        if (idx < he) {
-          const unsigned long long y = cw[adj[idx]];
-          if (cw_hi(y) == hh) hmin = min(hmin, cw_lo(y));
+      for (int32_t b = hs; b < he; b += 256) {
+        int32_t a[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          const int32_t idx = b + t * 64 + lane;
+          a[t] = idx < he ? adj[idx] : -1;
         }
+        unsigned long long y[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) y[t] = a[t] >= 0 ? cw[a[t]] : 0ull;
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+          if (a[t] >= 0 && cw_hi(y[t]) == hh) hmin = min(hmin, cw_lo(y[t]));
       }
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) hmin = min(hmin, __shfl_xor(hmin, o));
@@ -1124,14 +1217,27 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_push(DevCtl* __restrict__ dc, con
   if (dense) {
     wcc_pull_sweep(LIVE_IN(dc, L0, L1), n_live, begin, bend, adj, cw, qout, &dc->lv.out_q);
   } else {
-    // claim u for the next frontier when the push lowered its label
-    auto lower = [&](int32_t u, unsigned long long x) -> bool {
-      const unsigned long long y = cw[u];
-      if (cw_hi(y) != cw_hi(x) || cw_lo(x) >= cw_lo(y)) return false;
-      const unsigned long long old = atomicMin(&cw[u], x);
-      if (old <= x) return false;
-      const uint32_t bit = 1u << (u & 31);
-      return !(atomicOr(&Bout[u >> 5], bit) & bit);
+    // push label x to four in-neighbours u[t] (u[t] < 0: none); add[t] when
+    // the push lowered u[t]'s label and claimed it for the next frontier.
+    // Each round of loads or atomics is issued for all four together.
+    auto lower4 = [&](const int32_t (&u)[4], unsigned long long x, bool (&add)[4]) {
+      unsigned long long y[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) y[t] = u[t] >= 0 ? cw[u[t]] : 0ull;
+      bool want[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) want[t] = u[t] >= 0 && cw_hi(y[t]) == cw_hi(x) && cw_lo(x) < cw_lo(y[t]);
+      unsigned long long old[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) old[t] = want[t] ? atomicMin(&cw[u[t]], x) : 0ull;
+      uint32_t ob[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        want[t] = want[t] && old[t] > x;
+        ob[t] = want[t] ? atomicOr(&Bout[u[t] >> 5], 1u << (u[t] & 31)) : 0xFFFFFFFFu;
+      }
+#pragma unroll
+      for (int t = 0; t < 4; t++) add[t] = want[t] && !((ob[t] >> (u[t] & 31)) & 1u);
     };
     for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < nq; base += (int64_t)gridDim.x * BLOCK) {
       Out::maybe_flush(qout, &dc->lv.out_q, BLOCK);
@@ -1149,14 +1255,14 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_push(DevCtl* __restrict__ dc, con
       const int len = act ? e - s : 0;
       const int lim = len < 32 ? len : 0;
       const int wl = wave_max(lim);
-      for (int j = 0; j < wl; j++) {
-        bool add = false;
-        int32_t u = 0;
-        if (j < lim) {
-          u = radj[s + j];
-          add = lower(u, x);
-        }
-        Out::push(add, u, qout, &dc->lv.out_q);
+      for (int j = 0; j < wl; j += 4) {
+        int32_t u[4];
+        bool add[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) u[t] = j + t < lim ? radj[s + j + t] : -1;
+        lower4(u, x, add);
+#pragma unroll
+        for (int t = 0; t < 4; t++) Out::push(add[t], u[t], qout, &dc->lv.out_q);
       }
       uint64_t hm = __ballot(len >= 32);
       while (hm) {
@@ -1164,15 +1270,17 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_push(DevCtl* __restrict__ dc, con
         hm &= hm - 1;
         const int32_t hs = __shfl(s, L), he = __shfl(e, L);
         const unsigned long long hx = __shfl(x, L);
-        for (int32_t b = hs; b < he; b += 64) {
-          const int32_t idx = b + lane;
-          bool add = false;
-          int32_t u = 0;
-          if (idx < he) {
-            u = radj[idx];
-            add = lower(u, hx);
+        for (int32_t b = hs; b < he; b += 256) {
+          int32_t u[4];
+          bool add[4];
+#pragma unroll
+          for (int t = 0; t < 4; t++) {
+            const int32_t idx = b + t * 64 + lane;
+            u[t] = idx < he ? radj[idx] : -1;
           }
-          Out::push(add, u, qout, &dc->lv.out_q);
+          lower4(u, hx, add);
+#pragma unroll
+          for (int t = 0; t < 4; t++) Out::push(add[t], u[t], qout, &dc->lv.out_q);
         }
       }
     }
