@@ -381,8 +381,9 @@ int run_levels(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed) {
 int traverse_both(scc_ctx* c, bool multi, int64_t* levels, bool timed, const std::function<void()>& post) {
   // (both search controls were set by the step's set-up launch)
   // (a streaming pass with no completion count: more workgroups hide its latency)
-  launch_seed_pivots2(c->stream, grid_for(c->n_live, 2048u), c->dc, c->n, level_dir(c, true, true),
-                      level_dir(c, false, true), c->color, c->pkey, c->live[0], c->live[1]);
+  launch_seed_pivots2(c->stream, grid_for(c->n_live, 2048u), c->dc, c->n, multi ? 1 : 0,
+                      level_dir(c, true, true), level_dir(c, false, true), c->color, c->pkey, c->live[0],
+                      c->live[1]);
   return level_loop(c, true, true, multi, levels, timed, post);
 }
 

@@ -57,7 +57,7 @@ void launch_force_pivot(hipStream_t st, int32_t p, int32_t n, const int32_t* col
 void launch_seed_list(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* off, const int32_t* offe,
                       const int32_t* color, const int32_t* seeds, const unsigned long long* ns,
                       uint32_t* V, int32_t* q0);
-void launch_seed_pivots2(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, const LevelDir& F,
+void launch_seed_pivots2(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, int multi, const LevelDir& F,
                          const LevelDir& B, const int32_t* color, const unsigned long long* pkey,
                          const int32_t* L0, const int32_t* L1);
 void launch_level(hipStream_t st, unsigned grid, DevCtl* dc, int multi, const LevelDir& D,

@@ -452,8 +452,8 @@ __global__ void k_force_pivot(int32_t p, int32_t n, const int32_t* __restrict__ 
 // both frontiers (or, for a row of at least huge_row entries, the chunked
 // expansion of that direction's first level).  The summed row lengths give the
 // first direction-switch decision.
-__global__ __launch_bounds__(BLOCK) void k_seed_pivots2(DevCtl* __restrict__ dc, int32_t n, LevelDir F, LevelDir B,
-                                                        const int32_t* __restrict__ color,
+__global__ __launch_bounds__(BLOCK) void k_seed_pivots2(DevCtl* __restrict__ dc, int32_t n, int multi, LevelDir F,
+                                                        LevelDir B, const int32_t* __restrict__ color,
                                                         const unsigned long long* __restrict__ pkey,
                                                         const int32_t* __restrict__ L0,
                                                         const int32_t* __restrict__ L1) {
@@ -478,7 +478,9 @@ __global__ __launch_bounds__(BLOCK) void k_seed_pivots2(DevCtl* __restrict__ dc,
         ef += (unsigned long long)df;
         eb += (unsigned long long)db;
         cnt++;
-        dc->ctr[C_PIVOT] = (unsigned long long)v;  // the pivot, when there is one partition
+        // the pivot, when there is one partition (with many, thousands of
+        // same-address stores would serialise for a value nobody reads)
+        if (!multi) dc->ctr[C_PIVOT] = (unsigned long long)v;
       }
     }
     wave_append(piv && df < huge, v, F.q0, &dc->lv.nq);
@@ -602,6 +604,17 @@ __device__ __forceinline__ bool reached4(const uint32_t* __restrict__ V, const i
   return hit[0] || hit[1] || hit[2] || hit[3];
 }
 
+// Lanes per frontier vertex in a top-down level: 1, or a power of two up to 16
+// when the frontier (nq vertices, mf edges) leaves most of the search's
+// lanes idle and its rows are long enough to feed a group (a mean row of at
+// least 2G entries; a group takes up to 8G entries of a row, in two four-load
+// steps).
+__device__ __forceinline__ int td_group(unsigned long long nq, unsigned long long mf, unsigned long long lanes) {
+  int G = 1;
+  while (G < 16 && nq * (unsigned long long)(2 * G) <= lanes && mf >= nq * (unsigned long long)(4 * G)) G *= 2;
+  return G;
+}
+
 constexpr int TD_CHUNK = 2048;
 
 // One level.  Top-down: expand the frontier (rows of at least HUGE_ROW entries
@@ -635,7 +648,7 @@ __device__ __forceinline__ void level_part(DevCtl* __restrict__ dc, LevelCtl* __
   int32_t* __restrict__ c1 = D.c1;
   int32_t* __restrict__ h0 = D.h0;
   int32_t* __restrict__ h1 = D.h1;
-  __shared__ int s_done, s_mode, s_q, s_c, s_h, s_huge;
+  __shared__ int s_done, s_mode, s_q, s_c, s_h, s_huge, s_group;
   __shared__ unsigned long long s_nq, s_nh, s_nc;
   if (threadIdx.x == 0) {
     s_huge = dc->tune.huge_row;
@@ -647,6 +660,7 @@ __device__ __forceinline__ void level_part(DevCtl* __restrict__ dc, LevelCtl* __
     s_nq = lv->nq;
     s_nh = lv->nh;
     s_nc = lv->nc;
+    s_group = td_group(lv->nq, lv->mf, (unsigned long long)nblk * BLOCK);
   }
   __syncthreads();
   if (s_done) return;
@@ -693,7 +707,70 @@ __device__ __forceinline__ void level_part(DevCtl* __restrict__ dc, LevelCtl* __
     }
     // (b) the frontier
     const int64_t nq = (int64_t)s_nq;
-    for (int64_t base = (int64_t)bid * BLOCK; base < nq; base += (int64_t)nblk * BLOCK) {
+    const int G = s_group;
+    if (G > 1) {
+      // lane groups: G lanes share a frontier vertex and take its first
+      // 8G entries (two four-load steps); longer rows pass their rest to the
+      // whole wave as below
+      const int g = threadIdx.x & (G - 1);
+      const int per_blk = BLOCK / G;
+      const int limg = 8 * G;
+      for (int64_t base = (int64_t)bid * per_blk; base < nq; base += (int64_t)nblk * per_blk) {
+        QOut::maybe_flush(q_out, &lv->out_q, BLOCK);
+        const int64_t i = base + threadIdx.x / G;
+        const bool valid = i < nq;
+        int32_t u = 0, s = 0, e = 0, c = DONE;
+        if (valid) {
+          u = q_in[i];
+          s = off[u];
+          e = offe[u];
+          c = color[u];
+        }
+        const int32_t d = e - s;
+        const bool hug = valid && d >= s_huge;
+        wave_append(hug && g == 0, u, h_out, &lv->out_h);
+        const bool act = valid && !hug;
+        const int lim = act ? min(d, limg) : 0;
+        const int mx = wave_max(lim);
+        for (int j = 0; j < mx; j += 4 * G) {
+          int32_t k[4];
+          bool cl[4];
+#pragma unroll
+          for (int t = 0; t < 4; t++) {
+            const int jj = j + t * G + g;
+            k[t] = jj < lim ? nb[s + jj] : -1;
+          }
+          claim4(V, color, k, c, cl);
+#pragma unroll
+          for (int t = 0; t < 4; t++)
+            if (cl[t]) edges += (unsigned long long)(offe[k[t]] - off[k[t]]);
+#pragma unroll
+          for (int t = 0; t < 4; t++) QOut::push(cl[t], k[t], q_out, &lv->out_q);
+        }
+        uint64_t hm = __ballot(act && g == 0 && d > limg);
+        while (hm) {
+          const int L = __ffsll((unsigned long long)hm) - 1;
+          hm &= hm - 1;
+          const int32_t hs = __shfl(s, L) + limg, he = __shfl(e, L), hc = __shfl(c, L);
+          for (int32_t b = hs; b < he; b += 256) {
+            int32_t k[4];
+            bool cl[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+              const int32_t idx = b + t * 64 + lane;
+              k[t] = idx < he ? nb[idx] : -1;
+            }
+            claim4(V, color, k, hc, cl);
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+              if (cl[t]) edges += (unsigned long long)(offe[k[t]] - off[k[t]]);
+#pragma unroll
+            for (int t = 0; t < 4; t++) QOut::push(cl[t], k[t], q_out, &lv->out_q);
+          }
+        }
+      }
+    }
+    for (int64_t base = (int64_t)bid * BLOCK; G == 1 && base < nq; base += (int64_t)nblk * BLOCK) {
       QOut::maybe_flush(q_out, &lv->out_q, BLOCK);
       const int64_t i = base + threadIdx.x;
       const bool valid = i < nq;
@@ -1573,10 +1650,10 @@ void launch_seed_list(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* 
                       uint32_t* V, int32_t* q0) {
   LAUNCH(k_seed_list, grid, st, dc, off, offe, color, seeds, ns, V, q0);
 }
-void launch_seed_pivots2(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, const LevelDir& F,
+void launch_seed_pivots2(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, int multi, const LevelDir& F,
                          const LevelDir& B, const int32_t* color, const unsigned long long* pkey,
                          const int32_t* L0, const int32_t* L1) {
-  LAUNCH(k_seed_pivots2, grid, st, dc, n, F, B, color, pkey, L0, L1);
+  LAUNCH(k_seed_pivots2, grid, st, dc, n, multi, F, B, color, pkey, L0, L1);
 }
 void launch_level(hipStream_t st, unsigned grid, DevCtl* dc, int multi, const LevelDir& D,
                   const int32_t* color, const int32_t* L0, const int32_t* L1, int32_t max_chunks, int light) {
