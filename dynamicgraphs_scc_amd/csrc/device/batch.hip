@@ -137,6 +137,7 @@ int scc_batch_set(scc_ctx* c, int32_t num_graphs, const int32_t* offsets) {
   c->h_boff.clear();
   if (num_graphs == 0) return SCC_OK;  // back to one graph
   if (num_graphs < 0 || !offsets) return set_error(c, SCC_ERR_ARG, "bad batch");
+  if (c->relabeled) return set_error(c, SCC_ERR_STATE, "batched graphs need SCC_LAYOUT_INPUT");
   if (offsets[0] != 0 || offsets[num_graphs] != c->n)
     return set_error(c, SCC_ERR_ARG, "batch offsets must run from 0 to n");
   for (int32_t g = 0; g < num_graphs; g++)

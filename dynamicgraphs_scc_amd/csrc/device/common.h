@@ -425,6 +425,15 @@ struct scc_ctx {
   // neighbour meet hubs first.  NULL when absent (after updates).
   int32_t *tadj = nullptr, *tradj = nullptr;
 
+  // Vertex layout (layout.hip): with SCC_LAYOUT_DEGREE requested at upload the
+  // device graph is relabelled; perm[i] = caller id of internal vertex i,
+  // iperm[v] = internal id of caller vertex v (host copies beside them), and
+  // map_a / map_b are n + 1 ints of staging for mapped downloads.
+  int32_t layout_req = 0;
+  bool relabeled = false;
+  int32_t *perm = nullptr, *iperm = nullptr, *map_a = nullptr, *map_b = nullptr;
+  std::vector<int32_t> h_perm, h_iperm;
+
   // per-vertex state
   int32_t *color = nullptr, *label = nullptr, *canon = nullptr, *W = nullptr;
   unsigned long long* cw = nullptr;  // WCC iterations: colour << 32 | label

@@ -870,8 +870,15 @@ extern "C" int scc_update(scc_ctx* c, const int32_t* ins_src, const int32_t* ins
   // ---- 1. sorted, filtered batch lists ------------------------------------
   auto upload_keys = [&](const int32_t* a, const int32_t* b, int64_t k, bool drop_loops) -> int {
     if (k == 0) return SCC_OK;
-    std::memcpy(d->h_stage, a, 4ull * k);
-    std::memcpy(d->h_stage + k, b, 4ull * k);
+    if (c->relabeled) {  // caller ids to internal ids (validated above)
+      for (int64_t i = 0; i < k; i++) {
+        d->h_stage[i] = c->h_iperm[a[i]];
+        d->h_stage[k + i] = c->h_iperm[b[i]];
+      }
+    } else {
+      std::memcpy(d->h_stage, a, 4ull * k);
+      std::memcpy(d->h_stage + k, b, 4ull * k);
+    }
     int r;
     if ((r = hip_check(c, hipMemcpyAsync(d->src, d->h_stage, 4ull * k, hipMemcpyHostToDevice, s), "batch upload")) ||
         (r = hip_check(c, hipMemcpyAsync(d->dst, d->h_stage + k, 4ull * k, hipMemcpyHostToDevice, s), "batch upload")))

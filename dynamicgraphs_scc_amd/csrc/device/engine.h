@@ -2,6 +2,7 @@
 #pragma once
 #include <functional>
 #include <string>
+#include <vector>
 
 #include "common.h"
 #include "kernels.h"
@@ -22,6 +23,15 @@ int start_run(scc_ctx* c);  // reset_state, then the batch's per-graph colours w
 int alloc_graph(scc_ctx* c, int32_t n, int64_t m_cap);
 int build_order(scc_ctx* c);  // traversal copies of the rows, hubs first (order.hip)
 void drop_order(scc_ctx* c);
+
+// vertex layout (layout.hip): internal relabelling and the mappings at the boundary
+int build_layout(scc_ctx* c);  // relabel the uploaded rows when SCC_LAYOUT_DEGREE is requested
+void drop_layout(scc_ctx* c);
+int layout_to_caller(scc_ctx* c, const int32_t* in, bool map_values, int32_t* out_dev);
+int layout_canon_to_caller(scc_ctx* c, int32_t* out_dev);  // from canon[]: min caller id per component
+int layout_colors_to_internal(scc_ctx* c, const int32_t* colors_host);
+void layout_rows_to_caller(const scc_ctx* c, const std::vector<int32_t>& ib, const std::vector<int32_t>& ia,
+                           int32_t* ob, int32_t* oa);
 
 // phases (operate on the live list of the control block)
 int do_trim_compact(scc_ctx* c);  // drop colour -2 entries from the live list

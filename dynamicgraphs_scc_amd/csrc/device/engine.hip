@@ -167,6 +167,7 @@ int free_graph(scc_ctx* c) {
   dev_free(c, c->cw);
   c->cw = nullptr;
   drop_order(c);
+  drop_layout(c);
   dev_free(c, c->d_boff);
   c->d_boff = nullptr;
   c->batch_k = 0;
@@ -698,7 +699,8 @@ int scc_graph_upload(scc_ctx* c, int32_t n, int64_t m, const int32_t* begin, con
       (m > 0 && (rc = hip_check(c, hipMemcpyAsync(c->adj, node_idx, 4ull * m, hipMemcpyHostToDevice, c->stream), "upload"))) ||
       (m > 0 && (rc = hip_check(c, hipMemcpyAsync(c->radj, r_node_idx, 4ull * m, hipMemcpyHostToDevice, c->stream), "upload"))))
     return rc;
-  if ((rc = build_order(c))) return rc;
+  if ((rc = build_layout(c))) return rc;
+  if (!c->relabeled && (rc = build_order(c))) return rc;
   if ((rc = reset_state(c))) return rc;
   c->ran = false;
   return hip_check(c, hipStreamSynchronize(c->stream), "upload sync");
@@ -728,7 +730,8 @@ int scc_graph_upload_device(scc_ctx* c, int32_t n, int64_t m, const int32_t* d_b
       return rc;
     c->maxdeg = max_degree(n, b.data(), rb.data());
   }
-  if ((rc = build_order(c))) return rc;
+  if ((rc = build_layout(c))) return rc;
+  if (!c->relabeled && (rc = build_order(c))) return rc;
   if ((rc = reset_state(c))) return rc;
   c->ran = false;
   return hip_check(c, hipStreamSynchronize(c->stream), "upload sync");
@@ -746,8 +749,24 @@ int scc_graph_download(scc_ctx* c, int32_t* begin, int32_t* node_idx, int32_t* r
   if (!c) return SCC_ERR_ARG;
   if (!c->begin) return set_error(c, SCC_ERR_STATE, "no graph");
   (void)hipSetDevice(c->device);
-  if (c->end) return download_gapped(c, begin, node_idx, r_begin, r_node_idx);
   int rc = 0;
+  if (c->relabeled) {
+    // internal rows to host, then each caller row rebuilt and sorted
+    std::vector<int32_t> ib(c->n + 1), ia(c->m), rb(c->n + 1), ra(c->m);
+    if ((rc = c->end ? download_gapped(c, ib.data(), ia.data(), rb.data(), ra.data())
+                     : (hip_check(c, hipMemcpyAsync(ib.data(), c->begin, 4ull * (c->n + 1), hipMemcpyDeviceToHost, c->stream), "download") ||
+                        hip_check(c, hipMemcpyAsync(rb.data(), c->rbegin, 4ull * (c->n + 1), hipMemcpyDeviceToHost, c->stream), "download") ||
+                        (c->m && hip_check(c, hipMemcpyAsync(ia.data(), c->adj, 4ull * c->m, hipMemcpyDeviceToHost, c->stream), "download")) ||
+                        (c->m && hip_check(c, hipMemcpyAsync(ra.data(), c->radj, 4ull * c->m, hipMemcpyDeviceToHost, c->stream), "download")) ||
+                        hip_check(c, hipStreamSynchronize(c->stream), "download sync"))
+                           ? SCC_ERR_HIP
+                           : SCC_OK))
+      return rc;
+    layout_rows_to_caller(c, ib, ia, begin, node_idx);
+    layout_rows_to_caller(c, rb, ra, r_begin, r_node_idx);
+    return SCC_OK;
+  }
+  if (c->end) return download_gapped(c, begin, node_idx, r_begin, r_node_idx);
   if (begin && (rc = hip_check(c, hipMemcpyAsync(begin, c->begin, 4ull * (c->n + 1), hipMemcpyDeviceToHost, c->stream), "download"))) return rc;
   if (r_begin && (rc = hip_check(c, hipMemcpyAsync(r_begin, c->rbegin, 4ull * (c->n + 1), hipMemcpyDeviceToHost, c->stream), "download"))) return rc;
   if (node_idx && c->m && (rc = hip_check(c, hipMemcpyAsync(node_idx, c->adj, 4ull * c->m, hipMemcpyDeviceToHost, c->stream), "download"))) return rc;
@@ -775,10 +794,18 @@ int scc_state_set_colors(scc_ctx* c, const int32_t* colors) {
   for (int32_t v = 0; v < c->n && !c->multi_colors; v++) c->multi_colors = colors[v] != -1;
   c->colors_split = c->multi_colors;
   c->fresh = false;
+  if (c->relabeled) return layout_colors_to_internal(c, colors);
   if ((rc = hip_check(c, hipMemcpyAsync(c->color, colors, 4ull * c->n, hipMemcpyHostToDevice, c->stream),
                       "colours upload")))
     return rc;
   return hip_check(c, hipStreamSynchronize(c->stream), "colours sync");
+}
+
+int scc_ctx_set_layout(scc_ctx* c, int layout) {
+  if (!c) return SCC_ERR_ARG;
+  if (layout != SCC_LAYOUT_INPUT && layout != SCC_LAYOUT_DEGREE) return set_error(c, SCC_ERR_ARG, "bad layout");
+  c->layout_req = layout;
+  return SCC_OK;
 }
 
 #define REQUIRE_GRAPH(c)                                                    \
@@ -803,8 +830,10 @@ int scc_fb_global(scc_ctx* c, int policy, int32_t pivot_override, int64_t* scc_s
   c->labels_ready = false;
   int rc = do_trim_compact(c);
   if (rc) return rc;
-  return do_fb_round(c, policy, c->colors_split, pivot_override, scc_size, pivot_out, nullptr, false,
-                     nullptr);
+  if (c->relabeled && pivot_override >= 0) pivot_override = c->h_iperm[pivot_override];
+  rc = do_fb_round(c, policy, c->colors_split, pivot_override, scc_size, pivot_out, nullptr, false, nullptr);
+  if (!rc && c->relabeled && pivot_out && *pivot_out >= 0) *pivot_out = c->h_perm[*pivot_out];
+  return rc;
 }
 
 int scc_trim2(scc_ctx* c, int64_t* removed) {
@@ -821,6 +850,14 @@ int scc_wcc(scc_ctx* c, int64_t* parts) {
   int rc = do_trim_compact(c);
   if (rc) return rc;
   return do_wcc(c, parts);
+}
+
+int scc_color_round(scc_ctx* c, int64_t* resolved) {
+  REQUIRE_GRAPH(c);
+  c->labels_ready = false;
+  int rc = do_trim_compact(c);
+  if (rc) return rc;
+  return do_color_round(c, resolved);
 }
 
 int scc_partitions_resolve(scc_ctx* c, int policy, int mode, int trim_between, int64_t* rounds) {
@@ -933,6 +970,7 @@ int scc_run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_o
   st.ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
   st.ms_fb_kernels = c->kernel_ms;
   st.fb_launches = c->kernel_launches;
+  if (c->relabeled && st.pivot_first >= 0) st.pivot_first = c->h_perm[st.pivot_first];
   c->ran = true;
   if (st_out) *st_out = st;
   return SCC_OK;
@@ -945,8 +983,14 @@ int scc_labels_download(scc_ctx* c, int32_t* out, int canonical) {
   if (canonical && !c->labels_ready) {
     if ((rc = do_canon(c, nullptr))) return rc;
   }
-  if ((rc = hip_check(c, hipMemcpyAsync(out, canonical ? c->canon : c->label, 4ull * c->n,
-                                        hipMemcpyDeviceToHost, c->stream), "labels download")))
+  const int32_t* src = canonical ? c->canon : c->label;
+  if (c->relabeled) {
+    if ((rc = canonical ? layout_canon_to_caller(c, c->map_a) : layout_to_caller(c, c->label, true, c->map_a)))
+      return rc;
+    src = c->map_a;
+  }
+  if ((rc = hip_check(c, hipMemcpyAsync(out, src, 4ull * c->n, hipMemcpyDeviceToHost, c->stream),
+                      "labels download")))
     return rc;
   return hip_check(c, hipStreamSynchronize(c->stream), "labels sync");
 }
@@ -955,7 +999,12 @@ int scc_colors_download(scc_ctx* c, int32_t* out) {
   REQUIRE_GRAPH(c);
   if (!out) return SCC_ERR_ARG;
   int rc;
-  if ((rc = hip_check(c, hipMemcpyAsync(out, c->color, 4ull * c->n, hipMemcpyDeviceToHost, c->stream),
+  const int32_t* src = c->color;
+  if (c->relabeled) {
+    if ((rc = layout_to_caller(c, c->color, true, c->map_a))) return rc;
+    src = c->map_a;
+  }
+  if ((rc = hip_check(c, hipMemcpyAsync(out, src, 4ull * c->n, hipMemcpyDeviceToHost, c->stream),
                       "colours download")))
     return rc;
   return hip_check(c, hipStreamSynchronize(c->stream), "colours sync");
@@ -972,6 +1021,12 @@ int scc_labels_device(scc_ctx* c, const int32_t** d_labels) {
   int rc;
   if (!c->labels_ready && (rc = do_canon(c, nullptr))) return rc;
   *d_labels = c->canon;
+  if (c->relabeled) {
+    // caller-indexed copy, valid until the next call that maps labels or colours
+    if ((rc = layout_canon_to_caller(c, c->map_a))) return rc;
+    *d_labels = c->map_a;
+    return hip_check(c, hipStreamSynchronize(c->stream), "labels sync");
+  }
   return SCC_OK;
 }
 

@@ -1,0 +1,267 @@
+// Vertex layout (scc_ctx_set_layout): with SCC_LAYOUT_DEGREE the uploaded
+// graph is relabelled inside the context so that internal id i is the i-th
+// vertex by decreasing total degree (in + out), ties by caller id.  On skewed
+// graphs the hubs' per-vertex state (colours, reach bits, WCC words) then sits
+// in a few cache lines that every search level and sweep reuses, instead of
+// being spread over the whole arrays.  The rows are rebuilt sorted by internal
+// id, which is also hub-first order, so no traversal copies are needed.
+//
+// Every entry point keeps the caller's ids: perm[i] = caller id of internal
+// vertex i, iperm[v] = internal id of caller vertex v.  Labels and colours are
+// member ids, so both their index and their value are mapped on the way out;
+// canonical labels are the minimum caller id of each component.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.h"
+#include "engine.h"
+#include "kernels.h"
+
+namespace sccmi {
+
+namespace {
+
+#define LAUNCH(kern, grid, st, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, st, __VA_ARGS__)
+
+// ~(in + out degree): an ascending sort puts the largest degrees first
+__global__ __launch_bounds__(BLOCK) void k_neg_degree(int32_t n, const int32_t* __restrict__ begin,
+                                                      const int32_t* __restrict__ rbegin,
+                                                      uint32_t* __restrict__ key, int32_t* __restrict__ id) {
+  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < n; v += (int64_t)gridDim.x * BLOCK) {
+    key[v] = ~((uint32_t)(begin[v + 1] - begin[v]) + (uint32_t)(rbegin[v + 1] - rbegin[v]));
+    id[v] = (int32_t)v;
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_invert(int32_t n, const int32_t* __restrict__ perm,
+                                                  int32_t* __restrict__ iperm) {
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
+    iperm[perm[i]] = (int32_t)i;
+}
+
+// row lengths of the relabelled rows: len[i] = length of caller row perm[i];
+// len[n] = 0 so that an exclusive scan over n + 1 entries ends with m
+__global__ __launch_bounds__(BLOCK) void k_new_lengths(int32_t n, const int32_t* __restrict__ begin,
+                                                       const int32_t* __restrict__ perm,
+                                                       int32_t* __restrict__ len) {
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i <= n; i += (int64_t)gridDim.x * BLOCK) {
+    if (i == n) {
+      len[i] = 0;
+    } else {
+      const int32_t v = perm[i];
+      len[i] = begin[v + 1] - begin[v];
+    }
+  }
+}
+
+// key = internal row << 32 | internal neighbour; a wave per caller row
+__global__ __launch_bounds__(BLOCK) void k_edge_keys(int32_t n, const int32_t* __restrict__ begin,
+                                                     const int32_t* __restrict__ adj,
+                                                     const int32_t* __restrict__ iperm,
+                                                     unsigned long long* __restrict__ keys) {
+  const int lane = lane_id();
+  const int64_t wave = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * BLOCK) >> 6;
+  for (int64_t v = wave; v < n; v += nw) {
+    const int32_t s = begin[v], e = begin[v + 1];
+    const unsigned long long row = (unsigned long long)(uint32_t)iperm[v] << 32;
+    for (int32_t i = s + lane; i < e; i += 64) keys[i] = row | (uint32_t)iperm[adj[i]];
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_low_words(int64_t m, const unsigned long long* __restrict__ keys,
+                                                     int32_t* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < m; i += (int64_t)gridDim.x * BLOCK)
+    out[i] = (int32_t)(uint32_t)keys[i];
+}
+
+// out[perm[i]] = in[i], values that are vertex ids (>= 0) mapped through perm
+// when map_values (labels, colours); negative codes pass unchanged
+__global__ __launch_bounds__(BLOCK) void k_to_caller(int32_t n, const int32_t* __restrict__ perm,
+                                                     const int32_t* __restrict__ in, int map_values,
+                                                     int32_t* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+    const int32_t x = in[i];
+    out[perm[i]] = (map_values && x >= 0) ? perm[x] : x;
+  }
+}
+
+// out[iperm[v]] = in[v], values mapped through iperm (caller-set colours)
+__global__ __launch_bounds__(BLOCK) void k_to_internal(int32_t n, const int32_t* __restrict__ iperm,
+                                                       const int32_t* __restrict__ in, int32_t* __restrict__ out) {
+  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < n; v += (int64_t)gridDim.x * BLOCK) {
+    const int32_t x = in[v];
+    out[iperm[v]] = x >= 0 ? iperm[x] : x;
+  }
+}
+
+// mo[r] = min caller id over the members of the component whose internal
+// representative is r (canon[i] = r, or -1 while unresolved); mo must hold
+// INT32_MAX beforehand
+__global__ __launch_bounds__(BLOCK) void k_caller_min(int32_t n, const int32_t* __restrict__ canon,
+                                                      const int32_t* __restrict__ perm, int32_t* __restrict__ mo) {
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+    const int32_t r = canon[i], p = perm[i];
+    if (r >= 0 && mo[r] > p) atomicMin(&mo[r], p);
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_caller_apply(int32_t n, const int32_t* __restrict__ canon,
+                                                        const int32_t* __restrict__ perm,
+                                                        const int32_t* __restrict__ mo, int32_t* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+    const int32_t r = canon[i];
+    out[perm[i]] = r >= 0 ? mo[r] : -1;
+  }
+}
+
+// One direction: rows of (begin, adj) relabelled in place (begin, adj hold the
+// caller's rows on entry and the internal rows, sorted by id, on exit).
+int relabel_dir(scc_ctx* c, int32_t* begin, int32_t* adj) {
+  const int64_t m = c->m;
+  const int32_t n = c->n;
+  int rc;
+  unsigned long long *k0 = nullptr, *k1 = nullptr;
+  int32_t* len = nullptr;
+  void* tmp = nullptr;
+  auto done = [&](int r) {
+    dev_free(c, k0);
+    dev_free(c, k1);
+    dev_free(c, len);
+    dev_free(c, tmp);
+    return r;
+  };
+  if ((rc = dev_alloc(c, (void**)&len, 4ull * ((uint64_t)n + 1), "layout row lengths"))) return done(rc);
+  LAUNCH(k_new_lengths, grid_for((int64_t)n + 1, 4096), c->stream, n, begin, c->perm, len);
+  if (m > 0) {
+    if ((rc = dev_alloc(c, (void**)&k0, 8ull * m, "layout keys")) ||
+        (rc = dev_alloc(c, (void**)&k1, 8ull * m, "layout keys")))
+      return done(rc);
+    LAUNCH(k_edge_keys, grid_for((int64_t)n * 64, 4096), c->stream, n, begin, adj, c->iperm, k0);
+  }
+  int bits = 32;
+  while (bits < 64 && (1ull << (bits - 32)) < (unsigned long long)n) bits++;
+  hipcub::DoubleBuffer<unsigned long long> keys(k0, k1);
+  size_t tb_sort = 0, tb_scan = 0;
+  if (m > 0) (void)hipcub::DeviceRadixSort::SortKeys(nullptr, tb_sort, keys, (int)m, 0, bits, c->stream);
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tb_scan, len, begin, n + 1, c->stream);
+  if ((rc = dev_alloc(c, &tmp, std::max(tb_sort, tb_scan) + 256, "layout temp"))) return done(rc);
+  if (m > 0) {
+    if ((rc = hip_check(c, hipcub::DeviceRadixSort::SortKeys(tmp, tb_sort, keys, (int)m, 0, bits, c->stream),
+                        "layout sort")))
+      return done(rc);
+    LAUNCH(k_low_words, grid_for(m, 4096), c->stream, m, keys.Current(), adj);
+  }
+  // the lengths were read from the caller's offsets above; now overwrite them
+  if ((rc = hip_check(c, hipcub::DeviceScan::ExclusiveSum(tmp, tb_scan, len, begin, n + 1, c->stream),
+                      "layout scan")))
+    return done(rc);
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "layout sync"))) return done(rc);
+  return done(SCC_OK);
+}
+
+}  // namespace
+
+void drop_layout(scc_ctx* c) {
+  dev_free(c, c->perm);
+  dev_free(c, c->iperm);
+  dev_free(c, c->map_a);
+  dev_free(c, c->map_b);
+  c->perm = c->iperm = c->map_a = c->map_b = nullptr;
+  c->h_perm.clear();
+  c->h_iperm.clear();
+  c->relabeled = false;
+}
+
+int build_layout(scc_ctx* c) {
+  drop_layout(c);
+  if (c->layout_req != SCC_LAYOUT_DEGREE || c->n <= 0) return SCC_OK;
+  const int32_t n = c->n;
+  int rc;
+  uint32_t *key0 = nullptr, *key1 = nullptr;
+  int32_t* ids = nullptr;
+  void* tmp = nullptr;
+  auto done = [&](int r) {
+    dev_free(c, key0);
+    dev_free(c, key1);
+    dev_free(c, ids);
+    dev_free(c, tmp);
+    if (r) drop_layout(c);
+    return r;
+  };
+  if ((rc = dev_alloc(c, (void**)&c->perm, 4ull * n, "layout perm")) ||
+      (rc = dev_alloc(c, (void**)&c->iperm, 4ull * n, "layout iperm")) ||
+      (rc = dev_alloc(c, (void**)&c->map_a, 4ull * ((uint64_t)n + 1), "layout map")) ||
+      (rc = dev_alloc(c, (void**)&c->map_b, 4ull * ((uint64_t)n + 1), "layout map")) ||
+      (rc = dev_alloc(c, (void**)&key0, 4ull * n, "layout degree keys")) ||
+      (rc = dev_alloc(c, (void**)&key1, 4ull * n, "layout degree keys")) ||
+      (rc = dev_alloc(c, (void**)&ids, 4ull * n, "layout ids")))
+    return done(rc);
+  LAUNCH(k_neg_degree, grid_for(n, 4096), c->stream, n, c->begin, c->rbegin, key0, ids);
+  // radix sorts are stable: equal degrees keep the caller's id order
+  size_t tb = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key0, key1, ids, c->perm, n, 0, 32, c->stream);
+  if ((rc = dev_alloc(c, &tmp, tb + 256, "layout temp"))) return done(rc);
+  if ((rc = hip_check(c, hipcub::DeviceRadixSort::SortPairs(tmp, tb, key0, key1, ids, c->perm, n, 0, 32, c->stream),
+                      "layout degree sort")))
+    return done(rc);
+  LAUNCH(k_invert, grid_for(n, 4096), c->stream, n, c->perm, c->iperm);
+  dev_free(c, tmp);
+  tmp = nullptr;
+  if ((rc = relabel_dir(c, c->begin, c->adj)) || (rc = relabel_dir(c, c->rbegin, c->radj))) return done(rc);
+  c->h_perm.resize(n);
+  c->h_iperm.resize(n);
+  if ((rc = hip_check(c, hipMemcpyAsync(c->h_perm.data(), c->perm, 4ull * n, hipMemcpyDeviceToHost, c->stream),
+                      "layout download")) ||
+      (rc = hip_check(c, hipMemcpyAsync(c->h_iperm.data(), c->iperm, 4ull * n, hipMemcpyDeviceToHost, c->stream),
+                      "layout download")) ||
+      (rc = hip_check(c, hipStreamSynchronize(c->stream), "layout sync")))
+    return done(rc);
+  c->relabeled = true;
+  return done(SCC_OK);
+}
+
+int layout_to_caller(scc_ctx* c, const int32_t* in, bool map_values, int32_t* out_dev) {
+  LAUNCH(k_to_caller, grid_for(c->n, 4096), c->stream, c->n, c->perm, in, map_values ? 1 : 0, out_dev);
+  return hip_check(c, hipGetLastError(), "layout map");
+}
+
+int layout_canon_to_caller(scc_ctx* c, int32_t* out_dev) {
+  launch_fill_i32(c->stream, c->map_b, INT32_MAX, c->n);
+  LAUNCH(k_caller_min, grid_for(c->n, 4096), c->stream, c->n, c->canon, c->perm, c->map_b);
+  LAUNCH(k_caller_apply, grid_for(c->n, 4096), c->stream, c->n, c->canon, c->perm, c->map_b, out_dev);
+  return hip_check(c, hipGetLastError(), "layout canonical labels");
+}
+
+int layout_colors_to_internal(scc_ctx* c, const int32_t* colors_host) {
+  int rc;
+  if ((rc = hip_check(c, hipMemcpyAsync(c->map_b, colors_host, 4ull * c->n, hipMemcpyHostToDevice, c->stream),
+                      "colours upload")))
+    return rc;
+  LAUNCH(k_to_internal, grid_for(c->n, 4096), c->stream, c->n, c->iperm, c->map_b, c->color);
+  return hip_check(c, hipStreamSynchronize(c->stream), "colours sync");
+}
+
+// Rows of internal CSR (ib, ia) as caller rows (ob, oa), each sorted by id.
+void layout_rows_to_caller(const scc_ctx* c, const std::vector<int32_t>& ib, const std::vector<int32_t>& ia,
+                           int32_t* ob, int32_t* oa) {
+  const int32_t n = c->n;
+  std::vector<int32_t> off((size_t)n + 1, 0);
+  for (int32_t v = 0; v < n; v++) {
+    const int32_t i = c->h_iperm[v];
+    off[v + 1] = off[v] + (ib[i + 1] - ib[i]);
+  }
+  if (ob) std::copy(off.begin(), off.end(), ob);
+  if (!oa) return;
+  for (int32_t v = 0; v < n; v++) {
+    const int32_t i = c->h_iperm[v];
+    int32_t* row = oa + off[v];
+    const int32_t len = ib[i + 1] - ib[i];
+    for (int32_t j = 0; j < len; j++) row[j] = c->h_perm[ia[ib[i] + j]];
+    std::sort(row, row + len);
+  }
+}
+
+}  // namespace sccmi

@@ -45,16 +45,19 @@ PHASE_NAMES = {1: "trim1", 2: "fb_global", 3: "trim2", 4: "wcc", 5: "fb_round", 
 PIVOT_DEGREE_PRODUCT = 0
 PIVOT_MIN_ID = 1
 PIVOT_LEARNED = 2
+# vertex layout inside a context (scc_ctx_set_layout)
+LAYOUT_INPUT = 0
+LAYOUT_DEGREE = 1
 
 # every symbol include/sccmi.h declares
 EXPORTS = [
     "scc_ctx_create", "scc_ctx_destroy", "scc_last_error", "scc_ctx_bytes", "scc_bytes_needed",
     "scc_ctx_cancel", "scc_ctx_inject_fault", "scc_graph_upload", "scc_graph_upload_device",
     "scc_graph_info", "scc_graph_download", "scc_state_reset", "scc_state_set_colors",
-    "scc_trim1", "scc_fb_global", "scc_trim2", "scc_wcc", "scc_fb_partitions",
+    "scc_trim1", "scc_fb_global", "scc_trim2", "scc_wcc", "scc_color_round", "scc_fb_partitions",
     "scc_partitions_resolve",
     "scc_run_opts_default", "scc_run_method2", "scc_labels_download", "scc_colors_download",
-    "scc_count", "scc_labels_device", "scc_update", "scc_ctx_set_kernel_timing",
+    "scc_count", "scc_labels_device", "scc_update", "scc_ctx_set_kernel_timing", "scc_ctx_set_layout",
     "scc_ctx_synchronize", "scc_ctx_set_budget", "scc_update_bytes_needed",
     "scc_device_memory", "scc_phase_trace", "scc_sched_set_weights", "scc_sched_get_weights",
     "scc_sched_forward", "scc_sched_gradients", "scc_sched_train", "scc_sched_rollout",
@@ -152,6 +155,7 @@ def load_library():
         "scc_fb_global": (C.c_int, [vp, C.c_int, i32, C.POINTER(i64), C.POINTER(i32)]),
         "scc_trim2": (C.c_int, [vp, C.POINTER(i64)]),
         "scc_wcc": (C.c_int, [vp, C.POINTER(i64)]),
+        "scc_color_round": (C.c_int, [vp, C.POINTER(i64)]),
         "scc_fb_partitions": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(i64)]),
         "scc_partitions_resolve": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(i64)]),
         "scc_run_opts_default": (None, [C.POINTER(RunOpts)]),
@@ -160,6 +164,7 @@ def load_library():
         "scc_colors_download": (C.c_int, [vp, vp]),
         "scc_count": (C.c_int, [vp, C.POINTER(i64)]),
         "scc_labels_device": (C.c_int, [vp, C.POINTER(vp)]),
+        "scc_ctx_set_layout": (C.c_int, [vp, C.c_int]),
         "scc_update": (C.c_int, [vp, vp, vp, i64, vp, vp, i64, C.POINTER(UpdateStats)]),
         "scc_ctx_set_kernel_timing": (C.c_int, [vp, C.c_int]),
         "scc_ctx_synchronize": (C.c_int, [vp]),
@@ -248,6 +253,11 @@ class SccEngine:
         return int(self.lib.scc_ctx_bytes(self.h))
 
     # -- graph
+    def set_layout(self, layout: int) -> None:
+        """Vertex layout of the next upload (LAYOUT_INPUT or LAYOUT_DEGREE); ids,
+        labels and colours seen through this object stay the caller's."""
+        self._check(self.lib.scc_ctx_set_layout(self.h, int(layout)))
+
     def upload(self, g) -> None:
         b, a, rb, ra = (_i32(x) for x in (g.begin, g.node_idx, g.r_begin, g.r_node_idx))
         self._graph = (b, a, rb, ra)
@@ -300,6 +310,12 @@ class SccEngine:
     def do_global_wcc(self) -> int:
         t = C.c_int64()
         self._check(self.lib.scc_wcc(self.h, C.byref(t)))
+        return t.value
+
+    def color_round(self) -> int:
+        """One colouring step over every live partition; returns the vertices resolved."""
+        t = C.c_int64()
+        self._check(self.lib.scc_color_round(self.h, C.byref(t)))
         return t.value
 
     def start_workers_fw_bw_dfs(self, policy: int = PIVOT_DEGREE_PRODUCT,

@@ -87,6 +87,20 @@ int scc_graph_upload_device(scc_ctx* ctx, int32_t n, int64_t m, const int32_t* d
                             const int32_t* d_node_idx, const int32_t* d_r_begin,
                             const int32_t* d_r_node_idx);
 int scc_graph_info(const scc_ctx* ctx, int32_t* n, int64_t* m);
+/* Vertex layout inside the context (no counterpart in the reference: a device
+ * memory layout choice).  SCC_LAYOUT_DEGREE relabels the next uploaded graph
+ * by decreasing in + out degree (ties by id) so that hubs' per-vertex state
+ * shares cache lines; it pays on skewed (R-MAT) graphs.  Every entry point
+ * still takes and returns the caller's vertex ids, labels and colours, and
+ * canonical labels stay the minimum caller id of each component.  Batched
+ * graphs (scc_batch_set) need SCC_LAYOUT_INPUT. */
+typedef enum {
+  SCC_LAYOUT_INPUT = 0, /* device ids are the caller's ids (default) */
+  SCC_LAYOUT_DEGREE = 1 /* device ids by decreasing total degree */
+} scc_layout;
+/* Takes effect at the next scc_graph_upload / scc_graph_upload_device. */
+int scc_ctx_set_layout(scc_ctx* ctx, int layout);
+
 /* Download the current device CSR (after updates).  Any pointer may be NULL. */
 int scc_graph_download(scc_ctx* ctx, int32_t* begin, int32_t* node_idx, int32_t* r_begin,
                        int32_t* r_node_idx);
@@ -118,6 +132,12 @@ int scc_wcc(scc_ctx* ctx, int64_t* parts);
  * partition (all partitions at once), with trim1 between rounds when
  * trim_between != 0.  *rounds = rounds used. */
 int scc_fb_partitions(scc_ctx* ctx, int policy, int trim_between, int64_t* rounds);
+/* One colouring step over every live partition (no counterpart in the
+ * reference's CPU path; the step of SCC_PARTS_COLOR): W = minimum tag over each
+ * vertex's same-colour forward reach, each class recoloured by its root, and
+ * each root's SCC (its forward reach inside its class) resolved.
+ * *resolved = vertices resolved. */
+int scc_color_round(scc_ctx* ctx, int64_t* resolved);
 /* The partition phase with an explicit scc_partition_mode. */
 int scc_partitions_resolve(scc_ctx* ctx, int policy, int mode, int trim_between, int64_t* rounds);
 
