@@ -627,8 +627,10 @@ int scc_ctx_destroy(scc_ctx* c) {
 const char* scc_last_error(const scc_ctx* c) { return c ? c->err.c_str() : "null context"; }
 uint64_t scc_ctx_bytes(const scc_ctx* c) { return c ? c->bytes : 0; }
 uint64_t scc_bytes_needed(int32_t n, int64_t m) {
+  // graph, state, traversal copies or relabelling temporaries, scratch, the
+  // degree layout's maps (perm, iperm and two staging arrays), control block
   return 8ull * (uint64_t)(m > 0 ? m : 1) + 8ull * ((uint64_t)n + 1) + state_bytes(n) + order_bytes(m) +
-         4ull * ((uint64_t)n + 1) + sizeof(DevCtl);
+         4ull * ((uint64_t)n + 1) + 16ull * ((uint64_t)n + 1) + sizeof(DevCtl);
 }
 int scc_device_memory(int device, uint64_t* free_bytes, uint64_t* total_bytes) {
   if (!free_bytes || !total_bytes) return SCC_ERR_ARG;

@@ -99,13 +99,70 @@ __global__ __launch_bounds__(BLOCK) void k_to_internal(int32_t n, const int32_t*
 
 // mo[r] = min caller id over the members of the component whose internal
 // representative is r (canon[i] = r, or -1 while unresolved); mo must hold
-// INT32_MAX beforehand
+// INT32_MAX beforehand.  A giant component's members would all hit mo[r]:
+// lanes of a wave that share r reduce first and one lane issues the atomic,
+// and each workgroup keeps the most frequent r of its first 64 entries in LDS
+// with one global atomic at the end.
 __global__ __launch_bounds__(BLOCK) void k_caller_min(int32_t n, const int32_t* __restrict__ canon,
                                                       const int32_t* __restrict__ perm, int32_t* __restrict__ mo) {
-  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
-    const int32_t r = canon[i], p = perm[i];
-    if (r >= 0 && mo[r] > p) atomicMin(&mo[r], p);
+  __shared__ int32_t s_hot, s_hmin;
+  const int lane = lane_id();
+  if (threadIdx.x < 64) {  // wave 0 picks the hot representative
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const int32_t r = i < n ? canon[i] : -1;
+    bool pend = r >= 0;
+    int32_t best = -1;
+    int bestc = 1;
+    uint64_t act = __ballot(pend);
+    while (act) {
+      const int32_t R = __shfl(r, __ffsll((unsigned long long)act) - 1);
+      const bool same = pend && r == R;
+      const int cnt = __popcll(__ballot(same));
+      if (cnt > bestc) {
+        bestc = cnt;
+        best = R;
+      }
+      if (same) pend = false;
+      act = __ballot(pend);
+    }
+    if (threadIdx.x == 0) {
+      s_hot = best;
+      s_hmin = INT32_MAX;
+    }
   }
+  __syncthreads();
+  const int32_t hot = s_hot;
+  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n; base += (int64_t)gridDim.x * BLOCK) {
+    const int64_t i = base + threadIdx.x;
+    int32_t r = -1, p = INT32_MAX;
+    if (i < n) {
+      r = canon[i];
+      p = perm[i];
+    }
+    bool pend = r >= 0;
+    if (hot >= 0) {
+      const bool h = pend && r == hot;
+      int32_t x = h ? p : INT32_MAX;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o));
+      if (lane == 0 && x < INT32_MAX) atomicMin(&s_hmin, x);
+      if (h) pend = false;
+    }
+    uint64_t act = __ballot(pend);
+    while (act) {
+      const int leader = __ffsll((unsigned long long)act) - 1;
+      const int32_t R = __shfl(r, leader);
+      const bool mine = pend && r == R;
+      int32_t x = mine ? p : INT32_MAX;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o));
+      if (lane == leader && x < mo[R]) atomicMin(&mo[R], x);
+      if (mine) pend = false;
+      act = __ballot(pend);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && hot >= 0 && s_hmin < mo[hot]) atomicMin(&mo[hot], s_hmin);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_caller_apply(int32_t n, const int32_t* __restrict__ canon,

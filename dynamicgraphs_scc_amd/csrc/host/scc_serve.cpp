@@ -206,7 +206,8 @@ struct scc_server {
           return fail(rc, "load: no context on the device");
         }
         if (fault != INT_MIN) scc_ctx_inject_fault(g.ctx, fault);
-        if ((rc = scc_graph_upload(g.ctx, r.n, r.m, r.a.data(), r.b.data(), r.c.data(), r.d.data())) == SCC_OK) {
+        if ((rc = scc_ctx_set_layout(g.ctx, cfg.layout)) == SCC_OK &&
+            (rc = scc_graph_upload(g.ctx, r.n, r.m, r.a.data(), r.b.data(), r.c.data(), r.d.data())) == SCC_OK) {
           scc_run_opts o;
           scc_run_opts_default(&o);
           o.pivot_policy = cfg.pivot_policy;
@@ -296,6 +297,8 @@ void scc_server_config_default(scc_server_config* c) {
   c->hbm_budget_bytes = 0;
   c->max_batch = 100000;
   c->growth = 0.25;
+  c->layout = SCC_LAYOUT_INPUT;
+  c->reserved1 = 0;
 }
 
 int scc_server_create(const scc_server_config* cfg, scc_server** out) {
@@ -305,7 +308,7 @@ int scc_server_create(const scc_server_config* cfg, scc_server** out) {
   if (cfg) c = *cfg;
   else scc_server_config_default(&c);
   if (c.slots < 1 || c.slots > 64 || c.queue_limit < 1 || c.max_batch < 1 || c.growth < 0 ||
-      c.growth > 16) {
+      c.growth > 16 || (c.layout != SCC_LAYOUT_INPUT && c.layout != SCC_LAYOUT_DEGREE)) {
     t_err = "bad server configuration";
     return SCC_ERR_ARG;
   }

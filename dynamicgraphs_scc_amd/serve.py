@@ -32,7 +32,8 @@ EXPORTS = [
 class ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("queue_limit", C.c_int32),
                 ("pivot_policy", C.c_int32), ("hbm_budget_bytes", C.c_uint64),
-                ("max_batch", C.c_int64), ("growth", C.c_double)]
+                ("max_batch", C.c_int64), ("growth", C.c_double), ("layout", C.c_int32),
+                ("reserved1", C.c_int32)]
 
 
 class RequestInfo(C.Structure):
@@ -105,13 +106,13 @@ class SccServer:
 
     def __init__(self, device: int = 0, slots: int = 4, hbm_budget_bytes: int = 0,
                  max_batch: int = 100_000, growth: float = 0.25, queue_limit: int = 1 << 16,
-                 pivot_policy: int = PIVOT_DEGREE_PRODUCT):
+                 pivot_policy: int = PIVOT_DEGREE_PRODUCT, layout: int = 0):
         self.lib = load_serve_library()
         cfg = ServerConfig()
         self.lib.scc_server_config_default(C.byref(cfg))
         cfg.device, cfg.slots, cfg.queue_limit = device, slots, queue_limit
         cfg.pivot_policy, cfg.hbm_budget_bytes = pivot_policy, hbm_budget_bytes
-        cfg.max_batch, cfg.growth = max_batch, growth
+        cfg.max_batch, cfg.growth, cfg.layout = max_batch, growth, layout
         h = C.c_void_p()
         rc = self.lib.scc_server_create(C.byref(cfg), C.byref(h))
         if rc != 0:

@@ -55,6 +55,8 @@ typedef struct scc_server_config {
   uint64_t hbm_budget_bytes; /* sum of the graphs' reserves; 0 = 90% of the device's free memory */
   int64_t max_batch;         /* largest update list (inserts or deletes) accepted */
   double growth;             /* edge headroom reserved per graph, as a fraction of its edges */
+  int32_t layout;            /* scc_layout of every loaded graph (SCC_LAYOUT_INPUT by default) */
+  int32_t reserved1;
 } scc_server_config;
 
 typedef struct scc_request_info {

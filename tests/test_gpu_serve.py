@@ -164,3 +164,21 @@ def test_poisson_stream_statistics():
         tl, buf = srv.labels(gid, g.num_nodes)
         srv.result(tl)
     assert np.array_equal(buf, oracle.scc_canonical(final, 8))
+
+
+def test_degree_layout_server_equals_input_layout():
+    """Graphs loaded with the degree layout give the same labels after every
+    update as the input layout alone (caller ids throughout)."""
+    from dynamicgraphs_scc_amd.engine import LAYOUT_DEGREE
+    work = _workload(3, 4)
+    alone = [_alone(g, bs) for g, bs, _ in work]
+    with SccServer(slots=2, layout=LAYOUT_DEGREE) as srv:
+        for i, (g, bs, _) in enumerate(work):
+            gid, t = srv.load(g)
+            tickets = [srv.labels(gid, g.num_nodes)]
+            for a, b, c, d in bs:
+                srv.update(gid, a, b, c, d)
+                tickets.append(srv.labels(gid, g.num_nodes))
+            for j, (t, buf) in enumerate(tickets):
+                assert srv.result(t)["status"] == 0
+                assert np.array_equal(buf, alone[i][j]), f"graph {i} step {j}"

@@ -15,7 +15,8 @@ def _have_device():
 
 
 def test_bad_configuration_rejected():
-    for kw in ({"slots": 0}, {"slots": 65}, {"max_batch": 0}, {"growth": -1.0}, {"queue_limit": 0}):
+    for kw in ({"slots": 0}, {"slots": 65}, {"max_batch": 0}, {"growth": -1.0}, {"queue_limit": 0},
+               {"layout": 7}):
         with pytest.raises(SccError) as ei:
             serve.SccServer(**kw)
         assert ei.value.code == -1, kw
@@ -34,3 +35,4 @@ def test_default_config():
     cfg = serve.ServerConfig()
     lib.scc_server_config_default(ctypes.byref(cfg))
     assert cfg.slots >= 1 and cfg.max_batch == 100000 and cfg.hbm_budget_bytes == 0
+    assert cfg.layout == 0
