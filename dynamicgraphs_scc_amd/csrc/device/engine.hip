@@ -805,8 +805,15 @@ int scc_state_set_colors(scc_ctx* c, const int32_t* colors) {
 
 int scc_ctx_set_layout(scc_ctx* c, int layout) {
   if (!c) return SCC_ERR_ARG;
-  if (layout != SCC_LAYOUT_INPUT && layout != SCC_LAYOUT_DEGREE) return set_error(c, SCC_ERR_ARG, "bad layout");
+  if (layout != SCC_LAYOUT_INPUT && layout != SCC_LAYOUT_DEGREE && layout != SCC_LAYOUT_AUTO)
+    return set_error(c, SCC_ERR_ARG, "bad layout");
   c->layout_req = layout;
+  return SCC_OK;
+}
+
+int scc_graph_layout(const scc_ctx* c, int* layout) {
+  if (!c || !layout) return SCC_ERR_ARG;
+  *layout = c->relabeled ? SCC_LAYOUT_DEGREE : SCC_LAYOUT_INPUT;
   return SCC_OK;
 }
 

@@ -1,4 +1,5 @@
-// Vertex layout (scc_ctx_set_layout): with SCC_LAYOUT_DEGREE the uploaded
+// Vertex layout (scc_ctx_set_layout): with SCC_LAYOUT_DEGREE (or AUTO on a
+// skewed graph) the uploaded
 // graph is relabelled inside the context so that internal id i is the i-th
 // vertex by decreasing total degree (in + out), ties by caller id.  On skewed
 // graphs the hubs' per-vertex state (colours, reach bits, WCC words) then sits
@@ -249,7 +250,14 @@ void drop_layout(scc_ctx* c) {
 
 int build_layout(scc_ctx* c) {
   drop_layout(c);
-  if (c->layout_req != SCC_LAYOUT_DEGREE || c->n <= 0) return SCC_OK;
+  if (c->n <= 0) return SCC_OK;
+  if (c->layout_req == SCC_LAYOUT_AUTO) {
+    // skewed graphs only: hubs are what the relabelling packs together
+    const double mean = std::max(1.0, (double)c->m / (double)c->n);
+    if ((double)c->maxdeg <= 32.0 * mean) return SCC_OK;
+  } else if (c->layout_req != SCC_LAYOUT_DEGREE) {
+    return SCC_OK;
+  }
   const int32_t n = c->n;
   int rc;
   uint32_t *key0 = nullptr, *key1 = nullptr;

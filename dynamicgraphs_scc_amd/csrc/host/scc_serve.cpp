@@ -308,7 +308,7 @@ int scc_server_create(const scc_server_config* cfg, scc_server** out) {
   if (cfg) c = *cfg;
   else scc_server_config_default(&c);
   if (c.slots < 1 || c.slots > 64 || c.queue_limit < 1 || c.max_batch < 1 || c.growth < 0 ||
-      c.growth > 16 || (c.layout != SCC_LAYOUT_INPUT && c.layout != SCC_LAYOUT_DEGREE)) {
+      c.growth > 16 || c.layout < SCC_LAYOUT_INPUT || c.layout > SCC_LAYOUT_AUTO) {
     t_err = "bad server configuration";
     return SCC_ERR_ARG;
   }

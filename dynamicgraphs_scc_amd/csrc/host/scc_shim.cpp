@@ -35,7 +35,11 @@ void check(int rc, const char* what) {
 
 void bind(gm_graph& G) {
   if (!G.is_frozen() && !G.make_reverse_edges()) die("graph freeze");
-  if (!g_ctx && scc_ctx_create(0, 0, &g_ctx) != SCC_OK) die("no HIP device");
+  if (!g_ctx) {
+    if (scc_ctx_create(0, 0, &g_ctx) != SCC_OK) die("no HIP device");
+    // skewed graphs are relabelled by degree inside the context; ids stay the caller's
+    scc_ctx_set_layout(g_ctx, SCC_LAYOUT_AUTO);
+  }
   if (g_bound == &G && g_version == G.version()) return;
   const int32_t n = G.num_nodes();
   check(scc_graph_upload(g_ctx, n, G.num_edges(), G.begin, G.node_idx, G.r_begin, G.r_node_idx),

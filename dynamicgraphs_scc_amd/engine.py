@@ -48,6 +48,7 @@ PIVOT_LEARNED = 2
 # vertex layout inside a context (scc_ctx_set_layout)
 LAYOUT_INPUT = 0
 LAYOUT_DEGREE = 1
+LAYOUT_AUTO = 2
 
 # every symbol include/sccmi.h declares
 EXPORTS = [
@@ -57,7 +58,7 @@ EXPORTS = [
     "scc_trim1", "scc_fb_global", "scc_trim2", "scc_wcc", "scc_color_round", "scc_fb_partitions",
     "scc_partitions_resolve",
     "scc_run_opts_default", "scc_run_method2", "scc_labels_download", "scc_colors_download",
-    "scc_count", "scc_labels_device", "scc_update", "scc_ctx_set_kernel_timing", "scc_ctx_set_layout",
+    "scc_count", "scc_labels_device", "scc_update", "scc_ctx_set_kernel_timing", "scc_ctx_set_layout", "scc_graph_layout",
     "scc_ctx_synchronize", "scc_ctx_set_budget", "scc_update_bytes_needed",
     "scc_device_memory", "scc_phase_trace", "scc_sched_set_weights", "scc_sched_get_weights",
     "scc_sched_forward", "scc_sched_gradients", "scc_sched_train", "scc_sched_rollout",
@@ -165,6 +166,7 @@ def load_library():
         "scc_count": (C.c_int, [vp, C.POINTER(i64)]),
         "scc_labels_device": (C.c_int, [vp, C.POINTER(vp)]),
         "scc_ctx_set_layout": (C.c_int, [vp, C.c_int]),
+        "scc_graph_layout": (C.c_int, [vp, C.POINTER(C.c_int)]),
         "scc_update": (C.c_int, [vp, vp, vp, i64, vp, vp, i64, C.POINTER(UpdateStats)]),
         "scc_ctx_set_kernel_timing": (C.c_int, [vp, C.c_int]),
         "scc_ctx_synchronize": (C.c_int, [vp]),
@@ -257,6 +259,12 @@ class SccEngine:
         """Vertex layout of the next upload (LAYOUT_INPUT or LAYOUT_DEGREE); ids,
         labels and colours seen through this object stay the caller's."""
         self._check(self.lib.scc_ctx_set_layout(self.h, int(layout)))
+
+    def layout(self) -> int:
+        """Layout the uploaded graph is held in (LAYOUT_INPUT or LAYOUT_DEGREE)."""
+        x = C.c_int()
+        self._check(self.lib.scc_graph_layout(self.h, C.byref(x)))
+        return x.value
 
     def upload(self, g) -> None:
         b, a, rb, ra = (_i32(x) for x in (g.begin, g.node_idx, g.r_begin, g.r_node_idx))

@@ -95,11 +95,15 @@ int scc_graph_info(const scc_ctx* ctx, int32_t* n, int64_t* m);
  * canonical labels stay the minimum caller id of each component.  Batched
  * graphs (scc_batch_set) need SCC_LAYOUT_INPUT. */
 typedef enum {
-  SCC_LAYOUT_INPUT = 0, /* device ids are the caller's ids (default) */
-  SCC_LAYOUT_DEGREE = 1 /* device ids by decreasing total degree */
+  SCC_LAYOUT_INPUT = 0,  /* device ids are the caller's ids (default) */
+  SCC_LAYOUT_DEGREE = 1, /* device ids by decreasing total degree */
+  SCC_LAYOUT_AUTO = 2    /* SCC_LAYOUT_DEGREE when the largest in- or out-degree exceeds 32 times
+                            the mean degree (skewed graphs), else SCC_LAYOUT_INPUT */
 } scc_layout;
 /* Takes effect at the next scc_graph_upload / scc_graph_upload_device. */
 int scc_ctx_set_layout(scc_ctx* ctx, int layout);
+/* The layout the uploaded graph is held in (SCC_LAYOUT_INPUT or SCC_LAYOUT_DEGREE). */
+int scc_graph_layout(const scc_ctx* ctx, int* layout);
 
 /* Download the current device CSR (after updates).  Any pointer may be NULL. */
 int scc_graph_download(scc_ctx* ctx, int32_t* begin, int32_t* node_idx, int32_t* r_begin,

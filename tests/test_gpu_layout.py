@@ -141,3 +141,19 @@ def test_rmat22_full_size(deng):
     assert np.array_equal(lab, want)
     assert st["num_sccs"] == len(np.unique(want))
     assert st["scc_first"] == np.bincount(want).max()
+
+
+def test_auto_layout_follows_skew():
+    """SCC_LAYOUT_AUTO relabels a skewed R-MAT graph and keeps a planted graph's
+    ids; labels are exact either way."""
+    from dynamicgraphs_scc_amd.engine import LAYOUT_AUTO, LAYOUT_INPUT
+    with SccEngine() as e:
+        e.set_layout(LAYOUT_AUTO)
+        g = graphs.rmat(14, 8, seed=21)
+        lab, _ = run(e, g)
+        assert e.layout() == LAYOUT_DEGREE
+        assert np.array_equal(lab, oracle.scc_canonical(g, 8))
+        g, comp = graphs.planted(30000, seed=5, n_medium=600, with_components=True)
+        lab, _ = run(e, g)
+        assert e.layout() == LAYOUT_INPUT
+        assert np.array_equal(lab, graphs.canonicalize(comp))
