@@ -701,7 +701,10 @@ int scc_graph_upload(scc_ctx* c, int32_t n, int64_t m, const int32_t* begin, con
       (m > 0 && (rc = hip_check(c, hipMemcpyAsync(c->adj, node_idx, 4ull * m, hipMemcpyHostToDevice, c->stream), "upload"))) ||
       (m > 0 && (rc = hip_check(c, hipMemcpyAsync(c->radj, r_node_idx, 4ull * m, hipMemcpyHostToDevice, c->stream), "upload"))))
     return rc;
-  if ((rc = build_layout(c))) return rc;
+  if ((rc = build_layout(c))) {  // a half-relabelled graph must not stay usable
+    free_graph(c);
+    return rc;
+  }
   if (!c->relabeled && (rc = build_order(c))) return rc;
   if ((rc = reset_state(c))) return rc;
   c->ran = false;
@@ -732,7 +735,10 @@ int scc_graph_upload_device(scc_ctx* c, int32_t n, int64_t m, const int32_t* d_b
       return rc;
     c->maxdeg = max_degree(n, b.data(), rb.data());
   }
-  if ((rc = build_layout(c))) return rc;
+  if ((rc = build_layout(c))) {  // a half-relabelled graph must not stay usable
+    free_graph(c);
+    return rc;
+  }
   if (!c->relabeled && (rc = build_order(c))) return rc;
   if ((rc = reset_state(c))) return rc;
   c->ran = false;
