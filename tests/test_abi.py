@@ -106,3 +106,28 @@ int main() {{
                         "-L", LIB, "-lsccshim", "-lsccmi", "-lsccgen", f"-Wl,-rpath,{LIB}"], check=True)
         out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
     assert out == ["1"] * len(names) + ["0"]
+
+
+def test_enum_values_match_header():
+    """The Python constants for layouts, partition modes and pivot policies are
+    the values the C compiler gives the header's enums."""
+    import subprocess
+    import tempfile
+    from dynamicgraphs_scc_amd import engine
+    src = r"""
+#include <stdio.h>
+#include "sccmi.h"
+int main(){printf("%d %d %d %d %d %d %d %d %d %d\n", SCC_LAYOUT_INPUT, SCC_LAYOUT_DEGREE, SCC_LAYOUT_AUTO,
+  SCC_PARTS_FB, SCC_PARTS_COLOR, SCC_PARTS_BOTH, SCC_PARTS_LEARNED, SCC_PIVOT_DEGREE_PRODUCT,
+  SCC_PIVOT_MIN_ID, SCC_PIVOT_LEARNED);}
+"""
+    with tempfile.TemporaryDirectory() as td:
+        c = os.path.join(td, "e.c")
+        with open(c, "w") as f:
+            f.write(src)
+        exe = os.path.join(td, "e")
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    assert out == [engine.LAYOUT_INPUT, engine.LAYOUT_DEGREE, engine.LAYOUT_AUTO, engine.PARTS_FB,
+                   engine.PARTS_COLOR, engine.PARTS_BOTH, engine.PARTS_LEARNED,
+                   engine.PIVOT_DEGREE_PRODUCT, engine.PIVOT_MIN_ID, engine.PIVOT_LEARNED]
