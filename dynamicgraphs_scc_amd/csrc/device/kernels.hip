@@ -583,6 +583,57 @@ __device__ __forceinline__ void claim4(uint32_t* __restrict__ V, const int32_t* 
   }
 }
 
+// claim4 for four consecutive entries of one row held by one lane: entries
+// whose ids share a visited word are claimed with one atomicOr issued by the
+// first of them, and an id repeated among the four is claimed at most once.
+__device__ __forceinline__ void claim4_run(uint32_t* __restrict__ V, const int32_t* __restrict__ color,
+                                           const int32_t (&k)[4], int32_t c, bool (&cl)[4]) {
+  uint32_t w[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) w[t] = k[t] >= 0 ? V[k[t] >> 5] : 0xFFFFFFFFu;
+  bool want[4];
+  int32_t ck[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    want[t] = k[t] >= 0 && !((w[t] >> (k[t] & 31)) & 1u);
+    ck[t] = want[t] ? color[k[t]] : DONE;
+  }
+#pragma unroll
+  for (int t = 0; t < 4; t++) want[t] = want[t] && ck[t] == c;
+  // lead[t]: the group (word) of t starts at t; bits[t]: the group's bits
+  int g[4];
+  uint32_t bits[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    g[t] = t;
+    bits[t] = 0u;
+#pragma unroll
+    for (int u = 0; u < t; u++)
+      if (g[t] == t && want[u] && want[t] && (k[u] >> 5) == (k[t] >> 5)) g[t] = u;
+  }
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const uint32_t b = want[t] ? 1u << (k[t] & 31) : 0u;
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (g[t] == u) bits[u] |= b;
+  }
+  uint32_t old[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) old[t] = (want[t] && g[t] == t) ? atomicOr(&V[k[t] >> 5], bits[t]) : 0xFFFFFFFFu;
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    uint32_t og = old[0];
+#pragma unroll
+    for (int u = 1; u < 4; u++)
+      if (g[t] == u) og = old[u];
+    bool dup = false;
+#pragma unroll
+    for (int u = 0; u < t; u++) dup = dup || (want[u] && k[u] == k[t]);
+    cl[t] = want[t] && !dup && !((og >> (k[t] & 31)) & 1u);
+  }
+}
+
 // Whether any of four candidates parents k[t] (k[t] < 0: none) is visited
 // and, with several partitions, of colour c; the loads of all four are issued
 // together.
@@ -688,15 +739,21 @@ __device__ __forceinline__ void level_part(DevCtl* __restrict__ dc, LevelCtl* __
         const int32_t e = min(s + TD_CHUNK, offe[u]);
         if (s >= e) continue;  // uniform across the workgroup
         const int32_t c = color[u];
-        for (int32_t b = s; b < e; b += 4 * BLOCK) {
+        // each thread takes TD_CHUNK / BLOCK consecutive entries: the lanes of
+        // a wave are that many entries apart, so where the sorted row is dense
+        // (hub neighbours with nearby ids) they claim in different visited
+        // words instead of queueing on one
+        constexpr int PER = TD_CHUNK / BLOCK;
+        const int32_t tb = s + (int32_t)threadIdx.x * PER;
+        for (int32_t j = 0; j < PER; j += 4) {
           int32_t k[4];
           bool cl[4];
 #pragma unroll
           for (int t = 0; t < 4; t++) {
-            const int32_t idx = b + t * BLOCK + threadIdx.x;
+            const int32_t idx = tb + j + t;
             k[t] = idx < e ? nb[idx] : -1;
           }
-          claim4(V, color, k, c, cl);
+          claim4_run(V, color, k, c, cl);
 #pragma unroll
           for (int t = 0; t < 4; t++)
             if (cl[t]) edges += (unsigned long long)(offe[k[t]] - off[k[t]]);
