@@ -57,33 +57,20 @@ __global__ __launch_bounds__(BLOCK) void k_new_lengths(int32_t n, const int32_t*
   }
 }
 
-// key = internal row << 32 | internal neighbour.  A lane per caller row;
-// rows longer than 64 entries are written by the whole wave afterwards.
+// key = internal row << 32 | internal neighbour; a wave per caller row
+// (measured faster than a lane per row with the wave taking the long rows:
+// 1.59 vs 1.84 ms per direction at R-MAT-22)
 __global__ __launch_bounds__(BLOCK) void k_edge_keys(int32_t n, const int32_t* __restrict__ begin,
                                                      const int32_t* __restrict__ adj,
                                                      const int32_t* __restrict__ iperm,
                                                      unsigned long long* __restrict__ keys) {
   const int lane = lane_id();
-  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n; base += (int64_t)gridDim.x * BLOCK) {
-    const int64_t v = base + threadIdx.x;
-    int32_t s = 0, e = 0;
-    unsigned long long row = 0;
-    if (v < n) {
-      s = begin[v];
-      e = begin[v + 1];
-      row = (unsigned long long)(uint32_t)iperm[v] << 32;
-    }
-    const bool longrow = e - s > 64;
-    if (!longrow)
-      for (int32_t i = s; i < e; i++) keys[i] = row | (uint32_t)iperm[adj[i]];
-    uint64_t hm = __ballot(longrow);
-    while (hm) {
-      const int L = __ffsll((unsigned long long)hm) - 1;
-      hm &= hm - 1;
-      const int32_t hs = __shfl(s, L), he = __shfl(e, L);
-      const unsigned long long hrow = __shfl(row, L);
-      for (int32_t i = hs + lane; i < he; i += 64) keys[i] = hrow | (uint32_t)iperm[adj[i]];
-    }
+  const int64_t wave = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * BLOCK) >> 6;
+  for (int64_t v = wave; v < n; v += nw) {
+    const int32_t s = begin[v], e = begin[v + 1];
+    const unsigned long long row = (unsigned long long)(uint32_t)iperm[v] << 32;
+    for (int32_t i = s + lane; i < e; i += 64) keys[i] = row | (uint32_t)iperm[adj[i]];
   }
 }
 
@@ -212,7 +199,7 @@ int relabel_dir(scc_ctx* c, int32_t* begin, int32_t* adj) {
     if ((rc = dev_alloc(c, (void**)&k0, 8ull * m, "layout keys")) ||
         (rc = dev_alloc(c, (void**)&k1, 8ull * m, "layout keys")))
       return done(rc);
-    LAUNCH(k_edge_keys, grid_for(n, 4096), c->stream, n, begin, adj, c->iperm, k0);
+    LAUNCH(k_edge_keys, grid_for((int64_t)n * 64, 4096), c->stream, n, begin, adj, c->iperm, k0);
   }
   int bits = 32;
   while (bits < 64 && (1ull << (bits - 32)) < (unsigned long long)n) bits++;
