@@ -326,8 +326,8 @@ def main():
     ap.add_argument("--dyn-rate", type=float, default=60.0, help="Poisson arrival rate, batches/s")
     ap.add_argument("--dyn-fixed-batches", type=int, default=500)
     ap.add_argument("--dyn-fixed-rate", type=float, default=100.0)
-    ap.add_argument("--dyn-layout", default="input", choices=["input", "degree"],
-                    help="vertex layout of the served graph (degree: lower p50, higher p99 measured)")
+    ap.add_argument("--dyn-layout", default="degree", choices=["input", "degree"],
+                    help="vertex layout of the served graph")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "

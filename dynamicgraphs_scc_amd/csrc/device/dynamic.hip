@@ -258,36 +258,118 @@ __global__ __launch_bounds__(BLOCK) void k_row_plan(const int32_t* __restrict__ 
 // search over the group's offsets staged in LDS.  Positions past a row's
 // length (spare room) are skipped.
 constexpr int RB_ROWS = 256;
+// A group whose destination range is longer than this is left to
+// k_gather_heavy (with the degree vertex layout the first group holds the 256
+// largest hubs, millions of entries that one workgroup would copy alone).
+constexpr int32_t RB_HEAVY = 65536;
+constexpr int32_t RB_CHUNK = 8192;
+
+// Stage group [j0, j0 + cnt)'s offsets (relative to doff[j0]), sources and lengths in LDS.
+__device__ __forceinline__ void stage_group(const int32_t* __restrict__ rows, int64_t j0, int32_t cnt,
+                                            const int32_t* __restrict__ sbeg, const int32_t* __restrict__ send,
+                                            const int32_t* __restrict__ doff, int32_t o0, int32_t* s_off,
+                                            int32_t* s_src, int32_t* s_len) {
+  for (int t = threadIdx.x; t <= cnt; t += BLOCK) s_off[t] = doff[j0 + t] - o0;
+  for (int t = threadIdx.x; t < cnt; t += BLOCK) {
+    const int32_t u = rows ? rows[j0 + t] : (int32_t)(j0 + t);
+    s_src[t] = sbeg[u];
+    s_len[t] = send[u] - sbeg[u];
+  }
+}
+
+// Copy destination positions [e0, e1) of a staged group.
+__device__ __forceinline__ void copy_group(int32_t cnt, int32_t o0, int32_t e0, int32_t e1,
+                                           const int32_t* s_off, const int32_t* s_src, const int32_t* s_len,
+                                           const int32_t* __restrict__ src, int32_t* __restrict__ dst) {
+  for (int32_t e = e0 + threadIdx.x; e < e1; e += BLOCK) {
+    int32_t lo = 0, hi = cnt;
+    while (hi - lo > 1) {
+      const int32_t mid = (lo + hi) >> 1;
+      if (s_off[mid] <= e) lo = mid;
+      else hi = mid;
+    }
+    const int32_t i = e - s_off[lo];
+    if (i < s_len[lo]) dst[o0 + e] = src[s_src[lo] + i];
+  }
+}
+
 __global__ __launch_bounds__(BLOCK) void k_gather_rows(const int32_t* __restrict__ rows, int64_t nr,
                                                        const int32_t* __restrict__ sbeg,
                                                        const int32_t* __restrict__ send,
                                                        const int32_t* __restrict__ src,
                                                        const int32_t* __restrict__ doff,
-                                                       int32_t* __restrict__ dst) {
+                                                       int32_t* __restrict__ dst, int32_t* __restrict__ heavy) {
   __shared__ int32_t s_off[RB_ROWS + 1];
   __shared__ int32_t s_src[RB_ROWS], s_len[RB_ROWS];
   for (int64_t j0 = (int64_t)blockIdx.x * RB_ROWS; j0 < nr; j0 += (int64_t)gridDim.x * RB_ROWS) {
     const int32_t cnt = (int32_t)min<int64_t>(RB_ROWS, nr - j0);
     const int32_t o0 = doff[j0];
-    for (int t = threadIdx.x; t <= cnt; t += BLOCK) s_off[t] = doff[j0 + t] - o0;
-    for (int t = threadIdx.x; t < cnt; t += BLOCK) {
-      const int32_t u = rows ? rows[j0 + t] : (int32_t)(j0 + t);
-      s_src[t] = sbeg[u];
-      s_len[t] = send[u] - sbeg[u];
-    }
+    stage_group(rows, j0, cnt, sbeg, send, doff, o0, s_off, s_src, s_len);
     __syncthreads();
     const int32_t total = s_off[cnt];
-    for (int32_t e = threadIdx.x; e < total; e += BLOCK) {
-      int32_t lo = 0, hi = cnt;
-      while (hi - lo > 1) {
-        const int32_t mid = (lo + hi) >> 1;
-        if (s_off[mid] <= e) lo = mid;
-        else hi = mid;
-      }
-      const int32_t i = e - s_off[lo];
-      if (i < s_len[lo]) dst[o0 + e] = src[s_src[lo] + i];
+    if (total > RB_HEAVY) {  // (uniform) listed for k_gather_heavy
+      if (threadIdx.x == 0) heavy[1 + atomicAdd(&heavy[0], 1)] = (int32_t)(j0 / RB_ROWS);
+    } else {
+      copy_group(cnt, o0, 0, total, s_off, s_src, s_len, src, dst);
     }
     __syncthreads();
+  }
+}
+
+// The listed heavy groups, in RB_CHUNK-position pieces: the chunks of all
+// listed groups are numbered in list order and dealt round-robin over the grid,
+// so every workgroup gets about the same number whatever the groups' sizes.
+// Each batch of up to BLOCK groups has its chunk counts loaded in parallel and
+// prefix-summed in LDS; a workgroup stages only the groups it has chunks in.
+__global__ __launch_bounds__(BLOCK) void k_gather_heavy(const int32_t* __restrict__ rows, int64_t nr,
+                                                        const int32_t* __restrict__ sbeg,
+                                                        const int32_t* __restrict__ send,
+                                                        const int32_t* __restrict__ src,
+                                                        const int32_t* __restrict__ doff,
+                                                        int32_t* __restrict__ dst,
+                                                        const int32_t* __restrict__ heavy) {
+  __shared__ int32_t s_off[RB_ROWS + 1];
+  __shared__ int32_t s_src[RB_ROWS], s_len[RB_ROWS];
+  __shared__ int32_t s_incl[BLOCK], s_nch[BLOCK];
+  const int32_t nh = heavy[0];
+  const int64_t G = gridDim.x;
+  int64_t acc = 0;  // chunks of the batches before this one (the same in every thread)
+  for (int32_t h0 = 0; h0 < nh; h0 += BLOCK) {
+    const int32_t nb = min(BLOCK, nh - h0);
+    int32_t mine = 0;
+    if ((int32_t)threadIdx.x < nb) {
+      const int64_t j0 = (int64_t)heavy[1 + h0 + threadIdx.x] * RB_ROWS;
+      const int32_t cnt = (int32_t)min<int64_t>(RB_ROWS, nr - j0);
+      mine = (doff[j0 + cnt] - doff[j0] + RB_CHUNK - 1) / RB_CHUNK;
+    }
+    s_nch[threadIdx.x] = mine;
+    s_incl[threadIdx.x] = mine;
+    __syncthreads();
+    for (int o = 1; o < BLOCK; o <<= 1) {  // inclusive scan of the chunk counts
+      const int32_t add = (int)threadIdx.x >= o ? s_incl[threadIdx.x - o] : 0;
+      __syncthreads();
+      s_incl[threadIdx.x] += add;
+      __syncthreads();
+    }
+    for (int32_t k = 0; k < nb; k++) {
+      const int32_t nch = s_nch[k];
+      const int64_t first = acc + s_incl[k] - nch;  // number of the group's first chunk
+      int64_t c0 = ((int64_t)blockIdx.x - first) % G;
+      if (c0 < 0) c0 += G;
+      if (c0 >= nch) continue;  // uniform across the workgroup
+      const int64_t j0 = (int64_t)heavy[1 + h0 + k] * RB_ROWS;
+      const int32_t cnt = (int32_t)min<int64_t>(RB_ROWS, nr - j0);
+      const int32_t o0 = doff[j0];
+      const int32_t total = doff[j0 + cnt] - o0;
+      stage_group(rows, j0, cnt, sbeg, send, doff, o0, s_off, s_src, s_len);
+      __syncthreads();
+      for (int64_t ch = c0; ch < nch; ch += G)
+        copy_group(cnt, o0, (int32_t)ch * RB_CHUNK, min(total, ((int32_t)ch + 1) * RB_CHUNK), s_off, s_src,
+                   s_len, src, dst);
+      __syncthreads();  // before the next group is staged
+    }
+    acc += s_incl[BLOCK - 1];
+    __syncthreads();  // s_incl / s_nch are rewritten by the next batch
   }
 }
 
@@ -687,11 +769,23 @@ int regap_dir(scc_ctx* c, Dyn* d, bool fwd, int64_t min_over) {
   const int64_t nsize = (int64_t)total + over;
   if (nsize >= INT32_MAX) return fail(set_error(c, SCC_ERR_LIMIT, "gapped layout would exceed int32 offsets"));
   if ((rc = dev_alloc(c, (void**)&nadj, 4ull * (uint64_t)nsize, "gapped adjacency"))) return fail(rc);
+  // heavy-group list: count, then up to one entry per group
+  int32_t* heavy = nullptr;
+  if ((rc = dev_alloc(c, (void**)&heavy, 4ull * (uint64_t)((int64_t)n / RB_ROWS + 2), "regap heavy groups")))
+    return fail(rc);
+  launch_fill_i32(c->stream, heavy, 0, 1);
   LAUNCH(k_gather_rows, grid_for(n, 4096), c->stream, (const int32_t*)nullptr, (int64_t)n, beg, cur_end, adj,
-         nbeg, nadj);
+         nbeg, nadj, heavy);
+  LAUNCH(k_gather_heavy, 2048, c->stream, (const int32_t*)nullptr, (int64_t)n, beg, cur_end, adj, nbeg, nadj,
+         (const int32_t*)heavy);
   LAUNCH(k_room_ends, grid_for(n, 4096), c->stream, n, beg, cur_end, nbeg, nend, ncap);
-  if ((rc = hip_check(c, hipGetLastError(), "regap"))) return fail(rc);
-  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "regap sync"))) return fail(rc);
+  if ((rc = hip_check(c, hipGetLastError(), "regap"))) {
+    dev_free(c, heavy);
+    return fail(rc);
+  }
+  rc = hip_check(c, hipStreamSynchronize(c->stream), "regap sync");
+  dev_free(c, heavy);
+  if (rc) return fail(rc);
   dev_free(c, beg);
   dev_free(c, en);
   dev_free(c, adj);
