@@ -137,7 +137,6 @@ int scc_batch_set(scc_ctx* c, int32_t num_graphs, const int32_t* offsets) {
   c->h_boff.clear();
   if (num_graphs == 0) return SCC_OK;  // back to one graph
   if (num_graphs < 0 || !offsets) return set_error(c, SCC_ERR_ARG, "bad batch");
-  if (c->relabeled) return set_error(c, SCC_ERR_STATE, "batched graphs need SCC_LAYOUT_INPUT");
   if (offsets[0] != 0 || offsets[num_graphs] != c->n)
     return set_error(c, SCC_ERR_ARG, "batch offsets must run from 0 to n");
   for (int32_t g = 0; g < num_graphs; g++)
@@ -148,6 +147,14 @@ int scc_batch_set(scc_ctx* c, int32_t num_graphs, const int32_t* offsets) {
   if ((rc = hip_check(c, hipMemcpyAsync(c->d_boff, offsets, 4ull * ((uint64_t)num_graphs + 1),
                                         hipMemcpyHostToDevice, c->stream), "batch offsets")))
     return rc;
+  // the degree layout becomes a per-graph degree order, so that every graph
+  // keeps its id range inside the context as well
+  if (c->relabeled && (rc = layout_segment(c, num_graphs))) {
+    dev_free(c, c->d_boff);
+    c->d_boff = nullptr;
+    if (!c->relabeled) free_graph(c);  // a failed move leaves no usable graph
+    return rc;
+  }
   // no edge may leave its graph (either direction)
   if ((rc = zero_ctr(c))) return rc;
   LAUNCH(k_cross_edges, grid_for(c->n, 4096), c->stream, c->n, c->begin, row_end(c), c->adj, c->d_boff,

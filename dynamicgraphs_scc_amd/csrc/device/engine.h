@@ -21,6 +21,7 @@ int check_cancel(scc_ctx* c);
 int reset_state(scc_ctx* c);
 int start_run(scc_ctx* c);  // reset_state, then the batch's per-graph colours when a batch is set
 int alloc_graph(scc_ctx* c, int32_t n, int64_t m_cap);
+int free_graph(scc_ctx* c);
 int build_order(scc_ctx* c);  // traversal copies of the rows, hubs first (order.hip)
 void drop_order(scc_ctx* c);
 
@@ -30,6 +31,8 @@ void drop_layout(scc_ctx* c);
 int layout_to_caller(scc_ctx* c, const int32_t* in, bool map_values, int32_t* out_dev);
 int layout_canon_to_caller(scc_ctx* c, int32_t* out_dev);  // from canon[]: min caller id per component
 int layout_colors_to_internal(scc_ctx* c, const int32_t* colors_host);
+// re-relabel by (batch graph, decreasing degree) once c->d_boff holds k graphs' caller ranges
+int layout_segment(scc_ctx* c, int32_t k);
 void layout_rows_to_caller(const scc_ctx* c, const std::vector<int32_t>& ib, const std::vector<int32_t>& ia,
                            int32_t* ob, int32_t* oa);
 

@@ -177,6 +177,38 @@ __global__ __launch_bounds__(BLOCK) void k_caller_apply(int32_t n, const int32_t
   }
 }
 
+// Segmented order for batched graphs: key = graph << 32 | ~degree of
+// internal vertex i, value = its caller id (graph g owns caller ids
+// [boff[g], boff[g + 1])).
+__global__ __launch_bounds__(BLOCK) void k_segment_keys(int32_t n, const int32_t* __restrict__ perm,
+                                                        const int32_t* __restrict__ boff, int32_t k,
+                                                        const int32_t* __restrict__ begin,
+                                                        const int32_t* __restrict__ rbegin,
+                                                        unsigned long long* __restrict__ key,
+                                                        int32_t* __restrict__ val) {
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+    const int32_t v = perm[i];
+    int32_t lo = 0, hi = k;  // boff[lo] <= v < boff[hi]
+    while (hi - lo > 1) {
+      const int32_t mid = (lo + hi) >> 1;
+      if (boff[mid] <= v) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t deg = (uint32_t)(begin[i + 1] - begin[i]) + (uint32_t)(rbegin[i + 1] - rbegin[i]);
+    key[i] = ((unsigned long long)(uint32_t)lo << 32) | (uint32_t)~deg;
+    val[i] = v;
+  }
+}
+
+// rel[j] = cur_iperm[q[j]]: the current internal id of the vertex that the new
+// order puts at j
+__global__ __launch_bounds__(BLOCK) void k_compose(int32_t n, const int32_t* __restrict__ q,
+                                                   const int32_t* __restrict__ cur_iperm,
+                                                   int32_t* __restrict__ rel) {
+  for (int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x; j < n; j += (int64_t)gridDim.x * BLOCK)
+    rel[j] = cur_iperm[q[j]];
+}
+
 // One direction: rows of (begin, adj) relabelled in place (begin, adj hold the
 // caller's rows on entry and the internal rows, sorted by id, on exit).
 int relabel_dir(scc_ctx* c, int32_t* begin, int32_t* adj) {
@@ -300,6 +332,64 @@ int layout_canon_to_caller(scc_ctx* c, int32_t* out_dev) {
   LAUNCH(k_caller_min, grid_for(c->n, 4096), c->stream, c->n, c->canon, c->perm, c->map_b);
   LAUNCH(k_caller_apply, grid_for(c->n, 4096), c->stream, c->n, c->canon, c->perm, c->map_b, out_dev);
   return hip_check(c, hipGetLastError(), "layout canonical labels");
+}
+
+int layout_segment(scc_ctx* c, int32_t k) {
+  if (!c->relabeled) return SCC_OK;
+  if (c->end) return set_error(c, SCC_ERR_STATE, "batched graphs in the degree layout need an unchanged upload");
+  const int32_t n = c->n;
+  int rc;
+  unsigned long long *k0 = nullptr, *k1 = nullptr;
+  int32_t *v0 = nullptr, *q = nullptr;
+  void* tmp = nullptr;
+  auto done = [&](int r) {
+    dev_free(c, k0);
+    dev_free(c, k1);
+    dev_free(c, v0);
+    dev_free(c, q);
+    dev_free(c, tmp);
+    return r;
+  };
+  if ((rc = dev_alloc(c, (void**)&k0, 8ull * n, "segment keys")) ||
+      (rc = dev_alloc(c, (void**)&k1, 8ull * n, "segment keys")) ||
+      (rc = dev_alloc(c, (void**)&v0, 4ull * n, "segment ids")) ||
+      (rc = dev_alloc(c, (void**)&q, 4ull * n, "segment order")))
+    return done(rc);
+  LAUNCH(k_segment_keys, grid_for(n, 4096), c->stream, n, c->perm, c->d_boff, k, c->begin, c->rbegin, k0, v0);
+  int bits = 32;
+  while (bits < 64 && (1ull << (bits - 32)) < (unsigned long long)k) bits++;
+  size_t tb = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, q, n, 0, bits, c->stream);
+  if ((rc = dev_alloc(c, &tmp, tb + 256, "segment temp"))) return done(rc);
+  // stable: equal (graph, degree) keep the current order, i.e. caller id order
+  if ((rc = hip_check(c, hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, q, n, 0, bits, c->stream),
+                      "segment sort")))
+    return done(rc);
+  // rows move from the current internal order to the new one: relabel_dir with
+  // the relative permutation (map_a: new -> current, map_b: current -> new)
+  LAUNCH(k_compose, grid_for(n, 4096), c->stream, n, q, c->iperm, c->map_a);
+  LAUNCH(k_invert, grid_for(n, 4096), c->stream, n, c->map_a, c->map_b);
+  std::swap(c->perm, c->map_a);
+  std::swap(c->iperm, c->map_b);
+  rc = relabel_dir(c, c->begin, c->adj);
+  if (!rc) rc = relabel_dir(c, c->rbegin, c->radj);
+  std::swap(c->perm, c->map_a);
+  std::swap(c->iperm, c->map_b);
+  if (rc) {  // the rows are half moved: the graph is unusable
+    c->relabeled = false;
+    return done(rc);
+  }
+  // the new order itself: perm = q, iperm = its inverse
+  if ((rc = hip_check(c, hipMemcpyAsync(c->perm, q, 4ull * n, hipMemcpyDeviceToDevice, c->stream), "segment perm")))
+    return done(rc);
+  LAUNCH(k_invert, grid_for(n, 4096), c->stream, n, c->perm, c->iperm);
+  if ((rc = hip_check(c, hipMemcpyAsync(c->h_perm.data(), c->perm, 4ull * n, hipMemcpyDeviceToHost, c->stream),
+                      "layout download")) ||
+      (rc = hip_check(c, hipMemcpyAsync(c->h_iperm.data(), c->iperm, 4ull * n, hipMemcpyDeviceToHost, c->stream),
+                      "layout download")) ||
+      (rc = hip_check(c, hipStreamSynchronize(c->stream), "segment sync")))
+    return done(rc);
+  return done(SCC_OK);
 }
 
 int layout_colors_to_internal(scc_ctx* c, const int32_t* colors_host) {

@@ -92,8 +92,10 @@ int scc_graph_info(const scc_ctx* ctx, int32_t* n, int64_t* m);
  * by decreasing in + out degree (ties by id) so that hubs' per-vertex state
  * shares cache lines; it pays on skewed (R-MAT) graphs.  Every entry point
  * still takes and returns the caller's vertex ids, labels and colours, and
- * canonical labels stay the minimum caller id of each component.  Batched
- * graphs (scc_batch_set) need SCC_LAYOUT_INPUT. */
+ * canonical labels stay the minimum caller id of each component.  With a
+ * batch (scc_batch_set) the order becomes per graph (graph, then decreasing
+ * degree), so every graph keeps its id range; that needs an upload without
+ * updates since. */
 typedef enum {
   SCC_LAYOUT_INPUT = 0,  /* device ids are the caller's ids (default) */
   SCC_LAYOUT_DEGREE = 1, /* device ids by decreasing total degree */

@@ -125,12 +125,41 @@ def test_update_stream(deng):
             _same_graph(deng.download_graph(), cur)
 
 
-def test_batch_needs_input_layout(deng):
-    g = graphs.rmat(10, 4, seed=3)
-    deng.upload(g)
-    with pytest.raises(SccError) as ei:
-        deng.batch_set([0, 512, g.num_nodes])
-    assert ei.value.code == SCC_ERR_STATE
+@pytest.mark.parametrize("k,seed", [(2, 3), (24, 4)])
+def test_batch_in_degree_layout(deng, k, seed):
+    """A batch uploaded in the degree layout is re-ordered per graph (each graph
+    keeps its id range): labels and per-graph counts equal the oracle's."""
+    gs = graphs.batched_config(k=k, seed=seed, n_lo=1_000, n_hi=60_000)
+    u, off = graphs.disjoint_union(gs)
+    print(f"  degree-layout batch of {k}: n={u.num_nodes} m={u.num_edges}", flush=True)
+    deng.upload(u)
+    deng.batch_set(off)
+    deng.batch_run()
+    want = oracle.scc_canonical(u, 8)
+    assert np.array_equal(deng.labels(), want)
+    counts = deng.batch_counts()
+    for i in range(k):
+        seg = want[off[i]:off[i + 1]]
+        assert counts[i] == int(np.count_nonzero(seg == np.arange(off[i], off[i + 1])))
+    raw = deng.labels(canonical=False)
+    assert np.array_equal(want[raw], want)
+    _same_graph(deng.download_graph(), u)
+    # a second run and clearing the batch keep exact labels
+    deng.batch_set(None)
+    deng.run_method2()
+    assert np.array_equal(deng.labels(), want)
+
+
+def test_batch_in_degree_layout_rejects_cross_edges(deng):
+    a, b = graphs.rmat(9, 4, seed=1), graphs.rmat(9, 4, seed=2)
+    u, off = graphs.disjoint_union([a, b])
+    src, dst = u.edges()
+    bad = graphs.from_edges(u.num_nodes, np.append(src, 0), np.append(dst, u.num_nodes - 1))
+    deng.upload(bad)
+    with pytest.raises(SccError):
+        deng.batch_set(off)
+    deng.run_method2()  # still a usable single graph
+    assert np.array_equal(deng.labels(), scipy_canonical(bad))
 
 
 def test_rmat22_full_size(deng):
