@@ -193,22 +193,48 @@ __global__ __launch_bounds__(BLOCK) void k_trim1_step(DevCtl* __restrict__ dc, G
   unsigned long long trimmed = 0;
   using Out = LdsOut<0>;
   Out::init();
-  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
+  if (fresh) {
+    // fresh state: the live list is 0..n-1, every colour is -1 and none is
+    // done, so only empty rows decide this sweep (a row holding nothing but
+    // a self-loop survives it and falls to the next sweep, which reads
+    // colours; trimming later is always safe, trimming early never happens).
+    // Four vertices per thread and step (base + t * BLOCK + thread: loads
+    // coalesced, survivors pushed in id order), their row bounds loaded together.
+    for (int64_t base = (int64_t)blockIdx.x * 4 * BLOCK; base < n_in; base += (int64_t)gridDim.x * 4 * BLOCK) {
+      Out::maybe_flush(list_out, &dc->loop.acc1, 4 * BLOCK);
+      int32_t b[4], e[4], rb[4], re[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const int64_t v = base + t * BLOCK + threadIdx.x;
+        b[t] = e[t] = rb[t] = re[t] = 0;
+        if (v < n_in) {
+          b[t] = G.begin[v];
+          e[t] = G.end[v];
+          rb[t] = G.rbegin[v];
+          re[t] = G.rend[v];
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const int64_t v = base + t * BLOCK + threadIdx.x;
+        const bool valid = v < n_in;
+        const bool trim = valid && !(e[t] > b[t] && re[t] > rb[t]);
+        if (trim) {
+          label[v] = (int32_t)v;
+          color[v] = DONE;
+          trimmed++;
+        }
+        Out::push(valid && !trim, (int32_t)v, list_out, &dc->loop.acc1);
+      }
+    }
+  }
+  for (int64_t base = (int64_t)blockIdx.x * BLOCK; !fresh && base < n_in; base += (int64_t)gridDim.x * BLOCK) {
     Out::maybe_flush(list_out, &dc->loop.acc1, BLOCK);
     const int64_t i = base + threadIdx.x;
     const bool valid = i < n_in;
     int32_t v = 0, c = DONE;
     bool out_ok, in_ok;
-    if (fresh) {
-      // fresh state: the live list is 0..n-1, every colour is -1 and none is
-      // done, so only empty rows decide this sweep (a row holding nothing but
-      // a self-loop survives it and falls to the next sweep, which reads
-      // colours; trimming later is always safe, trimming early never happens)
-      v = (int32_t)i;
-      c = UNSET;
-      out_ok = valid && G.end[v] > G.begin[v];
-      in_ok = out_ok && G.rend[v] > G.rbegin[v];
-    } else {
+    {
       v = valid ? list_in[i] : 0;
       c = valid ? color[v] : DONE;
       const bool act = valid && c != DONE;
