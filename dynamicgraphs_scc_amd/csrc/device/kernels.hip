@@ -1123,7 +1123,7 @@ __global__ __launch_bounds__(BLOCK) void k_classify(DevCtl* __restrict__ dc, int
   // one global atomic each, instead of every wave hitting the same addresses
   __shared__ int32_t s_hot, s_hmin[3];
   if (threadIdx.x == 0) {
-    const int64_t i0 = (int64_t)blockIdx.x * BLOCK;
+    const int64_t i0 = (int64_t)blockIdx.x * 4 * BLOCK;
     const int32_t c0 = i0 < n_in ? color[list[i0]] : DONE;
     s_hot = c0 != DONE ? slot_of(c0, n) : -1;
   }
@@ -1133,33 +1133,45 @@ __global__ __launch_bounds__(BLOCK) void k_classify(DevCtl* __restrict__ dc, int
   unsigned long long nscc = 0;
   using Out = LdsOut<0>;
   Out::init();
-  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
-    Out::maybe_flush(list_out, &dc->ctr[C_LIST_OUT], BLOCK);
-    const int64_t i = base + threadIdx.x;
-    bool keep = false, hk = false;
-    int64_t rslot = 0;
-    int32_t v = 0;
-    if (i < n_in) {
-      v = list[i];
-      const int32_t c = color[v];
-      if (c != DONE) {
-        const int32_t s = slot_of(c, n);
-        const bool f = bit_test(vf, v), b = bit_test(vb, v);
-        if (f && b) {
-          label[v] = pivot_of_key(pkey[s]);
-          color[v] = DONE;
+  // four entries per thread and step (base + t * BLOCK + thread: survivors
+  // keep the list order), their dependent loads in flight together
+  for (int64_t base = (int64_t)blockIdx.x * 4 * BLOCK; base < n_in; base += (int64_t)gridDim.x * 4 * BLOCK) {
+    Out::maybe_flush(list_out, &dc->ctr[C_LIST_OUT], 4 * BLOCK);
+    int32_t v[4], c[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const int64_t i = base + t * BLOCK + threadIdx.x;
+      v[t] = i < n_in ? list[i] : -1;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) c[t] = v[t] >= 0 ? color[v[t]] : DONE;
+    bool f[4], b[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      f[t] = c[t] != DONE && bit_test(vf, v[t]);
+      b[t] = c[t] != DONE && bit_test(vb, v[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      bool keep = false, hk = false;
+      int64_t rslot = 0;
+      if (c[t] != DONE) {
+        const int32_t sl = slot_of(c[t], n);
+        if (f[t] && b[t]) {
+          label[v[t]] = pivot_of_key(pkey[sl]);
+          color[v[t]] = DONE;
           nscc++;
         } else {
-          const int t = f ? 0 : (b ? 1 : 2);
-          rslot = 3 * (int64_t)s + t;
+          const int k = f[t] ? 0 : (b[t] ? 1 : 2);
+          rslot = 3 * (int64_t)sl + k;
           keep = true;
-          hk = s == hot;
-          if (hk) atomicMin(&s_hmin[t], v);
+          hk = sl == hot;
+          if (hk) atomicMin(&s_hmin[k], v[t]);
         }
       }
+      grouped_min_i32(keep && !hk, rslot, v[t], rep);
+      Out::push(keep, v[t], list_out, &dc->ctr[C_LIST_OUT]);
     }
-    grouped_min_i32(keep && !hk, rslot, v, rep);
-    Out::push(keep, v, list_out, &dc->ctr[C_LIST_OUT]);
   }
   Out::flush(list_out, &dc->ctr[C_LIST_OUT]);
   if (threadIdx.x < 3 && hot >= 0) {
