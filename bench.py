@@ -396,9 +396,11 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_level2 (one level of the forward and one of the backward search of the global step per launch)",
                 "algorithmic_bytes": int(b_fb),
-                "algorithmic_bytes_per_launch": int(b_fb / max(1, statistics.median(launches))),
-                "avg_launch_us": round(1e3 * t_fb_ms / max(1, statistics.median(launches)), 3),
-                "launches_per_step": int(statistics.median(launches)),
+                # per launch: all timed runs' kernel time over all their launches (a
+                # step runs 6 or 7 level launches, the last sometimes finding nothing)
+                "algorithmic_bytes_per_launch": int(b_fb * len(launches) / max(1, sum(launches))),
+                "avg_launch_us": round(1e3 * sum(kt) / max(1, sum(launches)), 3),
+                "launches_per_step": round(sum(launches) / max(1, len(launches)), 2),
                 "kernel_ms_per_step": round(t_fb_ms, 4),
                 "sets": {"F": F, "B": B, "E_F": EF, "E_B": EB, "FnB": FB}}
     phases = {k: round(st[k], 4) for k in ("ms_trim1", "ms_fb_global", "ms_trim2", "ms_wcc",
