@@ -121,3 +121,37 @@ def test_update_after_phase_by_phase_run():
         want = graphs.from_edges(g.num_nodes, np.concatenate([src[keep], [5, 9]]),
                                  np.concatenate([dst[keep], [9, 5]]), dedup=True)
         assert np.array_equal(e.labels(), scipy_canonical(want))
+
+
+@pytest.mark.parametrize("kind", ["hubs_input", "rmat16_degree"])
+def test_rebuild_with_heavy_row_groups(kind):
+    """Graphs whose first 256 rows hold more than 65,536 entries: the gapped-row
+    rebuild of the first update copies such groups in chunks spread over the
+    grid.  Labels after every batch equal a recompute; the CSR equals the host's."""
+    from dynamicgraphs_scc_amd.engine import LAYOUT_DEGREE, LAYOUT_INPUT
+    rng = np.random.default_rng(77)
+    if kind == "hubs_input":
+        n = 20000
+        hs = np.repeat(np.arange(300, dtype=np.int32), 300)
+        hd = rng.integers(0, n, hs.size).astype(np.int32)
+        rs = rng.integers(0, n, 3 * n).astype(np.int32)
+        rd = rng.integers(0, n, 3 * n).astype(np.int32)
+        base = graphs.from_edges(n, np.concatenate([hs, rs, hd]), np.concatenate([hd, rd, hs]), dedup=True)
+        layout = LAYOUT_INPUT
+    else:
+        base = graphs.rmat(16, 16, seed=3)
+        layout = LAYOUT_DEGREE
+    assert np.diff(base.begin)[:256].sum() > 65536 or layout == LAYOUT_DEGREE
+    with SccEngine() as e:
+        e.set_layout(layout)
+        e.upload(base)
+        e.run_method2()
+        us = graphs.UpdateStream(base, seed=78, lo=200, hi=5000)
+        for b in range(4):
+            isrc, idst, dsrc, ddst = us.next()
+            print(f"  {kind} batch {b}: +{len(isrc)} -{len(dsrc)}", flush=True)
+            e.update(isrc, idst, dsrc, ddst)
+            cur = us.graph()
+            assert np.array_equal(e.labels(), oracle.scc_canonical(cur, 8)), f"batch {b}"
+            _check_graph(e, cur)
+        us.close()
