@@ -262,3 +262,25 @@ def test_two_contexts_independent():
         b.run_method2()
         assert np.array_equal(a.labels(), oracle.scc_canonical(g1))
         assert np.array_equal(b.labels(), oracle.scc_canonical(g2))
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_huge_row_with_duplicate_entries(layout):
+    """A pivot row above the huge-row threshold (8192 entries) that holds
+    repeated ids: the chunked expansion claims each id once, so the searches'
+    visited counts stay exact.  Input and degree layouts."""
+    rng = np.random.default_rng(5)
+    n = 20000
+    hub_out = np.sort(rng.integers(1, n, 12000)).astype(np.int32)   # duplicates stay adjacent
+    hub_in = rng.integers(1, n, 12000).astype(np.int32)
+    rs = rng.integers(1, n, 2 * n).astype(np.int32)
+    rd = rng.integers(0, n, 2 * n).astype(np.int32)
+    src = np.concatenate([np.zeros(12000, np.int32), hub_in, rs])
+    dst = np.concatenate([hub_out, np.zeros(12000, np.int32), rd])
+    g = graphs.from_edges(n, src, dst, dedup=False)
+    assert np.diff(g.begin)[0] == 12000 and len(np.unique(hub_out)) < 12000
+    with SccEngine() as e:
+        e.set_layout(layout)
+        lab, st = run(e, g)
+        assert np.array_equal(lab, scipy_canonical(g))
+        assert st["num_sccs"] == len(np.unique(lab))
