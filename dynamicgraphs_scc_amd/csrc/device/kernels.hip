@@ -408,43 +408,61 @@ __global__ __launch_bounds__(BLOCK) void k_pivot_key(const DevCtl* __restrict__ 
     zbits[w] = 0u;
   const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
   const int64_t n_in = (int64_t)dc->live.n;
-  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
-    const int64_t i = base + threadIdx.x;
-    bool act = false;
-    int32_t slot = 0;
-    unsigned long long key = 0;
-    if (i < n_in) {
-      const int32_t v = list[i];
-      const int32_t c = color[v];
-      if (c != DONE) {
-        unsigned long long score = 0;
-        if (policy == SCC_PIVOT_DEGREE_PRODUCT) {
-          const unsigned long long p = (unsigned long long)(G.end[v] - G.begin[v]) *
-                                       (unsigned long long)(G.rend[v] - G.rbegin[v]);
-          score = p > 0x1FFFFFFFFull ? 0x1FFFFFFFFull : p;
-        } else if (policy == SCC_PIVOT_LEARNED && scores) {
-          score = scores[v];
+  // PK list entries per thread and iteration: one slot check and reduction
+  // (four barriers) per PK * BLOCK entries
+  constexpr int PK = 4;
+  __shared__ int32_t s_slot;
+  __shared__ int s_mixed;
+  __shared__ unsigned long long s_best[WAVES];
+  for (int64_t base = (int64_t)blockIdx.x * PK * BLOCK; base < n_in;
+       base += (int64_t)gridDim.x * PK * BLOCK) {
+    bool act[PK];
+    int32_t slot[PK];
+    unsigned long long key[PK];
+    int32_t first = -1;
+#pragma unroll
+    for (int k = 0; k < PK; k++) {
+      const int64_t i = base + (int64_t)k * BLOCK + threadIdx.x;
+      act[k] = false;
+      slot[k] = 0;
+      key[k] = 0;
+      if (i < n_in) {
+        const int32_t v = list[i];
+        const int32_t c = color[v];
+        if (c != DONE) {
+          unsigned long long score = 0;
+          if (policy == SCC_PIVOT_DEGREE_PRODUCT) {
+            const unsigned long long p = (unsigned long long)(G.end[v] - G.begin[v]) *
+                                         (unsigned long long)(G.rend[v] - G.rbegin[v]);
+            score = p > 0x1FFFFFFFFull ? 0x1FFFFFFFFull : p;
+          } else if (policy == SCC_PIVOT_LEARNED && scores) {
+            score = scores[v];
+          }
+          key[k] = pivot_key(score, v);
+          slot[k] = slot_of(c, G.n);
+          act[k] = true;
+          if (first < 0) first = slot[k];
         }
-        key = pivot_key(score, v);
-        slot = slot_of(c, G.n);
-        act = true;
       }
     }
-    // one slot for the whole workgroup (the global step): reduce in LDS first
-    __shared__ int32_t s_slot;
-    __shared__ int s_mixed;
-    __shared__ unsigned long long s_best[WAVES];
+    // one slot for the whole workgroup (the global step): reduce in LDS first.
+    // The previous iteration ended on a barrier, so no wave still reads s_mixed.
     if (threadIdx.x == 0) s_mixed = 0;
-    const uint64_t am = __ballot(act);
-    const int32_t ws = am ? __shfl(slot, __ffsll((unsigned long long)am) - 1) : -1;
+    const uint64_t am = __ballot(first >= 0);
+    const int32_t ws = am ? __shfl(first, __ffsll((unsigned long long)am) - 1) : -1;
     __syncthreads();
     if (lane_id() == 0 && am) s_slot = ws;
     __syncthreads();
     const int32_t bs = s_slot;
-    if (__ballot(act && slot != bs) != 0 && lane_id() == 0) s_mixed = 1;
+    bool mis = false;
+#pragma unroll
+    for (int k = 0; k < PK; k++) mis |= act[k] && slot[k] != bs;
+    if (__ballot(mis) != 0 && lane_id() == 0) s_mixed = 1;
     __syncthreads();
     if (!s_mixed) {
-      unsigned long long x = act ? key : 0ull;
+      unsigned long long x = 0ull;
+#pragma unroll
+      for (int k = 0; k < PK; k++) x = (act[k] && key[k] > x) ? key[k] : x;
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) {
         const unsigned long long y = __shfl_xor(x, o);
@@ -457,10 +475,12 @@ __global__ __launch_bounds__(BLOCK) void k_pivot_key(const DevCtl* __restrict__ 
         for (int w = 0; w < WAVES; w++) b = s_best[w] > b ? s_best[w] : b;
         if (b && b > pkey[bs]) atomicMax(&pkey[bs], b);
       }
-      __syncthreads();
     } else {
-      grouped_max_u64(act, slot, key, pkey);
+#pragma unroll
+      for (int k = 0; k < PK; k++) grouped_max_u64(act[k], slot[k], key[k], pkey);
     }
+    // every wave has read s_mixed and s_best before the next iteration resets them
+    __syncthreads();
   }
 }
 
