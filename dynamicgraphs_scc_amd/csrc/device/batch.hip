@@ -129,8 +129,7 @@ extern "C" {
 
 int scc_batch_set(scc_ctx* c, int32_t num_graphs, const int32_t* offsets) {
   if (!c) return SCC_ERR_ARG;
-  if (!c->begin) return set_error(c, SCC_ERR_STATE, "no graph uploaded");
-  (void)hipSetDevice(c->device);
+  if (int r0 = require_graph(c)) return r0;
   dev_free(c, c->d_boff);
   c->d_boff = nullptr;
   c->batch_k = 0;
@@ -174,8 +173,7 @@ int scc_batch_set(scc_ctx* c, int32_t num_graphs, const int32_t* offsets) {
 
 int scc_batch_run(scc_ctx* c, const scc_run_opts* opts, scc_run_stats* st) {
   if (!c) return SCC_ERR_ARG;
-  if (!c->begin) return set_error(c, SCC_ERR_STATE, "no graph uploaded");
-  (void)hipSetDevice(c->device);
+  if (int r0 = require_graph(c)) return r0;
   int rc;
   if ((rc = start_run(c))) return rc;
   return scc_run_method2(c, opts, st);
@@ -225,8 +223,7 @@ int scc_ctx_set_tuning(scc_ctx* c, const scc_tuning* t) {
 int scc_autotune(scc_ctx* c, const scc_run_opts* opts, int32_t reps, scc_tuning* best_out, double* best_ms,
                  double* default_ms) {
   if (!c || reps < 1) return SCC_ERR_ARG;
-  if (!c->begin) return set_error(c, SCC_ERR_STATE, "no graph uploaded");
-  (void)hipSetDevice(c->device);
+  if (int r0 = require_graph(c)) return r0;
   int rc;
   auto run_ms = [&](double* ms) -> int {
     std::vector<double> ts;

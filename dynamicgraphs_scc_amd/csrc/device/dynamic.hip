@@ -53,6 +53,7 @@ enum DynCtr : int {
   D_OVERFLOW = 3,  // a room could not be taken
   D_MAXLEN = 4,    // longest changed row after the batch
   D_STAGED = 5,    // entries staged from the changed rows
+  D_DELENT = 6,    // row entries the delete keys of the direction remove (duplicates included)
   D_NUM = 8
 };
 
@@ -63,7 +64,11 @@ struct Dyn {
   unsigned long long *ins_sorted = nullptr;
   uint8_t* flags = nullptr;
   int32_t *src = nullptr, *dst = nullptr;  // staging of one list
-  int32_t *rdel = nullptr, *rins = nullptr;  // per-row counts of the direction being applied (n+1)
+  int32_t *rdel = nullptr, *rins = nullptr;  // per-row entries removed / added in the direction being applied (n+1)
+  // per delete key of the direction being applied: the row entries it removes
+  // (every copy of a duplicated edge), and their exclusive prefix sum over the
+  // sorted key list (cap_b + 1 each)
+  int32_t *dcopies = nullptr, *dcum = nullptr;
   int32_t* rowmark = nullptr;                // per row: 1 + index in the changed-row list, else 0
   int32_t* aff = nullptr;                    // per old-SCC flag
   int32_t *cap_f = nullptr, *cap_r = nullptr;  // room ends of the gapped layout
@@ -79,6 +84,7 @@ struct Dyn {
   size_t tmp_bytes = 0;
   int32_t n = -1;
   int32_t* h_stage = nullptr;  // pinned staging for batch upload (2 * cap_b ints)
+  int64_t rebuilds = 0;        // layout rebuilds of the current update (both directions)
 };
 
 constexpr unsigned long long BAD_KEY = ~0ull;
@@ -173,6 +179,29 @@ __global__ __launch_bounds__(BLOCK) void k_row_count(const unsigned long long* _
     atomicAdd(&cnt[key_hi(keys[i])], val);
 }
 
+// Per sorted, filtered delete key u->v: the entries equal to v in row u (a
+// graph may hold an edge several times; a delete removes every copy, as
+// src/common_main.h:379-386 drops every matching edge), added to the row's
+// removed count.  copies[k] = 0 closes the list for the exclusive scan.
+__global__ __launch_bounds__(BLOCK) void k_del_copies(const unsigned long long* __restrict__ keys, int64_t k,
+                                                      const int32_t* __restrict__ begin,
+                                                      const int32_t* __restrict__ end,
+                                                      const int32_t* __restrict__ adj,
+                                                      int32_t* __restrict__ copies, int32_t* __restrict__ rdel) {
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i <= k; i += (int64_t)gridDim.x * BLOCK) {
+    if (i == k) {
+      copies[k] = 0;
+      continue;
+    }
+    const int32_t u = key_hi(keys[i]), v = key_lo(keys[i]);
+    const int32_t e = end[u];
+    const int32_t p = lb_i32(adj, begin[u], e, v);
+    const int32_t q = lb_i32(adj, p, e, v + 1);  // v < n <= INT32_MAX - 1
+    copies[i] = q - p;
+    atomicAdd(&rdel[u], q - p);
+  }
+}
+
 __global__ void k_dctr_zero(unsigned long long* __restrict__ dctr, int mask) {
   const int i = threadIdx.x;
   if (i < D_NUM && ((mask >> i) & 1)) dctr[i] = 0;
@@ -182,7 +211,11 @@ __global__ void k_store_total(const int32_t* __restrict__ p, unsigned long long*
 }
 
 // ---- gapped CSR ------------------------------------------------------------------
-__device__ __forceinline__ int32_t room_of(int32_t len) { return len + 2 + len / 8; }
+// Room of a row of len entries: len + spare + spare8 eighths of len (the
+// context's scc_update_room; 2 and 1 by default).
+__device__ __forceinline__ int32_t room_of(int32_t len, int2 sp) {
+  return len + sp.x + (int32_t)(((int64_t)len * sp.y) >> 3);
+}
 
 // Changed rows: every row with a delete or an insert enters the list once.
 __global__ __launch_bounds__(BLOCK) void k_collect_rows(const unsigned long long* __restrict__ keys, int64_t k,
@@ -215,7 +248,7 @@ __global__ __launch_bounds__(BLOCK) void k_row_plan(const int32_t* __restrict__ 
                                                     const unsigned long long* __restrict__ ins, int64_t ni,
                                                     int4* __restrict__ rng,
                                                     unsigned long long* __restrict__ dctr, int bump_slot,
-                                                    long long size) {
+                                                    long long size, int2 sp) {
   for (int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x; j < nr; j += (int64_t)gridDim.x * BLOCK) {
     const int32_t u = rows[j];
     // the row's ranges in the sorted delete / insert lists
@@ -236,7 +269,7 @@ __global__ __launch_bounds__(BLOCK) void k_row_plan(const int32_t* __restrict__ 
       newbeg[j] = b;
       newcap[j] = cap[u];
     } else {
-      const int32_t room = room_of(nlen);
+      const int32_t room = room_of(nlen, sp);
       const unsigned long long at = atomicAdd(&dctr[bump_slot], (unsigned long long)room);
       if ((long long)(at + room) > size) {
         atomicOr(&dctr[D_OVERFLOW], 1ull);
@@ -398,13 +431,16 @@ __global__ __launch_bounds__(BLOCK) void k_stage_rows(const int32_t* __restrict_
 }
 
 // Survivors of the staged rows, written sorted to their new rooms:
-// old index - #deleted before it + #inserted before it.
+// old index - #entries deleted before it + #inserted before it.  The deleted
+// entries before key x of row r are dcum[first delete key >= x] - dcum[row's
+// first delete key] (every copy of a deleted key counts).
 __global__ __launch_bounds__(BLOCK) void k_merge_rows(const int32_t* __restrict__ rows, int32_t nr,
                                                       const int32_t* __restrict__ tmpoff,
                                                       const int32_t* __restrict__ stage,
                                                       const int32_t* __restrict__ newbeg,
                                                       const int4* __restrict__ rng,
                                                       const unsigned long long* __restrict__ del,
+                                                      const int32_t* __restrict__ dcum,
                                                       const unsigned long long* __restrict__ ins,
                                                       int32_t* __restrict__ adj) {
   const int32_t total = tmpoff[nr];
@@ -413,17 +449,21 @@ __global__ __launch_bounds__(BLOCK) void k_merge_rows(const int32_t* __restrict_
     const int4 g = rng[r];
     const int32_t k = stage[e];
     const unsigned long long key = ((unsigned long long)(uint32_t)rows[r] << 32) | (uint32_t)k;
-    int32_t dp = g.x;
+    int32_t dents = 0;
     if (g.y > g.x) {
-      dp = (int32_t)lb_u64(del, g.x, g.y, key);
-      if (dp < g.y && del[dp] == key) continue;  // deleted
+      const int32_t dp = (int32_t)lb_u64(del, g.x, g.y, key);
+      if (dp < g.y && del[dp] == key) continue;  // deleted (each copy)
+      dents = dcum[dp] - dcum[g.x];
     }
     const int32_t ip = g.w > g.z ? (int32_t)lb_u64(ins, g.z, g.w, key) : g.z;
-    adj[newbeg[r] + (e - tmpoff[r]) - (dp - g.x) + (ip - g.z)] = k;
+    adj[newbeg[r] + (e - tmpoff[r]) - dents + (ip - g.z)] = k;
   }
 }
 
-// Inserted entries at their merged rank in the new room.
+// Inserted entries at their merged rank in the new room: inserts before it +
+// staged entries below it - deleted entries below it (an insert key is never a
+// delete key: the filters drop deletes that are re-inserted and inserts that
+// are present).
 __global__ __launch_bounds__(BLOCK) void k_place_ins(const unsigned long long* __restrict__ ins, int64_t ni,
                                                      const int32_t* __restrict__ rowmark,
                                                      const int32_t* __restrict__ tmpoff,
@@ -431,6 +471,7 @@ __global__ __launch_bounds__(BLOCK) void k_place_ins(const unsigned long long* _
                                                      const int32_t* __restrict__ newbeg,
                                                      const int4* __restrict__ rng,
                                                      const unsigned long long* __restrict__ del,
+                                                     const int32_t* __restrict__ dcum,
                                                      int32_t* __restrict__ adj) {
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < ni; i += (int64_t)gridDim.x * BLOCK) {
     const unsigned long long key = ins[i];
@@ -438,7 +479,7 @@ __global__ __launch_bounds__(BLOCK) void k_place_ins(const unsigned long long* _
     const int32_t j = rowmark[u] - 1;
     const int4 g = rng[j];
     const int32_t t = (int32_t)i - g.z;
-    const int32_t dl = (int32_t)lb_u64(del, g.x, g.y, key) - g.x;
+    const int32_t dl = g.y > g.x ? dcum[lb_u64(del, g.x, g.y, key)] - dcum[g.x] : 0;
     const int32_t s0 = tmpoff[j], s1 = tmpoff[j + 1];
     const int32_t ol = lb_i32(stage, s0, s1, x) - s0;
     adj[newbeg[j] + t + ol - dl] = x;
@@ -467,9 +508,9 @@ __global__ __launch_bounds__(BLOCK) void k_commit_rows(const int32_t* __restrict
 // Room sizes of every row for a fresh layout (rooms[n] = 0 for the scan).
 __global__ __launch_bounds__(BLOCK) void k_room_sizes(int32_t n, const int32_t* __restrict__ begin,
                                                       const int32_t* __restrict__ end,
-                                                      int32_t* __restrict__ rooms) {
+                                                      int32_t* __restrict__ rooms, int2 sp) {
   for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v <= n; v += (int64_t)gridDim.x * BLOCK)
-    rooms[v] = v < n ? room_of(end[v] - begin[v]) : 0;
+    rooms[v] = v < n ? room_of(end[v] - begin[v], sp) : 0;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_room_ends(int32_t n, const int32_t* __restrict__ begin,
@@ -641,11 +682,15 @@ int ensure_dyn(scc_ctx* c, int64_t batch) {
     dev_free(c, d->flags);
     dev_free(c, d->src);
     dev_free(c, d->dst);
+    dev_free(c, d->dcopies);
+    dev_free(c, d->dcum);
     d->flags = nullptr;
-    d->src = d->dst = nullptr;
+    d->src = d->dst = d->dcopies = d->dcum = nullptr;
     if ((rc = dev_alloc(c, (void**)&d->flags, cap, "batch flags")) ||
         (rc = dev_alloc(c, (void**)&d->src, 4ull * cap, "batch src")) ||
-        (rc = dev_alloc(c, (void**)&d->dst, 4ull * cap, "batch dst")))
+        (rc = dev_alloc(c, (void**)&d->dst, 4ull * cap, "batch dst")) ||
+        (rc = dev_alloc(c, (void**)&d->dcopies, 4ull * (cap + 1), "delete copies")) ||
+        (rc = dev_alloc(c, (void**)&d->dcum, 4ull * (cap + 1), "delete copy offsets")))
       return rc;
     if (d->h_stage) (void)hipHostFree(d->h_stage);
     d->h_stage = nullptr;
@@ -689,6 +734,7 @@ uint64_t update_bytes(int64_t n, int64_t m, int64_t m_max, int64_t max_batch) {
   b += 2 * 2 * 4 * g;                           // adj + radj, old and new during a rebuild
   b += 4 * (uint64_t)mm;                        // staged rows (at most every entry)
   b += cap_b * (7 * 8 + 1 + 4 + 4);             // keys, flags, src, dst
+  b += 2 * 4 * (cap_b + 1);                     // delete copies and their offsets
   b += 5 * 4 * (2 * cap_b + 2);                 // changed-row arrays
   b += 2 * 8 * cap_b + 8 * N + (1u << 20);      // sort / select / scan temp
   return b + 4096;
@@ -754,7 +800,8 @@ int regap_dir(scc_ctx* c, Dyn* d, bool fwd, int64_t min_over) {
     return r;
   };
   // room sizes -> scan (in ncap as scratch) -> total
-  LAUNCH(k_room_sizes, grid_for((int64_t)n + 1, 4096), c->stream, n, beg, cur_end, ncap);
+  const int2 sp = make_int2(c->room.row_spare, c->room.row_spare_eighths);
+  LAUNCH(k_room_sizes, grid_for((int64_t)n + 1, 4096), c->stream, n, beg, cur_end, ncap, sp);
   size_t t = 0;
   (void)hipcub::DeviceScan::ExclusiveSum(nullptr, t, ncap, nbeg, n + 1);
   if ((rc = ensure_tmp(c, d, t + 256))) return fail(rc);
@@ -765,7 +812,8 @@ int regap_dir(scc_ctx* c, Dyn* d, bool fwd, int64_t min_over) {
   if ((rc = hip_check(c, hipMemcpyAsync(&total, nbeg + n, 4, hipMemcpyDeviceToHost, c->stream), "room total")) ||
       (rc = hip_check(c, hipStreamSynchronize(c->stream), "room total sync")))
     return fail(rc);
-  const int64_t over = std::max<int64_t>(std::max<int64_t>((int64_t)total / 8, 1 << 20), min_over);
+  const int64_t over = std::max<int64_t>(
+      std::max<int64_t>((int64_t)total * c->room.overflow_eighths / 8, c->room.overflow_min), min_over);
   const int64_t nsize = (int64_t)total + over;
   if (nsize >= INT32_MAX) return fail(set_error(c, SCC_ERR_LIMIT, "gapped layout would exceed int32 offsets"));
   if ((rc = dev_alloc(c, (void**)&nadj, 4ull * (uint64_t)nsize, "gapped adjacency"))) return fail(rc);
@@ -808,8 +856,16 @@ int apply_dir(scc_ctx* c, Dyn* d, bool fwd, const unsigned long long* del, int64
   hipStream_t s = c->stream;
   int rc;
   // changed rows and per-row counts
-  LAUNCH(k_dctr_zero, 1, s, d->dctr, 1 << D_ROWS);
-  if (nd) LAUNCH(k_row_count, grid_for(nd), s, del, nd, d->rdel, 1);
+  LAUNCH(k_dctr_zero, 1, s, d->dctr, (1 << D_ROWS) | (1 << D_DELENT));
+  if (nd) {
+    LAUNCH(k_del_copies, grid_for(nd + 1), s, del, nd, fwd ? c->begin : c->rbegin, fwd ? row_end(c) : rrow_end(c),
+           fwd ? c->adj : c->radj, d->dcopies, d->rdel);
+    size_t t = d->tmp_bytes;
+    if ((rc = hip_check(c, hipcub::DeviceScan::ExclusiveSum(d->tmp, t, d->dcopies, d->dcum, (int)(nd + 1), s),
+                        "delete copies scan")))
+      return rc;
+    LAUNCH(k_store_total, 1, s, d->dcum + nd, d->dctr + D_DELENT);
+  }
   if (ni) LAUNCH(k_row_count, grid_for(ni), s, ins, ni, d->rins, 1);
   if (nd) LAUNCH(k_collect_rows, grid_for(nd), s, del, nd, d->rowmark, d->rows, d->dctr + D_ROWS);
   if (ni) LAUNCH(k_collect_rows, grid_for(ni), s, ins, ni, d->rowmark, d->rows, d->dctr + D_ROWS);
@@ -821,7 +877,8 @@ int apply_dir(scc_ctx* c, Dyn* d, bool fwd, const unsigned long long* del, int64
     LAUNCH(k_row_plan, grid_for(nr), s, d->rows, nr, fwd ? c->begin : c->rbegin, fwd ? c->end : c->rend,
            fwd ? d->cap_f : d->cap_r, d->rdel, d->rins, d->rowmark, d->lens, d->newbeg, d->newcap, del, nd,
            ins, ni, d->rng, d->dctr,
-           fwd ? (int)D_BUMP_F : (int)D_BUMP_R, (long long)(fwd ? d->size_f : d->size_r));
+           fwd ? (int)D_BUMP_F : (int)D_BUMP_R, (long long)(fwd ? d->size_f : d->size_r),
+           make_int2(c->room.row_spare, c->room.row_spare_eighths));
     size_t t = d->tmp_bytes;
     if ((rc = hip_check(c, hipcub::DeviceScan::ExclusiveSum(d->tmp, t, d->lens, d->tmpoff, (int)(nr + 1), s),
                         "staging scan")))
@@ -833,6 +890,7 @@ int apply_dir(scc_ctx* c, Dyn* d, bool fwd, const unsigned long long* del, int64
     // out of overflow room: rebuild this direction with fresh room, plan again
     const int64_t staged = (int64_t)d->h_dctr[D_STAGED];
     if ((rc = regap_dir(c, d, fwd, 2 * (staged + ni + 3 * nr) + 4096))) return rc;
+    d->rebuilds++;
   }
   c->maxdeg = std::max<int32_t>(c->maxdeg, (int32_t)d->h_dctr[D_MAXLEN]);
   const int64_t staged = (int64_t)d->h_dctr[D_STAGED];
@@ -850,9 +908,12 @@ int apply_dir(scc_ctx* c, Dyn* d, bool fwd, const unsigned long long* del, int64
   const unsigned sg = grid_for(std::max<int64_t>(staged, 1), 2048);
   if (staged) {
     LAUNCH(k_stage_rows, sg, s, d->rows, (int32_t)nr, d->tmpoff, beg, adj, d->stage);
-    LAUNCH(k_merge_rows, sg, s, d->rows, (int32_t)nr, d->tmpoff, d->stage, d->newbeg, d->rng, del, ins, adj);
+    LAUNCH(k_merge_rows, sg, s, d->rows, (int32_t)nr, d->tmpoff, d->stage, d->newbeg, d->rng, del,
+           (const int32_t*)d->dcum, ins, adj);
   }
-  if (ni) LAUNCH(k_place_ins, grid_for(ni), s, ins, ni, d->rowmark, d->tmpoff, d->stage, d->newbeg, d->rng, del, adj);
+  if (ni)
+    LAUNCH(k_place_ins, grid_for(ni), s, ins, ni, d->rowmark, d->tmpoff, d->stage, d->newbeg, d->rng, del,
+           (const int32_t*)d->dcum, adj);
   LAUNCH(k_commit_rows, grid_for(nr), s, d->rows, nr, d->newbeg, d->newcap, d->rdel, d->rins, d->rowmark, beg, en,
          fwd ? d->cap_f : d->cap_r);
   return hip_check(c, hipGetLastError(), "apply rows");
@@ -869,7 +930,7 @@ void destroy_dynamic(scc_ctx* c) {
   Dyn* d = dyn_of(c);
   if (!d) return;
   void* ptrs[] = {d->k_in, d->k_tmp, d->del_f, d->del_r, d->ins_f, d->ins_r, d->ins_sorted,
-                  d->flags, d->src, d->dst, d->rdel, d->rins, d->rowmark, d->aff, d->cap_f, d->cap_r,
+                  d->flags, d->src, d->dst, d->dcopies, d->dcum, d->rdel, d->rins, d->rowmark, d->aff, d->cap_f, d->cap_r,
                   d->rows, d->tmpoff, d->newbeg, d->newcap, d->lens, d->rng, d->stage, d->dctr, d->tmp};
   for (void* p : ptrs) dev_free(c, p);
   if (d->h_stage) (void)hipHostFree(d->h_stage);
@@ -932,8 +993,7 @@ extern "C" int scc_update(scc_ctx* c, const int32_t* ins_src, const int32_t* ins
                           const int32_t* del_src, const int32_t* del_dst, int64_t n_del,
                           scc_update_stats* stats) {
   if (!c) return SCC_ERR_ARG;
-  if (!c->begin) return set_error(c, SCC_ERR_STATE, "no graph uploaded");
-  (void)hipSetDevice(c->device);
+  if (int r0 = require_graph(c)) return r0;
   if (!c->labels_ready) {
     // phase-by-phase callers: labels are a full decomposition once no vertex is live
     int r0 = do_trim_compact(c);
@@ -1019,16 +1079,24 @@ extern "C" int scc_update(scc_ctx* c, const int32_t* ins_src, const int32_t* ins
 
   // ---- 2. apply to the rows (gapped layout, converted on first use) ----------
   auto ta = std::chrono::steady_clock::now();
+  d->rebuilds = 0;
   if (!c->end) {
     if ((rc = regap_dir(c, d, true, 4 * d->cap_b)) || (rc = regap_dir(c, d, false, 4 * d->cap_b))) return rc;
   }
-  if ((rc = apply_dir(c, d, true, d->del_f, nd, d->ins_f, ni)) ||
-      (rc = apply_dir(c, d, false, d->del_r, nd, d->ins_r, ni)))
-    return rc;
-  c->m = c->m + ni - nd;
+  // From here on a failure leaves the two directions, the edge count or the
+  // labels out of step: the context refuses further graph calls until the
+  // graph is uploaded again (cleared at the end of a successful update).
+  c->graph_broken = true;
+  c->labels_ready = false;
+  if ((rc = apply_dir(c, d, true, d->del_f, nd, d->ins_f, ni))) return rc;
+  const int64_t del_entries = nd ? (int64_t)d->h_dctr[D_DELENT] : 0;  // every copy of a deleted edge
+  if ((rc = apply_dir(c, d, false, d->del_r, nd, d->ins_r, ni))) return rc;
+  st.deleted = del_entries;
+  c->m = c->m + ni - del_entries;
   c->m_cap = d->size_f;
   if ((rc = hip_check(c, hipStreamSynchronize(s), "apply sync"))) return rc;
   st.ms_apply = ms_since(ta);
+  st.rebuilds = d->rebuilds;
 
   // ---- 3. affected region ------------------------------------------------------
   auto tr = std::chrono::steady_clock::now();
@@ -1084,7 +1152,7 @@ extern "C" int scc_update(scc_ctx* c, const int32_t* ins_src, const int32_t* ins
     scc_run_opts_default(&o);
     o.pivot_policy = c->update_policy;
     scc_run_stats rs;
-    if ((rc = scc_run_method2(c, &o, &rs))) return rc;
+    if ((rc = run_method2(c, &o, &rs))) return rc;
     st.num_sccs = rs.num_sccs;
   } else {
     int64_t cnt = 0;
@@ -1092,7 +1160,31 @@ extern "C" int scc_update(scc_ctx* c, const int32_t* ins_src, const int32_t* ins
     st.num_sccs = cnt;
   }
   st.ms_recompute = ms_since(tc);
+  c->graph_broken = false;
   st.ms_total = ms_since(t0);
   if (stats) *stats = st;
   return SCC_OK;
+}
+
+extern "C" int scc_ctx_set_update_room(scc_ctx* c, const scc_update_room* r) {
+  if (!c) return SCC_ERR_ARG;
+  if (!r) {
+    scc_update_room_default(&c->room);
+    return SCC_OK;
+  }
+  if (r->row_spare < 0 || r->row_spare > (1 << 20) || r->row_spare_eighths < 0 || r->row_spare_eighths > 64 ||
+      r->overflow_eighths < 0 || r->overflow_eighths > 64 || r->overflow_min < 0 ||
+      r->overflow_min > (int64_t)1 << 30)
+    return set_error(c, SCC_ERR_ARG, "bad update room");
+  c->room = *r;
+  return SCC_OK;
+}
+
+extern "C" void scc_update_room_default(scc_update_room* r) {
+  if (!r) return;
+  std::memset(r, 0, sizeof(*r));
+  r->row_spare = 2;
+  r->row_spare_eighths = 1;
+  r->overflow_eighths = 1;
+  r->overflow_min = 1 << 20;
 }

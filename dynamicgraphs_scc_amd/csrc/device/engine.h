@@ -51,6 +51,7 @@ int do_wcc(scc_ctx* c, int64_t* parts);
 int do_color_round(scc_ctx* c, int64_t* scc);
 int do_fb_parts(scc_ctx* c, int policy, int mode, bool trim_between, int64_t* rounds);
 int do_canon(scc_ctx* c, int64_t* count);
+int run_method2(scc_ctx* c, const scc_run_opts* opts, scc_run_stats* st);  // scc_run_method2 unchecked
 
 // optional subsystems (dynamic.hip, sched.hip)
 void destroy_dynamic(scc_ctx* c);

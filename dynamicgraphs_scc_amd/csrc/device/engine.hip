@@ -189,6 +189,7 @@ int free_graph(scc_ctx* c) {
   c->m_cap = 0;
   c->n_live = 0;
   c->labels_ready = false;
+  c->graph_broken = false;
   return SCC_OK;
 }
 
@@ -749,14 +750,13 @@ int scc_graph_info(const scc_ctx* c, int32_t* n, int64_t* m) {
   if (!c) return SCC_ERR_ARG;
   if (n) *n = c->n;
   if (m) *m = c->m;
-  return SCC_OK;
+  return c->graph_broken ? SCC_ERR_STATE : SCC_OK;
 }
 
 int scc_graph_download(scc_ctx* c, int32_t* begin, int32_t* node_idx, int32_t* r_begin,
                        int32_t* r_node_idx) {
   if (!c) return SCC_ERR_ARG;
-  if (!c->begin) return set_error(c, SCC_ERR_STATE, "no graph");
-  (void)hipSetDevice(c->device);
+  if (int r0 = require_graph(c)) return r0;
   int rc = 0;
   if (c->relabeled) {
     // internal rows to host, then each caller row rebuilt and sorted
@@ -784,14 +784,13 @@ int scc_graph_download(scc_ctx* c, int32_t* begin, int32_t* node_idx, int32_t* r
 
 int scc_state_reset(scc_ctx* c) {
   if (!c) return SCC_ERR_ARG;
-  if (!c->begin) return set_error(c, SCC_ERR_STATE, "no graph uploaded");
-  (void)hipSetDevice(c->device);
+  if (int r0 = require_graph(c)) return r0;
   return reset_state(c);
 }
 
 int scc_state_set_colors(scc_ctx* c, const int32_t* colors) {
   if (!c || !colors) return SCC_ERR_ARG;
-  if (!c->begin) return set_error(c, SCC_ERR_STATE, "no graph uploaded");
+  if (int r0 = require_graph(c)) return r0;
   for (int32_t v = 0; v < c->n; v++)
     if (colors[v] < -1 || colors[v] >= c->n)
       return set_error(c, SCC_ERR_ARG, "colour out of range at " + std::to_string(v));
@@ -826,8 +825,7 @@ int scc_graph_layout(const scc_ctx* c, int* layout) {
 #define REQUIRE_GRAPH(c)                                                    \
   do {                                                                      \
     if (!(c)) return SCC_ERR_ARG;                                           \
-    if (!(c)->begin) return set_error((c), SCC_ERR_STATE, "no graph uploaded"); \
-    (void)hipSetDevice((c)->device);                                              \
+    if (int r0_ = require_graph(c)) return r0_;                             \
   } while (0)
 
 int scc_trim1(scc_ctx* c, int64_t trim_stop, int64_t* trimmed) {
@@ -905,6 +903,16 @@ void scc_run_opts_default(scc_run_opts* o) {
 
 int scc_run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_out) {
   REQUIRE_GRAPH(c);
+  return sccmi::run_method2(c, opts_in, st_out);
+}
+
+}  // extern "C"
+
+namespace sccmi {
+
+// The method-2 pipeline on a context already checked (scc_update runs it on
+// its affected region while the context counts as mid-update).
+int run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_out) {
   scc_run_opts opts;
   if (opts_in) opts = *opts_in;
   else scc_run_opts_default(&opts);
@@ -990,6 +998,10 @@ int scc_run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_o
   if (st_out) *st_out = st;
   return SCC_OK;
 }
+
+}  // namespace sccmi
+
+extern "C" {
 
 int scc_labels_download(scc_ctx* c, int32_t* out, int canonical) {
   REQUIRE_GRAPH(c);

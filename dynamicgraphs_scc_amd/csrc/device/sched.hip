@@ -982,8 +982,7 @@ int scc_sched_rollout(scc_ctx* c, const scc_run_opts* opts_in, uint64_t seed, fl
                       int32_t samples_per_round, float* X, float* Y, float* M, int64_t cap, int64_t* n_out,
                       scc_rollout_stats* rs) {
   if (!c || !n_out || cap < 0 || samples_per_round < 0 || (cap > 0 && (!X || !Y || !M))) return SCC_ERR_ARG;
-  if (!c->begin) return set_error(c, SCC_ERR_STATE, "no graph uploaded");
-  (void)hipSetDevice(c->device);
+  if (int r0 = require_graph(c)) return r0;
   int rc;
   if ((rc = ensure_sched(c))) return rc;
   Sched* s = sched_of(c);

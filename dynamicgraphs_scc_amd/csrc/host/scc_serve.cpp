@@ -178,7 +178,12 @@ struct scc_server {
           g.reserve = 0;
         }
       }
-      if (r->kind == SCC_REQ_UPDATE && rc == SCC_ERR_DEVICE_LOST) g.failed = true;
+      // an update that failed after it began to change the rows leaves the
+      // context refusing graph calls (scc_graph_info reports it): the graph is
+      // failed, so later requests are refused instead of served stale labels
+      if (r->kind == SCC_REQ_UPDATE && rc != SCC_OK &&
+          (rc == SCC_ERR_DEVICE_LOST || (g.ctx && scc_graph_info(g.ctx, nullptr, nullptr) != SCC_OK)))
+        g.failed = true;
       if (r->kind == SCC_REQ_UNLOAD && rc == SCC_OK) {
         reserved -= g.reserve;
         g.reserve = 0;

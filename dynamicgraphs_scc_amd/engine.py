@@ -63,7 +63,7 @@ EXPORTS = [
     "scc_device_memory", "scc_phase_trace", "scc_sched_set_weights", "scc_sched_get_weights",
     "scc_sched_forward", "scc_sched_gradients", "scc_sched_train", "scc_sched_rollout",
     "scc_batch_set", "scc_batch_run", "scc_batch_counts", "scc_ctx_get_tuning", "scc_ctx_set_tuning",
-    "scc_autotune", "scc_level_log",
+    "scc_autotune", "scc_level_log", "scc_update_room_default", "scc_ctx_set_update_room",
 ]
 
 
@@ -119,10 +119,16 @@ class Tuning(C.Structure):
 class UpdateStats(C.Structure):
     _fields_ = [("inserted", C.c_int64), ("deleted", C.c_int64), ("affected", C.c_int64),
                 ("num_sccs", C.c_int64), ("ms_total", C.c_double), ("ms_apply", C.c_double),
-                ("ms_region", C.c_double), ("ms_recompute", C.c_double), ("giant", C.c_int64)]
+                ("ms_region", C.c_double), ("ms_recompute", C.c_double), ("giant", C.c_int64),
+                ("rebuilds", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class UpdateRoom(C.Structure):
+    _fields_ = [("row_spare", C.c_int32), ("row_spare_eighths", C.c_int32),
+                ("overflow_eighths", C.c_int32), ("pad", C.c_int32), ("overflow_min", C.c_int64)]
 
 
 _lib = None
@@ -182,6 +188,8 @@ def load_library():
         "scc_ctx_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
         "scc_autotune": (C.c_int, [vp, C.POINTER(RunOpts), i32, C.POINTER(Tuning),
                                    C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "scc_update_room_default": (None, [C.POINTER(UpdateRoom)]),
+        "scc_ctx_set_update_room": (C.c_int, [vp, C.POINTER(UpdateRoom)]),
         "scc_sched_set_weights": (C.c_int, [vp, vp]),
         "scc_sched_get_weights": (C.c_int, [vp, vp]),
         "scc_sched_forward": (C.c_int, [vp, vp, i64, vp]),
@@ -383,6 +391,19 @@ class SccEngine:
         self._check(self.lib.scc_update(self.h, _ptr(isrc), _ptr(idst), isrc.shape[0],
                                         _ptr(dsrc), _ptr(ddst), dsrc.shape[0], C.byref(st)))
         return st.as_dict()
+
+    def set_update_room(self, **kw) -> None:
+        """Room of the gapped row layout updates keep (row_spare,
+        row_spare_eighths, overflow_eighths, overflow_min); no arguments restore
+        the defaults.  Takes effect at the next layout build."""
+        r = UpdateRoom()
+        self.lib.scc_update_room_default(C.byref(r))
+        for k, v in kw.items():
+            setattr(r, k, int(v))
+        self._check(self.lib.scc_ctx_set_update_room(self.h, C.byref(r)))
+
+    def set_budget(self, hbm_budget: int) -> None:
+        self._check(self.lib.scc_ctx_set_budget(self.h, int(hbm_budget)))
 
     def level_log(self) -> list:
         """Per level of the last reachability search (both searches of a forward-

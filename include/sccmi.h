@@ -86,6 +86,9 @@ int scc_graph_upload(scc_ctx* ctx, int32_t n, int64_t m, const int32_t* begin,
 int scc_graph_upload_device(scc_ctx* ctx, int32_t n, int64_t m, const int32_t* d_begin,
                             const int32_t* d_node_idx, const int32_t* d_r_begin,
                             const int32_t* d_r_node_idx);
+/* Returns SCC_ERR_STATE (with n and m still filled in) when an earlier
+ * scc_update failed after it began to rewrite the rows: every call that uses
+ * the graph is then refused with SCC_ERR_STATE until the next upload. */
 int scc_graph_info(const scc_ctx* ctx, int32_t* n, int64_t* m);
 /* Vertex layout inside the context (no counterpart in the reference: a device
  * memory layout choice).  SCC_LAYOUT_DEGREE relabels the next uploaded graph
@@ -304,7 +307,8 @@ int scc_labels_device(scc_ctx* ctx, const int32_t** d_labels);
 
 typedef struct scc_update_stats {
   int64_t inserted;        /* edges actually added (absent before) */
-  int64_t deleted;         /* edges actually removed (present before) */
+  int64_t deleted;         /* edge entries actually removed (present before; every copy of a
+                              duplicated edge counts) */
   int64_t affected;        /* vertices whose SCC was recomputed */
   int64_t num_sccs;        /* SCC count after the batch */
   double ms_total;         /* wall time of the call */
@@ -312,10 +316,18 @@ typedef struct scc_update_stats {
   double ms_region;        /* affected-region search */
   double ms_recompute;     /* SCC recompute inside the region */
   int64_t giant;           /* size of P, the SCC of the max in x out vertex, found first */
+  int64_t rebuilds;        /* row-layout rebuilds the batch needed (per direction; the first
+                              update's conversion to the gapped layout not counted) */
 } scc_update_stats;
 
 /* Apply one batch of edge inserts and deletes to the device graph and bring
  * the canonical labels up to date, recomputing only SCCs the batch can change.
+ * A delete removes every copy of a duplicated edge (stats->deleted counts
+ * the entries removed).  A failure after the rows began to change (an
+ * allocation beyond the budget, a HIP error, a cancel) leaves the context
+ * refusing graph calls with SCC_ERR_STATE until the graph is uploaded again
+ * (scc_graph_info reports it); failures before that (bad ids, no labels yet)
+ * leave the graph and labels as they were.
  * Replaces the reference's insert path (src/scc_incremental.cc:7-27 with
  * src/common_main.h:180-314, methods 5/6/7/11) and delete path
  * (src/scc_decremental.cc:7-24 with src/common_main.h:343-393, methods 8/9/10)
@@ -324,6 +336,25 @@ typedef struct scc_update_stats {
 int scc_update(scc_ctx* ctx, const int32_t* ins_src, const int32_t* ins_dst, int64_t n_ins,
                const int32_t* del_src, const int32_t* del_dst, int64_t n_del,
                scc_update_stats* stats);
+
+/* Room of the gapped row layout scc_update keeps (no reference counterpart:
+ * the reference rebuilds graphs on the host).  Row u of length L gets
+ * L + row_spare + row_spare_eighths * L / 8 entries; a row outgrowing its room
+ * moves to an overflow area of max(overflow_eighths / 8 of all rooms,
+ * overflow_min) entries; when that runs out the layout is rebuilt with fresh
+ * room.  Defaults 2, 1, 1, 1 << 20.  Less room saves HBM and rebuilds more
+ * often.  Takes effect at the next layout build (the first update after an
+ * upload, or a rebuild); scc_update_bytes_needed assumes the defaults. */
+typedef struct scc_update_room {
+  int32_t row_spare;
+  int32_t row_spare_eighths;
+  int32_t overflow_eighths;
+  int32_t pad;
+  int64_t overflow_min;
+} scc_update_room;
+void scc_update_room_default(scc_update_room* r);
+/* NULL restores the defaults. */
+int scc_ctx_set_update_room(scc_ctx* ctx, const scc_update_room* r);
 
 /* Upper bound of the device bytes scc_update adds to a context that holds a
  * graph of n vertices and m edges (scc_bytes_needed(n, m)), while the graph

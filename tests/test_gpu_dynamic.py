@@ -155,3 +155,133 @@ def test_rebuild_with_heavy_row_groups(kind):
             assert np.array_equal(e.labels(), oracle.scc_canonical(cur, 8)), f"batch {b}"
             _check_graph(e, cur)
         us.close()
+
+
+def _apply_host(n, src, dst, ins, dels):
+    """The reference's update semantics on a multigraph edge list: a delete drops
+    every copy of its edge (src/common_main.h:379-386), an insert adds an edge
+    that is absent (once, whatever the batch repeats), inserted self-loops are
+    ignored, and a delete whose edge the same batch inserts is dropped."""
+    keys = src.astype(np.int64) * n + dst
+    ik = np.unique(ins[0].astype(np.int64) * n + ins[1])
+    ik = ik[(ik // n) != (ik % n)]
+    dk = np.setdiff1d(np.unique(dels[0].astype(np.int64) * n + dels[1]), ik)
+    present = np.isin(ik, keys)
+    ik = ik[~present]
+    keep = ~np.isin(keys, dk)
+    removed = int((~keep).sum())
+    k2 = np.concatenate([keys[keep], ik])
+    return (k2 // n).astype(np.int32), (k2 % n).astype(np.int32), removed
+
+
+@pytest.mark.parametrize("layout", ["input", "degree"])
+def test_delete_duplicated_edges(layout):
+    """A graph loaded with duplicate edges (dedup off, as the reference's loader
+    keeps them): deleting a duplicated edge removes every copy.  The first batch
+    converts the rows to the gapped layout; the second deletes duplicates from a
+    row and inserts enough into it that it moves to the overflow area.  Labels
+    equal the oracle's and the downloaded CSR equals the host's after each."""
+    from dynamicgraphs_scc_amd.engine import LAYOUT_DEGREE, LAYOUT_INPUT
+    rng = np.random.default_rng(2024)
+    n = 3000
+    s0 = rng.integers(0, n, 9000).astype(np.int32)
+    d0 = rng.integers(0, n, 9000).astype(np.int32)
+    rep = rng.integers(1, 4, s0.size)  # every edge one to three times
+    src, dst = np.repeat(s0, rep), np.repeat(d0, rep)
+    # a hub row with many duplicated entries
+    hub = 7
+    hd = rng.integers(0, n, 40).astype(np.int32)
+    src = np.concatenate([src, np.full(80, hub, np.int32)])
+    dst = np.concatenate([dst, np.repeat(hd, 2)])
+    base = graphs.from_edges(n, src, dst, dedup=False)
+    src, dst = base.edges()
+    dup_keys = np.unique(src.astype(np.int64) * n + dst, return_counts=True)
+    dup = dup_keys[0][dup_keys[1] > 1]
+    assert dup.size > 1000
+    with SccEngine() as e:
+        e.set_layout(LAYOUT_DEGREE if layout == "degree" else LAYOUT_INPUT)
+        e.upload(base)
+        e.run_method2()
+        assert np.array_equal(e.labels(), oracle.scc_canonical(base, 8))
+        # batch 1: 600 duplicated edges deleted, 400 random inserts
+        pick = rng.choice(dup, 600, replace=False)
+        dels = ((pick // n).astype(np.int32), (pick % n).astype(np.int32))
+        ins = (rng.integers(0, n, 400).astype(np.int32), rng.integers(0, n, 400).astype(np.int32))
+        src, dst, removed = _apply_host(n, src, dst, ins, dels)
+        st = e.update(ins[0], ins[1], dels[0], dels[1])
+        cur = graphs.from_edges(n, src, dst, dedup=False)
+        assert st["deleted"] == removed and removed > 600
+        assert np.array_equal(e.labels(), oracle.scc_canonical(cur, 8))
+        _check_graph(e, cur)
+        # batch 2: the hub row loses its duplicated entries and gains 120 new ones
+        hub_nb = np.unique(dst[src == hub])
+        dels = (np.full(10, hub, np.int32), hub_nb[:10].astype(np.int32))
+        new = np.setdiff1d(np.arange(n, dtype=np.int32), hub_nb)[::23][:120]
+        ins = (np.full(new.size, hub, np.int32), new.astype(np.int32))
+        src, dst, removed = _apply_host(n, src, dst, ins, dels)
+        st = e.update(ins[0], ins[1], dels[0], dels[1])
+        cur = graphs.from_edges(n, src, dst, dedup=False)
+        assert st["deleted"] == removed and st["inserted"] == new.size
+        assert np.array_equal(e.labels(), oracle.scc_canonical(cur, 8))
+        _check_graph(e, cur)
+        # batch 3: a mixed batch on the rows as they now are
+        src0, dst0 = cur.edges()
+        pick = rng.choice(src0.size, 300, replace=False)
+        dels = (src0[pick], dst0[pick])
+        ins = (rng.integers(0, n, 300).astype(np.int32), rng.integers(0, n, 300).astype(np.int32))
+        src, dst, removed = _apply_host(n, src, dst, ins, dels)
+        st = e.update(ins[0], ins[1], dels[0], dels[1])
+        cur = graphs.from_edges(n, src, dst, dedup=False)
+        assert st["deleted"] == removed
+        assert np.array_equal(e.labels(), oracle.scc_canonical(cur, 8))
+        _check_graph(e, cur)
+
+
+def test_failed_update_refuses_later_calls():
+    """An update that fails part-way (a device failure injected after k HIP
+    calls, for every k until the update goes through) either fails before it
+    touches the graph, leaving graph and labels as they were, or leaves the
+    context refusing every graph call with SCC_ERR_STATE until the next upload.
+    Both outcomes occur, and a refused context serves again after an upload."""
+    base = graphs.rmat(11, 6, seed=31)
+    us = graphs.UpdateStream(base, seed=32, lo=500, hi=800)
+    isrc, idst, dsrc, ddst = us.next()
+    after_graph = us.graph()
+    us.close()
+    want_before = oracle.scc_canonical(base, 8)
+    want_after = oracle.scc_canonical(after_graph, 8)
+    seen = {"intact": 0, "refused": 0}
+    with SccEngine() as e:
+        for k in range(0, 2000):
+            e.upload(base)
+            e.run_method2()
+            e.inject_fault(k)
+            try:
+                e.update(isrc, idst, dsrc, ddst)
+                failed = False
+            except SccError as ex:
+                failed = True
+                assert ex.code == -6
+            e.inject_fault(-1)
+            if not failed:
+                assert np.array_equal(e.labels(), want_after)
+                break
+            try:
+                e.info()
+                intact = True
+            except SccError as ex:
+                assert ex.code == -5
+                intact = False
+            if intact:
+                seen["intact"] += 1
+                assert np.array_equal(e.labels(), want_before), f"k={k}"
+            else:
+                seen["refused"] += 1
+                for call in (e.labels, lambda: e.update([0], [1]), e.run_method2, e.download_graph):
+                    with pytest.raises(SccError) as ei:
+                        call()
+                    assert ei.value.code == -5, f"k={k}"
+        else:
+            raise AssertionError("the update never went through")
+    print(f"  outcomes: {seen}", flush=True)
+    assert seen["intact"] > 0 and seen["refused"] > 0
