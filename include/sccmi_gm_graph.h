@@ -25,8 +25,8 @@ typedef int32_t edge_t;
 
 class gm_graph {
  public:
-  static const node_t NIL_NODE = -1;
-  static const edge_t NIL_EDGE = -1;
+  static constexpr node_t NIL_NODE = -1;
+  static constexpr edge_t NIL_EDGE = -1;
 
   edge_t* begin = nullptr;
   node_t* node_idx = nullptr;
@@ -126,5 +126,12 @@ class gm_graph {
 };
 
 }  // namespace sccmi
+
+// The gm runtime calls the reference's driver makes (gm_graph/inc/gm_runtime.h:46-50):
+// OpenMP thread count and thread id of the host-side work.  Defined in libsccshim.so.
+void gm_rt_initialize();
+int gm_rt_get_num_threads();
+void gm_rt_set_num_threads(int n);
+int gm_rt_thread_id();
 
 #endif  // SCCMI_GM_GRAPH_H

@@ -75,37 +75,101 @@ def test_no_cpu_fallback_in_product_path():
                 assert not re.search(r"^\s*(import|from)\s+oracle\b", txt, re.M), fn
 
 
-def test_shim_defines_every_scc_h_entry_point():
-    """A C++ program that takes the address of every function scc.h declares
-    links against libsccshim.so (a missing definition fails the link) and runs
-    without touching the GPU."""
+# Every function the reference's src/scc.h:19-92 declares, and the queue calls of
+# src/my_work_queue.h:18-25 it includes, as (name, function-pointer type).
+REFERENCE_SCC_H = [
+    ("initialize_color", "void(*)()"), ("finalize_color", "void(*)()"),
+    ("get_new_color", "int(*)()"), ("get_curr_color", "int(*)()"),
+    ("choose_pivot_from_color", "node_t(*)(gm_graph&, int)"),
+    ("generate_compact_set", "NODE_SET*(*)(gm_graph&, int)"),
+    ("initialize_tarjan", "void(*)()"), ("finalize_tarjan", "void(*)()"),
+    ("do_tarjan_all", "void(*)(gm_graph&)"), ("do_tarjan", "void(*)(gm_graph&, int, node_t)"),
+    ("do_tarjan_parallel_color", "void(*)(gm_graph&, node_t*)"),
+    ("initialize_trim1", "void(*)()"), ("do_global_trim1", "int(*)(gm_graph&)"),
+    ("do_global_trim1_compact", "int(*)(gm_graph&)"),
+    ("repeat_global_trim1", "int(*)(gm_graph&, int)"),
+    ("repeat_global_trim1_compact", "int(*)(gm_graph&, int)"),
+    ("get_compact_trim_targets", "std::vector<node_t>&(*)()"),
+    ("repeat_local_trim1", "int(*)(gm_graph&, my_work*)"),
+    ("initialize_trim2", "void(*)()"), ("finalize_trim2", "void(*)()"),
+    ("do_global_trim2", "int(*)(gm_graph&)"), ("repeat_global_trim2", "int(*)(gm_graph&, int)"),
+    ("do_global_trim2_new", "int(*)(gm_graph&)"), ("repeat_global_trim2_new", "int(*)(gm_graph&, int)"),
+    ("initialize_WCC", "void(*)()"), ("finalize_WCC", "void(*)()"),
+    ("do_global_wcc", "void(*)(gm_graph&)"), ("create_work_items_from_wcc", "void(*)(gm_graph&)"),
+    ("get_WCC", "node_t*(*)()"),
+    ("do_fw_bw_single_thread", "int(*)(gm_graph&, my_work*, std::vector<my_work*>&)"),
+    ("start_workers_fw_bw", "void(*)(gm_graph&, int)"), ("start_workers_fw_bw_dfs", "void(*)(gm_graph&, int)"),
+    ("initialize_global_fb", "void(*)()"),
+    ("do_fw_bw_global_main", "int(*)(gm_graph&, int, int, bool)"),
+    ("create_works_after_bfs_trim", "void(*)(gm_graph&)"),
+    ("initialize_analyze", "void(*)()"), ("finalize_analyze", "void(*)()"),
+    ("print_scc_of_size", "void(*)(int)"), ("print_scc_of_nontrivial_size", "void(*)(int)"),
+    ("create_histogram_and_print", "void(*)()"), ("output_scc_list", "void(*)()"),
+    ("insert_idea1", "void(*)(gm_graph&, std::vector<std::pair<int,int>>, std::vector<std::pair<int,int>>)"),
+    ("insert_idea2", "void(*)(gm_graph&, std::vector<std::pair<int,int>>)"),
+    ("delete_idea1", "void(*)(gm_graph&, std::vector<std::vector<std::pair<int,int>>>)"),
+    ("is_work_q_empty_from_seq_context", "bool(*)()"), ("work_q_fetch", "my_work*(*)(int)"),
+    ("work_q_fetch_N", "void(*)(int, int, std::vector<my_work*>&)"),
+    ("work_q_put", "void(*)(int, my_work*)"), ("work_q_put_all", "void(*)(int, std::vector<my_work*>&)"),
+    ("work_q_init", "void(*)(int)"), ("work_q_print_max_depth", "void(*)()"), ("work_q_size", "int(*)()"),
+]
+
+
+def _build_and_run(src, td, name):
     import subprocess
+    cc = os.path.join(td, name + ".cpp")
+    with open(cc, "w") as f:
+        f.write(src)
+    exe = os.path.join(td, name)
+    subprocess.run(["g++", "-std=c++17", "-fopenmp", "-I", os.path.join(ROOT, "include"), cc, "-o", exe,
+                    "-L", LIB, "-lsccshim", "-lsccmi", "-lsccgen", f"-Wl,-rpath,{LIB}"], check=True)
+    return subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
+
+
+def test_shim_defines_every_scc_h_entry_point():
+    """Every function of the reference's scc.h (and of the work queue it
+    includes), bound to a pointer of the reference's exact type: a missing
+    definition fails the link and a different signature fails the build.  The
+    program runs without touching the GPU.  The other names include/scc.h
+    declares link too."""
     import tempfile
+    body = "\n".join(f'  {{ using T = {t}; T p = &{n}; printf("%d\\n", (int)(p != nullptr)); }}'
+                     for n, t in REFERENCE_SCC_H)
     with open(os.path.join(ROOT, "include", "scc.h")) as f:
         text = f.read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     text = re.sub(r"//[^\n]*", "", text)
-    names = sorted(set(re.findall(r"^\s*(?:void|int|long long)\s+([a-z_][a-zA-Z0-9_]*)\s*\(", text, re.M)))
-    assert {"repeat_global_trim1", "do_fw_bw_global_main", "start_workers_fw_bw_dfs",
-            "do_global_wcc", "output_scc_list", "insert_edges"} <= set(names)
-    body = "\n".join(f'  printf("%d\\n", (int)(reinterpret_cast<void*>(&{n}) != nullptr));' for n in names)
+    ours = sorted(set(re.findall(r"^\s*(?:void|int|long long|bool)\s+([a-z_][a-zA-Z0-9_]*)\s*\(", text, re.M)))
+    extra = [n for n in ours if n not in dict(REFERENCE_SCC_H)]
+    assert {"insert_edges", "delete_edges", "run_method2", "scc_shim_sync"} <= set(extra)
+    body += "\n" + "\n".join(f'  printf("%d\\n", (int)(reinterpret_cast<void*>(&{n}) != nullptr));'
+                             for n in extra)
     src = f"""#include <stdio.h>
 #include "scc.h"
 int main() {{
 {body}
-  printf("%d\\n", G_num_nodes);
+  printf("%d %d\\n", G_num_nodes, get_curr_color());
   return 0;
 }}
 """
     with tempfile.TemporaryDirectory() as td:
-        cc = os.path.join(td, "t.cpp")
-        with open(cc, "w") as f:
-            f.write(src)
-        exe = os.path.join(td, "t")
-        subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), cc, "-o", exe,
-                        "-L", LIB, "-lsccshim", "-lsccmi", "-lsccgen", f"-Wl,-rpath,{LIB}"], check=True)
-        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
-    assert out == ["1"] * len(names) + ["0"]
+        out = _build_and_run(src, td, "t")
+    assert out == ["1"] * (len(REFERENCE_SCC_H) + len(extra)) + ["0", "-1"]
+
+
+def test_reference_style_driver_builds():
+    """tests/shim_driver.cpp (the reference driver's call sequence) compiles and
+    links against include/scc.h and libsccshim.so; without arguments it prints
+    its usage and exits 2 (no GPU call)."""
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        exe = os.path.join(td, "drv")
+        subprocess.run(["g++", "-std=c++17", "-O1", "-fopenmp", "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "shim_driver.cpp"), "-o", exe, "-L", LIB, "-lsccshim",
+                        "-lsccmi", "-lsccgen", f"-Wl,-rpath,{LIB}"], check=True)
+        r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr
 
 
 def test_enum_values_match_header():

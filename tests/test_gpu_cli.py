@@ -94,3 +94,70 @@ def test_cli_update_methods(tmp_path):
     keep[sel] = False
     g3 = graphs.from_edges(n_ref, src[keep], dst[keep])
     assert parse(out)["scc_count"] == len(np.unique(oracle.scc_canonical(g3)))
+
+
+@pytest.fixture(scope="module")
+def shim_driver(tmp_path_factory):
+    """tests/shim_driver.cpp (the reference driver's scc.h call sequence) built
+    against include/scc.h and libsccshim.so (host g++, no device code)."""
+    lib = os.path.join(ROOT, "dynamicgraphs_scc_amd", "lib")
+    exe = str(tmp_path_factory.mktemp("drv") / "shim_driver")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-fopenmp", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "shim_driver.cpp"), "-o", exe, "-L", lib, "-lsccshim",
+                    "-lsccmi", "-lsccgen", f"-Wl,-rpath,{lib}"], check=True)
+    return exe
+
+
+def _run_driver(exe, args, cwd):
+    print("  shim_driver", args, flush=True)
+    r = subprocess.run([exe] + [str(a) for a in args], cwd=cwd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, f"{args}: {r.stdout[-2000:]} {r.stderr[-2000:]}"
+    return r.stdout
+
+
+@pytest.mark.parametrize("method", [0, 1, 2, 3, 4])
+def test_reference_style_driver_methods(tmp_path, shim_driver, method):
+    """Methods 0-4 as the reference's driver sequences its scc.h calls: the SCC
+    count (G_SCC[i] == i) and scc_list.txt equal the oracle's partition."""
+    g = graphs.planted(20000, seed=11, n_medium=300) if method == 2 else graphs.rmat(12, 5, seed=method + 20)
+    path = tmp_path / "edges.txt"
+    g.write_edgelist(str(path))
+    src, dst = g.edges()
+    n_ref = int(max(src.max(), dst.max())) + 1
+    want = oracle.scc_canonical(graphs.from_edges(n_ref, src, dst))
+    out = _run_driver(shim_driver, [path, method], tmp_path)
+    assert parse(out)["scc_count"] == len(np.unique(want))
+    if method == 2:
+        assert re.search(r"#queue_size = \d+", out)
+    dense = np.loadtxt(tmp_path / "scc_list.txt", dtype=np.int64, ndmin=2)
+    assert np.array_equal(oracle.dense_to_canonical(dense[:, 1].astype(np.int32)), want)
+
+
+@pytest.mark.parametrize("mode", ["ins", "del"])
+def test_reference_style_driver_update_graphs(tmp_path, shim_driver, mode):
+    """insert_idea1 / delete_idea1 build the updated graph as the reference's
+    update methods do (every SCC group's kept edges, not group 5 alone); the
+    decomposition of it equals the oracle's."""
+    base = graphs.rmat(11, 6, seed=17)
+    path = tmp_path / "edges.txt"
+    base.write_edgelist(str(path))
+    src, dst = base.edges()
+    n_ref = int(max(src.max(), dst.max())) + 1
+    rng = np.random.default_rng(5)
+    if mode == "ins":
+        e = rng.integers(0, n_ref, (400, 2))
+        s2, d2 = np.concatenate([src, e[:, 0]]), np.concatenate([dst, e[:, 1]])
+    else:
+        sel = rng.choice(len(src), 500, replace=False)
+        e = np.stack([src[sel], dst[sel]], 1)
+        keys = src.astype(np.int64) * n_ref + dst
+        keep = ~np.isin(keys, e[:, 0].astype(np.int64) * n_ref + e[:, 1])
+        s2, d2 = src[keep], dst[keep]
+    with open(tmp_path / "batch.txt", "w") as f:
+        for a, b in e:
+            f.write(f"{a + 1} {b + 1}\n")
+    out = _run_driver(shim_driver, [path, mode, tmp_path / "batch.txt"], tmp_path)
+    want = oracle.scc_canonical(graphs.from_edges(n_ref, s2, d2))
+    assert parse(out)["scc_count"] == len(np.unique(want))
+    dense = np.loadtxt(tmp_path / "scc_list.txt", dtype=np.int64, ndmin=2)
+    assert np.array_equal(oracle.dense_to_canonical(dense[:, 1].astype(np.int32)), want)
