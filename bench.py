@@ -2,8 +2,9 @@
 
 One step = one full pass of the hot path over the graph already resident in
 HBM: state reset, trim1 -> global forward-backward -> trim1/trim2/trim1 -> WCC
--> partition forward-backward rounds -> canonical labels, on the device
-(the reference's run() window, src/common_main.h:801-807).
+-> partition forward-backward rounds -> canonical labels in the caller's vertex
+ids, on the device (the reference's run() window, src/common_main.h:801-807,
+which ends with G_SCC indexed by the caller's vertices, src/scc.h:12).
 
 Prints one JSON line.  value = SCC GTEPS = M / (seconds per step) / 1e9.
 The graph is uploaded once, outside the timed region, into a context with the
@@ -12,8 +13,10 @@ is returned in the caller's ids); --layout input keeps the caller's ids.
 roofline: the global forward-backward traversal (the reference's 76 % phase),
 achieved = B_FB / (summed device time of its traversal kernels, HIP events on
 the engine's stream), with B_FB = 20|F| + 8E_F + 20|B| + 8E_B + 4|F n B|.
-cpu_baseline: the reference's own OpenMP binary (oracle/_ref/scc_ref) when it
-was built, else the oracle's OpenMP restatement, on this host's cores.
+cpu_baseline: the reference's own method 2 (its phase sources, timed by
+oracle/_ref/ref_bench) when it was built, else the oracle's OpenMP restatement,
+on this host's physical cores: median of 5 runs after a warm-up, a 1-thread
+figure, the CPU model, and one run on the planted graph of 8,388,608 vertices.
 
 dynamic: the serving workload (R-MAT scale 23 base resident in HBM, 2,000
 insert/delete batches with sizes log-uniform in [1k, 100k], half uniform
@@ -45,65 +48,115 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def host_cores() -> int:
+def host_info() -> dict:
+    """CPU model, the CPUs this process may run on, and the physical cores among
+    them (distinct (package, core) pairs in sysfs)."""
+    model = "unknown"
     try:
-        n = len(os.sched_getaffinity(0))
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cpus = sorted(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        cpus = list(range(os.cpu_count() or 1))
+    cores = set()
+    for c in cpus:
+        try:
+            base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+            with open(base + "physical_package_id") as f:
+                pkg = f.read().strip()
+            with open(base + "core_id") as f:
+                cores.add((pkg, f.read().strip()))
+        except OSError:
+            cores.add(("?", str(c)))
+    return {"model": model, "logical_cpus": len(cpus), "physical_cores": max(1, len(cores))}
 
 
-def cpu_baseline(g, cores: int, want_ref: bool = True):
-    """Time the CPU path on a bounded sample: the full graph, a few runs."""
+def cpu_baseline(g, gpu_labels, gp=None, gp_gpu_ms=None, want_ref: bool = True):
+    """The reference's own method 2 on this host's cores, same graph: one load,
+    one warm-up and five timed runs (median) on every physical core available
+    to the process (at most 16, the host share of one GPU), two more at one
+    thread; the labels of the last run are compared with the GPU's.  Then one
+    run on the planted graph (n = 8,388,608).  Falls back to the oracle's
+    restatement when the reference was not built."""
     import numpy as np
     import oracle
+    hi = host_info()
+    threads = min(hi["physical_cores"], 16)
     res = {}
-    ref_ok = want_ref and oracle.have_ref()
-    if ref_ok:
+    if want_ref and oracle.have_ref_bench():
         td = tempfile.mkdtemp(prefix="sccref_")
         try:
-            path = os.path.join(td, "edges.txt")
+            path = os.path.join(td, "g.gm")
             t = time.time()
-            g.write_edgelist(path)
-            log(f"cpu_baseline: wrote edge list in {time.time() - t:.1f}s")
-            times = []
-            for rep in range(2):
-                r = oracle.run_ref(path, threads=cores, method=2, flag="", cwd=td, timeout=300)
-                times.append(r["runtime_ms"])
-            res = {"value": g.num_edges / (min(times) / 1e3) / 1e9, "unit": "GTEPS",
-                   "cores": cores, "kind": "reference",
-                   "sample": (f"reference OpenMP binary (method 2, compiled from its sources) on "
-                              f"the same R-MAT-22 graph via its text loader; run() window "
-                              f"[T]running_time(ms); best of {len(times)} runs: "
-                              f"{', '.join(f'{x:.1f}' for x in times)} ms"),
-                   "ms": min(times)}
+            g.write_gm_binary(path)
+            r = oracle.run_ref_bench(path, threads, 6, labels_out=os.path.join(td, "lab.bin"))
+            runs = r["ms"][1:]
+            med = statistics.median(runs)
+            same = bool(np.array_equal(oracle.canonicalize(r["labels"]), gpu_labels))
+            log(f"cpu_baseline: reference {threads} threads {', '.join(f'{x:.1f}' for x in r['ms'])} ms "
+                f"({time.time() - t:.1f}s), labels equal to the GPU's: {same}")
+            t = time.time()
+            r1 = oracle.run_ref_bench(path, 1, 3)
+            ms1 = statistics.median(r1["ms"][1:])
+            log(f"cpu_baseline: reference 1 thread {', '.join(f'{x:.1f}' for x in r1['ms'])} ms "
+                f"({time.time() - t:.1f}s)")
+            res = {"value": g.num_edges / (med / 1e3) / 1e9, "unit": "GTEPS", "cores": threads,
+                   "kind": "reference",
+                   "sample": (f"the reference's method 2 (its phase sources, oracle/_ref/ref_bench) on the "
+                              f"same graph (n={g.num_nodes}, m={g.num_edges}), loaded once; run() window; median of {len(runs)} runs "
+                              f"after one warm-up at {threads} threads: "
+                              f"{', '.join(f'{x:.1f}' for x in runs)} ms; 1 thread: median of "
+                              f"{len(r1['ms']) - 1} after a warm-up"),
+                   "median_ms": med, "runs_ms": [round(x, 3) for x in runs],
+                   "ms_1thread": ms1, "gteps_1thread": g.num_edges / (ms1 / 1e3) / 1e9,
+                   "labels_equal_gpu": same}
+            if gp is not None:
+                t = time.time()
+                pp = os.path.join(td, "p.gm")
+                gp.write_gm_binary(pp)
+                rp = oracle.run_ref_bench(pp, threads, 1, timeout=900)
+                res["planted8m"] = {"n": gp.num_nodes, "m": gp.num_edges, "threads": threads,
+                                    "ms": rp["ms"][0], "gteps": gp.num_edges / (rp["ms"][0] / 1e3) / 1e9,
+                                    "sccs": rp["sccs"][0], "runs": 1,
+                                    "gpu_ms_plain": gp_gpu_ms,
+                                    "gpu_speedup": (rp["ms"][0] / gp_gpu_ms) if gp_gpu_ms else None}
+                log(f"cpu_baseline: planted 8M reference {rp['ms'][0]:.1f} ms ({time.time() - t:.1f}s)")
         except Exception as ex:  # fall back to the restatement
-            log(f"cpu_baseline: reference binary failed ({ex}); using the restatement")
+            log(f"cpu_baseline: reference harness failed ({ex}); using the restatement")
             res = {}
         finally:
             shutil.rmtree(td, ignore_errors=True)
     port_times = []
-    for rep in range(4):
-        _, ms, _ = oracle.method2(g, threads=cores)
+    for rep in range(6):
+        _, ms, _ = oracle.method2(g, threads=threads)
         port_times.append(ms)
-    port = {"value": g.num_edges / (statistics.median(port_times[1:]) / 1e3) / 1e9,
-            "ms": statistics.median(port_times[1:]), "cores": cores}
+    port_ms = statistics.median(port_times[1:])
     if not res:
-        res = {"value": port["value"], "unit": "GTEPS", "cores": cores, "kind": "port",
-               "sample": (f"oracle OpenMP restatement of method 2 on the same graph, median of "
-                          f"3 runs after a warm-up: {port['ms']:.1f} ms"),
-               "ms": port["ms"]}
-    res["port_gteps"] = port["value"]
-    res["port_ms"] = port["ms"]
+        res = {"value": g.num_edges / (port_ms / 1e3) / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port",
+               "sample": (f"oracle OpenMP restatement of method 2 on the same graph, median of 5 runs after "
+                          f"a warm-up at {threads} threads: {port_ms:.1f} ms"),
+               "median_ms": port_ms}
+    res.update({"model": hi["model"], "physical_cores": hi["physical_cores"],
+                "logical_cpus": hi["logical_cpus"],
+                "port_gteps": g.num_edges / (port_ms / 1e3) / 1e9, "port_ms": port_ms})
     return res
 
 
-def learned_leg(args, eng, g, ms_plain_rmat):
-    """Plain vs learned scheduler: R-MAT-22 (the engine's graph, same step as
-    `value`) and the planted-SCC graph of 8,388,608 vertices (seed 3)."""
+def learned_leg(args, eng, g, ms_plain_rmat, gp):
+    """Plain vs learned scheduler on R-MAT-22 (the engine's graph, same step as
+    `value`) and on the planted-SCC graph gp (8,388,608 vertices, seed 3), with
+    the network's two heads measured apart: the phase head alone
+    (SCC_PARTS_LEARNED, plain pivots), the pivot head alone (SCC_PIVOT_LEARNED,
+    plain step order) and both.  Labels must equal the plain run's."""
     import numpy as np
-    from dynamicgraphs_scc_amd import graphs
-    from dynamicgraphs_scc_amd.engine import PARTS_BOTH, PARTS_LEARNED, SccEngine, default_opts
+    from dynamicgraphs_scc_amd.engine import (PARTS_BOTH, PARTS_LEARNED, PIVOT_DEGREE_PRODUCT, PIVOT_LEARNED,
+                                              SccEngine, default_opts)
     from dynamicgraphs_scc_amd.sched import load_weights
 
     w = load_weights()
@@ -121,34 +174,38 @@ def learned_leg(args, eng, g, ms_plain_rmat):
             ts.append((time.perf_counter() - t0) * 1e3)
         return float(np.median(ts))
 
-    out = {}
-    lab_plain = eng.labels()
-    eng.set_sched_weights(w)
-    lo = default_opts(partition_mode=PARTS_LEARNED)
-    timed(eng, lo, 2)
-    ms_l = timed(eng, lo, max(5, args.steps))
-    same = bool(np.array_equal(lab_plain, eng.labels()))
-    eng.set_sched_weights(None)
-    out["rmat22"] = {"ms_plain": round(ms_plain_rmat, 4), "ms_learned": round(ms_l, 4),
-                     "speedup": round(ms_plain_rmat / ms_l, 3), "labels_equal": same,
-                     "learned_gteps": round(g.num_edges / (ms_l / 1e3) / 1e9, 4)}
-    gp = graphs.planted(8_388_608, seed=3)
+    variants = {"phase": default_opts(partition_mode=PARTS_LEARNED, pivot_policy=PIVOT_DEGREE_PRODUCT),
+                "pivot": default_opts(partition_mode=PARTS_BOTH, pivot_policy=PIVOT_LEARNED),
+                "both": default_opts(partition_mode=PARTS_LEARNED, pivot_policy=PIVOT_LEARNED)}
+
+    def heads(e, ms_plain, m, reps):
+        lab_plain = e.labels()
+        e.set_sched_weights(w)
+        out = {"ms_plain": round(ms_plain, 4)}
+        for k, o in variants.items():
+            timed(e, o, 1)
+            ms = timed(e, o, reps)
+            out[f"ms_{k}"] = round(ms, 4)
+            out[f"speedup_{k}"] = round(ms_plain / ms, 3)
+            out[f"labels_equal_{k}"] = bool(np.array_equal(lab_plain, e.labels()))
+        e.set_sched_weights(None)
+        best = min(variants, key=lambda k: out[f"ms_{k}"])
+        out["ms_learned"], out["speedup"] = out[f"ms_{best}"], out[f"speedup_{best}"]
+        out["learned_heads"] = best
+        out["labels_equal"] = all(out[f"labels_equal_{k}"] for k in variants)
+        out["gteps_plain"] = round(m / (ms_plain / 1e3) / 1e9, 4)
+        out["learned_gteps"] = round(m / (out["ms_learned"] / 1e3) / 1e9, 4)
+        return out
+
+    res = {"rmat22": heads(eng, ms_plain_rmat, g.num_edges, max(5, args.steps))}
     with SccEngine(device=0) as e:
         e.upload(gp)
         po = default_opts(partition_mode=PARTS_BOTH)
         timed(e, po, 1)
         ms_p = timed(e, po, 5)
-        lp = e.labels()
-        e.set_sched_weights(w)
-        timed(e, lo, 1)
-        ms_l = timed(e, lo, 5)
-        same = bool(np.array_equal(lp, e.labels()))
-    out["planted8m"] = {"n": gp.num_nodes, "m": gp.num_edges, "ms_plain": round(ms_p, 3),
-                        "ms_learned": round(ms_l, 3), "speedup": round(ms_p / ms_l, 3),
-                        "labels_equal": same,
-                        "gteps_plain": round(gp.num_edges / (ms_p / 1e3) / 1e9, 4),
-                        "gteps_learned": round(gp.num_edges / (ms_l / 1e3) / 1e9, 4)}
-    return out
+        res["planted8m"] = heads(e, ms_p, gp.num_edges, 5)
+        res["planted8m"].update({"n": gp.num_nodes, "m": gp.num_edges})
+    return res
 
 
 def batched_leg(args):
@@ -407,18 +464,24 @@ def main():
                                            "ms_fb_parts", "ms_canon")}
     log(f"step {ms_step:.3f} ms  GTEPS {gteps:.2f}  phases {phases}  fb_kernels {t_fb_ms:.3f} ms")
 
+    gp = None
+    if not (args.no_learned and args.no_cpu):
+        t = time.time()
+        gp = graphs.planted(8_388_608, seed=3)
+        log(f"generated planted 8M: n={gp.num_nodes} m={gp.num_edges} in {time.time() - t:.1f}s")
     learned = None
     if not args.no_learned:
         t = time.time()
-        learned = learned_leg(args, eng, g, ms_step)
+        learned = learned_leg(args, eng, g, ms_step, gp)
         log(f"learned leg in {time.time() - t:.1f}s: {learned}")
     eng.close()
     cpu = None
     if not args.no_cpu:
-        cores = host_cores()
         t = time.time()
-        cpu = cpu_baseline(g, cores, want_ref=not args.no_ref)
+        gp_ms = learned["planted8m"]["ms_plain"] if learned else None
+        cpu = cpu_baseline(g, labels, gp, gp_ms, want_ref=not args.no_ref)
         log(f"cpu baseline {cpu['kind']}: {cpu['value']:.3f} GTEPS in {time.time() - t:.1f}s")
+    del gp
 
     batched = None
     if not args.no_batched:

@@ -436,6 +436,8 @@ struct scc_ctx {
   int32_t layout_req = 0;
   bool relabeled = false;
   int32_t *perm = nullptr, *iperm = nullptr, *map_a = nullptr, *map_b = nullptr;
+  // canonical labels in caller ids (caller-indexed), written with canon[]
+  int32_t* canon_caller = nullptr;
   std::vector<int32_t> h_perm, h_iperm;
 
   // per-vertex state

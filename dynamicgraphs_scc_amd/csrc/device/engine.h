@@ -29,7 +29,6 @@ void drop_order(scc_ctx* c);
 int build_layout(scc_ctx* c);  // relabel the uploaded rows when SCC_LAYOUT_DEGREE is requested
 void drop_layout(scc_ctx* c);
 int layout_to_caller(scc_ctx* c, const int32_t* in, bool map_values, int32_t* out_dev);
-int layout_canon_to_caller(scc_ctx* c, int32_t* out_dev);  // from canon[]: min caller id per component
 int layout_colors_to_internal(scc_ctx* c, const int32_t* colors_host);
 // re-relabel by (batch graph, decreasing degree) once c->d_boff holds k graphs' caller ranges
 int layout_segment(scc_ctx* c, int32_t k);

@@ -565,7 +565,11 @@ int do_fb_parts(scc_ctx* c, int policy, int mode, bool trim_between, int64_t* ro
 int do_canon(scc_ctx* c, int64_t* count) {
   int rc;
   PhaseRec pr(c, SCC_PHASE_CANON);
-  launch_canon(c->stream, c->dc, c->n, c->label, c->scratch_i32, c->canon);
+  if (c->relabeled)  // the caller's canonical labels in the same passes
+    launch_canon_caller(c->stream, c->dc, c->n, c->label, reinterpret_cast<int2*>(c->pkey), c->perm, c->canon,
+                        c->canon_caller);
+  else
+    launch_canon(c->stream, c->dc, c->n, c->label, c->scratch_i32, c->canon);
   if ((rc = pull_ctl(c))) return rc;
   pr.finish((int64_t)c->h_dc->ctr[C_COUNT]);
   if (count) *count = (int64_t)c->h_dc->ctr[C_COUNT];
@@ -629,9 +633,10 @@ const char* scc_last_error(const scc_ctx* c) { return c ? c->err.c_str() : "null
 uint64_t scc_ctx_bytes(const scc_ctx* c) { return c ? c->bytes : 0; }
 uint64_t scc_bytes_needed(int32_t n, int64_t m) {
   // graph, state, traversal copies or relabelling temporaries, scratch, the
-  // degree layout's maps (perm, iperm and two staging arrays), control block
+  // degree layout's maps (perm, iperm, two staging arrays, caller-id labels),
+  // control block
   return 8ull * (uint64_t)(m > 0 ? m : 1) + 8ull * ((uint64_t)n + 1) + state_bytes(n) + order_bytes(m) +
-         4ull * ((uint64_t)n + 1) + 16ull * ((uint64_t)n + 1) + sizeof(DevCtl);
+         4ull * ((uint64_t)n + 1) + 20ull * ((uint64_t)n + 1) + sizeof(DevCtl);
 }
 int scc_device_memory(int device, uint64_t* free_bytes, uint64_t* total_bytes) {
   if (!free_bytes || !total_bytes) return SCC_ERR_ARG;
@@ -1012,9 +1017,12 @@ int scc_labels_download(scc_ctx* c, int32_t* out, int canonical) {
   }
   const int32_t* src = canonical ? c->canon : c->label;
   if (c->relabeled) {
-    if ((rc = canonical ? layout_canon_to_caller(c, c->map_a) : layout_to_caller(c, c->label, true, c->map_a)))
-      return rc;
-    src = c->map_a;
+    if (canonical) {
+      src = c->canon_caller;  // computed with canon
+    } else {
+      if ((rc = layout_to_caller(c, c->label, true, c->map_a))) return rc;
+      src = c->map_a;
+    }
   }
   if ((rc = hip_check(c, hipMemcpyAsync(out, src, 4ull * c->n, hipMemcpyDeviceToHost, c->stream),
                       "labels download")))
@@ -1047,13 +1055,7 @@ int scc_labels_device(scc_ctx* c, const int32_t** d_labels) {
   if (!d_labels) return SCC_ERR_ARG;
   int rc;
   if (!c->labels_ready && (rc = do_canon(c, nullptr))) return rc;
-  *d_labels = c->canon;
-  if (c->relabeled) {
-    // caller-indexed copy, valid until the next call that maps labels or colours
-    if ((rc = layout_canon_to_caller(c, c->map_a))) return rc;
-    *d_labels = c->map_a;
-    return hip_check(c, hipStreamSynchronize(c->stream), "labels sync");
-  }
+  *d_labels = c->relabeled ? c->canon_caller : c->canon;  // caller-indexed, computed with canon
   return SCC_OK;
 }
 

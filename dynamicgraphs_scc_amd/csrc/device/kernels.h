@@ -95,5 +95,9 @@ void launch_extract(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, i
 // and leaves the SCC count in dc->ctr[C_COUNT]
 void launch_canon(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label, int32_t* mn,
                   int32_t* out);
+// The same with the caller's canonical labels beside (degree vertex layout):
+// cout[perm[v]] = min caller id of v's component; mnc holds n int2 slots.
+void launch_canon_caller(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label, int2* mnc,
+                         const int32_t* perm, int32_t* out, int32_t* cout);
 
 }  // namespace sccmi

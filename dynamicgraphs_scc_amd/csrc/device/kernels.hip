@@ -1631,33 +1631,53 @@ __device__ int32_t block_hot_label(int32_t n, const int32_t* __restrict__ label)
   return s_hot;
 }
 
-// mn = identity (a label is always a vertex labelled with itself), counters zeroed.
-__global__ __launch_bounds__(BLOCK) void k_canon_init(DevCtl* __restrict__ dc, int32_t* __restrict__ mn, int32_t n) {
-  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < n; v += (int64_t)gridDim.x * BLOCK)
-    mn[v] = (int32_t)v;
+// mn = identity (a label is always a vertex labelled with itself), counters
+// zeroed.  With CALLER (the degree vertex layout, perm = internal -> caller
+// ids) each slot is a pair: the minimum internal id and the minimum caller id
+// of the vertices with that label, so the same passes give the caller's
+// canonical labels too.
+template <bool CALLER>
+__global__ __launch_bounds__(BLOCK) void k_canon_init(DevCtl* __restrict__ dc, int32_t* __restrict__ mn,
+                                                      int2* __restrict__ mnc, const int32_t* __restrict__ perm,
+                                                      int32_t n) {
+  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < n; v += (int64_t)gridDim.x * BLOCK) {
+    if (CALLER) mnc[v] = make_int2((int32_t)v, perm[v]);
+    else mn[v] = (int32_t)v;
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl_init_body(dc, 1, 0, 0, 0, 0.0);
 }
 
+template <bool CALLER>
 __global__ __launch_bounds__(BLOCK) void k_canon_min(int32_t n, const int32_t* __restrict__ label,
-                                                     int32_t* __restrict__ mn) {
+                                                     int32_t* __restrict__ mn, int2* __restrict__ mnc,
+                                                     const int32_t* __restrict__ perm) {
   const int lane = lane_id();
   const int32_t hot = block_hot_label(n, label);
-  __shared__ int32_t s_hmin;
-  if (threadIdx.x == 0) s_hmin = INT32_MAX;
+  __shared__ int32_t s_hmin, s_hcmin;
+  if (threadIdx.x == 0) s_hmin = s_hcmin = INT32_MAX;
   __syncthreads();
   for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n; base += (int64_t)gridDim.x * BLOCK) {
     const int64_t v = base + threadIdx.x;
-    int32_t l = -1;
-    if (v < n) l = label[v];
-    // mn starts as the identity (a label is always a vertex labelled with
-    // itself), so self-labelled vertices have nothing to contribute
+    int32_t l = -1, p = INT32_MAX;
+    if (v < n) {
+      l = label[v];
+      if (CALLER) p = perm[v];
+    }
+    // slots start as the label vertex's own ids (a label is always a vertex
+    // labelled with itself), so self-labelled vertices have nothing to contribute
     bool pending = (uint32_t)l < (uint32_t)n && l != (int32_t)v;
     if (hot >= 0) {
       const bool h = pending && l == hot;
-      int32_t x = h ? (int32_t)v : INT32_MAX;
+      int32_t x = h ? (int32_t)v : INT32_MAX, y = h ? p : INT32_MAX;
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o));
-      if (lane == 0 && x < INT32_MAX) atomicMin(&s_hmin, x);
+      for (int o = 32; o > 0; o >>= 1) {
+        x = min(x, __shfl_xor(x, o));
+        if (CALLER) y = min(y, __shfl_xor(y, o));
+      }
+      if (lane == 0 && x < INT32_MAX) {
+        atomicMin(&s_hmin, x);
+        if (CALLER) atomicMin(&s_hcmin, y);
+      }
       if (h) pending = false;
     }
     uint64_t act = __ballot(pending);
@@ -1665,46 +1685,83 @@ __global__ __launch_bounds__(BLOCK) void k_canon_min(int32_t n, const int32_t* _
       const int leader = __ffsll((unsigned long long)act) - 1;
       const int32_t L = __shfl(l, leader);
       const bool mine = pending && l == L;
-      int32_t x = mine ? (int32_t)v : INT32_MAX;
+      int32_t x = mine ? (int32_t)v : INT32_MAX, y = mine ? p : INT32_MAX;
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o));
-      if (lane == leader && x < mn[L]) atomicMin(&mn[L], x);
+      for (int o = 32; o > 0; o >>= 1) {
+        x = min(x, __shfl_xor(x, o));
+        if (CALLER) y = min(y, __shfl_xor(y, o));
+      }
+      if (lane == leader) {
+        if (CALLER) {
+          const int2 cur = mnc[L];
+          if (x < cur.x) atomicMin(&mnc[L].x, x);
+          if (y < cur.y) atomicMin(&mnc[L].y, y);
+        } else if (x < mn[L]) {
+          atomicMin(&mn[L], x);
+        }
+      }
       if (mine) pending = false;
       act = __ballot(pending);
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0 && s_hmin < INT32_MAX && s_hmin < mn[hot]) atomicMin(&mn[hot], s_hmin);
+  if (threadIdx.x == 0 && s_hmin < INT32_MAX) {
+    if (CALLER) {
+      const int2 cur = mnc[hot];
+      if (s_hmin < cur.x) atomicMin(&mnc[hot].x, s_hmin);
+      if (s_hcmin < cur.y) atomicMin(&mnc[hot].y, s_hcmin);
+    } else if (s_hmin < mn[hot]) {
+      atomicMin(&mn[hot], s_hmin);
+    }
+  }
 }
 
+// out[v] = canonical label (internal ids); with CALLER also cout[perm[v]] =
+// the minimum caller id of v's component (labels in the caller's ids).
+template <bool CALLER>
 __global__ __launch_bounds__(BLOCK) void k_canon_apply(DevCtl* __restrict__ dc, int32_t n,
                                                        const int32_t* __restrict__ label,
                                                        const int32_t* __restrict__ mn,
-                                                       int32_t* __restrict__ out) {
+                                                       const int2* __restrict__ mnc,
+                                                       const int32_t* __restrict__ perm,
+                                                       int32_t* __restrict__ out, int32_t* __restrict__ cout) {
   const int32_t hot = block_hot_label(n, label);
-  __shared__ int32_t s_hv;
-  if (threadIdx.x == 0) s_hv = hot >= 0 ? mn[hot] : -1;
+  __shared__ int2 s_hv;
+  if (threadIdx.x == 0) s_hv = hot >= 0 ? (CALLER ? mnc[hot] : make_int2(mn[hot], 0)) : make_int2(-1, -1);
   __syncthreads();
-  const int32_t hv = s_hv;
-  auto canon = [&](int32_t l) { return (l == hot) ? hv : (((uint32_t)l < (uint32_t)n) ? mn[l] : -1); };
+  const int2 hv = s_hv;
+  auto canon = [&](int32_t l) -> int2 {
+    if (l == hot) return hv;
+    if ((uint32_t)l >= (uint32_t)n) return make_int2(-1, -1);
+    return CALLER ? mnc[l] : make_int2(mn[l], 0);
+  };
   unsigned long long self = 0;
   // four labels per lane (16-byte loads and stores; the four gathers are
   // independent), then the tail
   const int64_t n4 = (int64_t)n / 4;
   const int4* __restrict__ label4 = reinterpret_cast<const int4*>(label);
+  const int4* __restrict__ perm4 = reinterpret_cast<const int4*>(perm);
   int4* __restrict__ out4 = reinterpret_cast<int4*>(out);
   for (int64_t q = (int64_t)blockIdx.x * BLOCK + threadIdx.x; q < n4; q += (int64_t)gridDim.x * BLOCK) {
     const int4 l = label4[q];
-    const int4 c = make_int4(canon(l.x), canon(l.y), canon(l.z), canon(l.w));
-    out4[q] = c;
+    const int2 c0 = canon(l.x), c1 = canon(l.y), c2 = canon(l.z), c3 = canon(l.w);
+    out4[q] = make_int4(c0.x, c1.x, c2.x, c3.x);
+    if (CALLER) {
+      const int4 pv = perm4[q];
+      cout[pv.x] = c0.y;
+      cout[pv.y] = c1.y;
+      cout[pv.z] = c2.y;
+      cout[pv.w] = c3.y;
+    }
     const int32_t v = (int32_t)(4 * q);
-    self += (c.x == v) + (c.y == v + 1) + (c.z == v + 2) + (c.w == v + 3);
+    self += (c0.x == v) + (c1.x == v + 1) + (c2.x == v + 2) + (c3.x == v + 3);
   }
   if (blockIdx.x == 0)
     for (int64_t v = 4 * n4 + threadIdx.x; v < n; v += BLOCK) {
-      const int32_t c = canon(label[v]);
-      out[v] = c;
-      self += (c == (int32_t)v) ? 1 : 0;
+      const int2 c = canon(label[v]);
+      out[v] = c.x;
+      if (CALLER) cout[perm[v]] = c.y;
+      self += (c.x == (int32_t)v) ? 1 : 0;
     }
   block_count_nowait(self, &dc->ctr[C_COUNT]);
 }
@@ -1842,9 +1899,26 @@ void launch_canon(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label, i
     hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, dc, 1, 0, 0ull, 0ll, 0.0);
     return;
   }
-  LAUNCH(k_canon_init, grid_for(n, 4096), st, dc, mn, n);
-  LAUNCH(k_canon_min, grid_for(n), st, n, label, mn);
-  LAUNCH(k_canon_apply, grid_for((n + 3) / 4, 1024), st, dc, n, label, mn, out);
+  int2* no2 = nullptr;
+  const int32_t* noperm = nullptr;
+  int32_t* nocout = nullptr;
+  LAUNCH(k_canon_init<false>, grid_for(n, 4096), st, dc, mn, no2, noperm, n);
+  LAUNCH(k_canon_min<false>, grid_for(n), st, n, label, mn, no2, noperm);
+  LAUNCH(k_canon_apply<false>, grid_for((n + 3) / 4, 1024), st, dc, n, label, (const int32_t*)mn,
+         (const int2*)no2, noperm, out, nocout);
+}
+
+void launch_canon_caller(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label, int2* mnc,
+                         const int32_t* perm, int32_t* out, int32_t* cout) {
+  if (n <= 0) {
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, dc, 1, 0, 0ull, 0ll, 0.0);
+    return;
+  }
+  int32_t* nomn = nullptr;
+  LAUNCH(k_canon_init<true>, grid_for(n, 4096), st, dc, nomn, mnc, perm, n);
+  LAUNCH(k_canon_min<true>, grid_for(n), st, n, label, nomn, mnc, perm);
+  LAUNCH(k_canon_apply<true>, grid_for((n + 3) / 4, 1024), st, dc, n, label, (const int32_t*)nomn,
+         (const int2*)mnc, perm, out, cout);
 }
 
 #undef LAUNCH

@@ -100,83 +100,6 @@ __global__ __launch_bounds__(BLOCK) void k_to_internal(int32_t n, const int32_t*
   }
 }
 
-// mo[r] = min caller id over the members of the component whose internal
-// representative is r (canon[i] = r, or -1 while unresolved); mo must hold
-// INT32_MAX beforehand.  A giant component's members would all hit mo[r]:
-// lanes of a wave that share r reduce first and one lane issues the atomic,
-// and each workgroup keeps the most frequent r of its first 64 entries in LDS
-// with one global atomic at the end.
-__global__ __launch_bounds__(BLOCK) void k_caller_min(int32_t n, const int32_t* __restrict__ canon,
-                                                      const int32_t* __restrict__ perm, int32_t* __restrict__ mo) {
-  __shared__ int32_t s_hot, s_hmin;
-  const int lane = lane_id();
-  if (threadIdx.x < 64) {  // wave 0 picks the hot representative
-    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    const int32_t r = i < n ? canon[i] : -1;
-    bool pend = r >= 0;
-    int32_t best = -1;
-    int bestc = 1;
-    uint64_t act = __ballot(pend);
-    while (act) {
-      const int32_t R = __shfl(r, __ffsll((unsigned long long)act) - 1);
-      const bool same = pend && r == R;
-      const int cnt = __popcll(__ballot(same));
-      if (cnt > bestc) {
-        bestc = cnt;
-        best = R;
-      }
-      if (same) pend = false;
-      act = __ballot(pend);
-    }
-    if (threadIdx.x == 0) {
-      s_hot = best;
-      s_hmin = INT32_MAX;
-    }
-  }
-  __syncthreads();
-  const int32_t hot = s_hot;
-  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n; base += (int64_t)gridDim.x * BLOCK) {
-    const int64_t i = base + threadIdx.x;
-    int32_t r = -1, p = INT32_MAX;
-    if (i < n) {
-      r = canon[i];
-      p = perm[i];
-    }
-    bool pend = r >= 0;
-    if (hot >= 0) {
-      const bool h = pend && r == hot;
-      int32_t x = h ? p : INT32_MAX;
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o));
-      if (lane == 0 && x < INT32_MAX) atomicMin(&s_hmin, x);
-      if (h) pend = false;
-    }
-    uint64_t act = __ballot(pend);
-    while (act) {
-      const int leader = __ffsll((unsigned long long)act) - 1;
-      const int32_t R = __shfl(r, leader);
-      const bool mine = pend && r == R;
-      int32_t x = mine ? p : INT32_MAX;
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o));
-      if (lane == leader && x < mo[R]) atomicMin(&mo[R], x);
-      if (mine) pend = false;
-      act = __ballot(pend);
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0 && hot >= 0 && s_hmin < mo[hot]) atomicMin(&mo[hot], s_hmin);
-}
-
-__global__ __launch_bounds__(BLOCK) void k_caller_apply(int32_t n, const int32_t* __restrict__ canon,
-                                                        const int32_t* __restrict__ perm,
-                                                        const int32_t* __restrict__ mo, int32_t* __restrict__ out) {
-  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
-    const int32_t r = canon[i];
-    out[perm[i]] = r >= 0 ? mo[r] : -1;
-  }
-}
-
 // Segmented order for batched graphs: key = graph << 32 | ~degree of
 // internal vertex i, value = its caller id (graph g owns caller ids
 // [boff[g], boff[g + 1])).
@@ -261,7 +184,8 @@ void drop_layout(scc_ctx* c) {
   dev_free(c, c->iperm);
   dev_free(c, c->map_a);
   dev_free(c, c->map_b);
-  c->perm = c->iperm = c->map_a = c->map_b = nullptr;
+  dev_free(c, c->canon_caller);
+  c->perm = c->iperm = c->map_a = c->map_b = c->canon_caller = nullptr;
   c->h_perm.clear();
   c->h_iperm.clear();
   c->relabeled = false;
@@ -294,6 +218,7 @@ int build_layout(scc_ctx* c) {
       (rc = dev_alloc(c, (void**)&c->iperm, 4ull * n, "layout iperm")) ||
       (rc = dev_alloc(c, (void**)&c->map_a, 4ull * ((uint64_t)n + 1), "layout map")) ||
       (rc = dev_alloc(c, (void**)&c->map_b, 4ull * ((uint64_t)n + 1), "layout map")) ||
+      (rc = dev_alloc(c, (void**)&c->canon_caller, 4ull * ((uint64_t)n + 1), "caller labels")) ||
       (rc = dev_alloc(c, (void**)&key0, 4ull * n, "layout degree keys")) ||
       (rc = dev_alloc(c, (void**)&key1, 4ull * n, "layout degree keys")) ||
       (rc = dev_alloc(c, (void**)&ids, 4ull * n, "layout ids")))
@@ -325,13 +250,6 @@ int build_layout(scc_ctx* c) {
 int layout_to_caller(scc_ctx* c, const int32_t* in, bool map_values, int32_t* out_dev) {
   LAUNCH(k_to_caller, grid_for(c->n, 4096), c->stream, c->n, c->perm, in, map_values ? 1 : 0, out_dev);
   return hip_check(c, hipGetLastError(), "layout map");
-}
-
-int layout_canon_to_caller(scc_ctx* c, int32_t* out_dev) {
-  launch_fill_i32(c->stream, c->map_b, INT32_MAX, c->n);
-  LAUNCH(k_caller_min, grid_for(c->n, 4096), c->stream, c->n, c->canon, c->perm, c->map_b);
-  LAUNCH(k_caller_apply, grid_for(c->n, 4096), c->stream, c->n, c->canon, c->perm, c->map_b, out_dev);
-  return hip_check(c, hipGetLastError(), "layout canonical labels");
 }
 
 int layout_segment(scc_ctx* c, int32_t k) {

@@ -39,6 +39,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_BIN = os.path.join(HERE, "_ref", "scc_ref")
+REF_BENCH = os.path.join(HERE, "_ref", "ref_bench")
 
 _lib = None
 i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -195,6 +196,29 @@ def run_ref(edgelist: str, threads: int = 1, method: int = 2, flag: str = "-p",
     if flag == "-p":
         dense = np.loadtxt(os.path.join(cwd, "scc_list.txt"), dtype=np.int64, ndmin=2)
         res["dense"] = dense[:, 1].astype(np.int32)
+    return res
+
+
+def have_ref_bench() -> bool:
+    return os.path.exists(REF_BENCH)
+
+
+def run_ref_bench(gm_path: str, threads: int, reps: int, labels_out: str | None = None,
+                  timeout: float = 600.0) -> dict:
+    """Time the reference's own method 2 with oracle/_ref/ref_bench (its phase
+    sources, one graph load, `reps` runs of src/scc_main.cc:272-351 each timed as
+    src/common_main.h:801-807 times run()).  Returns per-run ms and SCC counts;
+    with labels_out the last run's G_SCC is read back."""
+    if not have_ref_bench():
+        raise RuntimeError("reference timing harness not built (make -C oracle ref)")
+    args = [REF_BENCH, gm_path, str(threads), str(reps)] + ([labels_out] if labels_out else [])
+    out = subprocess.run(args, capture_output=True, text=True, timeout=timeout, check=True).stdout
+    runs = [(float(a), int(b)) for a, b in re.findall(r"^run \d+ ms ([0-9.]+) sccs (\d+)$", out, re.M)]
+    m = re.search(r"threads (\d+)", out)
+    res = {"ms": [r[0] for r in runs], "sccs": [r[1] for r in runs],
+           "threads": int(m.group(1)) if m else threads, "stdout": out}
+    if labels_out:
+        res["labels"] = np.fromfile(labels_out, dtype=np.int32)
     return res
 
 
