@@ -505,6 +505,9 @@ struct scc_ctx {
 
   std::vector<void*> allocs;
   std::vector<uint64_t> alloc_sizes;
+  // buffers are pieces of a static array (scc_run_device_noalloc): dev_alloc
+  // refuses, and no phase trace is kept
+  bool no_alloc = false;
 };
 
 namespace sccmi {

@@ -57,6 +57,8 @@ int hip_check(scc_ctx* c, hipError_t e, const char* what) {
 
 int dev_alloc(scc_ctx* c, void** p, uint64_t bytes, const char* what) {
   *p = nullptr;
+  if (c->no_alloc)
+    return set_error(c, SCC_ERR_STATE, std::string("allocation of ") + what + " in a call that allocates nothing");
   if (bytes == 0) bytes = 256;
   if (c->budget && c->bytes + bytes > c->budget)
     return set_error(c, SCC_ERR_BUDGET,
@@ -113,7 +115,7 @@ struct PhaseRec {
   std::chrono::steady_clock::time_point t0;
   PhaseRec(scc_ctx* cc, int k) : c(cc), kind(k), before(cc->n_live), t0(std::chrono::steady_clock::now()) {}
   void finish(int64_t aux) {
-    if (c->trace.size() >= 65536) return;
+    if (c->no_alloc || c->trace.size() >= 65536) return;
     scc_phase_rec r;
     r.kind = kind;
     r.round = c->round_idx;

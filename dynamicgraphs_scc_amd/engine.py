@@ -64,6 +64,7 @@ EXPORTS = [
     "scc_sched_forward", "scc_sched_gradients", "scc_sched_train", "scc_sched_rollout",
     "scc_batch_set", "scc_batch_run", "scc_batch_counts", "scc_ctx_get_tuning", "scc_ctx_set_tuning",
     "scc_autotune", "scc_level_log", "scc_update_room_default", "scc_ctx_set_update_room",
+    "scc_run_device_noalloc",
 ]
 
 
@@ -189,6 +190,8 @@ def load_library():
         "scc_autotune": (C.c_int, [vp, C.POINTER(RunOpts), i32, C.POINTER(Tuning),
                                    C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "scc_update_room_default": (None, [C.POINTER(UpdateRoom)]),
+        "scc_run_device_noalloc": (C.c_int, [C.c_int, vp, i32, i64, vp, vp, vp, vp, vp, C.POINTER(i64),
+                                             C.c_char_p, C.c_size_t]),
         "scc_ctx_set_update_room": (C.c_int, [vp, C.POINTER(UpdateRoom)]),
         "scc_sched_set_weights": (C.c_int, [vp, vp]),
         "scc_sched_get_weights": (C.c_int, [vp, vp]),

@@ -416,6 +416,28 @@ int scc_ctx_set_kernel_timing(scc_ctx* ctx, int enable);
 /* Block until the context's stream is idle. */
 int scc_ctx_synchronize(scc_ctx* ctx);
 
+/* ---- decomposition that allocates nothing ---------------------------------------
+ * (No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code preparing HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.)  The canonical SCC labels of a graph already in HBM, as
+ * scc_graph_upload_device + scc_run_method2 + scc_labels_download(canonical)
+ * give them on a context, computed with every scratch buffer (per-vertex
+ * state, lists, bitmaps, control block) in one static device array of
+ * SCC_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until the
+ * process ends.  No hipMalloc, hipFree, pinned or library allocation is made.
+ * The caller passes the int32 CSR + reverse CSR (device pointers, a valid CSR
+ * as scc_graph_upload_device takes) and d_labels (n int32, device); work runs
+ * on `stream` (a hipStream_t, NULL for the null stream) and the call returns
+ * once the labels are written.  n above SCC_NOALLOC_MAX_N fails with
+ * SCC_ERR_LIMIT.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU race and must be serialised by the caller.  A message for
+ * a failure is written to err (err_len bytes, may be NULL). */
+#define SCC_NOALLOC_MAX_N 262144
+#define SCC_NOALLOC_SCRATCH_BYTES (32u << 20)
+int scc_run_device_noalloc(int device, void* stream, int32_t n, int64_t m, const int32_t* d_begin,
+                           const int32_t* d_node_idx, const int32_t* d_r_begin, const int32_t* d_r_node_idx,
+                           int32_t* d_labels, int64_t* num_sccs, char* err, size_t err_len);
+
 #ifdef __cplusplus
 }
 #endif
