@@ -478,6 +478,12 @@ struct scc_ctx {
   int32_t trim_epoch = 0;                 // number of the last trim loop queued
   bool labels_ready = false;              // canon[] valid for the current graph
   bool ran = false;
+  // scc_state_reset defers its reset launch: the next call either performs it
+  // (require_graph) or, in scc_run_method2, fuses it into the first trim sweep
+  bool reset_pending = false;
+  // pkey[n] holds the global step's pivot key and the reach bitmaps are clear
+  // (left by that fused sweep when it ended the trim loop); cleared by every call
+  bool pivot_ready = false;
   // An update failed after it began to rewrite the rows: the directions, the
   // edge count or the labels may disagree, so graph calls are refused until
   // the next upload.
@@ -513,14 +519,9 @@ struct scc_ctx {
 namespace sccmi {
 int set_error(scc_ctx* c, int code, const std::string& msg);
 // Entry check of every call that uses the device graph: one is uploaded and
-// usable.  Makes the context's device current.
-inline int require_graph(scc_ctx* c) {
-  if (!c->begin) return set_error(c, SCC_ERR_STATE, "no graph uploaded");
-  if (c->graph_broken)
-    return set_error(c, SCC_ERR_STATE, "an update failed part-way; the graph must be uploaded again");
-  (void)hipSetDevice(c->device);
-  return SCC_OK;
-}
+// usable.  Makes the context's device current and performs a deferred reset
+// (materialize = false leaves it to the caller, scc_run_method2).
+int require_graph(scc_ctx* c, bool materialize = true);
 // Row-end arrays of the forward and reverse CSR.
 inline const int32_t* row_end(const scc_ctx* c) { return c->end ? c->end : c->begin + 1; }
 inline const int32_t* rrow_end(const scc_ctx* c) { return c->rend ? c->rend : c->rbegin + 1; }

@@ -236,8 +236,22 @@ int alloc_graph(scc_ctx* c, int32_t n, int64_t m_cap) {
   return SCC_OK;
 }
 
-int reset_state(scc_ctx* c) {
-  launch_reset(c->stream, c->dc, c->n, c->color, c->label, c->live[0]);
+int require_graph(scc_ctx* c, bool materialize) {
+  if (!c->begin) return set_error(c, SCC_ERR_STATE, "no graph uploaded");
+  if (c->graph_broken)
+    return set_error(c, SCC_ERR_STATE, "an update failed part-way; the graph must be uploaded again");
+  (void)hipSetDevice(c->device);
+  c->pivot_ready = false;
+  if (materialize && c->reset_pending) {
+    c->reset_pending = false;
+    launch_reset(c->stream, c->dc, c->n, c->color, c->label, c->live[0]);
+    return hip_check(c, hipGetLastError(), "reset");
+  }
+  return SCC_OK;
+}
+
+// The host side of a reset; the launch is left to the next call.
+static void reset_host_state(scc_ctx* c) {
   c->labels_ready = false;
   c->multi_colors = false;
   c->colors_split = false;
@@ -247,6 +261,13 @@ int reset_state(scc_ctx* c) {
   c->fresh = true;
   c->cur = 0;
   c->n_live = c->n;
+  c->pivot_ready = false;
+}
+
+int reset_state(scc_ctx* c) {
+  c->reset_pending = false;
+  launch_reset(c->stream, c->dc, c->n, c->color, c->label, c->live[0]);
+  reset_host_state(c);
   return hip_check(c, hipGetLastError(), "reset");
 }
 
@@ -269,10 +290,15 @@ int force_pivot(scc_ctx* c, int32_t pivot) {
 // repeat_global_trim1: sweeps until one trims <= stop.
 int do_trim1(scc_ctx* c, int64_t stop, int64_t* trimmed) {
   if (trimmed) *trimmed = 0;
+  c->pivot_ready = false;
   if (c->n_live == 0) return SCC_OK;
   PhaseRec pr(c, SCC_PHASE_TRIM1);
   const int32_t epoch = ++c->trim_epoch;
   const unsigned grid = grid_for(c->n_live, (unsigned)c->tune.grid_cap);
+  // a deferred reset goes into the first sweep, with the global pivot key
+  const bool fused = c->reset_pending;
+  c->reset_pending = false;
+  const int64_t words = ((int64_t)c->n + 31) / 32;
   // queue as many sweeps as the last loop of this kind needed (a sweep that
   // finds the loop ended returns at once)
   int32_t& hint = c->trim_hint[stop < 0 ? 1 : 0];
@@ -280,8 +306,9 @@ int do_trim1(scc_ctx* c, int64_t stop, int64_t* trimmed) {
   for (int guard = 0;; guard++) {
     if ((rc = check_cancel(c))) return rc;
     for (int i = 0; i < batch; i++) {
-      launch_trim1_step(c->stream, grid, c->dc, gp(c), c->color, c->label, c->live[0], c->live[1],
-                        c->fresh ? 1 : 0, (long long)stop, epoch);
+      const int fresh = c->fresh ? (fused && guard == 0 && i == 0 ? 2 : 1) : 0;
+      launch_trim1_step(c->stream, grid, c->dc, gp(c), c->color, c->label, c->live[0], c->live[1], fresh,
+                        (long long)stop, epoch, c->pkey, c->vf, 2 * words);
       c->fresh = false;
     }
     if ((rc = pull_ctl(c))) return rc;
@@ -291,6 +318,8 @@ int do_trim1(scc_ctx* c, int64_t stop, int64_t* trimmed) {
   }
   hint = std::min(64, c->h_dc->loop.iters);
   if (trimmed) *trimmed = (int64_t)c->h_dc->loop.total0;
+  // the fused sweep's pivot key stands when that sweep ended the loop
+  c->pivot_ready = fused && c->h_dc->loop.iters == 1;
   pr.finish(c->h_dc->loop.iters);
   return SCC_OK;
 }
@@ -385,9 +414,11 @@ int run_levels(scc_ctx* c, bool fwd, bool multi, int64_t* levels, bool timed) {
 int traverse_both(scc_ctx* c, bool multi, int64_t* levels, bool timed, const std::function<void()>& post) {
   // (both search controls were set by the step's set-up launch)
   // (a streaming pass with no completion count: more workgroups hide its latency)
-  launch_seed_pivots2(c->stream, grid_for(c->n_live, 2048u), c->dc, c->n, multi ? 1 : 0,
-                      level_dir(c, true, true), level_dir(c, false, true), c->color, c->pkey, c->live[0],
-                      c->live[1]);
+  if (multi)
+    launch_seed_pivots2(c->stream, grid_for(c->n_live, 2048u), c->dc, c->n, 1, level_dir(c, true, true),
+                        level_dir(c, false, true), c->color, c->pkey, c->live[0], c->live[1]);
+  else  // one partition: its pivot is pkey[n]
+    launch_seed_single(c->stream, c->dc, c->n, level_dir(c, true, true), level_dir(c, false, true), c->pkey);
   return level_loop(c, true, true, multi, levels, timed, post);
 }
 
@@ -421,11 +452,16 @@ int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int6
   // counters, both search controls and (one partition) its slot; several
   // partitions reset their slots from the live list
   const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
-  launch_fb_setup(c->stream, c->dc, c->n, c->pkey, c->rep, multi ? 0 : 1, avg / c->tune.bu_alpha);
+  // the fused first trim sweep left the global pivot key and clear bitmaps
+  const bool ready = !multi && c->pivot_ready && pivot_override < 0 && policy == SCC_PIVOT_DEGREE_PRODUCT;
+  c->pivot_ready = false;
+  launch_fb_setup(c->stream, c->dc, c->n, c->pkey, c->rep, multi ? 0 : (ready ? 2 : 1), avg / c->tune.bu_alpha);
   if (multi)
     launch_reset_slots(c->stream, grid, c->dc, c->n, c->color, c->pkey, c->rep, c->live[0], c->live[1], 0);
   const int64_t words = ((int64_t)c->n + 31) / 32;
-  if (pivot_override >= 0) {
+  if (ready) {
+    // nothing to launch
+  } else if (pivot_override >= 0) {
     if ((rc = force_pivot(c, pivot_override))) return rc;
     launch_fill_u32(c->stream, c->vf, 0u, 2 * words);
   } else {
@@ -791,8 +827,11 @@ int scc_graph_download(scc_ctx* c, int32_t* begin, int32_t* node_idx, int32_t* r
 
 int scc_state_reset(scc_ctx* c) {
   if (!c) return SCC_ERR_ARG;
-  if (int r0 = require_graph(c)) return r0;
-  return reset_state(c);
+  if (int r0 = require_graph(c, false)) return r0;
+  // deferred: the next call performs it, or scc_run_method2 fuses it into its first sweep
+  reset_host_state(c);
+  c->reset_pending = true;
+  return SCC_OK;
 }
 
 int scc_state_set_colors(scc_ctx* c, const int32_t* colors) {
@@ -909,7 +948,8 @@ void scc_run_opts_default(scc_run_opts* o) {
 }
 
 int scc_run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_out) {
-  REQUIRE_GRAPH(c);
+  if (!c) return SCC_ERR_ARG;
+  if (int r0 = require_graph(c, false)) return r0;  // a deferred reset goes into the first trim sweep
   return sccmi::run_method2(c, opts_in, st_out);
 }
 

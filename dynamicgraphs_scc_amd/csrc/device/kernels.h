@@ -32,8 +32,15 @@ void launch_iota(hipStream_t st, int32_t* p, int32_t n);
 void launch_ctl_init(hipStream_t st, DevCtl* dc, int what, int32_t live_sel, unsigned long long live_n,
                      long long stop, double bu_ratio);
 // one sweep of trim loop number epoch (a new number starts a loop)
+// fresh: 0 a sweep over the live list, 1 the first sweep after a reset (row
+// lengths only), 2 the same with the reset fused in (colours / labels of every
+// vertex written, the global pivot key into pkey[n], zwords words at zbits cleared)
 void launch_trim1_step(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, int32_t* color,
-                       int32_t* label, int32_t* L0, int32_t* L1, int fresh, long long stop, int32_t epoch);
+                       int32_t* label, int32_t* L0, int32_t* L1, int fresh, long long stop, int32_t epoch,
+                       unsigned long long* pkey, uint32_t* zbits, int64_t zwords);
+// Seeds of the global step (one partition) from pkey[n]
+void launch_seed_single(hipStream_t st, DevCtl* dc, int32_t n, const LevelDir& F, const LevelDir& B,
+                        const unsigned long long* pkey);
 void launch_compact(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* color, int32_t* L0,
                     int32_t* L1);
 void launch_trim2(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, int32_t* color,

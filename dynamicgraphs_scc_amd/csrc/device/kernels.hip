@@ -152,6 +152,15 @@ __global__ __launch_bounds__(BLOCK) void k_reset(DevCtl* __restrict__ dc, int32_
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl_init_body(dc, 1 | 2, 0, (unsigned long long)n, 0, 0.0);
 }
 
+// Before a sweep of k_trim1_step with fresh == 2: the control block as the reset
+// leaves it (counters zero, live list 0..n-1 in live[0]) and the global pivot
+// slot empty.
+__global__ void k_fresh_init(DevCtl* __restrict__ dc, int32_t n, unsigned long long* __restrict__ pkey) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  ctl_init_body(dc, 1 | 2, 0, (unsigned long long)n, 0, 0.0);
+  pkey[n] = 0ull;
+}
+
 // Set-up of a forward-backward step in one launch: zeroed counters, both
 // search controls, and (single: every live vertex has colour -1) the one
 // partition slot in use.
@@ -159,8 +168,8 @@ __global__ void k_fb_setup(DevCtl* __restrict__ dc, int32_t n, unsigned long lon
                            int32_t* __restrict__ rep, int single, double bu_ratio) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   ctl_init_body(dc, 1 | 8 | 32, 0, 0, 0, bu_ratio);
-  if (single) {
-    pkey[n] = 0ull;
+  if (single) {  // single == 2: pkey[n] already holds the pivot key (k_trim1_step, fresh == 2)
+    if (single == 1) pkey[n] = 0ull;
     rep[3 * (int64_t)n + 0] = rep[3 * (int64_t)n + 1] = rep[3 * (int64_t)n + 2] = INT32_MAX;
   }
 }
@@ -177,11 +186,20 @@ __global__ void k_fb_setup(DevCtl* __restrict__ dc, int32_t n, unsigned long lon
 // loop state of another epoch is stale, so the first sweep of a loop needs no
 // separate initialisation (the per-sweep accumulators are left zero by every
 // sweep's last workgroup).
+//
+// fresh == 2 is the first sweep straight after a reset that was not launched:
+// it also writes colour -1 / label -1 of every survivor (the reset's work),
+// takes the global step's pivot key (max in-degree x out-degree over the
+// survivors, into pkey[n], which the launch before zeroed) and clears the
+// step's reach bitmaps (zwords words at zbits), so that a decomposition whose
+// trim loop ends after this sweep needs no reset or pivot-key pass.
 __global__ __launch_bounds__(BLOCK) void k_trim1_step(DevCtl* __restrict__ dc, GraphPtrs G,
                                                       int32_t* __restrict__ color,
                                                       int32_t* __restrict__ label,
                                                       int32_t* __restrict__ L0, int32_t* __restrict__ L1,
-                                                      int fresh, long long stop, int32_t epoch) {
+                                                      int fresh, long long stop, int32_t epoch,
+                                                      unsigned long long* __restrict__ pkey,
+                                                      uint32_t* __restrict__ zbits, int64_t zwords) {
   __shared__ int s_done;
   if (threadIdx.x == 0) s_done = dc->loop.epoch == epoch ? dc->loop.done : 0;
   __syncthreads();
@@ -193,6 +211,10 @@ __global__ __launch_bounds__(BLOCK) void k_trim1_step(DevCtl* __restrict__ dc, G
   unsigned long long trimmed = 0;
   using Out = LdsOut<0>;
   Out::init();
+  unsigned long long best = 0ull;  // fresh == 2: this thread's largest pivot key
+  if (fresh == 2)
+    for (int64_t w = (int64_t)blockIdx.x * BLOCK + threadIdx.x; w < zwords; w += (int64_t)gridDim.x * BLOCK)
+      zbits[w] = 0u;
   if (fresh) {
     // fresh state: the live list is 0..n-1, every colour is -1 and none is
     // done, so only empty rows decide this sweep (a row holding nothing but
@@ -223,6 +245,12 @@ __global__ __launch_bounds__(BLOCK) void k_trim1_step(DevCtl* __restrict__ dc, G
           label[v] = (int32_t)v;
           color[v] = DONE;
           trimmed++;
+        } else if (fresh == 2 && valid) {
+          label[v] = -1;
+          color[v] = UNSET;
+          const unsigned long long p = (unsigned long long)(e[t] - b[t]) * (unsigned long long)(re[t] - rb[t]);
+          const unsigned long long k = pivot_key(p > 0x1FFFFFFFFull ? 0x1FFFFFFFFull : p, (int32_t)v);
+          best = k > best ? k : best;
         }
         Out::push(valid && !trim, (int32_t)v, list_out, &dc->loop.acc1);
       }
@@ -262,6 +290,21 @@ __global__ __launch_bounds__(BLOCK) void k_trim1_step(DevCtl* __restrict__ dc, G
     Out::push(act && !trim, v, list_out, &dc->loop.acc1);
   }
   Out::flush(list_out, &dc->loop.acc1);
+  if (fresh == 2) {  // the workgroup's largest key, one atomic (the slot is read first)
+    __shared__ unsigned long long s_best[WAVES];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned long long y = __shfl_xor(best, o);
+      best = y > best ? y : best;
+    }
+    if (lane_id() == 0) s_best[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long x = 0ull;
+      for (int w = 0; w < WAVES; w++) x = s_best[w] > x ? s_best[w] : x;
+      if (x > pkey[G.n]) atomicMax(&pkey[G.n], x);
+    }
+  }
   block_count(trimmed, &dc->loop.acc0);
   if (last_block2(dc->done) && threadIdx.x == 0) {
     LoopCtl& l = dc->loop;
@@ -538,6 +581,30 @@ __global__ __launch_bounds__(BLOCK) void k_seed_pivots2(DevCtl* __restrict__ dc,
   block_count(eb, &dc->lv2.mf);
   block_count(cnt, &dc->lv.visited);
   block_count(cnt, &dc->lv2.visited);
+}
+
+// The seeds of the global step's two searches when there is one partition
+// (colour -1, slot n): its pivot, from pkey[n], with no pass over the live list.
+__global__ void k_seed_single(DevCtl* __restrict__ dc, int32_t n, LevelDir F, LevelDir B,
+                              const unsigned long long* __restrict__ pkey) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const unsigned long long key = pkey[n];
+  if (key == 0ull) return;  // no live vertex
+  const int32_t v = pivot_of_key(key);
+  const uint32_t bit = 1u << (v & 31);
+  atomicOr(&F.V[v >> 5], bit);
+  atomicOr(&B.V[v >> 5], bit);
+  const int32_t df = F.offe[v] - F.off[v], db = B.offe[v] - B.off[v];
+  const int32_t huge = dc->tune.huge_row;
+  if (df < huge) F.q0[dc->lv.nq++] = v;
+  else F.h0[dc->lv.nh++] = v;
+  if (db < huge) B.q0[dc->lv2.nq++] = v;
+  else B.h0[dc->lv2.nh++] = v;
+  dc->lv.mf += (unsigned long long)df;
+  dc->lv2.mf += (unsigned long long)db;
+  dc->lv.visited += 1;
+  dc->lv2.visited += 1;
+  dc->ctr[C_PIVOT] = (unsigned long long)v;
 }
 
 // Seed a search from an explicit vertex list (count at *ns): vertices not yet
@@ -1784,8 +1851,14 @@ void launch_ctl_init(hipStream_t st, DevCtl* dc, int what, int32_t live_sel, uns
   hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, dc, what, live_sel, live_n, stop, bu_ratio);
 }
 void launch_trim1_step(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, int32_t* color,
-                       int32_t* label, int32_t* L0, int32_t* L1, int fresh, long long stop, int32_t epoch) {
-  LAUNCH(k_trim1_step, grid, st, dc, G, color, label, L0, L1, fresh, stop, epoch);
+                       int32_t* label, int32_t* L0, int32_t* L1, int fresh, long long stop, int32_t epoch,
+                       unsigned long long* pkey, uint32_t* zbits, int64_t zwords) {
+  if (fresh == 2) hipLaunchKernelGGL(k_fresh_init, dim3(1), dim3(64), 0, st, dc, G.n, pkey);
+  LAUNCH(k_trim1_step, grid, st, dc, G, color, label, L0, L1, fresh, stop, epoch, pkey, zbits, zwords);
+}
+void launch_seed_single(hipStream_t st, DevCtl* dc, int32_t n, const LevelDir& F, const LevelDir& B,
+                        const unsigned long long* pkey) {
+  hipLaunchKernelGGL(k_seed_single, dim3(1), dim3(64), 0, st, dc, n, F, B, pkey);
 }
 void launch_compact(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* color, int32_t* L0,
                     int32_t* L1) {
