@@ -591,7 +591,7 @@ __global__ __launch_bounds__(BLOCK) void k_region(DevCtl* __restrict__ dc, int32
                                                   const uint32_t* __restrict__ vf,
                                                   const uint32_t* __restrict__ vb, bool have_fb,
                                                   int32_t* __restrict__ color, int32_t* __restrict__ label,
-                                                  int32_t* __restrict__ live, int32_t sel) {
+                                                  int32_t* __restrict__ live, int32_t sel, CanonKeys K) {
   using Out = LdsOut<0>;
   Out::init();
   for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n; base += (int64_t)gridDim.x * BLOCK) {
@@ -603,6 +603,8 @@ __global__ __launch_bounds__(BLOCK) void k_region(DevCtl* __restrict__ dc, int32
       in = aff[cv] != 0 || (have_fb && bit_test(vf, (int32_t)v) && bit_test(vb, (int32_t)v));
       color[v] = in ? UNSET : DONE;
       label[v] = in ? -1 : cv;
+      // the kept SCC's label is its canonical member, whose key is the SCC's least
+      if (!in && cv == (int32_t)v) K.ckey[v] = canon_key(K.perm, (int32_t)v);
     }
     Out::push(in, (int32_t)v, live, &dc->ctr[C_LIST_OUT]);
   }
@@ -1136,7 +1138,7 @@ extern "C" int scc_update(scc_ctx* c, const int32_t* ins_src, const int32_t* ins
     have_fb = true;
   }
   LAUNCH(k_region, grid_for(c->n), s, c->dc, c->n, c->canon, d->aff, c->vf, c->vb, have_fb, c->color,
-         c->label, c->live[0], 0);
+         c->label, c->live[0], 0, ck(c));
   // clear the old-SCC flags (indexed by old canonical labels, before canon changes)
   if (nd) LAUNCH(k_clear_aff, grid_for(nd), s, d->del_f, nd, c->canon, d->aff);
   if ((rc = pull_ctl(c))) return rc;

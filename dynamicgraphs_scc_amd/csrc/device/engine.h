@@ -14,6 +14,7 @@ int hip_check(scc_ctx* c, hipError_t e, const char* what);
 int dev_alloc(scc_ctx* c, void** p, uint64_t bytes, const char* what);
 void dev_free(scc_ctx* c, void* p);
 GraphPtrs gp(const scc_ctx* c);
+CanonKeys ck(const scc_ctx* c);  // canonical-key pointers of the context
 int pull_ctl(scc_ctx* c);   // control block -> host (syncs the stream)
 int zero_ctr(scc_ctx* c);   // zero DevCtl::ctr (queued)
 int set_live(scc_ctx* c, int sel, int64_t n);

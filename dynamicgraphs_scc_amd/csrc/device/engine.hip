@@ -90,8 +90,10 @@ void dev_free(scc_ctx* c, void* p) {
 }
 
 GraphPtrs gp(const scc_ctx* c) {
-  return GraphPtrs{c->begin, row_end(c), tr_adj(c), c->rbegin, rrow_end(c), tr_radj(c), c->n, c->maxdeg};
+  return GraphPtrs{c->begin, row_end(c), tr_adj(c), c->rbegin, rrow_end(c), tr_radj(c), c->n, c->maxdeg,
+                   c->relabeled ? c->perm : nullptr, c->ckey};
 }
+CanonKeys ck(const scc_ctx* c) { return CanonKeys{c->relabeled ? c->perm : nullptr, c->ckey}; }
 
 // Copy the control block to the host and refresh the host copies of the live list.
 int pull_ctl(scc_ctx* c) {
@@ -156,6 +158,7 @@ uint64_t state_bytes(int32_t n) {
          + 6 * N * 4          // bq[2] bcand[2] bhuge[2]
          + 2 * words * 4      // vf vb
          + 8 * N + 12 * N     // pkey rep
+         + 8 * N              // ckey
          + 4 * N              // scratch
          + 8 * N;             // packed WCC words
 }
@@ -184,8 +187,10 @@ int free_graph(scc_ctx* c) {
   }
   dev_free(c, c->vf);  // vb shares this allocation
   dev_free(c, c->pkey);
+  dev_free(c, c->ckey);
   c->vf = c->vb = nullptr;
   c->pkey = nullptr;
+  c->ckey = nullptr;
   c->n = 0;
   c->m = 0;
   c->m_cap = 0;
@@ -230,6 +235,7 @@ int alloc_graph(scc_ctx* c, int32_t n, int64_t m_cap) {
   ALLOC(c->rep, 12 * N);
   ALLOC(c->scratch_i32, 4 * N);
   ALLOC(c->cw, 8 * N);
+  ALLOC(c->ckey, 8 * N);
 #undef ALLOC
   c->n = n;
   c->m_cap = m_cap;
@@ -416,9 +422,10 @@ int traverse_both(scc_ctx* c, bool multi, int64_t* levels, bool timed, const std
   // (a streaming pass with no completion count: more workgroups hide its latency)
   if (multi)
     launch_seed_pivots2(c->stream, grid_for(c->n_live, 2048u), c->dc, c->n, 1, level_dir(c, true, true),
-                        level_dir(c, false, true), c->color, c->pkey, c->live[0], c->live[1]);
+                        level_dir(c, false, true), c->color, c->pkey, c->live[0], c->live[1], ck(c));
   else  // one partition: its pivot is pkey[n]
-    launch_seed_single(c->stream, c->dc, c->n, level_dir(c, true, true), level_dir(c, false, true), c->pkey);
+    launch_seed_single(c->stream, c->dc, c->n, level_dir(c, true, true), level_dir(c, false, true), c->pkey,
+                       ck(c));
   return level_loop(c, true, true, multi, levels, timed, post);
 }
 
@@ -476,7 +483,7 @@ int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int6
   auto post = [&]() {
     if (sets) launch_fb_stats(c->stream, grid, c->dc, gp(c), c->vf, c->vb, c->live[0], c->live[1]);
     launch_classify(c->stream, grid, c->dc, c->n, c->color, c->label, c->vf, c->vb, c->pkey, c->rep,
-                    c->live[0], c->live[1]);
+                    c->live[0], c->live[1], ck(c));
     launch_recolor(c->stream, grid, c->dc, c->n, c->color, c->vf, c->vb, c->rep, c->live[0], c->live[1]);
   };
   if ((rc = traverse_both(c, multi, levels, timed, post))) return rc;
@@ -526,7 +533,7 @@ static int wcc_fixpoint(scc_ctx* c, bool seed) {
     launch_fill_u32(c->stream, c->vf, 0u, ((int64_t)c->n + 31) / 32);
   }
   launch_wcc_color(c->stream, grid, c->dc, c->color, c->W, seed ? 1 : 0, c->begin, row_end(c), c->vf, c->q[0],
-                   c->live[0], c->live[1]);
+                   c->live[0], c->live[1], ck(c));
   c->colors_split = true;
   return hip_check(c, hipGetLastError(), "wcc colour");
 }
@@ -554,7 +561,8 @@ int do_color_round(scc_ctx* c, int64_t* scc) {
   if ((rc = wcc_fixpoint(c, true))) return rc;
   const int64_t iters = c->h_dc->loop.iters;
   if ((rc = run_levels(c, true, true, nullptr, false))) return rc;
-  launch_extract(c->stream, grid_for(c->n_live, (unsigned)c->tune.grid_cap), c->dc, c->color, c->label, c->vf, c->live[0], c->live[1]);
+  launch_extract(c->stream, grid_for(c->n_live, (unsigned)c->tune.grid_cap), c->dc, c->color, c->label, c->vf,
+                 c->live[0], c->live[1], ck(c));
   if ((rc = pull_ctl(c))) return rc;
   if (scc) *scc = (int64_t)c->h_dc->ctr[C_SCC];
   c->classes = true;
@@ -603,11 +611,9 @@ int do_fb_parts(scc_ctx* c, int policy, int mode, bool trim_between, int64_t* ro
 int do_canon(scc_ctx* c, int64_t* count) {
   int rc;
   PhaseRec pr(c, SCC_PHASE_CANON);
-  if (c->relabeled)  // the caller's canonical labels in the same passes
-    launch_canon_caller(c->stream, c->dc, c->n, c->label, reinterpret_cast<int2*>(c->pkey), c->perm, c->canon,
-                        c->canon_caller);
-  else
-    launch_canon(c->stream, c->dc, c->n, c->label, c->scratch_i32, c->canon);
+  // (in the degree layout the caller's canonical labels in the same pass)
+  launch_canon(c->stream, c->dc, c->n, c->label, c->ckey, c->relabeled ? c->perm : nullptr, c->canon,
+               c->canon_caller);
   if ((rc = pull_ctl(c))) return rc;
   pr.finish((int64_t)c->h_dc->ctr[C_COUNT]);
   if (count) *count = (int64_t)c->h_dc->ctr[C_COUNT];

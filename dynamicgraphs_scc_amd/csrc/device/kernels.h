@@ -14,7 +14,27 @@ struct GraphPtrs {
   const int32_t *begin, *end, *adj, *rbegin, *rend, *radj;
   int32_t n;
   int32_t maxdeg;
+  // canonical-label keys (see CanonKeys)
+  const int32_t* perm;
+  unsigned long long* ckey;
 };
+
+// Canonical-label keys.  key(v) = caller id of v << 32 | v (the caller id is
+// perm[v] in the degree vertex layout, v itself otherwise).  For every label l
+// in use, ckey[l] holds the minimum key over the vertices labelled l: the phase
+// that gives a label sets it (trims store their vertex's key, a step's pivots
+// and a colouring's roots start their slot with their own key, the members'
+// keys are min-reduced in), so the canonical label of v is the low half of
+// ckey[label[v]] (internal id of the member with the smallest caller id, the
+// smallest id in the input layout) and its caller-id label the high half.
+struct CanonKeys {
+  const int32_t* perm;
+  unsigned long long* ckey;
+};
+__device__ __forceinline__ unsigned long long canon_key(const int32_t* __restrict__ perm, int32_t v) {
+  const uint32_t cid = perm ? (uint32_t)perm[v] : (uint32_t)v;
+  return ((unsigned long long)cid << 32) | (uint32_t)v;
+}
 
 // grid size for a list of n items (grid-stride beyond the cap)
 inline unsigned grid_for(int64_t n, unsigned cap = 2048) {
@@ -40,7 +60,7 @@ void launch_trim1_step(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, i
                        unsigned long long* pkey, uint32_t* zbits, int64_t zwords);
 // Seeds of the global step (one partition) from pkey[n]
 void launch_seed_single(hipStream_t st, DevCtl* dc, int32_t n, const LevelDir& F, const LevelDir& B,
-                        const unsigned long long* pkey);
+                        const unsigned long long* pkey, CanonKeys K);
 void launch_compact(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* color, int32_t* L0,
                     int32_t* L1);
 void launch_trim2(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, int32_t* color,
@@ -66,7 +86,7 @@ void launch_seed_list(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* 
                       uint32_t* V, int32_t* q0);
 void launch_seed_pivots2(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, int multi, const LevelDir& F,
                          const LevelDir& B, const int32_t* color, const unsigned long long* pkey,
-                         const int32_t* L0, const int32_t* L1);
+                         const int32_t* L0, const int32_t* L1, CanonKeys K);
 void launch_level(hipStream_t st, unsigned grid, DevCtl* dc, int multi, const LevelDir& D,
                   const int32_t* color, const int32_t* L0, const int32_t* L1, int32_t max_chunks, int light);
 // both searches of a step, one level each; grid = workgroups per search x 2,
@@ -77,7 +97,7 @@ void launch_fb_stats(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, con
                      const uint32_t* vb, const int32_t* L0, const int32_t* L1);
 void launch_classify(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, int32_t* color,
                      int32_t* label, const uint32_t* vf, const uint32_t* vb,
-                     const unsigned long long* pkey, int32_t* rep, int32_t* L0, int32_t* L1);
+                     const unsigned long long* pkey, int32_t* rep, int32_t* L0, int32_t* L1, CanonKeys K);
 void launch_recolor(hipStream_t st, unsigned grid, const DevCtl* dc, int32_t n, int32_t* color,
                     const uint32_t* vf, const uint32_t* vb, const int32_t* rep, const int32_t* L0,
                     const int32_t* L1);
@@ -95,16 +115,13 @@ void launch_wcc_store(hipStream_t st, unsigned grid, const DevCtl* dc, const uns
                       const int32_t* L0, const int32_t* L1);
 void launch_wcc_color(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, const int32_t* W,
                       int seed, const int32_t* off, const int32_t* offe, uint32_t* V, int32_t* q0, const int32_t* L0,
-                      const int32_t* L1);
+                      const int32_t* L1, CanonKeys K);
 void launch_extract(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, int32_t* label,
-                    const uint32_t* V, int32_t* L0, int32_t* L1);
-// canonical labels into out (mn: n ints of scratch); zeroes the counters first
-// and leaves the SCC count in dc->ctr[C_COUNT]
-void launch_canon(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label, int32_t* mn,
-                  int32_t* out);
-// The same with the caller's canonical labels beside (degree vertex layout):
-// cout[perm[v]] = min caller id of v's component; mnc holds n int2 slots.
-void launch_canon_caller(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label, int2* mnc,
-                         const int32_t* perm, int32_t* out, int32_t* cout);
+                    const uint32_t* V, int32_t* L0, int32_t* L1, CanonKeys K);
+// canonical labels into out, and (perm != NULL) the caller-id labels into
+// cout[perm[v]], from the keys the phases left (CanonKeys); zeroes the
+// counters first and leaves the SCC count in dc->ctr[C_COUNT]
+void launch_canon(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label, const unsigned long long* ckey,
+                  const int32_t* perm, int32_t* out, int32_t* cout);
 
 }  // namespace sccmi

@@ -244,6 +244,7 @@ __global__ __launch_bounds__(BLOCK) void k_trim1_step(DevCtl* __restrict__ dc, G
         if (trim) {
           label[v] = (int32_t)v;
           color[v] = DONE;
+          G.ckey[v] = canon_key(G.perm, (int32_t)v);
           trimmed++;
         } else if (fresh == 2 && valid) {
           label[v] = -1;
@@ -285,6 +286,7 @@ __global__ __launch_bounds__(BLOCK) void k_trim1_step(DevCtl* __restrict__ dc, G
     if (trim) {
       label[v] = v;
       color[v] = DONE;
+      G.ckey[v] = canon_key(G.perm, v);
       trimmed++;
     }
     Out::push(act && !trim, v, list_out, &dc->loop.acc1);
@@ -400,6 +402,8 @@ __global__ __launch_bounds__(BLOCK) void k_trim2(DevCtl* __restrict__ dc, GraphP
       label[k] = v;
       color[v] = DONE;
       color[k] = DONE;
+      const unsigned long long kv = canon_key(G.perm, v), kk = canon_key(G.perm, k);
+      G.ckey[v] = kv < kk ? kv : kk;
       removed += 2;
     }
   }
@@ -545,7 +549,7 @@ __global__ __launch_bounds__(BLOCK) void k_seed_pivots2(DevCtl* __restrict__ dc,
                                                         LevelDir B, const int32_t* __restrict__ color,
                                                         const unsigned long long* __restrict__ pkey,
                                                         const int32_t* __restrict__ L0,
-                                                        const int32_t* __restrict__ L1) {
+                                                        const int32_t* __restrict__ L1, CanonKeys K) {
   const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
   const int64_t n_in = (int64_t)dc->live.n;
   const int32_t huge = dc->tune.huge_row;
@@ -559,6 +563,7 @@ __global__ __launch_bounds__(BLOCK) void k_seed_pivots2(DevCtl* __restrict__ dc,
       const int32_t c = color[v];
       piv = c != DONE && pivot_of_key(pkey[slot_of(c, n)]) == v;
       if (piv) {
+        K.ckey[v] = canon_key(K.perm, v);  // the pivot's SCC starts its key slot
         const uint32_t bit = 1u << (v & 31);
         atomicOr(&F.V[v >> 5], bit);
         atomicOr(&B.V[v >> 5], bit);
@@ -586,11 +591,12 @@ __global__ __launch_bounds__(BLOCK) void k_seed_pivots2(DevCtl* __restrict__ dc,
 // The seeds of the global step's two searches when there is one partition
 // (colour -1, slot n): its pivot, from pkey[n], with no pass over the live list.
 __global__ void k_seed_single(DevCtl* __restrict__ dc, int32_t n, LevelDir F, LevelDir B,
-                              const unsigned long long* __restrict__ pkey) {
+                              const unsigned long long* __restrict__ pkey, CanonKeys K) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const unsigned long long key = pkey[n];
   if (key == 0ull) return;  // no live vertex
   const int32_t v = pivot_of_key(key);
+  K.ckey[v] = canon_key(K.perm, v);
   const uint32_t bit = 1u << (v & 31);
   atomicOr(&F.V[v >> 5], bit);
   atomicOr(&B.V[v >> 5], bit);
@@ -1200,19 +1206,23 @@ __global__ __launch_bounds__(BLOCK) void k_classify(DevCtl* __restrict__ dc, int
                                                     const uint32_t* __restrict__ vb,
                                                     const unsigned long long* __restrict__ pkey,
                                                     int32_t* __restrict__ rep, int32_t* __restrict__ L0,
-                                                    int32_t* __restrict__ L1) {
+                                                    int32_t* __restrict__ L1, CanonKeys K) {
   if (searches_open(dc)) return;
   const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
   int32_t* __restrict__ list_out = LIVE_OUT(dc, L0, L1);
   const int64_t n_in = (int64_t)dc->live.n;
   // the partition of the workgroup's first entry (in the global step, every
-  // entry's): its three representatives are reduced in LDS and written with
-  // one global atomic each, instead of every wave hitting the same addresses
-  __shared__ int32_t s_hot, s_hmin[3];
+  // entry's): its three representatives and its SCC's canonical key are
+  // reduced in LDS and written with one global atomic each, instead of every
+  // wave hitting the same addresses
+  __shared__ int32_t s_hot, s_hmin[3], s_hpiv;
+  __shared__ unsigned long long s_hkey;
   if (threadIdx.x == 0) {
     const int64_t i0 = (int64_t)blockIdx.x * 4 * BLOCK;
     const int32_t c0 = i0 < n_in ? color[list[i0]] : DONE;
     s_hot = c0 != DONE ? slot_of(c0, n) : -1;
+    s_hpiv = c0 != DONE ? pivot_of_key(pkey[s_hot]) : -1;
+    s_hkey = ~0ull;
   }
   if (threadIdx.x < 3) s_hmin[threadIdx.x] = INT32_MAX;
   __syncthreads();
@@ -1222,6 +1232,7 @@ __global__ __launch_bounds__(BLOCK) void k_classify(DevCtl* __restrict__ dc, int
   Out::init();
   // four entries per thread and step (base + t * BLOCK + thread: survivors
   // keep the list order), their dependent loads in flight together
+  unsigned long long hkey = ~0ull;  // this thread's least key among the hot SCC's members
   for (int64_t base = (int64_t)blockIdx.x * 4 * BLOCK; base < n_in; base += (int64_t)gridDim.x * 4 * BLOCK) {
     Out::maybe_flush(list_out, &dc->ctr[C_LIST_OUT], 4 * BLOCK);
     int32_t v[4], c[4];
@@ -1240,14 +1251,22 @@ __global__ __launch_bounds__(BLOCK) void k_classify(DevCtl* __restrict__ dc, int
     }
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-      bool keep = false, hk = false;
+      bool keep = false, hk = false, inscc = false;
       int64_t rslot = 0;
+      int32_t piv = 0;
       if (c[t] != DONE) {
         const int32_t sl = slot_of(c[t], n);
         if (f[t] && b[t]) {
-          label[v[t]] = pivot_of_key(pkey[sl]);
+          piv = sl == hot ? s_hpiv : pivot_of_key(pkey[sl]);
+          label[v[t]] = piv;
           color[v[t]] = DONE;
           nscc++;
+          if (sl == hot) {
+            const unsigned long long kv = canon_key(K.perm, v[t]);
+            hkey = kv < hkey ? kv : hkey;
+          } else {
+            inscc = true;
+          }
         } else {
           const int k = f[t] ? 0 : (b[t] ? 1 : 2);
           rslot = 3 * (int64_t)sl + k;
@@ -1257,15 +1276,23 @@ __global__ __launch_bounds__(BLOCK) void k_classify(DevCtl* __restrict__ dc, int
         }
       }
       grouped_min_i32(keep && !hk, rslot, v[t], rep);
+      grouped_min_u64(inscc, piv, inscc ? canon_key(K.perm, v[t]) : 0ull, K.ckey);
       Out::push(keep, v[t], list_out, &dc->ctr[C_LIST_OUT]);
     }
   }
-  Out::flush(list_out, &dc->ctr[C_LIST_OUT]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long y = __shfl_xor(hkey, o);
+    hkey = y < hkey ? y : hkey;
+  }
+  if (lane_id() == 0 && hkey != ~0ull) atomicMin(&s_hkey, hkey);
+  Out::flush(list_out, &dc->ctr[C_LIST_OUT]);  // (begins with a barrier)
   if (threadIdx.x < 3 && hot >= 0) {
     const int32_t x = s_hmin[threadIdx.x];
     int32_t* r = &rep[3 * (int64_t)hot + threadIdx.x];
     if (x < *r) atomicMin(r, x);
   }
+  if (threadIdx.x == 3 && hot >= 0 && s_hkey != ~0ull && s_hkey < K.ckey[s_hpiv]) atomicMin(&K.ckey[s_hpiv], s_hkey);
   block_count(nscc, &dc->ctr[C_SCC]);
   if (last_block2(dc->done) && threadIdx.x == 0) {
     dc->live.n = ld_acq(&dc->ctr[C_LIST_OUT]);
@@ -1581,7 +1608,7 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_color(DevCtl* __restrict__ dc, in
                                                      const int32_t* __restrict__ off, const int32_t* __restrict__ offe,
                                                      uint32_t* __restrict__ V, int32_t* __restrict__ q0,
                                                      const int32_t* __restrict__ L0,
-                                                     const int32_t* __restrict__ L1) {
+                                                     const int32_t* __restrict__ L1, CanonKeys K) {
   const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
   const int64_t n_in = (int64_t)dc->live.n;
   unsigned long long roots = 0, edges = 0;
@@ -1601,6 +1628,7 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_color(DevCtl* __restrict__ dc, in
         if (root) {
           roots++;
           if (seed) {
+            K.ckey[v] = canon_key(K.perm, v);  // the root's SCC starts its key slot
             atomicOr(&V[v >> 5], 1u << (v & 31));
             edges += (unsigned long long)(offe[v] - off[v]);
           }
@@ -1624,34 +1652,59 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_color(DevCtl* __restrict__ dc, in
 __global__ __launch_bounds__(BLOCK) void k_extract(DevCtl* __restrict__ dc, int32_t* __restrict__ color,
                                                    int32_t* __restrict__ label,
                                                    const uint32_t* __restrict__ V, int32_t* __restrict__ L0,
-                                                   int32_t* __restrict__ L1) {
+                                                   int32_t* __restrict__ L1, CanonKeys K) {
   const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
   int32_t* __restrict__ list_out = LIVE_OUT(dc, L0, L1);
   const int64_t n_in = (int64_t)dc->live.n;
   unsigned long long nscc = 0;
   using Out = LdsOut<0>;
   Out::init();
+  // the class of the workgroup's first entry: its SCC's key reduced in LDS
+  __shared__ int32_t s_hot;
+  __shared__ unsigned long long s_hkey;
+  if (threadIdx.x == 0) {
+    const int64_t i0 = (int64_t)blockIdx.x * BLOCK;
+    s_hot = i0 < n_in ? color[list[i0]] : DONE;
+    s_hkey = ~0ull;
+  }
+  __syncthreads();
+  const int32_t hot = s_hot;
+  unsigned long long hkey = ~0ull;  // this thread's least key among the hot SCC's members
   for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
     Out::maybe_flush(list_out, &dc->ctr[C_LIST_OUT], BLOCK);
     const int64_t i = base + threadIdx.x;
-    bool keep = false;
-    int32_t v = 0;
+    bool keep = false, inscc = false;
+    int32_t v = 0, c = DONE;
     if (i < n_in) {
       v = list[i];
-      const int32_t c = color[v];
+      c = color[v];
       if (c != DONE) {
         if (bit_test(V, v)) {
           label[v] = c;
           color[v] = DONE;
           nscc++;
+          if (c == hot) {
+            const unsigned long long kv = canon_key(K.perm, v);
+            hkey = kv < hkey ? kv : hkey;
+          } else {
+            inscc = true;
+          }
         } else {
           keep = true;
         }
       }
     }
+    grouped_min_u64(inscc, c, inscc ? canon_key(K.perm, v) : 0ull, K.ckey);
     Out::push(keep, v, list_out, &dc->ctr[C_LIST_OUT]);
   }
-  Out::flush(list_out, &dc->ctr[C_LIST_OUT]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long y = __shfl_xor(hkey, o);
+    hkey = y < hkey ? y : hkey;
+  }
+  if (lane_id() == 0 && hkey != ~0ull) atomicMin(&s_hkey, hkey);
+  Out::flush(list_out, &dc->ctr[C_LIST_OUT]);  // (begins with a barrier)
+  if (threadIdx.x == 0 && hot != DONE && s_hkey != ~0ull && s_hkey < K.ckey[hot]) atomicMin(&K.ckey[hot], s_hkey);
   block_count(nscc, &dc->ctr[C_SCC]);
   if (last_block2(dc->done) && threadIdx.x == 0) {
     dc->live.n = ld_acq(&dc->ctr[C_LIST_OUT]);
@@ -1662,13 +1715,11 @@ __global__ __launch_bounds__(BLOCK) void k_extract(DevCtl* __restrict__ dc, int3
 }
 
 // ---- canonical labels -----------------------------------------------------------
-// mn[l] = min vertex labelled l.  Most labels are shared by one vertex; the
-// forward-backward pivots label whole SCCs, so lanes of a wave that share a
-// label first reduce among themselves and one lane issues the atomic (test
-// first: the slot usually already holds a smaller id).  A label that many
-// workgroups meet in many lanes (a giant SCC's) would make one address hot:
-// each workgroup takes the most frequent shared label of its first 64
-// entries as its hot label and handles it in LDS, with one global access.
+// One pass: the canonical label of v is the low half of ckey[label[v]] (the
+// phases that gave the labels min-reduced their members' keys there, see
+// CanonKeys), its caller-id label the high half, stored at cout[perm[v]] in
+// the degree vertex layout.  A label that many workgroups meet (a giant SCC's)
+// is read once per workgroup.
 
 // The most frequent label held by at least two of entries
 // [blockIdx.x * BLOCK, + 64) that are not self-labelled, or -1.  All threads.
@@ -1698,109 +1749,21 @@ __device__ int32_t block_hot_label(int32_t n, const int32_t* __restrict__ label)
   return s_hot;
 }
 
-// mn = identity (a label is always a vertex labelled with itself), counters
-// zeroed.  With CALLER (the degree vertex layout, perm = internal -> caller
-// ids) each slot is a pair: the minimum internal id and the minimum caller id
-// of the vertices with that label, so the same passes give the caller's
-// canonical labels too.
 template <bool CALLER>
-__global__ __launch_bounds__(BLOCK) void k_canon_init(DevCtl* __restrict__ dc, int32_t* __restrict__ mn,
-                                                      int2* __restrict__ mnc, const int32_t* __restrict__ perm,
-                                                      int32_t n) {
-  for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < n; v += (int64_t)gridDim.x * BLOCK) {
-    if (CALLER) mnc[v] = make_int2((int32_t)v, perm[v]);
-    else mn[v] = (int32_t)v;
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctl_init_body(dc, 1, 0, 0, 0, 0.0);
-}
-
-template <bool CALLER>
-__global__ __launch_bounds__(BLOCK) void k_canon_min(int32_t n, const int32_t* __restrict__ label,
-                                                     int32_t* __restrict__ mn, int2* __restrict__ mnc,
-                                                     const int32_t* __restrict__ perm) {
-  const int lane = lane_id();
+__global__ __launch_bounds__(BLOCK) void k_canon_keys(DevCtl* __restrict__ dc, int32_t n,
+                                                      const int32_t* __restrict__ label,
+                                                      const unsigned long long* __restrict__ ckey,
+                                                      const int32_t* __restrict__ perm,
+                                                      int32_t* __restrict__ out, int32_t* __restrict__ cout) {
   const int32_t hot = block_hot_label(n, label);
-  __shared__ int32_t s_hmin, s_hcmin;
-  if (threadIdx.x == 0) s_hmin = s_hcmin = INT32_MAX;
+  __shared__ unsigned long long s_hk;
+  if (threadIdx.x == 0) s_hk = hot >= 0 ? ckey[hot] : ~0ull;
   __syncthreads();
-  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n; base += (int64_t)gridDim.x * BLOCK) {
-    const int64_t v = base + threadIdx.x;
-    int32_t l = -1, p = INT32_MAX;
-    if (v < n) {
-      l = label[v];
-      if (CALLER) p = perm[v];
-    }
-    // slots start as the label vertex's own ids (a label is always a vertex
-    // labelled with itself), so self-labelled vertices have nothing to contribute
-    bool pending = (uint32_t)l < (uint32_t)n && l != (int32_t)v;
-    if (hot >= 0) {
-      const bool h = pending && l == hot;
-      int32_t x = h ? (int32_t)v : INT32_MAX, y = h ? p : INT32_MAX;
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        x = min(x, __shfl_xor(x, o));
-        if (CALLER) y = min(y, __shfl_xor(y, o));
-      }
-      if (lane == 0 && x < INT32_MAX) {
-        atomicMin(&s_hmin, x);
-        if (CALLER) atomicMin(&s_hcmin, y);
-      }
-      if (h) pending = false;
-    }
-    uint64_t act = __ballot(pending);
-    while (act) {
-      const int leader = __ffsll((unsigned long long)act) - 1;
-      const int32_t L = __shfl(l, leader);
-      const bool mine = pending && l == L;
-      int32_t x = mine ? (int32_t)v : INT32_MAX, y = mine ? p : INT32_MAX;
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        x = min(x, __shfl_xor(x, o));
-        if (CALLER) y = min(y, __shfl_xor(y, o));
-      }
-      if (lane == leader) {
-        if (CALLER) {
-          const int2 cur = mnc[L];
-          if (x < cur.x) atomicMin(&mnc[L].x, x);
-          if (y < cur.y) atomicMin(&mnc[L].y, y);
-        } else if (x < mn[L]) {
-          atomicMin(&mn[L], x);
-        }
-      }
-      if (mine) pending = false;
-      act = __ballot(pending);
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0 && s_hmin < INT32_MAX) {
-    if (CALLER) {
-      const int2 cur = mnc[hot];
-      if (s_hmin < cur.x) atomicMin(&mnc[hot].x, s_hmin);
-      if (s_hcmin < cur.y) atomicMin(&mnc[hot].y, s_hcmin);
-    } else if (s_hmin < mn[hot]) {
-      atomicMin(&mn[hot], s_hmin);
-    }
-  }
-}
-
-// out[v] = canonical label (internal ids); with CALLER also cout[perm[v]] =
-// the minimum caller id of v's component (labels in the caller's ids).
-template <bool CALLER>
-__global__ __launch_bounds__(BLOCK) void k_canon_apply(DevCtl* __restrict__ dc, int32_t n,
-                                                       const int32_t* __restrict__ label,
-                                                       const int32_t* __restrict__ mn,
-                                                       const int2* __restrict__ mnc,
-                                                       const int32_t* __restrict__ perm,
-                                                       int32_t* __restrict__ out, int32_t* __restrict__ cout) {
-  const int32_t hot = block_hot_label(n, label);
-  __shared__ int2 s_hv;
-  if (threadIdx.x == 0) s_hv = hot >= 0 ? (CALLER ? mnc[hot] : make_int2(mn[hot], 0)) : make_int2(-1, -1);
-  __syncthreads();
-  const int2 hv = s_hv;
-  auto canon = [&](int32_t l) -> int2 {
-    if (l == hot) return hv;
-    if ((uint32_t)l >= (uint32_t)n) return make_int2(-1, -1);
-    return CALLER ? mnc[l] : make_int2(mn[l], 0);
+  const unsigned long long hk = s_hk;
+  // -1 (both halves) for a vertex not yet labelled
+  auto key = [&](int32_t l) -> unsigned long long {
+    if (l == hot) return hk;
+    return (uint32_t)l < (uint32_t)n ? ckey[l] : ~0ull;
   };
   unsigned long long self = 0;
   // four labels per lane (16-byte loads and stores; the four gathers are
@@ -1811,24 +1774,25 @@ __global__ __launch_bounds__(BLOCK) void k_canon_apply(DevCtl* __restrict__ dc, 
   int4* __restrict__ out4 = reinterpret_cast<int4*>(out);
   for (int64_t q = (int64_t)blockIdx.x * BLOCK + threadIdx.x; q < n4; q += (int64_t)gridDim.x * BLOCK) {
     const int4 l = label4[q];
-    const int2 c0 = canon(l.x), c1 = canon(l.y), c2 = canon(l.z), c3 = canon(l.w);
-    out4[q] = make_int4(c0.x, c1.x, c2.x, c3.x);
+    const unsigned long long k0 = key(l.x), k1 = key(l.y), k2 = key(l.z), k3 = key(l.w);
+    const int4 c = make_int4((int32_t)k0, (int32_t)k1, (int32_t)k2, (int32_t)k3);
+    out4[q] = c;
     if (CALLER) {
       const int4 pv = perm4[q];
-      cout[pv.x] = c0.y;
-      cout[pv.y] = c1.y;
-      cout[pv.z] = c2.y;
-      cout[pv.w] = c3.y;
+      cout[pv.x] = (int32_t)(k0 >> 32);
+      cout[pv.y] = (int32_t)(k1 >> 32);
+      cout[pv.z] = (int32_t)(k2 >> 32);
+      cout[pv.w] = (int32_t)(k3 >> 32);
     }
     const int32_t v = (int32_t)(4 * q);
-    self += (c0.x == v) + (c1.x == v + 1) + (c2.x == v + 2) + (c3.x == v + 3);
+    self += (c.x == v) + (c.y == v + 1) + (c.z == v + 2) + (c.w == v + 3);
   }
   if (blockIdx.x == 0)
     for (int64_t v = 4 * n4 + threadIdx.x; v < n; v += BLOCK) {
-      const int2 c = canon(label[v]);
-      out[v] = c.x;
-      if (CALLER) cout[perm[v]] = c.y;
-      self += (c.x == (int32_t)v) ? 1 : 0;
+      const unsigned long long k = key(label[v]);
+      out[v] = (int32_t)k;
+      if (CALLER) cout[perm[v]] = (int32_t)(k >> 32);
+      self += ((int32_t)k == (int32_t)v) ? 1 : 0;
     }
   block_count_nowait(self, &dc->ctr[C_COUNT]);
 }
@@ -1857,8 +1821,8 @@ void launch_trim1_step(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, i
   LAUNCH(k_trim1_step, grid, st, dc, G, color, label, L0, L1, fresh, stop, epoch, pkey, zbits, zwords);
 }
 void launch_seed_single(hipStream_t st, DevCtl* dc, int32_t n, const LevelDir& F, const LevelDir& B,
-                        const unsigned long long* pkey) {
-  hipLaunchKernelGGL(k_seed_single, dim3(1), dim3(64), 0, st, dc, n, F, B, pkey);
+                        const unsigned long long* pkey, CanonKeys K) {
+  hipLaunchKernelGGL(k_seed_single, dim3(1), dim3(64), 0, st, dc, n, F, B, pkey, K);
 }
 void launch_compact(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* color, int32_t* L0,
                     int32_t* L1) {
@@ -1897,8 +1861,8 @@ void launch_seed_list(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* 
 }
 void launch_seed_pivots2(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, int multi, const LevelDir& F,
                          const LevelDir& B, const int32_t* color, const unsigned long long* pkey,
-                         const int32_t* L0, const int32_t* L1) {
-  LAUNCH(k_seed_pivots2, grid, st, dc, n, multi, F, B, color, pkey, L0, L1);
+                         const int32_t* L0, const int32_t* L1, CanonKeys K) {
+  LAUNCH(k_seed_pivots2, grid, st, dc, n, multi, F, B, color, pkey, L0, L1, K);
 }
 void launch_level(hipStream_t st, unsigned grid, DevCtl* dc, int multi, const LevelDir& D,
                   const int32_t* color, const int32_t* L0, const int32_t* L1, int32_t max_chunks, int light) {
@@ -1931,8 +1895,8 @@ void launch_fb_stats(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, con
 }
 void launch_classify(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, int32_t* color,
                      int32_t* label, const uint32_t* vf, const uint32_t* vb,
-                     const unsigned long long* pkey, int32_t* rep, int32_t* L0, int32_t* L1) {
-  LAUNCH(k_classify, grid, st, dc, n, color, label, vf, vb, pkey, rep, L0, L1);
+                     const unsigned long long* pkey, int32_t* rep, int32_t* L0, int32_t* L1, CanonKeys K) {
+  LAUNCH(k_classify, grid, st, dc, n, color, label, vf, vb, pkey, rep, L0, L1, K);
 }
 void launch_recolor(hipStream_t st, unsigned grid, const DevCtl* dc, int32_t n, int32_t* color,
                     const uint32_t* vf, const uint32_t* vb, const int32_t* rep, const int32_t* L0,
@@ -1959,39 +1923,21 @@ void launch_wcc_store(hipStream_t st, unsigned grid, const DevCtl* dc, const uns
 }
 void launch_wcc_color(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, const int32_t* W,
                       int seed, const int32_t* off, const int32_t* offe, uint32_t* V, int32_t* q0, const int32_t* L0,
-                      const int32_t* L1) {
-  LAUNCH(k_wcc_color, grid, st, dc, color, W, seed, off, offe, V, q0, L0, L1);
+                      const int32_t* L1, CanonKeys K) {
+  LAUNCH(k_wcc_color, grid, st, dc, color, W, seed, off, offe, V, q0, L0, L1, K);
 }
 void launch_extract(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, int32_t* label,
-                    const uint32_t* V, int32_t* L0, int32_t* L1) {
-  LAUNCH(k_extract, grid, st, dc, color, label, V, L0, L1);
+                    const uint32_t* V, int32_t* L0, int32_t* L1, CanonKeys K) {
+  LAUNCH(k_extract, grid, st, dc, color, label, V, L0, L1, K);
 }
-void launch_canon(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label, int32_t* mn,
-                  int32_t* out) {
-  if (n <= 0) {
-    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, dc, 1, 0, 0ull, 0ll, 0.0);
-    return;
-  }
-  int2* no2 = nullptr;
-  const int32_t* noperm = nullptr;
-  int32_t* nocout = nullptr;
-  LAUNCH(k_canon_init<false>, grid_for(n, 4096), st, dc, mn, no2, noperm, n);
-  LAUNCH(k_canon_min<false>, grid_for(n), st, n, label, mn, no2, noperm);
-  LAUNCH(k_canon_apply<false>, grid_for((n + 3) / 4, 1024), st, dc, n, label, (const int32_t*)mn,
-         (const int2*)no2, noperm, out, nocout);
-}
-
-void launch_canon_caller(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label, int2* mnc,
-                         const int32_t* perm, int32_t* out, int32_t* cout) {
-  if (n <= 0) {
-    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, dc, 1, 0, 0ull, 0ll, 0.0);
-    return;
-  }
-  int32_t* nomn = nullptr;
-  LAUNCH(k_canon_init<true>, grid_for(n, 4096), st, dc, nomn, mnc, perm, n);
-  LAUNCH(k_canon_min<true>, grid_for(n), st, n, label, nomn, mnc, perm);
-  LAUNCH(k_canon_apply<true>, grid_for((n + 3) / 4, 1024), st, dc, n, label, (const int32_t*)nomn,
-         (const int2*)mnc, perm, out, cout);
+void launch_canon(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label, const unsigned long long* ckey,
+                  const int32_t* perm, int32_t* out, int32_t* cout) {
+  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, dc, 1, 0, 0ull, 0ll, 0.0);
+  if (n <= 0) return;
+  if (perm)
+    LAUNCH(k_canon_keys<true>, grid_for((n + 3) / 4, 1024), st, dc, n, label, ckey, perm, out, cout);
+  else
+    LAUNCH(k_canon_keys<false>, grid_for((n + 3) / 4, 1024), st, dc, n, label, ckey, perm, out, cout);
 }
 
 #undef LAUNCH

@@ -141,6 +141,7 @@ extern "C" int scc_run_device_noalloc(int device, void* stream, int32_t n, int64
   c.pkey = k.take<unsigned long long>(N);
   c.rep = k.take<int32_t>(3 * N);
   c.cw = k.take<unsigned long long>(N);
+  c.ckey = k.take<unsigned long long>(N);
   if (k.off > (uint64_t)SCC_NOALLOC_SCRATCH_BYTES) {  // guarded by the static check below
     t_err = "scratch layout exceeds the array";
     return SCC_ERR_LIMIT;
@@ -179,6 +180,6 @@ extern "C" int scc_run_device_noalloc(int device, void* stream, int32_t n, int64
 }
 
 namespace sccmi {
-static_assert((uint64_t)SCC_NOALLOC_MAX_N * 120ull + (2ull << 20) <= (uint64_t)SCC_NOALLOC_SCRATCH_BYTES,
+static_assert((uint64_t)SCC_NOALLOC_MAX_N * 120ull + (2ull << 20) <= (uint64_t)SCC_NOALLOC_SCRATCH_BYTES,  // 112 B/vertex carved
               "no-alloc scratch array too small for SCC_NOALLOC_MAX_N");
 }  // namespace sccmi
