@@ -360,28 +360,6 @@ __device__ __forceinline__ void grouped_max_u64(bool act, int32_t slot, unsigned
   }
 }
 
-// atomicMin counterpart for 64-bit slots.
-__device__ __forceinline__ void grouped_min_u64(bool act, int64_t slot, unsigned long long val,
-                                                unsigned long long* target) {
-  const int lane = lane_id();
-  bool pend = act;
-  uint64_t m = __ballot(pend);
-  while (m) {
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    const int64_t S = __shfl(slot, leader);
-    const bool mine = pend && slot == S;
-    unsigned long long x = mine ? val : ~0ull;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const unsigned long long y = __shfl_xor(x, o);
-      x = y < x ? y : x;
-    }
-    if (lane == leader && x < target[S]) atomicMin(&target[S], x);
-    if (mine) pend = false;
-    m = __ballot(pend);
-  }
-}
-
 // atomicMin counterpart for int32 slots.
 __device__ __forceinline__ void grouped_min_i32(bool act, int64_t slot, int32_t val, int32_t* target) {
   const int lane = lane_id();

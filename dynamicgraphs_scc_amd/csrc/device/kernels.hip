@@ -1276,7 +1276,10 @@ __global__ __launch_bounds__(BLOCK) void k_classify(DevCtl* __restrict__ dc, int
         }
       }
       grouped_min_i32(keep && !hk, rslot, v[t], rep);
-      grouped_min_u64(inscc, piv, inscc ? canon_key(K.perm, v[t]) : 0ull, K.ckey);
+      if (inscc) {  // small SCCs (many labels per wave): a direct atomic, read first
+        const unsigned long long kv = canon_key(K.perm, v[t]);
+        if (kv < K.ckey[piv]) atomicMin(&K.ckey[piv], kv);
+      }
       Out::push(keep, v[t], list_out, &dc->ctr[C_LIST_OUT]);
     }
   }
@@ -1694,7 +1697,10 @@ __global__ __launch_bounds__(BLOCK) void k_extract(DevCtl* __restrict__ dc, int3
         }
       }
     }
-    grouped_min_u64(inscc, c, inscc ? canon_key(K.perm, v) : 0ull, K.ckey);
+    if (inscc) {  // small SCCs (many labels per wave): a direct atomic, read first
+      const unsigned long long kv = canon_key(K.perm, v);
+      if (kv < K.ckey[c]) atomicMin(&K.ckey[c], kv);
+    }
     Out::push(keep, v, list_out, &dc->ctr[C_LIST_OUT]);
   }
 #pragma unroll
