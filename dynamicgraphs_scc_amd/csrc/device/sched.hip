@@ -809,19 +809,30 @@ bool sched_ready(const scc_ctx* c) {
 // forward-backward step) only when its mean vote is far from even.
 constexpr double SCHED_MARGIN = 0.25;
 
+// A step kind repeated right after it resolved less than this share of the
+// live set is replaced by the other kind (a progress guard: a network that
+// keeps choosing forward-backward steps on a graph of many small partitions
+// would otherwise approach the reference's per-partition FB, orders of
+// magnitude slower).
+constexpr double SCHED_MIN_PROGRESS = 0.02;
+
 int sched_choose(scc_ctx* c, int* action) {
   Sched* s = sched_of(c);
   double vote = 0;
   int rc = infer(c, s, false, &vote);
   if (rc) return rc;
-  int plain = 0;
+  int plain = 0, last = -1;
+  double progress = 1.0;
   for (auto it = c->trace.rbegin(); it != c->trace.rend(); ++it) {
     if (it->kind == SCC_PHASE_FB_ROUND || it->kind == SCC_PHASE_COLOR_ROUND) {
-      plain = it->kind == SCC_PHASE_FB_ROUND ? 1 : 0;
+      last = it->kind == SCC_PHASE_FB_ROUND ? 0 : 1;
+      plain = 1 - last;
+      progress = (double)(it->live_before - it->live_after) / (double)std::max<int64_t>(1, it->live_before);
       break;
     }
   }
   *action = vote > 0.5 + SCHED_MARGIN ? 1 : (vote < 0.5 - SCHED_MARGIN ? 0 : plain);
+  if (*action == last && progress < SCHED_MIN_PROGRESS) *action = 1 - last;
   return SCC_OK;
 }
 

@@ -145,14 +145,16 @@ def test_learned_policy_without_weights_falls_back():
 
 
 def test_shipped_weights_gain_over_plain():
-    """The trained network against the method without it, on a held-out planted
-    graph: equal labels, and no slower (measured gain is reported)."""
+    """The trained network against the method without it on a held-out planted
+    graph (n = 8,388,608, seed 3; the training graphs are planted graphs of
+    0.1-1.5 M vertices with other seeds): equal labels, and a gain of at least
+    5 % in the median of 5 runs (measured 1.11-1.12x on one MI355X)."""
     import time
     from dynamicgraphs_scc_amd.engine import PARTS_BOTH
     from dynamicgraphs_scc_amd.sched import load_weights
     w = load_weights()
     assert w is not None
-    g = graphs.planted(1_000_000, seed=4, n_medium=25_000)
+    g = graphs.planted(8_388_608, seed=3)
 
     def timed(e, opts):
         ts = []
@@ -173,4 +175,4 @@ def test_shipped_weights_gain_over_plain():
         tl = timed(e, default_opts(partition_mode=PARTS_LEARNED))
         assert np.array_equal(lp, e.labels())
     print(f"  plain {tp * 1e3:.2f} ms, learned {tl * 1e3:.2f} ms, speedup {tp / tl:.3f}", flush=True)
-    assert tl <= 1.05 * tp
+    assert tp / tl >= 1.05
