@@ -16,6 +16,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "sccmi.h"
@@ -86,11 +87,12 @@ struct scc_server {
   }
 
   // ---- scheduling (called with mu held) ---------------------------------------
-  // Extra bytes an update needs before it can run (0 when within headroom).
-  uint64_t growth_need(const Graph& g, const Request& r, int64_t* new_max) const {
+  // Extra bytes an update of n_ins inserts needs before it can run (0 when
+  // within headroom).
+  uint64_t growth_need(const Graph& g, int64_t n_ins, int64_t* new_max) const {
     *new_max = g.m_max;
-    if (g.m + (int64_t)r.a.size() <= g.m_max) return 0;
-    const int64_t want = g.m + (int64_t)r.a.size();
+    if (g.m + n_ins <= g.m_max) return 0;
+    const int64_t want = g.m + n_ins;
     *new_max = want + (int64_t)(cfg.growth * (double)want);
     const uint64_t r2 = graph_reserve(g.n, g.m, *new_max);
     return r2 > g.reserve ? r2 - g.reserve : 0;
@@ -108,7 +110,7 @@ struct scc_server {
         if (reserved + need > budget) continue;
       } else if (r->kind == SCC_REQ_UPDATE && !g.failed && g.ctx) {
         int64_t nm;
-        const uint64_t extra = growth_need(g, *r, &nm);
+        const uint64_t extra = growth_need(g, (int64_t)r->a.size(), &nm);
         // an update that could never fit runs and fails; one that fits later waits
         if (extra && reserved + extra > budget && g.reserve + extra <= budget) continue;
       }
@@ -128,6 +130,10 @@ struct scc_server {
       Graph& g = graphs[r->graph];
       g.busy = true;
       g.q.pop_front();
+      std::vector<Request*> group{r};
+      std::unique_ptr<Request> merged;
+      if (r->kind == SCC_REQ_UPDATE && !g.failed && g.ctx && cfg.coalesce > 1) merged = coalesce(g, group);
+      Request* run = merged ? merged.get() : r;
       if (r->kind == SCC_REQ_LOAD) {
         loads.pop_front();
         g.m_max = r->m + (int64_t)(cfg.growth * (double)r->m);
@@ -137,7 +143,7 @@ struct scc_server {
       int64_t grow_to = g.m_max;
       uint64_t extra = 0;
       if (r->kind == SCC_REQ_UPDATE && !g.failed && g.ctx) {
-        extra = growth_need(g, *r, &grow_to);
+        extra = growth_need(g, (int64_t)run->a.size(), &grow_to);
         if (extra && reserved + extra <= budget) {
           reserved += extra;
           g.reserve += extra;
@@ -146,29 +152,44 @@ struct scc_server {
           extra = UINT64_MAX;  // cannot fit even alone
         }
       }
-      r->status = SCC_REQ_RUNNING;
-      r->info.start_ms = now();
-      queued--;
-      running++;
+      const double t_start = now();
+      for (Request* x : group) {
+        x->status = SCC_REQ_RUNNING;
+        x->info.start_ms = t_start;
+        x->info.group = r->kind == SCC_REQ_UPDATE ? (int32_t)group.size() : 0;
+        queued--;
+      }
+      running++;  // one device request, whatever the group size
       max_running = std::max(max_running, running);
       const int fault = g.pending_fault;
       g.pending_fault = INT_MIN;
       lk.unlock();
 
-      const int rc = execute(g, *r, fault, extra == UINT64_MAX);
+      const int rc = execute(g, *run, fault, extra == UINT64_MAX);
 
       lk.lock();
-      r->info.end_ms = now();
-      r->info.ms_latency = r->info.end_ms - r->info.arrival_ms;
-      r->info.status = r->status = rc;
+      const double t_end = now();
       running--;
-      if (rc == SCC_OK) completed++;
-      else failed++;
+      for (Request* x : group) {
+        if (x != run) {
+          x->info.num_sccs = run->info.num_sccs;
+          x->info.inserted = run->info.inserted;
+          x->info.deleted = run->info.deleted;
+          x->info.affected = run->info.affected;
+        }
+        x->info.end_ms = t_end;
+        x->info.ms_latency = t_end - x->info.arrival_ms;
+        x->info.status = x->status = rc;
+        if (rc == SCC_OK) completed++;
+        else failed++;
+        if (x->kind == SCC_REQ_UPDATE && rc == SCC_OK) {
+          upd_lat.push_back(x->info.ms_latency);
+          if (upd_first < 0 || x->info.arrival_ms < upd_first) upd_first = x->info.arrival_ms;
+        }
+      }
       if (r->kind == SCC_REQ_UPDATE && rc == SCC_OK) {
-        upd_lat.push_back(r->info.ms_latency);
-        upd_edges += r->info.inserted + r->info.deleted;
-        if (upd_first < 0 || r->info.arrival_ms < upd_first) upd_first = r->info.arrival_ms;
-        upd_last = std::max(upd_last, r->info.end_ms);
+        upd_edges += run->info.inserted + run->info.deleted;  // a group's edges once
+        upd_last = std::max(upd_last, t_end);
       }
       if (r->kind == SCC_REQ_LOAD) {
         if (rc == SCC_OK) g.m = r->m;
@@ -188,16 +209,87 @@ struct scc_server {
         reserved -= g.reserve;
         g.reserve = 0;
       }
-      // drop bulk inputs; the request stays queryable
-      std::vector<int32_t>().swap(r->a);
-      std::vector<int32_t>().swap(r->b);
-      std::vector<int32_t>().swap(r->c);
-      std::vector<int32_t>().swap(r->d);
+      // drop bulk inputs; the requests stay queryable
+      for (Request* x : group) {
+        std::vector<int32_t>().swap(x->a);
+        std::vector<int32_t>().swap(x->b);
+        std::vector<int32_t>().swap(x->c);
+        std::vector<int32_t>().swap(x->d);
+      }
+      merged.reset();
       g.busy = false;
       if (r->kind == SCC_REQ_UNLOAD && g.q.empty()) graphs.erase(r->graph);
       cv_done.notify_all();
       cv_work.notify_all();
     }
+  }
+
+  // ---- coalescing (mu held) ------------------------------------------------------
+  static bool ids_in_range(const Request& x, int32_t n) {
+    for (const std::vector<int32_t>* v : {&x.a, &x.b, &x.c, &x.d})
+      for (int32_t u : *v)
+        if (u < 0 || u >= n) return false;
+    return true;
+  }
+
+  // Moves the UPDATEs queued right behind group[0] on g into the group and
+  // returns the one update that applies them all, or nullptr when group[0]
+  // stays alone.  Per edge key the last member's intent wins; within a member
+  // an insert beats a delete (scc_update drops a delete its own batch
+  // inserts); inserted self-loops carry no intent (scc_update ignores them).
+  // A member that inserts a key an earlier member deletes is left for the
+  // next group: one by one, a duplicated edge deleted and inserted again keeps
+  // one copy, while one batch holding the insert would keep all copies.
+  std::unique_ptr<Request> coalesce(Graph& g, std::vector<Request*>& group) {
+    Request* r = group[0];
+    if (g.q.empty() || g.q.front()->kind != SCC_REQ_UPDATE || !ids_in_range(*r, g.n)) return nullptr;
+    const int64_t n = g.n;
+    enum : uint8_t { DEL = 1, INS = 2 };
+    std::unordered_map<int64_t, uint8_t> last;  // key -> last intent
+    int64_t n_ins = 0, n_del = 0;  // upper bounds of the merged lists
+    auto admit = [&](const Request& x, bool check) {
+      if (check) {
+        if (n_ins + (int64_t)x.a.size() > cfg.max_batch || n_del + (int64_t)x.c.size() > cfg.max_batch)
+          return false;
+        if (!ids_in_range(x, g.n)) return false;
+        for (size_t i = 0; i < x.a.size(); i++) {
+          if (x.a[i] == x.b[i]) continue;
+          auto it = last.find((int64_t)x.a[i] * n + x.b[i]);
+          if (it != last.end() && it->second == DEL) return false;
+        }
+        int64_t nm;
+        const uint64_t extra = growth_need(g, n_ins + (int64_t)x.a.size(), &nm);
+        if (extra && reserved + extra > budget) return false;
+      }
+      for (size_t i = 0; i < x.c.size(); i++) last[(int64_t)x.c[i] * n + x.d[i]] = DEL;
+      for (size_t i = 0; i < x.a.size(); i++)
+        if (x.a[i] != x.b[i]) last[(int64_t)x.a[i] * n + x.b[i]] = INS;
+      n_ins += (int64_t)x.a.size();
+      n_del += (int64_t)x.c.size();
+      return true;
+    };
+    admit(*r, false);
+    while ((int)group.size() < cfg.coalesce && !g.q.empty() && g.q.front()->kind == SCC_REQ_UPDATE &&
+           admit(*g.q.front(), true)) {
+      group.push_back(g.q.front());
+      g.q.pop_front();
+    }
+    if (group.size() == 1) return nullptr;
+    auto m = std::make_unique<Request>();
+    m->kind = SCC_REQ_UPDATE;
+    m->graph = r->graph;
+    m->ticket = r->ticket;
+    for (const auto& kv : last) {
+      const int32_t u = (int32_t)(kv.first / n), v = (int32_t)(kv.first % n);
+      if (kv.second == INS) {
+        m->a.push_back(u);
+        m->b.push_back(v);
+      } else {
+        m->c.push_back(u);
+        m->d.push_back(v);
+      }
+    }
+    return m;
   }
 
   // ---- execution (mu not held; the graph is marked busy) ------------------------
@@ -303,7 +395,7 @@ void scc_server_config_default(scc_server_config* c) {
   c->max_batch = 100000;
   c->growth = 0.25;
   c->layout = SCC_LAYOUT_INPUT;
-  c->reserved1 = 0;
+  c->coalesce = 32;
 }
 
 int scc_server_create(const scc_server_config* cfg, scc_server** out) {
@@ -313,7 +405,7 @@ int scc_server_create(const scc_server_config* cfg, scc_server** out) {
   if (cfg) c = *cfg;
   else scc_server_config_default(&c);
   if (c.slots < 1 || c.slots > 64 || c.queue_limit < 1 || c.max_batch < 1 || c.growth < 0 ||
-      c.growth > 16 || c.layout < SCC_LAYOUT_INPUT || c.layout > SCC_LAYOUT_AUTO) {
+      c.growth > 16 || c.layout < SCC_LAYOUT_INPUT || c.layout > SCC_LAYOUT_AUTO || c.coalesce > (1 << 20)) {
     t_err = "bad server configuration";
     return SCC_ERR_ARG;
   }

@@ -33,17 +33,17 @@ class ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("queue_limit", C.c_int32),
                 ("pivot_policy", C.c_int32), ("hbm_budget_bytes", C.c_uint64),
                 ("max_batch", C.c_int64), ("growth", C.c_double), ("layout", C.c_int32),
-                ("reserved1", C.c_int32)]
+                ("coalesce", C.c_int32)]
 
 
 class RequestInfo(C.Structure):
     _fields_ = [("kind", C.c_int32), ("status", C.c_int32), ("graph", C.c_int32),
-                ("reserved0", C.c_int32), ("num_sccs", C.c_int64), ("inserted", C.c_int64),
+                ("group", C.c_int32), ("num_sccs", C.c_int64), ("inserted", C.c_int64),
                 ("deleted", C.c_int64), ("affected", C.c_int64), ("arrival_ms", C.c_double),
                 ("start_ms", C.c_double), ("end_ms", C.c_double), ("ms_latency", C.c_double)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved0"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class ServerStats(C.Structure):
@@ -106,13 +106,14 @@ class SccServer:
 
     def __init__(self, device: int = 0, slots: int = 4, hbm_budget_bytes: int = 0,
                  max_batch: int = 100_000, growth: float = 0.25, queue_limit: int = 1 << 16,
-                 pivot_policy: int = PIVOT_DEGREE_PRODUCT, layout: int = 0):
+                 pivot_policy: int = PIVOT_DEGREE_PRODUCT, layout: int = 0, coalesce: int = 32):
         self.lib = load_serve_library()
         cfg = ServerConfig()
         self.lib.scc_server_config_default(C.byref(cfg))
         cfg.device, cfg.slots, cfg.queue_limit = device, slots, queue_limit
         cfg.pivot_policy, cfg.hbm_budget_bytes = pivot_policy, hbm_budget_bytes
         cfg.max_batch, cfg.growth, cfg.layout = max_batch, growth, layout
+        cfg.coalesce = coalesce
         h = C.c_void_p()
         rc = self.lib.scc_server_create(C.byref(cfg), C.byref(h))
         if rc != 0:

@@ -19,6 +19,18 @@
  * beyond its headroom reserves more; memory freed by UNLOAD (or by failed
  * loads) admits waiting requests.  LOADs are admitted in arrival order.
  *
+ * Coalescing: when an UPDATE starts, the UPDATEs queued right behind it on the
+ * same graph (no other request between them) are merged into the same device
+ * update, up to `coalesce` requests and max_batch edges per list.  Per edge
+ * the last request's intent wins (within one request an insert beats a
+ * delete, as in scc_update; inserted self-loops carry none), so the graph and
+ * labels after the group equal those after applying its requests one by one.
+ * A group ends before a request with an out-of-range vertex id (it runs alone
+ * and fails alone), before one whose edges would exceed max_batch or the HBM
+ * budget, and before one that inserts an edge an earlier member deletes (a
+ * duplicated edge then keeps one copy one by one, all copies in one batch).
+ * Every member reports its own latency; members share start and end times.
+ *
  * Latency of a request = arrival (the submit time, or the arrival time the
  * caller passes, on the server clock scc_server_now_ms) to labels final
  * (the update's results are on the device and synchronised).
@@ -56,16 +68,17 @@ typedef struct scc_server_config {
   int64_t max_batch;         /* largest update list (inserts or deletes) accepted */
   double growth;             /* edge headroom reserved per graph, as a fraction of its edges */
   int32_t layout;            /* scc_layout of every loaded graph (SCC_LAYOUT_INPUT by default) */
-  int32_t reserved1;
+  int32_t coalesce;          /* most UPDATE requests merged into one device update (<= 1: none; default 32) */
 } scc_server_config;
 
 typedef struct scc_request_info {
   int32_t kind;      /* scc_request_kind */
   int32_t status;    /* SCC_REQ_DONE / QUEUED / RUNNING or a negative scc_status */
   int32_t graph;
-  int32_t reserved0;
+  int32_t group;     /* UPDATE: requests applied by the same device update (1 when alone) */
   int64_t num_sccs;  /* LOAD / UPDATE: SCCs after the request */
-  int64_t inserted, deleted, affected;  /* UPDATE: effective changes, region size */
+  int64_t inserted, deleted, affected;  /* UPDATE: effective changes, region size (of the whole
+                                           group when group > 1) */
   double arrival_ms, start_ms, end_ms;  /* server clock */
   double ms_latency;                    /* end - arrival */
 } scc_request_info;

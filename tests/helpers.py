@@ -79,3 +79,20 @@ def same_partition(a, b) -> bool:
     """True when two labelings induce the same partition."""
     return np.array_equal(graphs.canonicalize(np.unique(a, return_inverse=True)[1].astype(np.int32)),
                           graphs.canonicalize(np.unique(b, return_inverse=True)[1].astype(np.int32)))
+
+
+def apply_update_host(n, src, dst, ins, dels):
+    """The reference's update semantics on a multigraph edge list: a delete drops
+    every copy of its edge (src/common_main.h:379-386), an insert adds an edge
+    that is absent (once, whatever the batch repeats), inserted self-loops are
+    ignored, and a delete whose edge the same batch inserts is dropped."""
+    keys = src.astype(np.int64) * n + dst
+    ik = np.unique(ins[0].astype(np.int64) * n + ins[1])
+    ik = ik[(ik // n) != (ik % n)]
+    dk = np.setdiff1d(np.unique(dels[0].astype(np.int64) * n + dels[1]), ik)
+    present = np.isin(ik, keys)
+    ik = ik[~present]
+    keep = ~np.isin(keys, dk)
+    removed = int((~keep).sum())
+    k2 = np.concatenate([keys[keep], ik])
+    return (k2 // n).astype(np.int32), (k2 % n).astype(np.int32), removed

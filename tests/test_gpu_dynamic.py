@@ -7,7 +7,7 @@ import pytest
 import oracle
 from dynamicgraphs_scc_amd import graphs
 from dynamicgraphs_scc_amd.engine import SccEngine, SccError
-from helpers import scipy_canonical
+from helpers import apply_update_host, scipy_canonical
 
 pytestmark = pytest.mark.gpu
 
@@ -157,23 +157,6 @@ def test_rebuild_with_heavy_row_groups(kind):
         us.close()
 
 
-def _apply_host(n, src, dst, ins, dels):
-    """The reference's update semantics on a multigraph edge list: a delete drops
-    every copy of its edge (src/common_main.h:379-386), an insert adds an edge
-    that is absent (once, whatever the batch repeats), inserted self-loops are
-    ignored, and a delete whose edge the same batch inserts is dropped."""
-    keys = src.astype(np.int64) * n + dst
-    ik = np.unique(ins[0].astype(np.int64) * n + ins[1])
-    ik = ik[(ik // n) != (ik % n)]
-    dk = np.setdiff1d(np.unique(dels[0].astype(np.int64) * n + dels[1]), ik)
-    present = np.isin(ik, keys)
-    ik = ik[~present]
-    keep = ~np.isin(keys, dk)
-    removed = int((~keep).sum())
-    k2 = np.concatenate([keys[keep], ik])
-    return (k2 // n).astype(np.int32), (k2 % n).astype(np.int32), removed
-
-
 @pytest.mark.parametrize("layout", ["input", "degree"])
 def test_delete_duplicated_edges(layout):
     """A graph loaded with duplicate edges (dedup off, as the reference's loader
@@ -207,7 +190,7 @@ def test_delete_duplicated_edges(layout):
         pick = rng.choice(dup, 600, replace=False)
         dels = ((pick // n).astype(np.int32), (pick % n).astype(np.int32))
         ins = (rng.integers(0, n, 400).astype(np.int32), rng.integers(0, n, 400).astype(np.int32))
-        src, dst, removed = _apply_host(n, src, dst, ins, dels)
+        src, dst, removed = apply_update_host(n, src, dst, ins, dels)
         st = e.update(ins[0], ins[1], dels[0], dels[1])
         cur = graphs.from_edges(n, src, dst, dedup=False)
         assert st["deleted"] == removed and removed > 600
@@ -218,7 +201,7 @@ def test_delete_duplicated_edges(layout):
         dels = (np.full(10, hub, np.int32), hub_nb[:10].astype(np.int32))
         new = np.setdiff1d(np.arange(n, dtype=np.int32), hub_nb)[::23][:120]
         ins = (np.full(new.size, hub, np.int32), new.astype(np.int32))
-        src, dst, removed = _apply_host(n, src, dst, ins, dels)
+        src, dst, removed = apply_update_host(n, src, dst, ins, dels)
         st = e.update(ins[0], ins[1], dels[0], dels[1])
         cur = graphs.from_edges(n, src, dst, dedup=False)
         assert st["deleted"] == removed and st["inserted"] == new.size
@@ -229,7 +212,7 @@ def test_delete_duplicated_edges(layout):
         pick = rng.choice(src0.size, 300, replace=False)
         dels = (src0[pick], dst0[pick])
         ins = (rng.integers(0, n, 300).astype(np.int32), rng.integers(0, n, 300).astype(np.int32))
-        src, dst, removed = _apply_host(n, src, dst, ins, dels)
+        src, dst, removed = apply_update_host(n, src, dst, ins, dels)
         st = e.update(ins[0], ins[1], dels[0], dels[1])
         cur = graphs.from_edges(n, src, dst, dedup=False)
         assert st["deleted"] == removed

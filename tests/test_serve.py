@@ -16,7 +16,7 @@ def _have_device():
 
 def test_bad_configuration_rejected():
     for kw in ({"slots": 0}, {"slots": 65}, {"max_batch": 0}, {"growth": -1.0}, {"queue_limit": 0},
-               {"layout": 7}):
+               {"layout": 7}, {"coalesce": (1 << 20) + 1}):
         with pytest.raises(SccError) as ei:
             serve.SccServer(**kw)
         assert ei.value.code == -1, kw
@@ -35,4 +35,32 @@ def test_default_config():
     cfg = serve.ServerConfig()
     lib.scc_server_config_default(ctypes.byref(cfg))
     assert cfg.slots >= 1 and cfg.max_batch == 100000 and cfg.hbm_budget_bytes == 0
-    assert cfg.layout == 0
+    assert cfg.layout == 0 and cfg.coalesce == 32
+
+
+def test_update_host_model_matches_brute_force():
+    """The host model of one update that the merged-update server test and the
+    dynamic tests check against, against a per-edge loop over a small
+    multigraph: deletes drop every copy unless the batch inserts the edge,
+    inserts add an absent edge once, inserted self-loops do nothing."""
+    import numpy as np
+
+    from helpers import apply_update_host
+    rng = np.random.default_rng(5)
+    n = 12
+    for _ in range(50):
+        src = rng.integers(0, n, 40).astype(np.int32)
+        dst = rng.integers(0, n, 40).astype(np.int32)
+        ins = (rng.integers(0, n, 8).astype(np.int32), rng.integers(0, n, 8).astype(np.int32))
+        dels = (rng.integers(0, n, 8).astype(np.int32), rng.integers(0, n, 8).astype(np.int32))
+        edges = list(zip(src.tolist(), dst.tolist()))
+        inserting = {(u, v) for u, v in zip(*[x.tolist() for x in ins]) if u != v}
+        gone = {(u, v) for u, v in zip(*[x.tolist() for x in dels])} - inserting
+        kept = [e for e in edges if e not in gone]
+        removed = len(edges) - len(kept)
+        for e in sorted(inserting):
+            if e not in kept:
+                kept.append(e)
+        s2, d2, r2 = apply_update_host(n, src, dst, ins, dels)
+        assert r2 == removed
+        assert sorted(zip(s2.tolist(), d2.tolist())) == sorted(kept)
