@@ -100,6 +100,24 @@ struct Tune {
   int32_t pad;
 };
 
+// A decomposition queued as one chain of launches (no host read-back between
+// its phases): a gate launch after each phase checks that the phase finished
+// within the launches queued for it.  When it did not (or when the chain's
+// last phase leaves vertices live), the gate stops the chain: halt = 2 * gate
+// + 1, and every later guarded launch returns at once, so the state is the one
+// the gate saw and the host resumes from there.  halt = 2: no vertex is live,
+// so the later phases are skipped but canonicalisation runs.
+constexpr int CHAIN_GATES = 6;
+struct ChainCtl {
+  int32_t halt;
+  int32_t gate;  // last gate reached
+  int32_t ok[CHAIN_GATES];         // the gate's phase had finished
+  int32_t iters[CHAIN_GATES];      // trim loop: sweeps; search: levels (fw + bw)
+  unsigned long long live[CHAIN_GATES];   // live vertices at the gate
+  unsigned long long count[CHAIN_GATES];  // trimmed (trim loops, trim2) or SCC members (search)
+  unsigned long long pivot;        // the global step's pivot
+};
+
 struct DevCtl {
   Tune tune;
   LiveCtl live;
@@ -115,7 +133,11 @@ struct DevCtl {
   LevelCtl lv2;
   uint32_t done2[9 * 16];
   unsigned long long lvlog2[64][4];
+  ChainCtl chain;
 };
+
+// Guard of a launch that may be queued in a chain: true when it must return.
+__device__ __forceinline__ bool chain_halted(const DevCtl* __restrict__ dc) { return dc->chain.halt != 0; }
 
 // The arrays one reachability direction reads and writes: rows to expand
 // (off/offe/nb), the opposite rows a bottom-up level scans (roff/roffe/rnb), the
@@ -476,6 +498,15 @@ struct scc_ctx {
   int32_t lv_hint[3] = {8, 8, 8};         // level launches per search batch (fw, bw, both)
   int32_t it_hint = 4;                    // WCC iterations per batch
   int32_t trim_hint[2] = {3, 3};          // trim sweeps per batch (absolute stop, relative stop)
+  // the queued chain of a fresh decomposition (ChainCtl), sized by the last one:
+  // its last gate (0 first trim loop, 1 global search, 2 second trim loop, 3
+  // trim2 and the third trim loop), sweeps per trim loop, level launches, and
+  // live vertices at each gate (launch grids)
+  int32_t chain_shape = 3;
+  int32_t chain_sweeps[3] = {1, 1, 1};
+  int32_t chain_levels = 8;
+  int64_t chain_live[sccmi::CHAIN_GATES] = {-1, -1, -1, -1, -1, -1};
+  bool chain_off = false;                 // scc_ctx_set_chain(ctx, 0)
   int32_t trim_epoch = 0;                 // number of the last trim loop queued
   bool labels_ready = false;              // canon[] valid for the current graph
   bool ran = false;

@@ -503,6 +503,28 @@ int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int6
   return SCC_OK;
 }
 
+// The rest of a global step whose searches a queued chain left open (they
+// continue from the device state): more level launches, then the
+// classification behind them.
+static int fb_continue(scc_ctx* c, int64_t* scc, int64_t* levels) {
+  c->fresh = false;
+  PhaseRec pr(c, SCC_PHASE_FB_GLOBAL);
+  const unsigned grid = grid_for(c->n_live, (unsigned)c->tune.grid_cap);
+  auto post = [&]() {
+    launch_classify(c->stream, grid, c->dc, c->n, c->color, c->label, c->vf, c->vb, c->pkey, c->rep, c->live[0],
+                    c->live[1], ck(c));
+    launch_recolor(c->stream, grid, c->dc, c->n, c->color, c->vf, c->vb, c->rep, c->live[0], c->live[1]);
+  };
+  int64_t lv = 0;
+  const int rc = level_loop(c, true, true, false, &lv, false, post);
+  if (rc) return rc;
+  *scc = (int64_t)c->h_dc->ctr[C_SCC];
+  *levels = lv;
+  c->colors_split = true;
+  pr.finish(lv);
+  return SCC_OK;
+}
+
 // WCC colouring of every live partition (W to its fixpoint, colour = root).
 // With seed, the roots also seed the forward search of a colouring round.
 static int wcc_fixpoint(scc_ctx* c, bool seed) {
@@ -621,6 +643,149 @@ int do_canon(scc_ctx* c, int64_t* count) {
   return SCC_OK;
 }
 
+// ---- a fresh decomposition queued as one chain -------------------------------------
+// Stages of method 2 in order; run_method2 runs them from a given one.
+enum Stage : int { ST_TRIM_FIRST = 0, ST_FB, ST_FB_CONTINUE, ST_TRIM_SECOND, ST_TRIM2, ST_TRIM_THIRD, ST_WCC, ST_DONE };
+
+// Queue the trims, the global step, the trims after it and the canonical labels
+// behind each other with a gate after each phase (ChainCtl), read the control
+// block back once, and return the stage the host continues from: ST_DONE when
+// the chain ran to its end (labels final), else the stage after the gate that
+// stopped it.  The chain's length and each phase's launches come from the last
+// decomposition on this context; a phase that needs more launches stops the
+// chain at its gate and continues with host read-backs as usual.
+static int run_chain(scc_ctx* c, const scc_run_opts& o, scc_run_stats* st, int* resume) {
+  const int shape = c->chain_shape;
+  auto grid_at = [&](int gate) {
+    const int64_t live = gate < 0 || c->chain_live[gate] < 0 ? c->n : c->chain_live[gate];
+    return grid_for(std::max<int64_t>(live, 1), (unsigned)c->tune.grid_cap);
+  };
+  const int64_t words = ((int64_t)c->n + 31) / 32;
+  const long long stop = (long long)o.trim_stop;
+  auto trim_loop = [&](int gate, int k) {
+    const int32_t epoch = ++c->trim_epoch;
+    for (int i = 0; i < c->chain_sweeps[k]; i++) {
+      const int fresh = c->fresh ? 2 : 0;
+      launch_trim1_step(c->stream, grid_at(gate - 1), c->dc, gp(c), c->color, c->label, c->live[0], c->live[1], fresh,
+                        stop, epoch, c->pkey, c->vf, 2 * words);
+      c->fresh = false;
+    }
+    return epoch;
+  };
+  c->reset_pending = false;
+  c->pivot_ready = false;
+  auto t0 = std::chrono::steady_clock::now();
+  // gate 0: the first trim loop; its first sweep does the deferred reset and
+  // leaves the global pivot key, which stands when it is the loop's only sweep
+  const int32_t e0 = trim_loop(0, 0);
+  launch_chain_gate(c->stream, c->dc, 0, GATE_TRIM, e0, 1, shape == 0);
+  if (shape >= 1) {  // gate 1: the global forward-backward step
+    const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
+    const unsigned grid = grid_at(0);
+    launch_fb_setup(c->stream, c->dc, c->n, c->pkey, c->rep, 2, avg / c->tune.bu_alpha);
+    const LevelDir F = level_dir(c, true, true), B = level_dir(c, false, true);
+    launch_seed_single(c->stream, c->dc, c->n, F, B, c->pkey, ck(c));
+    const int32_t max_chunks = std::max<int32_t>(1, (c->maxdeg + 2047) / 2048);
+    for (int i = 0; i < c->chain_levels; i++)
+      launch_level2(c->stream, 2 * grid, c->dc, 0, F, B, c->color, c->live[0], c->live[1], max_chunks, c->tune.light);
+    launch_classify(c->stream, grid, c->dc, c->n, c->color, c->label, c->vf, c->vb, c->pkey, c->rep, c->live[0],
+                    c->live[1], ck(c));
+    launch_recolor(c->stream, grid, c->dc, c->n, c->color, c->vf, c->vb, c->rep, c->live[0], c->live[1]);
+    launch_chain_gate(c->stream, c->dc, 1, GATE_SEARCH, 0, 0, shape == 1);
+  }
+  if (shape >= 2) {  // gate 2: trim1 after the step
+    const int32_t e2 = trim_loop(2, 1);
+    launch_chain_gate(c->stream, c->dc, 2, GATE_TRIM, e2, 0, shape == 2);
+  }
+  if (shape >= 3 && o.use_trim2) {  // gates 3 and 4: trim2, then trim1
+    launch_ctl_init(c->stream, c->dc, 1, 0, 0, 0, 0.0);
+    launch_trim2(c->stream, grid_at(2), c->dc, gp(c), c->color, c->label, c->live[0], c->live[1]);
+    launch_chain_gate(c->stream, c->dc, 3, GATE_TRIM2, 0, 0, 0);
+    const int32_t e4 = trim_loop(4, 2);
+    launch_chain_gate(c->stream, c->dc, 4, GATE_TRIM, e4, 0, 1);
+  }
+  // canonical labels: skipped by a chain that stopped (halt odd)
+  launch_canon(c->stream, c->dc, c->n, c->label, c->ckey, c->relabeled ? c->perm : nullptr, c->canon,
+               c->canon_caller);
+  int rc = pull_ctl(c);
+  if (rc) return rc;
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  const ChainCtl ch = c->h_dc->chain;
+  const int last = ch.gate;  // the last gate that ran
+  // (the last gate queued always decides: halt 2, or odd at a gate that ran)
+  if (last < 0 || last >= CHAIN_GATES || (ch.halt != 2 && (ch.halt & 1) == 0))
+    return set_error(c, SCC_ERR_STATE, "queued chain ended without a decision");
+  // what the gates saw
+  st->trimmed_first = last >= 0 ? (int64_t)ch.count[0] : 0;
+  if (last >= 1) {
+    st->scc_first = (int64_t)ch.count[1];
+    st->pivot_first = (int32_t)ch.pivot;
+    st->fb_levels = ch.iters[1];
+  }
+  st->trimmed_second = (last >= 2 ? (int64_t)ch.count[2] : 0) + (last >= 3 ? (int64_t)ch.count[3] : 0) +
+                       (last >= 4 ? (int64_t)ch.count[4] : 0);
+  for (int g = 0; g <= last && g < CHAIN_GATES; g++) c->chain_live[g] = (int64_t)ch.live[g];
+  // the phase trace (the chain synchronises once: its wall time goes to the last record)
+  static const int kinds[CHAIN_GATES] = {SCC_PHASE_TRIM1, SCC_PHASE_FB_GLOBAL, SCC_PHASE_TRIM1, SCC_PHASE_TRIM2,
+                                         SCC_PHASE_TRIM1, 0};
+  int64_t before = c->n;
+  for (int g = 0; g <= last && g < 5 && c->trace.size() < 65536 && !c->no_alloc; g++) {
+    scc_phase_rec r;
+    r.kind = kinds[g];
+    r.round = -1;
+    r.live_before = before;
+    r.live_after = (int64_t)ch.live[g];
+    r.aux = g == 3 ? (int64_t)ch.count[3] : ch.iters[g];
+    r.ms = g == last ? ms : 0.0;
+    c->trace.push_back(r);
+    before = r.live_after;
+  }
+  if (ch.halt == 2) {  // ran to its end: every vertex decided, labels final
+    if (!c->no_alloc && c->trace.size() < 65536) {
+      scc_phase_rec r{SCC_PHASE_CANON, -1, 0, 0, (int64_t)c->h_dc->ctr[C_COUNT], 0.0};
+      c->trace.push_back(r);
+    }
+    st->num_sccs = (int64_t)c->h_dc->ctr[C_COUNT];
+    c->labels_ready = true;
+    c->colors_split = last >= 1;
+    // next time, stop where the vertices ran out
+    c->chain_shape = last >= 3 ? 3 : last;
+    if (last >= 1) c->chain_levels = std::min(64, std::max(2, std::max(c->h_dc->lv.levels, c->h_dc->lv2.levels)) + 1);
+    launch_ctl_init(c->stream, c->dc, 64, 0, 0, 0, 0.0);  // (no halt for later calls)
+    *resume = ST_DONE;
+    st->chain_stop = 5;
+    return hip_check(c, hipGetLastError(), "chain");
+  }
+  // stopped at gate `last` (halt odd): continue from there
+  launch_ctl_init(c->stream, c->dc, 64, 0, 0, 0, 0.0);
+  st->chain_stop = last;
+  const bool ok = ch.ok[last] != 0;
+  const LoopCtl& l = c->h_dc->loop;
+  switch (last) {
+    case 0:
+      // the fused sweep's pivot key stands when it alone ended the loop
+      c->pivot_ready = ok;
+      *resume = ok || l.done ? ST_FB : ST_TRIM_FIRST;
+      if (ok) c->chain_shape = 3;
+      break;
+    case 1:
+      c->colors_split = true;
+      *resume = ok ? ST_TRIM_SECOND : ST_FB_CONTINUE;
+      if (ok) c->chain_shape = 3;
+      break;
+    case 2:
+      *resume = ok ? ST_TRIM2 : ST_TRIM_SECOND;
+      c->colors_split = true;
+      if (ok) c->chain_shape = 3;
+      break;
+    default:  // gate 4 (gate 3 never stops the chain)
+      c->colors_split = true;
+      *resume = ok ? ST_WCC : ST_TRIM_THIRD;
+      break;
+  }
+  return hip_check(c, hipGetLastError(), "chain");
+}
+
 }  // namespace sccmi
 
 // ---------------------------------------------------------------------------------
@@ -647,7 +812,7 @@ int scc_ctx_create(int device, uint64_t hbm_budget_bytes, scc_ctx** out) {
   c->h_ctr = c->h_dc->ctr;
   std::memset(c->h_dc, 0, sizeof(DevCtl));
   for (auto& e : c->ev) (void)hipEventCreate(&e);
-  launch_ctl_init(c->stream, c->dc, 1 | 2 | 4 | 8 | 16, 0, 0, 0, 0.0);
+  launch_ctl_init(c->stream, c->dc, 1 | 2 | 4 | 8 | 16 | 64, 0, 0, 0, 0.0);
   if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "init"))) {
     scc_ctx_destroy(c);
     return rc;
@@ -756,7 +921,10 @@ int scc_graph_upload(scc_ctx* c, int32_t n, int64_t m, const int32_t* begin, con
     return rc;
   }
   if (!c->relabeled && (rc = build_order(c))) return rc;
-  if ((rc = reset_state(c))) return rc;
+  // the fresh state is deferred like scc_state_reset's: the next call makes it,
+  // or scc_run_method2 fuses it into its first sweep
+  reset_host_state(c);
+  c->reset_pending = true;
   c->ran = false;
   return hip_check(c, hipStreamSynchronize(c->stream), "upload sync");
 }
@@ -790,7 +958,10 @@ int scc_graph_upload_device(scc_ctx* c, int32_t n, int64_t m, const int32_t* d_b
     return rc;
   }
   if (!c->relabeled && (rc = build_order(c))) return rc;
-  if ((rc = reset_state(c))) return rc;
+  // the fresh state is deferred like scc_state_reset's: the next call makes it,
+  // or scc_run_method2 fuses it into its first sweep
+  reset_host_state(c);
+  c->reset_pending = true;
   c->ran = false;
   return hip_check(c, hipStreamSynchronize(c->stream), "upload sync");
 }
@@ -974,6 +1145,7 @@ int run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_out) 
   scc_run_stats st;
   std::memset(&st, 0, sizeof(st));
   st.pivot_first = -1;
+  st.chain_stop = -1;
   c->labels_ready = false;
   c->kernel_ms = 0;
   c->kernel_launches = 0;
@@ -992,35 +1164,64 @@ int run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_out) 
     }
     return ms;
   };
+  // A fresh decomposition on the plain schedule queues its phases as one chain
+  // (run_chain) and continues here only when the chain stopped early.
+  int from = ST_TRIM_FIRST;
+  const bool chain = !c->chain_off && !tp && !c->kernel_timing && c->reset_pending && c->fresh && c->n > 0 &&
+                     !c->colors_split && !c->multi_colors && opts.pivot_policy == SCC_PIVOT_DEGREE_PRODUCT &&
+                     opts.partition_mode != SCC_PARTS_LEARNED;
+  if (chain) {
+    if ((rc = check_cancel(c))) return rc;
+    if ((rc = run_chain(c, opts, &st, &from))) return rc;
+  }
   do {
+    if (from == ST_DONE) break;
     mark(0);
     // (trim sweeps drop finished vertices from the list, so no compaction first)
-    if ((rc = do_trim1(c, opts.trim_stop, &st.trimmed_first))) break;
+    if (from <= ST_TRIM_FIRST) {
+      int64_t t = 0;
+      if ((rc = do_trim1(c, opts.trim_stop, &t))) break;
+      st.trimmed_first += t;
+      if (chain) c->chain_sweeps[0] = std::min(64, std::max(1, c->h_dc->loop.iters));
+    }
     mark(1);
     st.ms_trim1 = elapsed(0, 1);
     if (c->n_live == 0) break;
     if ((rc = check_cancel(c))) break;
-    // the global step runs over colour -1; with caller-set colours it is a
-    // partition round
-    int64_t sets[5] = {0, 0, 0, 0, 0};
-    if ((rc = do_fb_round(c, opts.pivot_policy, c->colors_split, -1, &st.scc_first, &st.pivot_first,
-                          &st.fb_levels, c->kernel_timing, tp || c->kernel_timing ? sets : nullptr)))
-      break;
+    if (from <= ST_FB) {
+      // the global step runs over colour -1; with caller-set colours it is a
+      // partition round
+      int64_t sets[5] = {0, 0, 0, 0, 0};
+      if ((rc = do_fb_round(c, opts.pivot_policy, c->colors_split, -1, &st.scc_first, &st.pivot_first,
+                            &st.fb_levels, c->kernel_timing, tp || c->kernel_timing ? sets : nullptr)))
+        break;
+      st.fb_f = sets[0];
+      st.fb_b = sets[1];
+      st.fb_ef = sets[2];
+      st.fb_eb = sets[3];
+      st.fb_fb = sets[4];
+    } else if (from == ST_FB_CONTINUE) {
+      // the chain's search needed more level launches than it queued
+      if ((rc = fb_continue(c, &st.scc_first, &st.fb_levels))) break;
+    }
+    if (chain && from <= ST_FB_CONTINUE)
+      c->chain_levels = std::min(64, std::max(2, std::max(c->h_dc->lv.levels, c->h_dc->lv2.levels)) + 1);
     mark(2);
     st.ms_fb_global = elapsed(1, 2);
-    st.fb_f = sets[0];
-    st.fb_b = sets[1];
-    st.fb_ef = sets[2];
-    st.fb_eb = sets[3];
-    st.fb_fb = sets[4];
     int64_t t = 0;
-    if ((rc = do_trim1(c, opts.trim_stop, &t))) break;
-    st.trimmed_second = t;
-    if (opts.use_trim2 && c->n_live > 0) {
-      if ((rc = do_trim2(c, &t))) break;
-      st.trimmed_second += t;
+    if (from <= ST_TRIM_SECOND) {
       if ((rc = do_trim1(c, opts.trim_stop, &t))) break;
       st.trimmed_second += t;
+      if (chain) c->chain_sweeps[1] = std::min(64, std::max(1, c->h_dc->loop.iters));
+    }
+    if (opts.use_trim2 && c->n_live > 0 && from <= ST_TRIM_THIRD) {
+      if (from <= ST_TRIM2) {
+        if ((rc = do_trim2(c, &t))) break;
+        st.trimmed_second += t;
+      }
+      if ((rc = do_trim1(c, opts.trim_stop, &t))) break;
+      st.trimmed_second += t;
+      if (chain) c->chain_sweeps[2] = std::min(64, std::max(1, c->h_dc->loop.iters));
     }
     mark(3);
     st.ms_trim2 = elapsed(2, 3);
@@ -1039,7 +1240,7 @@ int run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_out) 
   } while (false);
   if (rc) return rc;
   mark(4);
-  if ((rc = do_canon(c, &st.num_sccs))) return rc;
+  if (from != ST_DONE && (rc = do_canon(c, &st.num_sccs))) return rc;
   mark(5);
   st.ms_canon = elapsed(4, 5);
   auto t1 = std::chrono::steady_clock::now();
@@ -1110,6 +1311,19 @@ int scc_labels_device(scc_ctx* c, const int32_t** d_labels) {
 int scc_ctx_set_kernel_timing(scc_ctx* c, int enable) {
   if (!c) return SCC_ERR_ARG;
   c->kernel_timing = enable != 0;
+  return SCC_OK;
+}
+
+int scc_ctx_set_chain(scc_ctx* c, int enable, int32_t shape, int32_t level_launches, int32_t sweeps) {
+  if (!c) return SCC_ERR_ARG;
+  if (shape > 3 || level_launches > 64 || sweeps > 64 || shape < -1 || level_launches == 0 || sweeps == 0 ||
+      level_launches < -1 || sweeps < -1)
+    return set_error(c, SCC_ERR_ARG, "chain: shape in [0, 3], launches and sweeps in [1, 64] (-1 keeps)");
+  c->chain_off = enable == 0;
+  if (shape >= 0) c->chain_shape = shape;
+  if (level_launches > 0) c->chain_levels = level_launches;
+  if (sweeps > 0)
+    for (auto& x : c->chain_sweeps) x = sweeps;
   return SCC_OK;
 }
 

@@ -79,6 +79,10 @@ void launch_reset(hipStream_t st, DevCtl* dc, int32_t n, int32_t* color, int32_t
 // single the slot of colour -1 reset
 void launch_fb_setup(hipStream_t st, DevCtl* dc, int32_t n, unsigned long long* pkey, int32_t* rep, int single,
                      double bu_ratio);
+// A gate of a queued chain (ChainCtl): kind GATE_TRIM (a trim loop of the given
+// epoch; single: it must have ended after one sweep), GATE_SEARCH, GATE_TRIM2.
+enum : int { GATE_TRIM = 0, GATE_SEARCH = 1, GATE_TRIM2 = 2 };
+void launch_chain_gate(hipStream_t st, DevCtl* dc, int gate, int kind, int32_t epoch, int single, int last);
 void launch_force_pivot(hipStream_t st, int32_t p, int32_t n, const int32_t* color,
                         unsigned long long* pkey);
 void launch_seed_list(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* off, const int32_t* offe,

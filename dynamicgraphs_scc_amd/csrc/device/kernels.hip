@@ -103,7 +103,7 @@ __device__ void init_level(LevelCtl& v, double bu_ratio) {
 }
 
 // what: bit 0 counters, 1 live list, 2 loop control, 3 and 5 the two level
-// controls, 4 tuning defaults.  One thread.
+// controls, 4 tuning defaults, 6 the chain state (no halt).  One thread.
 __device__ void ctl_init_body(DevCtl* dc, int what, int32_t live_sel, unsigned long long live_n, long long stop,
                               double bu_ratio) {
   if (what & 1) {
@@ -133,6 +133,16 @@ __device__ void ctl_init_body(DevCtl* dc, int what, int32_t live_sel, unsigned l
   }
   if (what & 8) init_level(dc->lv, bu_ratio);
   if (what & 32) init_level(dc->lv2, bu_ratio);
+  if (what & 64) {
+    ChainCtl& ch = dc->chain;
+    ch.halt = 0;
+    ch.gate = -1;
+    for (int i = 0; i < CHAIN_GATES; i++) {
+      ch.ok[i] = ch.iters[i] = 0;
+      ch.live[i] = ch.count[i] = 0;
+    }
+    ch.pivot = 0;
+  }
 }
 
 __global__ void k_ctl_init(DevCtl* dc, int what, int32_t live_sel, unsigned long long live_n,
@@ -149,15 +159,15 @@ __global__ __launch_bounds__(BLOCK) void k_reset(DevCtl* __restrict__ dc, int32_
     label[v] = -1;
     live0[v] = (int32_t)v;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctl_init_body(dc, 1 | 2, 0, (unsigned long long)n, 0, 0.0);
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl_init_body(dc, 1 | 2 | 64, 0, (unsigned long long)n, 0, 0.0);
 }
 
 // Before a sweep of k_trim1_step with fresh == 2: the control block as the reset
-// leaves it (counters zero, live list 0..n-1 in live[0]) and the global pivot
-// slot empty.
+// leaves it (counters zero, live list 0..n-1 in live[0], no chain halt) and the
+// global pivot slot empty.
 __global__ void k_fresh_init(DevCtl* __restrict__ dc, int32_t n, unsigned long long* __restrict__ pkey) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  ctl_init_body(dc, 1 | 2, 0, (unsigned long long)n, 0, 0.0);
+  ctl_init_body(dc, 1 | 2 | 64, 0, (unsigned long long)n, 0, 0.0);
   pkey[n] = 0ull;
 }
 
@@ -166,12 +176,45 @@ __global__ void k_fresh_init(DevCtl* __restrict__ dc, int32_t n, unsigned long l
 // partition slot in use.
 __global__ void k_fb_setup(DevCtl* __restrict__ dc, int32_t n, unsigned long long* __restrict__ pkey,
                            int32_t* __restrict__ rep, int single, double bu_ratio) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (threadIdx.x != 0 || blockIdx.x != 0 || chain_halted(dc)) return;
   ctl_init_body(dc, 1 | 8 | 32, 0, 0, 0, bu_ratio);
   if (single) {  // single == 2: pkey[n] already holds the pivot key (k_trim1_step, fresh == 2)
     if (single == 1) pkey[n] = 0ull;
     rep[3 * (int64_t)n + 0] = rep[3 * (int64_t)n + 1] = rep[3 * (int64_t)n + 2] = INT32_MAX;
   }
+}
+
+// ---- chain gates -------------------------------------------------------------------
+// One thread, after a phase of a queued chain (ChainCtl): records what the
+// phase did and stops the chain when it did not finish within its launches
+// (a trim loop still open, or, with single, ended after more than its first
+// sweep, whose pivot key then no longer stands; a search still open), or when
+// this is the chain's last gate and vertices are still live.  No live vertex:
+// the later phases are skipped (halt 2).
+__global__ void k_chain_gate(DevCtl* __restrict__ dc, int gate, int kind, int32_t epoch, int single,
+                             int last) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || chain_halted(dc)) return;
+  ChainCtl& ch = dc->chain;
+  bool ok = true;
+  if (kind == GATE_TRIM) {
+    const LoopCtl& l = dc->loop;
+    ok = l.epoch == epoch && l.done && (!single || l.iters == 1);
+    ch.iters[gate] = l.iters;
+    ch.count[gate] = l.total0;
+  } else if (kind == GATE_SEARCH) {
+    ok = dc->lv.done && dc->lv2.done;
+    ch.iters[gate] = dc->lv.levels + dc->lv2.levels;
+    ch.count[gate] = dc->ctr[C_SCC];
+    ch.pivot = dc->ctr[C_PIVOT];
+  } else {  // GATE_TRIM2: one launch
+    ch.count[gate] = dc->ctr[C_TRIMMED];
+  }
+  const unsigned long long live = dc->live.n;
+  ch.ok[gate] = ok;
+  ch.live[gate] = live;
+  ch.gate = gate;
+  if (!ok || (last && live > 0)) ch.halt = 2 * gate + 1;
+  else if (live == 0) ch.halt = 2;
 }
 
 // ---- trim1 -------------------------------------------------------------------
@@ -201,7 +244,7 @@ __global__ __launch_bounds__(BLOCK) void k_trim1_step(DevCtl* __restrict__ dc, G
                                                       unsigned long long* __restrict__ pkey,
                                                       uint32_t* __restrict__ zbits, int64_t zwords) {
   __shared__ int s_done;
-  if (threadIdx.x == 0) s_done = dc->loop.epoch == epoch ? dc->loop.done : 0;
+  if (threadIdx.x == 0) s_done = chain_halted(dc) || (dc->loop.epoch == epoch ? dc->loop.done : 0);
   __syncthreads();
   if (s_done) return;
   const int32_t* __restrict__ list_in = LIVE_IN(dc, L0, L1);
@@ -384,6 +427,7 @@ __global__ __launch_bounds__(BLOCK) void k_trim2(DevCtl* __restrict__ dc, GraphP
                                                  int32_t* __restrict__ color, int32_t* __restrict__ label,
                                                  const int32_t* __restrict__ L0,
                                                  const int32_t* __restrict__ L1) {
+  if (chain_halted(dc)) return;
   const int32_t* __restrict__ list_in = LIVE_IN(dc, L0, L1);
   const int64_t n_in = (int64_t)dc->live.n;
   unsigned long long removed = 0;
@@ -592,7 +636,7 @@ __global__ __launch_bounds__(BLOCK) void k_seed_pivots2(DevCtl* __restrict__ dc,
 // (colour -1, slot n): its pivot, from pkey[n], with no pass over the live list.
 __global__ void k_seed_single(DevCtl* __restrict__ dc, int32_t n, LevelDir F, LevelDir B,
                               const unsigned long long* __restrict__ pkey, CanonKeys K) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (threadIdx.x != 0 || blockIdx.x != 0 || chain_halted(dc)) return;
   const unsigned long long key = pkey[n];
   if (key == 0ull) return;  // no live vertex
   const int32_t v = pivot_of_key(key);
@@ -1150,6 +1194,7 @@ __global__ __launch_bounds__(BLOCK) void k_level2(DevCtl* __restrict__ dc, int m
                                                   const int32_t* __restrict__ color,
                                                   const int32_t* __restrict__ L0, const int32_t* __restrict__ L1,
                                                   int32_t max_chunks) {
+  if (chain_halted(dc)) return;
   const bool bw = (blockIdx.x >> 3) & 1u;
   const unsigned bid = ((blockIdx.x >> 4) << 3) | (blockIdx.x & 7u);
   level_part<s_light>(dc, bw ? &dc->lv2 : &dc->lv, bw ? dc->done2 : dc->done, bw ? dc->lvlog2 : dc->lvlog, multi,
@@ -1161,7 +1206,7 @@ __global__ __launch_bounds__(BLOCK) void k_level2(DevCtl* __restrict__ dc, int m
 // ended; they do nothing until both have (every workgroup reads the same
 // flags: the level launches before them in the stream have finished).
 __device__ __forceinline__ bool searches_open(const DevCtl* __restrict__ dc) {
-  return !(dc->lv.done && dc->lv2.done);
+  return chain_halted(dc) || !(dc->lv.done && dc->lv2.done);
 }
 
 // Sizes of a step's reach sets for the traffic model: |F|, |B|, E_F (out-degrees
@@ -1761,6 +1806,7 @@ __global__ __launch_bounds__(BLOCK) void k_canon_keys(DevCtl* __restrict__ dc, i
                                                       const unsigned long long* __restrict__ ckey,
                                                       const int32_t* __restrict__ perm,
                                                       int32_t* __restrict__ out, int32_t* __restrict__ cout) {
+  if (dc->chain.halt & 1) return;  // a chain stopped before its end
   const int32_t hot = block_hot_label(n, label);
   __shared__ unsigned long long s_hk;
   if (threadIdx.x == 0) s_hk = hot >= 0 ? ckey[hot] : ~0ull;
@@ -1855,6 +1901,9 @@ void launch_reset(hipStream_t st, DevCtl* dc, int32_t n, int32_t* color, int32_t
 void launch_fb_setup(hipStream_t st, DevCtl* dc, int32_t n, unsigned long long* pkey, int32_t* rep, int single,
                      double bu_ratio) {
   hipLaunchKernelGGL(k_fb_setup, dim3(1), dim3(64), 0, st, dc, n, pkey, rep, single, bu_ratio);
+}
+void launch_chain_gate(hipStream_t st, DevCtl* dc, int gate, int kind, int32_t epoch, int single, int last) {
+  hipLaunchKernelGGL(k_chain_gate, dim3(1), dim3(64), 0, st, dc, gate, kind, epoch, single, last);
 }
 void launch_force_pivot(hipStream_t st, int32_t p, int32_t n, const int32_t* color,
                         unsigned long long* pkey) {

@@ -158,7 +158,7 @@ extern "C" int scc_run_device_noalloc(int device, void* stream, int32_t n, int64
   };
   // every control field and completion counter from scratch: the array keeps
   // the last call's contents
-  launch_ctl_init(c.stream, c.dc, 1 | 2 | 4 | 8 | 16 | 32, 0, 0, 0, 0.0);
+  launch_ctl_init(c.stream, c.dc, 1 | 2 | 4 | 8 | 16 | 32 | 64, 0, 0, 0, 0.0);
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, c.stream, d_maxdeg);
   hipLaunchKernelGGL(k_max_degree, dim3(grid_for(n, 1024)), dim3(BLOCK), 0, c.stream, n, d_begin, d_r_begin,
                      d_maxdeg);

@@ -64,7 +64,7 @@ EXPORTS = [
     "scc_sched_forward", "scc_sched_gradients", "scc_sched_train", "scc_sched_rollout",
     "scc_batch_set", "scc_batch_run", "scc_batch_counts", "scc_ctx_get_tuning", "scc_ctx_set_tuning",
     "scc_autotune", "scc_level_log", "scc_update_room_default", "scc_ctx_set_update_room",
-    "scc_run_device_noalloc",
+    "scc_run_device_noalloc", "scc_ctx_set_chain",
 ]
 
 
@@ -82,7 +82,7 @@ class RunOpts(C.Structure):
 
 class RunStats(C.Structure):
     _fields_ = [("trimmed_first", C.c_int64), ("scc_first", C.c_int64),
-                ("pivot_first", C.c_int32), ("pad0", C.c_int32),
+                ("pivot_first", C.c_int32), ("chain_stop", C.c_int32),
                 ("trimmed_second", C.c_int64), ("wcc_parts", C.c_int64),
                 ("fb_rounds", C.c_int64), ("num_sccs", C.c_int64), ("fb_levels", C.c_int64),
                 ("ms_total", C.c_double), ("ms_trim1", C.c_double),
@@ -93,7 +93,7 @@ class RunStats(C.Structure):
                 ("fb_eb", C.c_int64), ("fb_fb", C.c_int64)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad0"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class PhaseRec(C.Structure):
@@ -176,6 +176,7 @@ def load_library():
         "scc_graph_layout": (C.c_int, [vp, C.POINTER(C.c_int)]),
         "scc_update": (C.c_int, [vp, vp, vp, i64, vp, vp, i64, C.POINTER(UpdateStats)]),
         "scc_ctx_set_kernel_timing": (C.c_int, [vp, C.c_int]),
+        "scc_ctx_set_chain": (C.c_int, [vp, C.c_int, C.c_int32, C.c_int32, C.c_int32]),
         "scc_ctx_synchronize": (C.c_int, [vp]),
         "scc_ctx_set_budget": (C.c_int, [vp, C.c_uint64]),
         "scc_update_bytes_needed": (C.c_uint64, [i32, i64, i64, i64]),
@@ -359,6 +360,12 @@ class SccEngine:
 
     def set_kernel_timing(self, on: bool):
         self._check(self.lib.scc_ctx_set_kernel_timing(self.h, int(on)))
+
+    def set_chain(self, on: bool = True, shape: int = -1, level_launches: int = -1, sweeps: int = -1):
+        """Queue a fresh decomposition as one chain of launches (default on);
+        shape, level launches and sweeps per trim loop < 0 keep the values the
+        last decomposition left (scc_ctx_set_chain)."""
+        self._check(self.lib.scc_ctx_set_chain(self.h, int(on), shape, level_launches, sweeps))
 
     def synchronize(self):
         self._check(self.lib.scc_ctx_synchronize(self.h))

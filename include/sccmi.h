@@ -182,7 +182,8 @@ typedef struct scc_run_stats {
   int64_t trimmed_first;   /* repeat_global_trim1 count */
   int64_t scc_first;       /* size of the global FB's SCC */
   int32_t pivot_first;     /* pivot of the global FB */
-  int32_t pad0;
+  int32_t chain_stop;      /* queued chain (scc_ctx_set_chain): -1 none, 0-4 the gate that stopped
+                              it (the run went on phase by phase), 5 it ran to its end */
   int64_t trimmed_second;  /* trim1 + trim2 + trim1 after the global FB */
   int64_t wcc_parts;       /* partitions after WCC */
   int64_t fb_rounds;       /* partition FB rounds */
@@ -415,6 +416,21 @@ int scc_level_log(scc_ctx* ctx, int64_t* out, int32_t* levels);
 int scc_ctx_set_kernel_timing(scc_ctx* ctx, int enable);
 /* Block until the context's stream is idle. */
 int scc_ctx_synchronize(scc_ctx* ctx);
+
+/* Queued chain (no reference counterpart: the reference returns to the host
+ * after every phase).  scc_run_method2 from a fresh state on the plain
+ * schedule (degree-product pivots, no phase or kernel timing, partition mode
+ * not learned) queues the first trim loop, the global step, the trims after it
+ * and the canonical labels as one chain of launches and reads the control
+ * block back once.  A gate launch after each phase stops the chain when the
+ * phase needs more launches than were queued for it, or when vertices are
+ * still live after the chain's last phase; the call then continues phase by
+ * phase with read-backs, as without the chain.  The chain's length (shape: its
+ * last gate, 0 = first trim loop, 1 = global step, 2 = second trim loop, 3 =
+ * trim2 and the third trim loop), level launches and sweeps per trim loop
+ * follow the last decomposition on the context; values < 0 keep them.
+ * Results do not depend on any of this.  enable = 0 turns the chain off. */
+int scc_ctx_set_chain(scc_ctx* ctx, int enable, int32_t shape, int32_t level_launches, int32_t sweeps);
 
 /* ---- decomposition that allocates nothing ---------------------------------------
  * (No counterpart in the reference.  For callers that cannot allocate device
