@@ -675,45 +675,49 @@ static int run_chain(scc_ctx* c, const scc_run_opts& o, scc_run_stats* st, int* 
   c->reset_pending = false;
   c->pivot_ready = false;
   auto t0 = std::chrono::steady_clock::now();
+  // the last gate queued marks the chain's end (and zeroes the counters for
+  // the canonicalisation behind it)
+  const int lastg = shape >= 3 && o.use_trim2 ? 4 : std::min(shape, 2);
   // gate 0: the first trim loop; its first sweep does the deferred reset and
   // leaves the global pivot key, which stands when it is the loop's only sweep
   const int32_t e0 = trim_loop(0, 0);
-  launch_chain_gate(c->stream, c->dc, 0, GATE_TRIM, e0, 1, shape == 0);
-  if (shape >= 1) {  // gate 1: the global forward-backward step
+  if (lastg == 0) {
+    launch_chain_gate(c->stream, c->dc, 0, GATE_TRIM, e0, 1, 1, 1);
+  } else {  // gate 0 with the global forward-backward step's set-up and seeding
     const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
     const unsigned grid = grid_at(0);
-    launch_fb_setup(c->stream, c->dc, c->n, c->pkey, c->rep, 2, avg / c->tune.bu_alpha);
     const LevelDir F = level_dir(c, true, true), B = level_dir(c, false, true);
-    launch_seed_single(c->stream, c->dc, c->n, F, B, c->pkey, ck(c));
+    launch_chain_gate_fb(c->stream, c->dc, e0, 0, c->n, c->pkey, c->rep, avg / c->tune.bu_alpha, F, B, ck(c));
     const int32_t max_chunks = std::max<int32_t>(1, (c->maxdeg + 2047) / 2048);
     for (int i = 0; i < c->chain_levels; i++)
       launch_level2(c->stream, 2 * grid, c->dc, 0, F, B, c->color, c->live[0], c->live[1], max_chunks, c->tune.light);
     launch_classify(c->stream, grid, c->dc, c->n, c->color, c->label, c->vf, c->vb, c->pkey, c->rep, c->live[0],
                     c->live[1], ck(c));
     launch_recolor(c->stream, grid, c->dc, c->n, c->color, c->vf, c->vb, c->rep, c->live[0], c->live[1]);
-    launch_chain_gate(c->stream, c->dc, 1, GATE_SEARCH, 0, 0, shape == 1);
+    launch_chain_gate(c->stream, c->dc, 1, GATE_SEARCH, 0, 0, lastg == 1, lastg == 1);
   }
-  if (shape >= 2) {  // gate 2: trim1 after the step
+  if (lastg >= 2) {  // gate 2: trim1 after the step (its counters zeroed for trim2 or the labels)
     const int32_t e2 = trim_loop(2, 1);
-    launch_chain_gate(c->stream, c->dc, 2, GATE_TRIM, e2, 0, shape == 2);
+    launch_chain_gate(c->stream, c->dc, 2, GATE_TRIM, e2, 0, lastg == 2, 1);
   }
-  if (shape >= 3 && o.use_trim2) {  // gates 3 and 4: trim2, then trim1
-    launch_ctl_init(c->stream, c->dc, 1, 0, 0, 0, 0.0);
+  if (lastg == 4) {  // gates 3 and 4: trim2, then trim1
     launch_trim2(c->stream, grid_at(2), c->dc, gp(c), c->color, c->label, c->live[0], c->live[1]);
-    launch_chain_gate(c->stream, c->dc, 3, GATE_TRIM2, 0, 0, 0);
+    launch_chain_gate(c->stream, c->dc, 3, GATE_TRIM2, 0, 0, 0, 0);
     const int32_t e4 = trim_loop(4, 2);
-    launch_chain_gate(c->stream, c->dc, 4, GATE_TRIM, e4, 0, 1);
+    launch_chain_gate(c->stream, c->dc, 4, GATE_TRIM, e4, 0, 1, 1);
   }
-  // canonical labels: skipped by a chain that stopped (halt odd)
+  // canonical labels: skipped by a chain that stopped (halt odd); a chain that
+  // ran to its end has its halt cleared by this launch
   launch_canon(c->stream, c->dc, c->n, c->label, c->ckey, c->relabeled ? c->perm : nullptr, c->canon,
-               c->canon_caller);
+               c->canon_caller, false);
   int rc = pull_ctl(c);
   if (rc) return rc;
   const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   const ChainCtl ch = c->h_dc->chain;
   const int last = ch.gate;  // the last gate that ran
-  // (the last gate queued always decides: halt 2, or odd at a gate that ran)
-  if (last < 0 || last >= CHAIN_GATES || (ch.halt != 2 && (ch.halt & 1) == 0))
+  // (the last gate queued always decides: halt 2, which the canonicalisation
+  // clears, or odd at a gate that ran)
+  if (last < 0 || last >= CHAIN_GATES || (ch.halt != 0 && ch.halt != 2 && (ch.halt & 1) == 0))
     return set_error(c, SCC_ERR_STATE, "queued chain ended without a decision");
   // what the gates saw
   st->trimmed_first = last >= 0 ? (int64_t)ch.count[0] : 0;
@@ -725,6 +729,18 @@ static int run_chain(scc_ctx* c, const scc_run_opts& o, scc_run_stats* st, int* 
   st->trimmed_second = (last >= 2 ? (int64_t)ch.count[2] : 0) + (last >= 3 ? (int64_t)ch.count[3] : 0) +
                        (last >= 4 ? (int64_t)ch.count[4] : 0);
   for (int g = 0; g <= last && g < CHAIN_GATES; g++) c->chain_live[g] = (int64_t)ch.live[g];
+  // sweeps per trim loop: a loop that ended inside the chain needed iters[g]
+  // (the first loop too when its gate failed only because it took more than one
+  // sweep); one still open gets twice what it had, since the loop that
+  // continues it afterwards counts only its own sweeps
+  {
+    const int gates[3] = {0, 2, 4};
+    for (int k = 0; k < 3 && gates[k] <= last; k++) {
+      const int g = gates[k], it = std::max(1, ch.iters[g]);
+      const bool open = g == last && !ch.ok[g] && !(g == 0 && c->h_dc->loop.done);
+      c->chain_sweeps[k] = std::min(64, open ? 2 * it : it);
+    }
+  }
   // the phase trace (the chain synchronises once: its wall time goes to the last record)
   static const int kinds[CHAIN_GATES] = {SCC_PHASE_TRIM1, SCC_PHASE_FB_GLOBAL, SCC_PHASE_TRIM1, SCC_PHASE_TRIM2,
                                          SCC_PHASE_TRIM1, 0};
@@ -740,7 +756,7 @@ static int run_chain(scc_ctx* c, const scc_run_opts& o, scc_run_stats* st, int* 
     c->trace.push_back(r);
     before = r.live_after;
   }
-  if (ch.halt == 2) {  // ran to its end: every vertex decided, labels final
+  if ((ch.halt & 1) == 0) {  // ran to its end: every vertex decided, labels final
     if (!c->no_alloc && c->trace.size() < 65536) {
       scc_phase_rec r{SCC_PHASE_CANON, -1, 0, 0, (int64_t)c->h_dc->ctr[C_COUNT], 0.0};
       c->trace.push_back(r);
@@ -751,7 +767,6 @@ static int run_chain(scc_ctx* c, const scc_run_opts& o, scc_run_stats* st, int* 
     // next time, stop where the vertices ran out
     c->chain_shape = last >= 3 ? 3 : last;
     if (last >= 1) c->chain_levels = std::min(64, std::max(2, std::max(c->h_dc->lv.levels, c->h_dc->lv2.levels)) + 1);
-    launch_ctl_init(c->stream, c->dc, 64, 0, 0, 0, 0.0);  // (no halt for later calls)
     *resume = ST_DONE;
     st->chain_stop = 5;
     return hip_check(c, hipGetLastError(), "chain");
@@ -1182,7 +1197,6 @@ int run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_out) 
       int64_t t = 0;
       if ((rc = do_trim1(c, opts.trim_stop, &t))) break;
       st.trimmed_first += t;
-      if (chain) c->chain_sweeps[0] = std::min(64, std::max(1, c->h_dc->loop.iters));
     }
     mark(1);
     st.ms_trim1 = elapsed(0, 1);
@@ -1212,7 +1226,6 @@ int run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_out) 
     if (from <= ST_TRIM_SECOND) {
       if ((rc = do_trim1(c, opts.trim_stop, &t))) break;
       st.trimmed_second += t;
-      if (chain) c->chain_sweeps[1] = std::min(64, std::max(1, c->h_dc->loop.iters));
     }
     if (opts.use_trim2 && c->n_live > 0 && from <= ST_TRIM_THIRD) {
       if (from <= ST_TRIM2) {
@@ -1221,7 +1234,6 @@ int run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_out) 
       }
       if ((rc = do_trim1(c, opts.trim_stop, &t))) break;
       st.trimmed_second += t;
-      if (chain) c->chain_sweeps[2] = std::min(64, std::max(1, c->h_dc->loop.iters));
     }
     mark(3);
     st.ms_trim2 = elapsed(2, 3);

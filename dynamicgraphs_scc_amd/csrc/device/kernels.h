@@ -82,7 +82,12 @@ void launch_fb_setup(hipStream_t st, DevCtl* dc, int32_t n, unsigned long long* 
 // A gate of a queued chain (ChainCtl): kind GATE_TRIM (a trim loop of the given
 // epoch; single: it must have ended after one sweep), GATE_SEARCH, GATE_TRIM2.
 enum : int { GATE_TRIM = 0, GATE_SEARCH = 1, GATE_TRIM2 = 2 };
-void launch_chain_gate(hipStream_t st, DevCtl* dc, int gate, int kind, int32_t epoch, int single, int last);
+// zero_ctr: then zero the counters (for the canonicalisation queued next).
+void launch_chain_gate(hipStream_t st, DevCtl* dc, int gate, int kind, int32_t epoch, int single, int last,
+                       int zero_ctr);
+// Gate 0 with, when the chain goes on, the global step's set-up and seeding.
+void launch_chain_gate_fb(hipStream_t st, DevCtl* dc, int32_t epoch, int last, int32_t n, unsigned long long* pkey,
+                          int32_t* rep, double bu_ratio, const LevelDir& F, const LevelDir& B, CanonKeys K);
 void launch_force_pivot(hipStream_t st, int32_t p, int32_t n, const int32_t* color,
                         unsigned long long* pkey);
 void launch_seed_list(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* off, const int32_t* offe,
@@ -126,6 +131,7 @@ void launch_extract(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, i
 // cout[perm[v]], from the keys the phases left (CanonKeys); zeroes the
 // counters first and leaves the SCC count in dc->ctr[C_COUNT]
 void launch_canon(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label, const unsigned long long* ckey,
-                  const int32_t* perm, int32_t* out, int32_t* cout);
+                  const int32_t* perm, int32_t* out, int32_t* cout,
+                  bool zero_ctr = true);
 
 }  // namespace sccmi

@@ -174,14 +174,18 @@ __global__ void k_fresh_init(DevCtl* __restrict__ dc, int32_t n, unsigned long l
 // Set-up of a forward-backward step in one launch: zeroed counters, both
 // search controls, and (single: every live vertex has colour -1) the one
 // partition slot in use.
-__global__ void k_fb_setup(DevCtl* __restrict__ dc, int32_t n, unsigned long long* __restrict__ pkey,
-                           int32_t* __restrict__ rep, int single, double bu_ratio) {
-  if (threadIdx.x != 0 || blockIdx.x != 0 || chain_halted(dc)) return;
+__device__ void fb_setup_body(DevCtl* __restrict__ dc, int32_t n, unsigned long long* __restrict__ pkey,
+                              int32_t* __restrict__ rep, int single, double bu_ratio) {
   ctl_init_body(dc, 1 | 8 | 32, 0, 0, 0, bu_ratio);
   if (single) {  // single == 2: pkey[n] already holds the pivot key (k_trim1_step, fresh == 2)
     if (single == 1) pkey[n] = 0ull;
     rep[3 * (int64_t)n + 0] = rep[3 * (int64_t)n + 1] = rep[3 * (int64_t)n + 2] = INT32_MAX;
   }
+}
+__global__ void k_fb_setup(DevCtl* __restrict__ dc, int32_t n, unsigned long long* __restrict__ pkey,
+                           int32_t* __restrict__ rep, int single, double bu_ratio) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || chain_halted(dc)) return;
+  fb_setup_body(dc, n, pkey, rep, single, bu_ratio);
 }
 
 // ---- chain gates -------------------------------------------------------------------
@@ -190,10 +194,9 @@ __global__ void k_fb_setup(DevCtl* __restrict__ dc, int32_t n, unsigned long lon
 // (a trim loop still open, or, with single, ended after more than its first
 // sweep, whose pivot key then no longer stands; a search still open), or when
 // this is the chain's last gate and vertices are still live.  No live vertex:
-// the later phases are skipped (halt 2).
-__global__ void k_chain_gate(DevCtl* __restrict__ dc, int gate, int kind, int32_t epoch, int single,
-                             int last) {
-  if (threadIdx.x != 0 || blockIdx.x != 0 || chain_halted(dc)) return;
+// the later phases are skipped (halt 2).  One thread; nothing when halted.
+__device__ void gate_body(DevCtl* __restrict__ dc, int gate, int kind, int32_t epoch, int single, int last) {
+  if (chain_halted(dc)) return;
   ChainCtl& ch = dc->chain;
   bool ok = true;
   if (kind == GATE_TRIM) {
@@ -215,6 +218,14 @@ __global__ void k_chain_gate(DevCtl* __restrict__ dc, int gate, int kind, int32_
   ch.gate = gate;
   if (!ok || (last && live > 0)) ch.halt = 2 * gate + 1;
   else if (live == 0) ch.halt = 2;
+}
+// zero_ctr: then zero the counters, whatever the gate decided (the chain's
+// canonicalisation, queued next, counts from zero)
+__global__ void k_chain_gate(DevCtl* __restrict__ dc, int gate, int kind, int32_t epoch, int single, int last,
+                             int zero_ctr) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  gate_body(dc, gate, kind, epoch, single, last);
+  if (zero_ctr) ctl_init_body(dc, 1, 0, 0, 0, 0.0);
 }
 
 // ---- trim1 -------------------------------------------------------------------
@@ -634,9 +645,8 @@ __global__ __launch_bounds__(BLOCK) void k_seed_pivots2(DevCtl* __restrict__ dc,
 
 // The seeds of the global step's two searches when there is one partition
 // (colour -1, slot n): its pivot, from pkey[n], with no pass over the live list.
-__global__ void k_seed_single(DevCtl* __restrict__ dc, int32_t n, LevelDir F, LevelDir B,
-                              const unsigned long long* __restrict__ pkey, CanonKeys K) {
-  if (threadIdx.x != 0 || blockIdx.x != 0 || chain_halted(dc)) return;
+__device__ void seed_single_body(DevCtl* __restrict__ dc, int32_t n, const LevelDir& F, const LevelDir& B,
+                                 const unsigned long long* __restrict__ pkey, CanonKeys K) {
   const unsigned long long key = pkey[n];
   if (key == 0ull) return;  // no live vertex
   const int32_t v = pivot_of_key(key);
@@ -655,6 +665,25 @@ __global__ void k_seed_single(DevCtl* __restrict__ dc, int32_t n, LevelDir F, Le
   dc->lv.visited += 1;
   dc->lv2.visited += 1;
   dc->ctr[C_PIVOT] = (unsigned long long)v;
+}
+__global__ void k_seed_single(DevCtl* __restrict__ dc, int32_t n, LevelDir F, LevelDir B,
+                              const unsigned long long* __restrict__ pkey, CanonKeys K) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || chain_halted(dc)) return;
+  seed_single_body(dc, n, F, B, pkey, K);
+}
+
+// A chain's first gate (its first trim loop, which must have ended after the
+// fused first sweep) with, when it lets the chain go on, the global step's
+// set-up and its pivot's seeding: k_chain_gate, k_fb_setup (single 2) and
+// k_seed_single in one launch.
+__global__ void k_chain_gate_fb(DevCtl* __restrict__ dc, int32_t epoch, int last, int32_t n,
+                                unsigned long long* __restrict__ pkey, int32_t* __restrict__ rep, double bu_ratio,
+                                LevelDir F, LevelDir B, CanonKeys K) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  gate_body(dc, 0, GATE_TRIM, epoch, 1, last);
+  if (chain_halted(dc)) return;
+  fb_setup_body(dc, n, pkey, rep, 2, bu_ratio);
+  seed_single_body(dc, n, F, B, pkey, K);
 }
 
 // Seed a search from an explicit vertex list (count at *ns): vertices not yet
@@ -1807,6 +1836,9 @@ __global__ __launch_bounds__(BLOCK) void k_canon_keys(DevCtl* __restrict__ dc, i
                                                       const int32_t* __restrict__ perm,
                                                       int32_t* __restrict__ out, int32_t* __restrict__ cout) {
   if (dc->chain.halt & 1) return;  // a chain stopped before its end
+  // a chain that ran to its end (halt 2) leaves no halt for later launches:
+  // every workgroup reads 0 or 2 here and runs either way
+  if (blockIdx.x == 0 && threadIdx.x == 0 && dc->chain.halt == 2) dc->chain.halt = 0;
   const int32_t hot = block_hot_label(n, label);
   __shared__ unsigned long long s_hk;
   if (threadIdx.x == 0) s_hk = hot >= 0 ? ckey[hot] : ~0ull;
@@ -1902,8 +1934,13 @@ void launch_fb_setup(hipStream_t st, DevCtl* dc, int32_t n, unsigned long long* 
                      double bu_ratio) {
   hipLaunchKernelGGL(k_fb_setup, dim3(1), dim3(64), 0, st, dc, n, pkey, rep, single, bu_ratio);
 }
-void launch_chain_gate(hipStream_t st, DevCtl* dc, int gate, int kind, int32_t epoch, int single, int last) {
-  hipLaunchKernelGGL(k_chain_gate, dim3(1), dim3(64), 0, st, dc, gate, kind, epoch, single, last);
+void launch_chain_gate(hipStream_t st, DevCtl* dc, int gate, int kind, int32_t epoch, int single, int last,
+                       int zero_ctr) {
+  hipLaunchKernelGGL(k_chain_gate, dim3(1), dim3(64), 0, st, dc, gate, kind, epoch, single, last, zero_ctr);
+}
+void launch_chain_gate_fb(hipStream_t st, DevCtl* dc, int32_t epoch, int last, int32_t n, unsigned long long* pkey,
+                          int32_t* rep, double bu_ratio, const LevelDir& F, const LevelDir& B, CanonKeys K) {
+  hipLaunchKernelGGL(k_chain_gate_fb, dim3(1), dim3(64), 0, st, dc, epoch, last, n, pkey, rep, bu_ratio, F, B, K);
 }
 void launch_force_pivot(hipStream_t st, int32_t p, int32_t n, const int32_t* color,
                         unsigned long long* pkey) {
@@ -1986,8 +2023,8 @@ void launch_extract(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, i
   LAUNCH(k_extract, grid, st, dc, color, label, V, L0, L1, K);
 }
 void launch_canon(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label, const unsigned long long* ckey,
-                  const int32_t* perm, int32_t* out, int32_t* cout) {
-  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, dc, 1, 0, 0ull, 0ll, 0.0);
+                  const int32_t* perm, int32_t* out, int32_t* cout, bool zero_ctr) {
+  if (zero_ctr) hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, dc, 1, 0, 0ull, 0ll, 0.0);
   if (n <= 0) return;
   if (perm)
     LAUNCH(k_canon_keys<true>, grid_for((n + 3) / 4, 1024), st, dc, n, label, ckey, perm, out, cout);

@@ -92,16 +92,16 @@ int scc_graph_upload_device(scc_ctx* ctx, int32_t n, int64_t m, const int32_t* d
 int scc_graph_info(const scc_ctx* ctx, int32_t* n, int64_t* m);
 /* Vertex layout inside the context (no counterpart in the reference: a device
  * memory layout choice).  SCC_LAYOUT_DEGREE relabels the next uploaded graph
- * by decreasing in + out degree (ties by id) so that hubs' per-vertex state
- * shares cache lines; it pays on skewed (R-MAT) graphs.  Every entry point
- * still takes and returns the caller's vertex ids, labels and colours, and
- * canonical labels stay the minimum caller id of each component.  With a
- * batch (scc_batch_set) the order becomes per graph (graph, then decreasing
- * degree), so every graph keeps its id range; that needs an upload without
- * updates since. */
+ * by decreasing degree class (bit length of in + out degree), caller id order
+ * within a class, so that hubs' per-vertex state shares cache lines; it pays
+ * on skewed (R-MAT) graphs.  Every entry point still takes and returns the
+ * caller's vertex ids, labels and colours, and canonical labels stay the
+ * minimum caller id of each component.  With a batch (scc_batch_set) the order
+ * becomes per graph (graph, then decreasing degree class), so every graph
+ * keeps its id range; that needs an upload without updates since. */
 typedef enum {
   SCC_LAYOUT_INPUT = 0,  /* device ids are the caller's ids (default) */
-  SCC_LAYOUT_DEGREE = 1, /* device ids by decreasing total degree */
+  SCC_LAYOUT_DEGREE = 1, /* device ids by decreasing total degree class */
   SCC_LAYOUT_AUTO = 2    /* SCC_LAYOUT_DEGREE when the largest in- or out-degree exceeds 32 times
                             the mean degree (skewed graphs), else SCC_LAYOUT_INPUT */
 } scc_layout;
