@@ -516,6 +516,9 @@ struct scc_ctx {
   // pkey[n] holds the global step's pivot key and the reach bitmaps are clear
   // (left by that fused sweep when it ended the trim loop); cleared by every call
   bool pivot_ready = false;
+  // canon (internal order) not computed: the last decomposition's chain wrote
+  // the caller-order labels only (materialize_canon)
+  bool canon_stale = false;
   // An update failed after it began to rewrite the rows: the directions, the
   // edge count or the labels may disagree, so graph calls are refused until
   // the next upload.

@@ -1004,6 +1004,7 @@ extern "C" int scc_update(scc_ctx* c, const int32_t* ins_src, const int32_t* ins
       return set_error(c, SCC_ERR_STATE, "scc_update needs a finished decomposition (run first)");
     if ((r0 = do_canon(c, nullptr))) return r0;
   }
+  if (int r1 = materialize_canon(c)) return r1;
   if (n_ins < 0 || n_del < 0 || (n_ins && (!ins_src || !ins_dst)) || (n_del && (!del_src || !del_dst)))
     return set_error(c, SCC_ERR_ARG, "bad batch");
   for (int64_t i = 0; i < n_ins; i++)

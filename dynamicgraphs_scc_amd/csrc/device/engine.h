@@ -51,6 +51,9 @@ int do_wcc(scc_ctx* c, int64_t* parts);
 int do_color_round(scc_ctx* c, int64_t* scc);
 int do_fb_parts(scc_ctx* c, int policy, int mode, bool trim_between, int64_t* rounds);
 int do_canon(scc_ctx* c, int64_t* count);
+// The internal-order canonical labels (c->canon) when the last decomposition
+// left only the caller-order ones (a chain in the degree layout).
+int materialize_canon(scc_ctx* c);
 int run_method2(scc_ctx* c, const scc_run_opts* opts, scc_run_stats* st);  // scc_run_method2 unchecked
 
 // optional subsystems (dynamic.hip, sched.hip)

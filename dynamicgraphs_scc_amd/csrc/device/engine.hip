@@ -268,6 +268,7 @@ static void reset_host_state(scc_ctx* c) {
   c->cur = 0;
   c->n_live = c->n;
   c->pivot_ready = false;
+  c->canon_stale = false;
 }
 
 int reset_state(scc_ctx* c) {
@@ -640,7 +641,15 @@ int do_canon(scc_ctx* c, int64_t* count) {
   pr.finish((int64_t)c->h_dc->ctr[C_COUNT]);
   if (count) *count = (int64_t)c->h_dc->ctr[C_COUNT];
   c->labels_ready = true;
+  c->canon_stale = false;
   return SCC_OK;
+}
+
+int materialize_canon(scc_ctx* c) {
+  if (!c->canon_stale) return SCC_OK;
+  launch_canon(c->stream, c->dc, c->n, c->label, c->ckey, nullptr, c->canon, nullptr);
+  c->canon_stale = false;
+  return hip_check(c, hipGetLastError(), "internal canonical labels");
 }
 
 // ---- a fresh decomposition queued as one chain -------------------------------------
@@ -707,9 +716,13 @@ static int run_chain(scc_ctx* c, const scc_run_opts& o, scc_run_stats* st, int* 
     launch_chain_gate(c->stream, c->dc, 4, GATE_TRIM, e4, 0, 1, 1);
   }
   // canonical labels: skipped by a chain that stopped (halt odd); a chain that
-  // ran to its end has its halt cleared by this launch
-  launch_canon(c->stream, c->dc, c->n, c->label, c->ckey, c->relabeled ? c->perm : nullptr, c->canon,
-               c->canon_caller, false);
+  // ran to its end has its halt cleared by this launch.  In the degree layout
+  // only the caller-order labels (sequential writes); the internal-order ones
+  // follow when an update needs them.
+  if (c->relabeled)
+    launch_canon_caller(c->stream, c->dc, c->n, c->label, c->ckey, c->iperm, c->canon_caller, false);
+  else
+    launch_canon(c->stream, c->dc, c->n, c->label, c->ckey, nullptr, c->canon, nullptr, false);
   int rc = pull_ctl(c);
   if (rc) return rc;
   const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -763,6 +776,7 @@ static int run_chain(scc_ctx* c, const scc_run_opts& o, scc_run_stats* st, int* 
     }
     st->num_sccs = (int64_t)c->h_dc->ctr[C_COUNT];
     c->labels_ready = true;
+    c->canon_stale = c->relabeled;
     c->colors_split = last >= 1;
     // next time, stop where the vertices ran out
     c->chain_shape = last >= 3 ? 3 : last;

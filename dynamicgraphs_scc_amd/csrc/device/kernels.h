@@ -130,6 +130,10 @@ void launch_extract(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, i
 // canonical labels into out, and (perm != NULL) the caller-id labels into
 // cout[perm[v]], from the keys the phases left (CanonKeys); zeroes the
 // counters first and leaves the SCC count in dc->ctr[C_COUNT]
+// Caller-order canonical labels only (degree vertex layout): cout[u] from the
+// label of iperm[u]; the internal-order labels are left to launch_canon.
+void launch_canon_caller(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label,
+                         const unsigned long long* ckey, const int32_t* iperm, int32_t* cout, bool zero_ctr);
 void launch_canon(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label, const unsigned long long* ckey,
                   const int32_t* perm, int32_t* out, int32_t* cout,
                   bool zero_ctr = true);

@@ -1881,6 +1881,49 @@ __global__ __launch_bounds__(BLOCK) void k_canon_keys(DevCtl* __restrict__ dc, i
   block_count_nowait(self, &dc->ctr[C_COUNT]);
 }
 
+// Caller-order canonical labels in the degree vertex layout: cout[u] is the
+// high half (the minimum caller id) of the key of the label of internal vertex
+// iperm[u].  The writes are sequential; the label and key reads follow the
+// degree classes, within each of which caller order is internal order.  Eight
+// caller ids per thread, each stage's loads issued together.  Counts the
+// caller ids that are their component's label.  In a chain: nothing when it
+// stopped (halt odd), and a finished chain's halt is cleared.
+__global__ __launch_bounds__(BLOCK) void k_canon_caller(DevCtl* __restrict__ dc, int32_t n,
+                                                        const int32_t* __restrict__ label,
+                                                        const unsigned long long* __restrict__ ckey,
+                                                        const int32_t* __restrict__ iperm,
+                                                        int32_t* __restrict__ cout) {
+  if (dc->chain.halt & 1) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && dc->chain.halt == 2) dc->chain.halt = 0;
+  auto key = [&](int32_t l) -> unsigned long long { return (uint32_t)l < (uint32_t)n ? ckey[l] : ~0ull; };
+  unsigned long long self = 0;
+  const int64_t n8 = (int64_t)n / 8;
+  const int4* __restrict__ ip4 = reinterpret_cast<const int4*>(iperm);
+  int4* __restrict__ c4 = reinterpret_cast<int4*>(cout);
+  for (int64_t q = (int64_t)blockIdx.x * BLOCK + threadIdx.x; q < n8; q += (int64_t)gridDim.x * BLOCK) {
+    const int4 a = ip4[2 * q], b = ip4[2 * q + 1];
+    const int32_t v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    int32_t l[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) l[k] = label[v[k]];
+    int32_t o[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) o[k] = (int32_t)(key(l[k]) >> 32);
+    c4[2 * q] = make_int4(o[0], o[1], o[2], o[3]);
+    c4[2 * q + 1] = make_int4(o[4], o[5], o[6], o[7]);
+    const int32_t u = (int32_t)(8 * q);
+#pragma unroll
+    for (int k = 0; k < 8; k++) self += o[k] == u + k;
+  }
+  if (blockIdx.x == 0)
+    for (int64_t u = 8 * n8 + threadIdx.x; u < n; u += BLOCK) {
+      const int32_t o = (int32_t)(key(label[iperm[u]]) >> 32);
+      cout[u] = o;
+      self += o == (int32_t)u;
+    }
+  block_count_nowait(self, &dc->ctr[C_COUNT]);
+}
+
 // ---- host launchers -------------------------------------------------------------
 
 #define LAUNCH(kern, grid, st, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, st, __VA_ARGS__)
@@ -2021,6 +2064,12 @@ void launch_wcc_color(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color,
 void launch_extract(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, int32_t* label,
                     const uint32_t* V, int32_t* L0, int32_t* L1, CanonKeys K) {
   LAUNCH(k_extract, grid, st, dc, color, label, V, L0, L1, K);
+}
+void launch_canon_caller(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label,
+                         const unsigned long long* ckey, const int32_t* iperm, int32_t* cout, bool zero_ctr) {
+  if (zero_ctr) hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, dc, 1, 0, 0ull, 0ll, 0.0);
+  if (n <= 0) return;
+  LAUNCH(k_canon_caller, grid_for(n / 8 > 0 ? n / 8 : 1, 2048), st, dc, n, label, ckey, iperm, cout);
 }
 void launch_canon(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label, const unsigned long long* ckey,
                   const int32_t* perm, int32_t* out, int32_t* cout, bool zero_ctr) {
