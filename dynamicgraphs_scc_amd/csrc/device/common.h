@@ -144,6 +144,7 @@ __device__ __forceinline__ bool chain_halted(const DevCtl* __restrict__ dc) { re
 // visited bitmap, and ping-pong frontier, candidate and deferred-row lists.
 struct LevelDir {
   const int32_t *off, *offe, *nb, *roff, *roffe, *rnb;
+  const int32_t* rhead;  // first entry of each opposite row (rnb[roff[v]]; -1 for an empty row), or null
   uint32_t* V;
   int32_t *q0, *q1, *c0, *c1, *h0, *h1;
 };
@@ -450,6 +451,8 @@ struct scc_ctx {
   // neighbour (ties by id), so scans that stop at the first live / reached
   // neighbour meet hubs first.  NULL when absent (after updates).
   int32_t *tadj = nullptr, *tradj = nullptr;
+  // first entry of every traversal row (out, in; -1 when empty), with the copies
+  int32_t *head_out = nullptr, *head_in = nullptr;
 
   // Vertex layout (layout.hip): with SCC_LAYOUT_DEGREE requested at upload the
   // device graph is relabelled; perm[i] = caller id of internal vertex i,
@@ -505,6 +508,7 @@ struct scc_ctx {
   int32_t chain_shape = 3;
   int32_t chain_sweeps[3] = {1, 1, 1};
   int32_t chain_levels = 8;
+  int32_t chain_level_grid[64] = {};      // workgroups per direction of each level launch (0: full)
   int64_t chain_live[sccmi::CHAIN_GATES] = {-1, -1, -1, -1, -1, -1};
   bool chain_off = false;                 // scc_ctx_set_chain(ctx, 0)
   int32_t trim_epoch = 0;                 // number of the last trim loop queued

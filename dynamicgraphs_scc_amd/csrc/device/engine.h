@@ -54,6 +54,8 @@ int do_canon(scc_ctx* c, int64_t* count);
 // The internal-order canonical labels (c->canon) when the last decomposition
 // left only the caller-order ones (a chain in the degree layout).
 int materialize_canon(scc_ctx* c);
+// Row heads of the traversal rows (order.hip), after build_order / the layout.
+int build_heads(scc_ctx* c);
 int run_method2(scc_ctx* c, const scc_run_opts* opts, scc_run_stats* st);  // scc_run_method2 unchecked
 
 // optional subsystems (dynamic.hip, sched.hip)

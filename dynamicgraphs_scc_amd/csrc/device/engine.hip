@@ -355,6 +355,7 @@ static LevelDir level_dir(scc_ctx* c, bool fwd, bool own_lists) {
   d.roff = fwd ? c->rbegin : c->begin;
   d.roffe = fwd ? rrow_end(c) : row_end(c);
   d.rnb = fwd ? tr_radj(c) : tr_adj(c);
+  d.rhead = fwd ? c->head_in : c->head_out;
   d.V = fwd ? c->vf : c->vb;
   const bool b = !fwd && own_lists;
   d.q0 = b ? c->bq[0] : c->q[0];
@@ -653,6 +654,36 @@ int materialize_canon(scc_ctx* c) {
 }
 
 // ---- a fresh decomposition queued as one chain -------------------------------------
+// From the global step just read back: level launches for the next chain (as
+// many as the longer search took: its last launch admitted nothing) and each
+// launch's workgroups per direction, from the work the two searches' levels
+// had (a top-down frontier or bottom-up candidate list, 256 per workgroup,
+// at least 8; the full grid where a level split hub rows over the grid).
+// A level that grows on another graph only runs with fewer workgroups.
+static void chain_learn_search(scc_ctx* c) {
+  const DevCtl& d = *c->h_dc;
+  const int levels = std::max(d.lv.levels, d.lv2.levels);
+  c->chain_levels = std::min(64, std::max(1, levels));
+  auto need = [&](const unsigned long long* lg, int have, int i) -> int64_t {
+    if (i >= have) return 0;
+    const unsigned long long in = lg[1];
+    if ((lg[0] & 1) == MODE_TD) {
+      if ((in >> 32) > 0) return INT64_MAX;  // rows split over the grid
+      return (int64_t)(in & 0xFFFFFFFFull);
+    }
+    return (int64_t)in;
+  };
+  for (int i = 0; i < 64; i++) {
+    const int64_t w = std::max(need(d.lvlog[i], d.lv.levels, i), need(d.lvlog2[i], d.lv2.levels, i));
+    if (w == INT64_MAX) {
+      c->chain_level_grid[i] = 0;
+      continue;
+    }
+    const int64_t g = ((w + 255) / 256 + 7) / 8 * 8;
+    c->chain_level_grid[i] = (int32_t)std::max<int64_t>(8, std::min<int64_t>(g, 1 << 20));
+  }
+}
+
 // Stages of method 2 in order; run_method2 runs them from a given one.
 enum Stage : int { ST_TRIM_FIRST = 0, ST_FB, ST_FB_CONTINUE, ST_TRIM_SECOND, ST_TRIM2, ST_TRIM_THIRD, ST_WCC, ST_DONE };
 
@@ -698,8 +729,11 @@ static int run_chain(scc_ctx* c, const scc_run_opts& o, scc_run_stats* st, int* 
     const LevelDir F = level_dir(c, true, true), B = level_dir(c, false, true);
     launch_chain_gate_fb(c->stream, c->dc, e0, 0, c->n, c->pkey, c->rep, avg / c->tune.bu_alpha, F, B, ck(c));
     const int32_t max_chunks = std::max<int32_t>(1, (c->maxdeg + 2047) / 2048);
-    for (int i = 0; i < c->chain_levels; i++)
-      launch_level2(c->stream, 2 * grid, c->dc, 0, F, B, c->color, c->live[0], c->live[1], max_chunks, c->tune.light);
+    for (int i = 0; i < c->chain_levels; i++) {
+      const unsigned gi = c->chain_level_grid[i] > 0 ? std::min<unsigned>(grid, (unsigned)c->chain_level_grid[i]) : grid;
+      launch_level2(c->stream, 2 * std::max(8u, gi), c->dc, 0, F, B, c->color, c->live[0], c->live[1], max_chunks,
+                    c->tune.light);
+    }
     launch_classify(c->stream, grid, c->dc, c->n, c->color, c->label, c->vf, c->vb, c->pkey, c->rep, c->live[0],
                     c->live[1], ck(c));
     launch_recolor(c->stream, grid, c->dc, c->n, c->color, c->vf, c->vb, c->rep, c->live[0], c->live[1]);
@@ -780,7 +814,7 @@ static int run_chain(scc_ctx* c, const scc_run_opts& o, scc_run_stats* st, int* 
     c->colors_split = last >= 1;
     // next time, stop where the vertices ran out
     c->chain_shape = last >= 3 ? 3 : last;
-    if (last >= 1) c->chain_levels = std::min(64, std::max(2, std::max(c->h_dc->lv.levels, c->h_dc->lv2.levels)) + 1);
+    if (last >= 1) chain_learn_search(c);
     *resume = ST_DONE;
     st->chain_stop = 5;
     return hip_check(c, hipGetLastError(), "chain");
@@ -872,9 +906,9 @@ uint64_t scc_ctx_bytes(const scc_ctx* c) { return c ? c->bytes : 0; }
 uint64_t scc_bytes_needed(int32_t n, int64_t m) {
   // graph, state, traversal copies or relabelling temporaries, scratch, the
   // degree layout's maps (perm, iperm, two staging arrays, caller-id labels),
-  // control block
+  // row heads, control block
   return 8ull * (uint64_t)(m > 0 ? m : 1) + 8ull * ((uint64_t)n + 1) + state_bytes(n) + order_bytes(m) +
-         4ull * ((uint64_t)n + 1) + 20ull * ((uint64_t)n + 1) + sizeof(DevCtl);
+         4ull * ((uint64_t)n + 1) + 20ull * ((uint64_t)n + 1) + 8ull * ((uint64_t)n + 1) + sizeof(DevCtl);
 }
 int scc_device_memory(int device, uint64_t* free_bytes, uint64_t* total_bytes) {
   if (!free_bytes || !total_bytes) return SCC_ERR_ARG;
@@ -950,6 +984,7 @@ int scc_graph_upload(scc_ctx* c, int32_t n, int64_t m, const int32_t* begin, con
     return rc;
   }
   if (!c->relabeled && (rc = build_order(c))) return rc;
+  if ((rc = build_heads(c))) return rc;
   // the fresh state is deferred like scc_state_reset's: the next call makes it,
   // or scc_run_method2 fuses it into its first sweep
   reset_host_state(c);
@@ -987,6 +1022,7 @@ int scc_graph_upload_device(scc_ctx* c, int32_t n, int64_t m, const int32_t* d_b
     return rc;
   }
   if (!c->relabeled && (rc = build_order(c))) return rc;
+  if ((rc = build_heads(c))) return rc;
   // the fresh state is deferred like scc_state_reset's: the next call makes it,
   // or scc_run_method2 fuses it into its first sweep
   reset_host_state(c);
@@ -1233,7 +1269,7 @@ int run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_out) 
       if ((rc = fb_continue(c, &st.scc_first, &st.fb_levels))) break;
     }
     if (chain && from <= ST_FB_CONTINUE)
-      c->chain_levels = std::min(64, std::max(2, std::max(c->h_dc->lv.levels, c->h_dc->lv2.levels)) + 1);
+      chain_learn_search(c);
     mark(2);
     st.ms_fb_global = elapsed(1, 2);
     int64_t t = 0;

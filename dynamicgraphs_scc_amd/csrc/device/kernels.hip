@@ -263,7 +263,8 @@ __global__ __launch_bounds__(BLOCK) void k_trim1_step(DevCtl* __restrict__ dc, G
   const int64_t n_in = (int64_t)dc->live.n;
   auto live = [color](int32_t k, int32_t pv, int32_t pc) { return k != pv && color[k] == pc; };
   unsigned long long trimmed = 0;
-  using Out = LdsOut<0>;
+  // (room for a whole sweep's survivors of most workgroups: one flush, at the end)
+  using Out = LdsOut<0, 8192>;
   Out::init();
   unsigned long long best = 0ull;  // fresh == 2: this thread's largest pivot key
   if (fresh == 2)
@@ -1082,15 +1083,17 @@ __device__ __forceinline__ void level_part(DevCtl* __restrict__ dc, LevelCtl* __
       COut::maybe_flush(c_out, &lv->out_c, BLOCK);
       const int64_t i = base + threadIdx.x;
       const bool valid = i < ncin;
-      int32_t v = 0, c = DONE, s = 0, e = 0;
+      int32_t v = 0, c = DONE, s = 0, e = 0, hd = -1;
       bool act = false;
       if (valid) {
         v = c_in[i];
-        // independent loads: visited bit, colour (several partitions only), row bounds
+        // independent loads: visited bit, colour (several partitions only), row
+        // bounds, the row's first entry (from the heads: no row-start line)
         const bool vis = bit_test(V, v);
         c = multi ? color[v] : UNSET;
         s = roff[v];
         e = roffe[v];
+        if (D.rhead) hd = D.rhead[v];
         act = !vis && c != DONE;
       }
       // phase 1: each lane tests the first s_light entries of its row
@@ -1101,7 +1104,7 @@ __device__ __forceinline__ void level_part(DevCtl* __restrict__ dc, LevelCtl* __
         // the first entry alone (hub-first rows put a likely parent there),
         // then four per step with their loads issued together
         if (lim > 0) {
-          const int32_t k0[4] = {rnb[s], -1, -1, -1};
+          const int32_t k0[4] = {D.rhead ? hd : rnb[s], -1, -1, -1};
           found = reached4(V, color, multi, k0, c);
         }
         if (__ballot(lim > 1 && !found) != 0) {
@@ -1302,7 +1305,7 @@ __global__ __launch_bounds__(BLOCK) void k_classify(DevCtl* __restrict__ dc, int
   __syncthreads();
   const int32_t hot = s_hot;
   unsigned long long nscc = 0;
-  using Out = LdsOut<0>;
+  using Out = LdsOut<0, 8192>;  // (few flushes: each is a barrier and a waited atomic)
   Out::init();
   // four entries per thread and step (base + t * BLOCK + thread: survivors
   // keep the list order), their dependent loads in flight together

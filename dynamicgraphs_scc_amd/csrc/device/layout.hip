@@ -310,6 +310,8 @@ int layout_segment(scc_ctx* c, int32_t k) {
   if ((rc = hip_check(c, hipMemcpyAsync(c->perm, q, 4ull * n, hipMemcpyDeviceToDevice, c->stream), "segment perm")))
     return done(rc);
   LAUNCH(k_invert, grid_for(n, 4096), c->stream, n, c->perm, c->iperm);
+  // the rows moved: their heads follow
+  if ((rc = build_heads(c))) return done(rc);
   if ((rc = hip_check(c, hipMemcpyAsync(c->h_perm.data(), c->perm, 4ull * n, hipMemcpyDeviceToHost, c->stream),
                       "layout download")) ||
       (rc = hip_check(c, hipMemcpyAsync(c->h_iperm.data(), c->iperm, 4ull * n, hipMemcpyDeviceToHost, c->stream),
