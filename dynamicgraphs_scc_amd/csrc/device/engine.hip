@@ -375,7 +375,7 @@ static LevelDir level_dir(scc_ctx* c, bool fwd, bool own_lists) {
 static int level_loop(scc_ctx* c, bool both, bool fwd, bool multi, int64_t* levels, bool timed,
                       const std::function<void()>& post = nullptr) {
   const unsigned grid = grid_for(c->n_live, (unsigned)c->tune.grid_cap);
-  const int32_t max_chunks = std::max<int32_t>(1, (c->maxdeg + 2047) / 2048);
+  const int32_t max_chunks = std::max<int32_t>(1, (c->maxdeg + 1023) / 1024);
   const LevelDir F = level_dir(c, fwd || both, both), B = level_dir(c, false, true);
   int& hint = c->lv_hint[both ? 2 : (fwd ? 0 : 1)];
   int batch = std::max(2, hint), rc;
@@ -485,7 +485,7 @@ int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int6
   auto post = [&]() {
     if (sets) launch_fb_stats(c->stream, grid, c->dc, gp(c), c->vf, c->vb, c->live[0], c->live[1]);
     launch_classify(c->stream, grid, c->dc, c->n, c->color, c->label, c->vf, c->vb, c->pkey, c->rep,
-                    c->live[0], c->live[1], ck(c));
+                    c->live[0], c->live[1], ck(c), multi ? 0 : 1);
     launch_recolor(c->stream, grid, c->dc, c->n, c->color, c->vf, c->vb, c->rep, c->live[0], c->live[1]);
   };
   if ((rc = traverse_both(c, multi, levels, timed, post))) return rc;
@@ -514,7 +514,7 @@ static int fb_continue(scc_ctx* c, int64_t* scc, int64_t* levels) {
   const unsigned grid = grid_for(c->n_live, (unsigned)c->tune.grid_cap);
   auto post = [&]() {
     launch_classify(c->stream, grid, c->dc, c->n, c->color, c->label, c->vf, c->vb, c->pkey, c->rep, c->live[0],
-                    c->live[1], ck(c));
+                    c->live[1], ck(c), 1);
     launch_recolor(c->stream, grid, c->dc, c->n, c->color, c->vf, c->vb, c->rep, c->live[0], c->live[1]);
   };
   int64_t lv = 0;
@@ -728,14 +728,14 @@ static int run_chain(scc_ctx* c, const scc_run_opts& o, scc_run_stats* st, int* 
     const unsigned grid = grid_at(0);
     const LevelDir F = level_dir(c, true, true), B = level_dir(c, false, true);
     launch_chain_gate_fb(c->stream, c->dc, e0, 0, c->n, c->pkey, c->rep, avg / c->tune.bu_alpha, F, B, ck(c));
-    const int32_t max_chunks = std::max<int32_t>(1, (c->maxdeg + 2047) / 2048);
+    const int32_t max_chunks = std::max<int32_t>(1, (c->maxdeg + 1023) / 1024);
     for (int i = 0; i < c->chain_levels; i++) {
       const unsigned gi = c->chain_level_grid[i] > 0 ? std::min<unsigned>(grid, (unsigned)c->chain_level_grid[i]) : grid;
       launch_level2(c->stream, 2 * std::max(8u, gi), c->dc, 0, F, B, c->color, c->live[0], c->live[1], max_chunks,
                     c->tune.light);
     }
     launch_classify(c->stream, grid, c->dc, c->n, c->color, c->label, c->vf, c->vb, c->pkey, c->rep, c->live[0],
-                    c->live[1], ck(c));
+                    c->live[1], ck(c), 1);
     launch_recolor(c->stream, grid, c->dc, c->n, c->color, c->vf, c->vb, c->rep, c->live[0], c->live[1]);
     launch_chain_gate(c->stream, c->dc, 1, GATE_SEARCH, 0, 0, lastg == 1, lastg == 1);
   }

@@ -106,7 +106,8 @@ void launch_fb_stats(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, con
                      const uint32_t* vb, const int32_t* L0, const int32_t* L1);
 void launch_classify(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, int32_t* color,
                      int32_t* label, const uint32_t* vf, const uint32_t* vb,
-                     const unsigned long long* pkey, int32_t* rep, int32_t* L0, int32_t* L1, CanonKeys K);
+                     const unsigned long long* pkey, int32_t* rep, int32_t* L0, int32_t* L1, CanonKeys K,
+                     int single);
 void launch_recolor(hipStream_t st, unsigned grid, const DevCtl* dc, int32_t n, int32_t* color,
                     const uint32_t* vf, const uint32_t* vb, const int32_t* rep, const int32_t* L0,
                     const int32_t* L1);

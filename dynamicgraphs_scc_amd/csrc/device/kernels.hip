@@ -859,7 +859,7 @@ __device__ __forceinline__ int td_group(unsigned long long nq, unsigned long lon
   return G;
 }
 
-constexpr int TD_CHUNK = 2048;
+constexpr int TD_CHUNK = 1024;  // (2048 measured 7 us slower per R-MAT-22 step, 512 3 us)
 
 // One level.  Top-down: expand the frontier (rows of at least HUGE_ROW entries
 // are deferred to the next launch, which cuts them into TD_CHUNK pieces spread
@@ -1283,22 +1283,28 @@ __global__ __launch_bounds__(BLOCK) void k_classify(DevCtl* __restrict__ dc, int
                                                     const uint32_t* __restrict__ vb,
                                                     const unsigned long long* __restrict__ pkey,
                                                     int32_t* __restrict__ rep, int32_t* __restrict__ L0,
-                                                    int32_t* __restrict__ L1, CanonKeys K) {
+                                                    int32_t* __restrict__ L1, CanonKeys K, int single) {
   if (searches_open(dc)) return;
   const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
   int32_t* __restrict__ list_out = LIVE_OUT(dc, L0, L1);
   const int64_t n_in = (int64_t)dc->live.n;
-  // the partition of the workgroup's first entry (in the global step, every
-  // entry's): its three representatives and its SCC's canonical key are
+  constexpr int PER = 8;  // entries per thread and step
+  // the partition of the workgroup's first entry (single: the one partition,
+  // colour -1): its three representatives and its SCC's canonical key are
   // reduced in LDS and written with one global atomic each, instead of every
   // wave hitting the same addresses
   __shared__ int32_t s_hot, s_hmin[3], s_hpiv;
   __shared__ unsigned long long s_hkey;
   if (threadIdx.x == 0) {
-    const int64_t i0 = (int64_t)blockIdx.x * 4 * BLOCK;
-    const int32_t c0 = i0 < n_in ? color[list[i0]] : DONE;
-    s_hot = c0 != DONE ? slot_of(c0, n) : -1;
-    s_hpiv = c0 != DONE ? pivot_of_key(pkey[s_hot]) : -1;
+    if (single) {
+      s_hot = n;
+      s_hpiv = pivot_of_key(pkey[n]);
+    } else {
+      const int64_t i0 = (int64_t)blockIdx.x * PER * BLOCK;
+      const int32_t c0 = i0 < n_in ? color[list[i0]] : DONE;
+      s_hot = c0 != DONE ? slot_of(c0, n) : -1;
+      s_hpiv = c0 != DONE ? pivot_of_key(pkey[s_hot]) : -1;
+    }
     s_hkey = ~0ull;
   }
   if (threadIdx.x < 3) s_hmin[threadIdx.x] = INT32_MAX;
@@ -1307,45 +1313,50 @@ __global__ __launch_bounds__(BLOCK) void k_classify(DevCtl* __restrict__ dc, int
   unsigned long long nscc = 0;
   using Out = LdsOut<0, 8192>;  // (few flushes: each is a barrier and a waited atomic)
   Out::init();
-  // four entries per thread and step (base + t * BLOCK + thread: survivors
-  // keep the list order), their dependent loads in flight together
+  // eight entries per thread and step (base + t * BLOCK + thread: survivors
+  // keep the list order); each round of loads (entries; then colours, both
+  // reach words and caller ids) is issued for all eight before any is used
   unsigned long long hkey = ~0ull;  // this thread's least key among the hot SCC's members
-  for (int64_t base = (int64_t)blockIdx.x * 4 * BLOCK; base < n_in; base += (int64_t)gridDim.x * 4 * BLOCK) {
-    Out::maybe_flush(list_out, &dc->ctr[C_LIST_OUT], 4 * BLOCK);
-    int32_t v[4], c[4];
+  for (int64_t base = (int64_t)blockIdx.x * PER * BLOCK; base < n_in; base += (int64_t)gridDim.x * PER * BLOCK) {
+    Out::maybe_flush(list_out, &dc->ctr[C_LIST_OUT], PER * BLOCK);
+    int32_t v[PER], c[PER], pm[PER];
+    uint32_t wf[PER], wb[PER];
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
+    for (int t = 0; t < PER; t++) {
       const int64_t i = base + t * BLOCK + threadIdx.x;
       v[t] = i < n_in ? list[i] : -1;
     }
 #pragma unroll
-    for (int t = 0; t < 4; t++) c[t] = v[t] >= 0 ? color[v[t]] : DONE;
-    bool f[4], b[4];
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-      f[t] = c[t] != DONE && bit_test(vf, v[t]);
-      b[t] = c[t] != DONE && bit_test(vb, v[t]);
+    for (int t = 0; t < PER; t++) {
+      c[t] = DONE;
+      wf[t] = wb[t] = 0u;
+      pm[t] = v[t];
+      if (v[t] >= 0) {
+        c[t] = color[v[t]];
+        wf[t] = vf[v[t] >> 5];
+        wb[t] = vb[v[t] >> 5];
+        if (K.perm) pm[t] = K.perm[v[t]];
+      }
     }
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
+    for (int t = 0; t < PER; t++) {
+      const bool f = c[t] != DONE && ((wf[t] >> (v[t] & 31)) & 1u);
+      const bool b = c[t] != DONE && ((wb[t] >> (v[t] & 31)) & 1u);
+      const unsigned long long kv = ((unsigned long long)(uint32_t)pm[t] << 32) | (uint32_t)v[t];
       bool keep = false, hk = false, inscc = false;
       int64_t rslot = 0;
       int32_t piv = 0;
       if (c[t] != DONE) {
         const int32_t sl = slot_of(c[t], n);
-        if (f[t] && b[t]) {
+        if (f && b) {
           piv = sl == hot ? s_hpiv : pivot_of_key(pkey[sl]);
           label[v[t]] = piv;
           color[v[t]] = DONE;
           nscc++;
-          if (sl == hot) {
-            const unsigned long long kv = canon_key(K.perm, v[t]);
-            hkey = kv < hkey ? kv : hkey;
-          } else {
-            inscc = true;
-          }
+          if (sl == hot) hkey = kv < hkey ? kv : hkey;
+          else inscc = true;
         } else {
-          const int k = f[t] ? 0 : (b[t] ? 1 : 2);
+          const int k = f ? 0 : (b ? 1 : 2);
           rslot = 3 * (int64_t)sl + k;
           keep = true;
           hk = sl == hot;
@@ -1353,10 +1364,7 @@ __global__ __launch_bounds__(BLOCK) void k_classify(DevCtl* __restrict__ dc, int
         }
       }
       grouped_min_i32(keep && !hk, rslot, v[t], rep);
-      if (inscc) {  // small SCCs (many labels per wave): a direct atomic, read first
-        const unsigned long long kv = canon_key(K.perm, v[t]);
-        if (kv < K.ckey[piv]) atomicMin(&K.ckey[piv], kv);
-      }
+      if (inscc && kv < K.ckey[piv]) atomicMin(&K.ckey[piv], kv);  // small SCCs: a direct atomic, read first
       Out::push(keep, v[t], list_out, &dc->ctr[C_LIST_OUT]);
     }
   }
@@ -2033,8 +2041,9 @@ void launch_fb_stats(hipStream_t st, unsigned grid, DevCtl* dc, GraphPtrs G, con
 }
 void launch_classify(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, int32_t* color,
                      int32_t* label, const uint32_t* vf, const uint32_t* vb,
-                     const unsigned long long* pkey, int32_t* rep, int32_t* L0, int32_t* L1, CanonKeys K) {
-  LAUNCH(k_classify, grid, st, dc, n, color, label, vf, vb, pkey, rep, L0, L1, K);
+                     const unsigned long long* pkey, int32_t* rep, int32_t* L0, int32_t* L1, CanonKeys K,
+                     int single) {
+  LAUNCH(k_classify, grid, st, dc, n, color, label, vf, vb, pkey, rep, L0, L1, K, single);
 }
 void launch_recolor(hipStream_t st, unsigned grid, const DevCtl* dc, int32_t n, int32_t* color,
                     const uint32_t* vf, const uint32_t* vb, const int32_t* rep, const int32_t* L0,
