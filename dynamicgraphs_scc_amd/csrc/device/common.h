@@ -23,6 +23,8 @@ constexpr int HUGE_ROW = 8192; // default: rows at least this long are expanded 
 constexpr int GRID_CAP = 640;  // default: workgroups of a sweep launch (2.5 per CU; more only adds
                                // per-workgroup set-up and flushes to the short levels)
 
+constexpr int CANON_WIN = 4096;  // caller ids per workgroup of k_canon_window
+constexpr int CANON_RUNS = 64;   // runs of increasing caller ids the window table can hold
 constexpr int32_t WCC_TAG = 0x20000000, WCC_MASK = 0x1FFFFFFF;  // src/scc_weak.cc:57-62
 
 // Counter slots of DevCtl::ctr (unsigned long long).
@@ -464,6 +466,13 @@ struct scc_ctx {
   // canonical labels in caller ids (caller-indexed), written with canon[]
   int32_t* canon_caller = nullptr;
   std::vector<int32_t> h_perm, h_iperm;
+  // caller-id windows of the canonical labels (k_canon_window): for window w
+  // of CANON_WIN caller ids and run r (a maximal internal range whose caller
+  // ids increase: a degree class), cwin[w * cwin_runs + r] = the first
+  // internal id of run r whose caller id is in window w or later; null when
+  // the order has more than CANON_RUNS runs
+  int32_t* cwin = nullptr;
+  int32_t cwin_runs = 0, cwin_nwin = 0;
 
   // per-vertex state
   int32_t *color = nullptr, *label = nullptr, *canon = nullptr, *W = nullptr;

@@ -753,7 +753,10 @@ static int run_chain(scc_ctx* c, const scc_run_opts& o, scc_run_stats* st, int* 
   // ran to its end has its halt cleared by this launch.  In the degree layout
   // only the caller-order labels (sequential writes); the internal-order ones
   // follow when an update needs them.
-  if (c->relabeled)
+  if (c->relabeled && c->cwin)
+    launch_canon_window(c->stream, c->dc, c->n, c->cwin_runs, c->cwin_nwin, c->cwin, c->label, c->ckey, c->perm,
+                        c->canon_caller);
+  else if (c->relabeled)
     launch_canon_caller(c->stream, c->dc, c->n, c->label, c->ckey, c->iperm, c->canon_caller, false);
   else
     launch_canon(c->stream, c->dc, c->n, c->label, c->ckey, nullptr, c->canon, nullptr, false);

@@ -131,6 +131,10 @@ void launch_extract(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, i
 // canonical labels into out, and (perm != NULL) the caller-id labels into
 // cout[perm[v]], from the keys the phases left (CanonKeys); zeroes the
 // counters first and leaves the SCC count in dc->ctr[C_COUNT]
+// Caller-order canonical labels by caller-id windows (degree layout with a
+// window table, scc_ctx::cwin); counts into ctr[C_COUNT] like launch_canon.
+void launch_canon_window(hipStream_t st, DevCtl* dc, int32_t n, int32_t runs, int32_t nwin, const int32_t* table,
+                         const int32_t* label, const unsigned long long* ckey, const int32_t* perm, int32_t* cout);
 // Caller-order canonical labels only (degree vertex layout): cout[u] from the
 // label of iperm[u]; the internal-order labels are left to launch_canon.
 void launch_canon_caller(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label,

@@ -1935,6 +1935,81 @@ __global__ __launch_bounds__(BLOCK) void k_canon_caller(DevCtl* __restrict__ dc,
   block_count_nowait(self, &dc->ctr[C_COUNT]);
 }
 
+// Caller-order canonical labels by windows of CANON_WIN caller ids, one
+// workgroup each.  Inside a run of internal ids whose caller ids increase (a
+// degree class), the members of a window are one contiguous internal range
+// (the table T gives where each run enters each window), so the workgroup
+// reads labels, keys and caller ids sequentially, run by run, assembles the
+// window's labels in LDS and writes them out in caller order.  Counts the
+// caller ids that are their component's label.  In a chain: nothing when it
+// stopped (halt odd), and a finished chain's halt is cleared.
+__global__ __launch_bounds__(BLOCK) void k_canon_window(DevCtl* __restrict__ dc, int32_t n, int32_t runs,
+                                                        const int32_t* __restrict__ T,
+                                                        const int32_t* __restrict__ label,
+                                                        const unsigned long long* __restrict__ ckey,
+                                                        const int32_t* __restrict__ perm,
+                                                        int32_t* __restrict__ cout) {
+  if (dc->chain.halt & 1) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && dc->chain.halt == 2) dc->chain.halt = 0;
+  __shared__ int32_t s_buf[CANON_WIN];
+  __shared__ int32_t s_pre[CANON_RUNS + 1], s_base[CANON_RUNS];
+  const int64_t w = blockIdx.x;
+  const int64_t U0 = w * CANON_WIN;
+  if (threadIdx.x < runs) {
+    const int32_t a = T[w * runs + threadIdx.x], b = T[(w + 1) * runs + threadIdx.x];
+    s_base[threadIdx.x] = a;
+    s_pre[threadIdx.x + 1] = b - a;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    s_pre[0] = 0;
+    for (int r = 0; r < runs; r++) s_pre[r + 1] += s_pre[r];
+  }
+  __syncthreads();
+  const int32_t total = s_pre[runs];
+  auto key = [&](int32_t l) -> unsigned long long { return (uint32_t)l < (uint32_t)n ? ckey[l] : ~0ull; };
+  // four entries per thread and step, each round of loads issued together
+  for (int32_t j0 = threadIdx.x; j0 < total; j0 += 4 * BLOCK) {
+    int32_t v[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const int32_t j = j0 + t * BLOCK;
+      v[t] = -1;
+      if (j < total) {
+        int lo = 0, hi = runs;  // s_pre[lo] <= j < s_pre[hi]
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (s_pre[mid] <= j) lo = mid;
+          else hi = mid;
+        }
+        v[t] = s_base[lo] + (j - s_pre[lo]);
+      }
+    }
+    int32_t l[4], u[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      l[t] = v[t] >= 0 ? label[v[t]] : -1;
+      u[t] = v[t] >= 0 ? perm[v[t]] : -1;
+    }
+    unsigned long long k[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) k[t] = key(l[t]);
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const int64_t pos = (int64_t)u[t] - U0;
+      if (v[t] >= 0 && pos >= 0 && pos < total) s_buf[pos] = (int32_t)(k[t] >> 32);
+    }
+  }
+  __syncthreads();
+  unsigned long long self = 0;
+  for (int32_t i = threadIdx.x; i < total; i += BLOCK) {
+    const int32_t o = s_buf[i];
+    cout[U0 + i] = o;
+    self += o == (int32_t)(U0 + i);
+  }
+  block_count_nowait(self, &dc->ctr[C_COUNT]);
+}
+
 // ---- host launchers -------------------------------------------------------------
 
 #define LAUNCH(kern, grid, st, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, st, __VA_ARGS__)
@@ -2076,6 +2151,11 @@ void launch_wcc_color(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color,
 void launch_extract(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, int32_t* label,
                     const uint32_t* V, int32_t* L0, int32_t* L1, CanonKeys K) {
   LAUNCH(k_extract, grid, st, dc, color, label, V, L0, L1, K);
+}
+void launch_canon_window(hipStream_t st, DevCtl* dc, int32_t n, int32_t runs, int32_t nwin, const int32_t* table,
+                         const int32_t* label, const unsigned long long* ckey, const int32_t* perm, int32_t* cout) {
+  if (n <= 0 || nwin <= 0) return;
+  LAUNCH(k_canon_window, nwin, st, dc, n, runs, table, label, ckey, perm, cout);
 }
 void launch_canon_caller(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label,
                          const unsigned long long* ckey, const int32_t* iperm, int32_t* cout, bool zero_ctr) {

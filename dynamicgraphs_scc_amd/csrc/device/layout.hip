@@ -194,10 +194,58 @@ void drop_layout(scc_ctx* c) {
   dev_free(c, c->map_a);
   dev_free(c, c->map_b);
   dev_free(c, c->canon_caller);
+  dev_free(c, c->cwin);
+  c->cwin = nullptr;
+  c->cwin_runs = c->cwin_nwin = 0;
   c->perm = c->iperm = c->map_a = c->map_b = c->canon_caller = nullptr;
   c->h_perm.clear();
   c->h_iperm.clear();
   c->relabeled = false;
+}
+
+// The window table of k_canon_window from the host copy of perm: runs are the
+// maximal internal ranges over which perm increases (the degree classes); for
+// each window boundary B = w * CANON_WIN, a run's entry is its first internal
+// id whose caller id is >= B.  More than CANON_RUNS runs (a batch's per-graph
+// order): no table, and the canonical labels take the general path.
+static int build_canon_windows(scc_ctx* c) {
+  dev_free(c, c->cwin);
+  c->cwin = nullptr;
+  c->cwin_runs = c->cwin_nwin = 0;
+  const int32_t n = c->n;
+  if (n <= 0 || (int64_t)c->h_perm.size() != n) return SCC_OK;
+  const int32_t* p = c->h_perm.data();
+  std::vector<int32_t> start{0};
+  for (int32_t v = 1; v < n; v++)
+    if (p[v] < p[v - 1]) {
+      if ((int)start.size() >= CANON_RUNS) return SCC_OK;
+      start.push_back(v);
+    }
+  const int32_t runs = (int32_t)start.size();
+  start.push_back(n);
+  const int64_t nwin = ((int64_t)n + CANON_WIN - 1) / CANON_WIN;
+  std::vector<int32_t> T((size_t)(nwin + 1) * runs);
+  for (int32_t r = 0; r < runs; r++) {
+    int32_t pos = start[r];
+    const int32_t e = start[r + 1];
+    for (int64_t w = 0; w <= nwin; w++) {
+      const int64_t bound = std::min<int64_t>(n, w * CANON_WIN);
+      while (pos < e && p[pos] < bound) pos++;
+      T[(size_t)w * runs + r] = pos;
+    }
+  }
+  int rc;
+  if ((rc = dev_alloc(c, (void**)&c->cwin, 4ull * T.size(), "label windows"))) return rc;
+  if ((rc = hip_check(c, hipMemcpyAsync(c->cwin, T.data(), 4ull * T.size(), hipMemcpyHostToDevice, c->stream),
+                      "label windows")) ||
+      (rc = hip_check(c, hipStreamSynchronize(c->stream), "label windows sync"))) {
+    dev_free(c, c->cwin);
+    c->cwin = nullptr;
+    return rc;
+  }
+  c->cwin_runs = runs;
+  c->cwin_nwin = (int32_t)nwin;
+  return SCC_OK;
 }
 
 int build_layout(scc_ctx* c) {
@@ -252,6 +300,7 @@ int build_layout(scc_ctx* c) {
                       "layout download")) ||
       (rc = hip_check(c, hipStreamSynchronize(c->stream), "layout sync")))
     return done(rc);
+  if ((rc = build_canon_windows(c))) return done(rc);
   c->relabeled = true;
   return done(SCC_OK);
 }
@@ -318,7 +367,8 @@ int layout_segment(scc_ctx* c, int32_t k) {
                       "layout download")) ||
       (rc = hip_check(c, hipStreamSynchronize(c->stream), "segment sync")))
     return done(rc);
-  return done(SCC_OK);
+  // (per graph and class: usually more runs than the table holds)
+  return done(build_canon_windows(c));
 }
 
 int layout_colors_to_internal(scc_ctx* c, const int32_t* colors_host) {
