@@ -736,18 +736,27 @@ static int run_chain(scc_ctx* c, const scc_run_opts& o, scc_run_stats* st, int* 
     }
     launch_classify(c->stream, grid, c->dc, c->n, c->color, c->label, c->vf, c->vb, c->pkey, c->rep, c->live[0],
                     c->live[1], ck(c), 1);
-    launch_recolor(c->stream, grid, c->dc, c->n, c->color, c->vf, c->vb, c->rep, c->live[0], c->live[1]);
-    launch_chain_gate(c->stream, c->dc, 1, GATE_SEARCH, 0, 0, lastg == 1, lastg == 1);
+    // gate 1 (when the chain goes on past it) runs inside the recolouring launch
+    launch_recolor(c->stream, grid, c->dc, c->n, c->color, c->vf, c->vb, c->rep, c->live[0], c->live[1],
+                   lastg >= 2);
+    if (lastg == 1) launch_chain_gate(c->stream, c->dc, 1, GATE_SEARCH, 0, 0, 1, 1);
   }
+  // the last gate, when it follows a trim loop, runs inside the window
+  // canonicalisation (which needs no counter reset: its count was zeroed by the
+  // step's set-up and nothing since adds to it)
+  const bool fold_last = c->relabeled && c->cwin && (lastg == 2 || lastg == 4);
+  int32_t e_last = 0;
   if (lastg >= 2) {  // gate 2: trim1 after the step (its counters zeroed for trim2 or the labels)
     const int32_t e2 = trim_loop(2, 1);
-    launch_chain_gate(c->stream, c->dc, 2, GATE_TRIM, e2, 0, lastg == 2, 1);
+    e_last = e2;
+    if (!(fold_last && lastg == 2)) launch_chain_gate(c->stream, c->dc, 2, GATE_TRIM, e2, 0, lastg == 2, 1);
   }
   if (lastg == 4) {  // gates 3 and 4: trim2, then trim1
     launch_trim2(c->stream, grid_at(2), c->dc, gp(c), c->color, c->label, c->live[0], c->live[1]);
     launch_chain_gate(c->stream, c->dc, 3, GATE_TRIM2, 0, 0, 0, 0);
     const int32_t e4 = trim_loop(4, 2);
-    launch_chain_gate(c->stream, c->dc, 4, GATE_TRIM, e4, 0, 1, 1);
+    e_last = e4;
+    if (!fold_last) launch_chain_gate(c->stream, c->dc, 4, GATE_TRIM, e4, 0, 1, 1);
   }
   // canonical labels: skipped by a chain that stopped (halt odd); a chain that
   // ran to its end has its halt cleared by this launch.  In the degree layout
@@ -755,7 +764,7 @@ static int run_chain(scc_ctx* c, const scc_run_opts& o, scc_run_stats* st, int* 
   // follow when an update needs them.
   if (c->relabeled && c->cwin)
     launch_canon_window(c->stream, c->dc, c->n, c->cwin_runs, c->cwin_nwin, c->cwin, c->label, c->ckey, c->perm,
-                        c->canon_caller);
+                        c->pkey + c->n, c->canon_caller, fold_last ? lastg : -1, e_last);
   else if (c->relabeled)
     launch_canon_caller(c->stream, c->dc, c->n, c->label, c->ckey, c->iperm, c->canon_caller, false);
   else

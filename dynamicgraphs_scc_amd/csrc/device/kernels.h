@@ -108,9 +108,10 @@ void launch_classify(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, int32
                      int32_t* label, const uint32_t* vf, const uint32_t* vb,
                      const unsigned long long* pkey, int32_t* rep, int32_t* L0, int32_t* L1, CanonKeys K,
                      int single);
-void launch_recolor(hipStream_t st, unsigned grid, const DevCtl* dc, int32_t n, int32_t* color,
+void launch_recolor(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, int32_t* color,
                     const uint32_t* vf, const uint32_t* vb, const int32_t* rep, const int32_t* L0,
-                    const int32_t* L1);
+                    const int32_t* L1,
+                    int gate1 = 0);
 void launch_cw_init(hipStream_t st, int32_t n, const int32_t* begin, const int32_t* bend, const int32_t* color,
                     unsigned long long* cw);
 void launch_wcc_pull(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* begin, const int32_t* bend,
@@ -133,8 +134,10 @@ void launch_extract(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, i
 // counters first and leaves the SCC count in dc->ctr[C_COUNT]
 // Caller-order canonical labels by caller-id windows (degree layout with a
 // window table, scc_ctx::cwin); counts into ctr[C_COUNT] like launch_canon.
+// gkey: the global step's pivot key (its SCC's label is read once per workgroup), 0 for none.
 void launch_canon_window(hipStream_t st, DevCtl* dc, int32_t n, int32_t runs, int32_t nwin, const int32_t* table,
-                         const int32_t* label, const unsigned long long* ckey, const int32_t* perm, int32_t* cout);
+                         const int32_t* label, const unsigned long long* ckey, const int32_t* perm,
+                         const unsigned long long* gkey, int32_t* cout, int gate = -1, int32_t epoch = 0);
 // Caller-order canonical labels only (degree vertex layout): cout[u] from the
 // label of iperm[u]; the internal-order labels are left to launch_canon.
 void launch_canon_caller(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label,

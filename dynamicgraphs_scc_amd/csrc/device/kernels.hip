@@ -1390,13 +1390,19 @@ __global__ __launch_bounds__(BLOCK) void k_classify(DevCtl* __restrict__ dc, int
   }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_recolor(const DevCtl* __restrict__ dc, int32_t n,
+// gate1: also the chain's gate after the global step (not its last gate), by
+// thread 0 of workgroup 0 first.  A workgroup that reads the halt it may set
+// returns, one that read it before runs: the same either way, since the gate
+// stops the chain only when the searches are still open (nothing to recolour
+// then) or no vertex is live (nothing to recolour either).
+__global__ __launch_bounds__(BLOCK) void k_recolor(DevCtl* __restrict__ dc, int32_t n,
                                                    int32_t* __restrict__ color,
                                                    const uint32_t* __restrict__ vf,
                                                    const uint32_t* __restrict__ vb,
                                                    const int32_t* __restrict__ rep,
                                                    const int32_t* __restrict__ L0,
-                                                   const int32_t* __restrict__ L1) {
+                                                   const int32_t* __restrict__ L1, int gate1) {
+  if (gate1 && blockIdx.x == 0 && threadIdx.x == 0) gate_body(dc, 1, GATE_SEARCH, 0, 0, 0);
   if (searches_open(dc)) return;
   const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
   const int64_t n_in = (int64_t)dc->live.n;
@@ -1940,7 +1946,10 @@ __global__ __launch_bounds__(BLOCK) void k_canon_caller(DevCtl* __restrict__ dc,
 // degree class), the members of a window are one contiguous internal range
 // (the table T gives where each run enters each window), so the workgroup
 // reads labels, keys and caller ids sequentially, run by run, assembles the
-// window's labels in LDS and writes them out in caller order.  Counts the
+// window's labels in LDS and writes them out in caller order.  The label of
+// the global step's SCC (its pivot, from *gkey = the step's pivot key, 0 for
+// none) is the label of most members: its key is read once per workgroup
+// instead of by every wave (one L2 line would serve all of them).  Counts the
 // caller ids that are their component's label.  In a chain: nothing when it
 // stopped (halt odd), and a finished chain's halt is cleared.
 __global__ __launch_bounds__(BLOCK) void k_canon_window(DevCtl* __restrict__ dc, int32_t n, int32_t runs,
@@ -1948,13 +1957,33 @@ __global__ __launch_bounds__(BLOCK) void k_canon_window(DevCtl* __restrict__ dc,
                                                         const int32_t* __restrict__ label,
                                                         const unsigned long long* __restrict__ ckey,
                                                         const int32_t* __restrict__ perm,
-                                                        int32_t* __restrict__ cout) {
-  if (dc->chain.halt & 1) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0 && dc->chain.halt == 2) dc->chain.halt = 0;
+                                                        const unsigned long long* __restrict__ gkey,
+                                                        int32_t* __restrict__ cout, int gate,
+                                                        int32_t epoch) {
+  // gate >= 0: the chain's last gate (a trim loop of this epoch) runs here
+  // first, in thread 0 of workgroup 0.  Each workgroup decides from the halt
+  // its thread 0 reads; when this gate stops the chain, workgroups that read
+  // before it may still write labels, which the host's continuation rewrites
+  __shared__ int s_skip;
+  if (threadIdx.x == 0) {
+    if (gate >= 0 && blockIdx.x == 0) gate_body(dc, gate, GATE_TRIM, epoch, 0, 1);
+    const int32_t h = dc->chain.halt;
+    s_skip = h & 1;
+    if (blockIdx.x == 0 && h == 2) dc->chain.halt = 0;
+  }
+  __syncthreads();
+  if (s_skip) return;
   __shared__ int32_t s_buf[CANON_WIN];
   __shared__ int32_t s_pre[CANON_RUNS + 1], s_base[CANON_RUNS];
+  __shared__ int32_t s_hot;
+  __shared__ unsigned long long s_hk;
   const int64_t w = blockIdx.x;
   const int64_t U0 = w * CANON_WIN;
+  if (threadIdx.x == 64) {  // (a thread of the second wave: the first loads the table)
+    const unsigned long long g = *gkey;
+    s_hot = g ? pivot_of_key(g) : -1;
+    s_hk = g ? ckey[s_hot] : ~0ull;
+  }
   if (threadIdx.x < runs) {
     const int32_t a = T[w * runs + threadIdx.x], b = T[(w + 1) * runs + threadIdx.x];
     s_base[threadIdx.x] = a;
@@ -1967,7 +1996,12 @@ __global__ __launch_bounds__(BLOCK) void k_canon_window(DevCtl* __restrict__ dc,
   }
   __syncthreads();
   const int32_t total = s_pre[runs];
-  auto key = [&](int32_t l) -> unsigned long long { return (uint32_t)l < (uint32_t)n ? ckey[l] : ~0ull; };
+  const int32_t hot = s_hot;
+  const unsigned long long hk = s_hk;
+  auto key = [&](int32_t l) -> unsigned long long {
+    if (l == hot) return hk;
+    return (uint32_t)l < (uint32_t)n ? ckey[l] : ~0ull;
+  };
   // four entries per thread and step, each round of loads issued together
   for (int32_t j0 = threadIdx.x; j0 < total; j0 += 4 * BLOCK) {
     int32_t v[4];
@@ -2120,10 +2154,10 @@ void launch_classify(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, int32
                      int single) {
   LAUNCH(k_classify, grid, st, dc, n, color, label, vf, vb, pkey, rep, L0, L1, K, single);
 }
-void launch_recolor(hipStream_t st, unsigned grid, const DevCtl* dc, int32_t n, int32_t* color,
+void launch_recolor(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, int32_t* color,
                     const uint32_t* vf, const uint32_t* vb, const int32_t* rep, const int32_t* L0,
-                    const int32_t* L1) {
-  LAUNCH(k_recolor, grid, st, dc, n, color, vf, vb, rep, L0, L1);
+                    const int32_t* L1, int gate1) {
+  LAUNCH(k_recolor, grid, st, dc, n, color, vf, vb, rep, L0, L1, gate1);
 }
 void launch_cw_init(hipStream_t st, int32_t n, const int32_t* begin, const int32_t* bend, const int32_t* color,
                     unsigned long long* cw) {
@@ -2153,9 +2187,10 @@ void launch_extract(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, i
   LAUNCH(k_extract, grid, st, dc, color, label, V, L0, L1, K);
 }
 void launch_canon_window(hipStream_t st, DevCtl* dc, int32_t n, int32_t runs, int32_t nwin, const int32_t* table,
-                         const int32_t* label, const unsigned long long* ckey, const int32_t* perm, int32_t* cout) {
+                         const int32_t* label, const unsigned long long* ckey, const int32_t* perm,
+                         const unsigned long long* gkey, int32_t* cout, int gate, int32_t epoch) {
   if (n <= 0 || nwin <= 0) return;
-  LAUNCH(k_canon_window, nwin, st, dc, n, runs, table, label, ckey, perm, cout);
+  LAUNCH(k_canon_window, nwin, st, dc, n, runs, table, label, ckey, perm, gkey, cout, gate, epoch);
 }
 void launch_canon_caller(hipStream_t st, DevCtl* dc, int32_t n, const int32_t* label,
                          const unsigned long long* ckey, const int32_t* iperm, int32_t* cout, bool zero_ctr) {

@@ -81,12 +81,16 @@ def test_chain_equals_phase_by_phase(layout):
         assert on.run_method2()["chain_stop"] == 5
 
 
+@pytest.mark.parametrize("layout", [LAYOUT_INPUT, LAYOUT_DEGREE])
 @pytest.mark.parametrize("shape", [0, 1, 2, 3])
 @pytest.mark.parametrize("levels,sweeps", [(1, 1), (2, 4), (8, 1), (64, 64)])
-def test_chain_stopped_at_each_gate(shape, levels, sweeps):
+def test_chain_stopped_at_each_gate(layout, shape, levels, sweeps):
     """Chains of every length with too few (or many) level launches and sweeps:
-    wherever a gate stops them, the run ends with the oracle's labels."""
+    wherever a gate stops them, the run ends with the oracle's labels.  In the
+    degree layout the last gate runs inside the canonical-label launch and the
+    gate after the global step inside the recolouring."""
     with SccEngine() as e:
+        e.set_layout(layout)
         for name, g in _cases().items():
             if name == "golden_rmat12":
                 continue
