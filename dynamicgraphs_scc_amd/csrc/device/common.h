@@ -520,6 +520,7 @@ struct scc_ctx {
   int32_t chain_level_grid[64] = {};      // workgroups per direction of each level launch (0: full)
   int64_t chain_live[sccmi::CHAIN_GATES] = {-1, -1, -1, -1, -1, -1};
   bool chain_off = false;                 // scc_ctx_set_chain(ctx, 0)
+  bool chain_ended = false;               // the last chain ran to its end (no partition steps followed)
   int32_t trim_epoch = 0;                 // number of the last trim loop queued
   bool labels_ready = false;              // canon[] valid for the current graph
   bool ran = false;

@@ -715,12 +715,22 @@ static int run_chain(scc_ctx* c, const scc_run_opts& o, scc_run_stats* st, int* 
   c->reset_pending = false;
   c->pivot_ready = false;
   auto t0 = std::chrono::steady_clock::now();
+  // under the learned phase schedule each phase's device time goes into the
+  // phase trace (the scheduler's round features read it): events between the
+  // phases, read after the chain's one synchronisation.  Skipped when the last
+  // chain on this context ran to its end (no partition step read the trace).
+  const bool timed = o.partition_mode == SCC_PARTS_LEARNED && !c->chain_ended;
+  auto mark = [&](int i) {
+    if (timed) (void)hipEventRecord(c->ev[2 + i], c->stream);
+  };
+  mark(0);
   // the last gate queued marks the chain's end (and zeroes the counters for
   // the canonicalisation behind it)
   const int lastg = shape >= 3 && o.use_trim2 ? 4 : std::min(shape, 2);
   // gate 0: the first trim loop; its first sweep does the deferred reset and
   // leaves the global pivot key, which stands when it is the loop's only sweep
   const int32_t e0 = trim_loop(0, 0);
+  mark(1);
   if (lastg == 0) {
     launch_chain_gate(c->stream, c->dc, 0, GATE_TRIM, e0, 1, 1, 1);
   } else {  // gate 0 with the global forward-backward step's set-up and seeding
@@ -740,6 +750,7 @@ static int run_chain(scc_ctx* c, const scc_run_opts& o, scc_run_stats* st, int* 
     launch_recolor(c->stream, grid, c->dc, c->n, c->color, c->vf, c->vb, c->rep, c->live[0], c->live[1],
                    lastg >= 2);
     if (lastg == 1) launch_chain_gate(c->stream, c->dc, 1, GATE_SEARCH, 0, 0, 1, 1);
+    mark(2);
   }
   // the last gate, when it follows a trim loop, runs inside the window
   // canonicalisation (which needs no counter reset: its count was zeroed by the
@@ -748,13 +759,16 @@ static int run_chain(scc_ctx* c, const scc_run_opts& o, scc_run_stats* st, int* 
   int32_t e_last = 0;
   if (lastg >= 2) {  // gate 2: trim1 after the step (its counters zeroed for trim2 or the labels)
     const int32_t e2 = trim_loop(2, 1);
+    mark(3);
     e_last = e2;
     if (!(fold_last && lastg == 2)) launch_chain_gate(c->stream, c->dc, 2, GATE_TRIM, e2, 0, lastg == 2, 1);
   }
   if (lastg == 4) {  // gates 3 and 4: trim2, then trim1
     launch_trim2(c->stream, grid_at(2), c->dc, gp(c), c->color, c->label, c->live[0], c->live[1]);
     launch_chain_gate(c->stream, c->dc, 3, GATE_TRIM2, 0, 0, 0, 0);
+    mark(4);
     const int32_t e4 = trim_loop(4, 2);
+    mark(5);
     e_last = e4;
     if (!fold_last) launch_chain_gate(c->stream, c->dc, 4, GATE_TRIM, e4, 0, 1, 1);
   }
@@ -812,6 +826,10 @@ static int run_chain(scc_ctx* c, const scc_run_opts& o, scc_run_stats* st, int* 
     r.live_after = (int64_t)ch.live[g];
     r.aux = g == 3 ? (int64_t)ch.count[3] : ch.iters[g];
     r.ms = g == last ? ms : 0.0;
+    if (timed) {  // phase g lies between marks g and g + 1
+      float e = 0.f;
+      if (hipEventElapsedTime(&e, c->ev[2 + g], c->ev[3 + g]) == hipSuccess) r.ms = e;
+    }
     c->trace.push_back(r);
     before = r.live_after;
   }
@@ -821,6 +839,7 @@ static int run_chain(scc_ctx* c, const scc_run_opts& o, scc_run_stats* st, int* 
       c->trace.push_back(r);
     }
     st->num_sccs = (int64_t)c->h_dc->ctr[C_COUNT];
+    c->chain_ended = true;
     c->labels_ready = true;
     c->canon_stale = c->relabeled;
     c->colors_split = last >= 1;
@@ -832,6 +851,7 @@ static int run_chain(scc_ctx* c, const scc_run_opts& o, scc_run_stats* st, int* 
     return hip_check(c, hipGetLastError(), "chain");
   }
   // stopped at gate `last` (halt odd): continue from there
+  c->chain_ended = false;
   launch_ctl_init(c->stream, c->dc, 64, 0, 0, 0, 0.0);
   st->chain_stop = last;
   const bool ok = ch.ok[last] != 0;
@@ -1241,12 +1261,12 @@ int run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_out) 
     }
     return ms;
   };
-  // A fresh decomposition on the plain schedule queues its phases as one chain
-  // (run_chain) and continues here only when the chain stopped early.
+  // A fresh decomposition with plain global pivots queues its phases as one
+  // chain (run_chain) and continues here only when the chain stopped early (the
+  // learned phase schedule only decides the partition steps after it).
   int from = ST_TRIM_FIRST;
   const bool chain = !c->chain_off && !tp && !c->kernel_timing && c->reset_pending && c->fresh && c->n > 0 &&
-                     !c->colors_split && !c->multi_colors && opts.pivot_policy == SCC_PIVOT_DEGREE_PRODUCT &&
-                     opts.partition_mode != SCC_PARTS_LEARNED;
+                     !c->colors_split && !c->multi_colors && opts.pivot_policy != SCC_PIVOT_MIN_ID;
   if (chain) {
     if ((rc = check_cancel(c))) return rc;
     if ((rc = run_chain(c, opts, &st, &from))) return rc;
@@ -1266,9 +1286,13 @@ int run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_out) 
     if ((rc = check_cancel(c))) break;
     if (from <= ST_FB) {
       // the global step runs over colour -1; with caller-set colours it is a
-      // partition round
+      // partition round.  Learned pivots rank the candidates of partition
+      // rounds; the global step keeps the plain rule (one partition, whose
+      // max in x out vertex the network's degree features rank first anyway)
       int64_t sets[5] = {0, 0, 0, 0, 0};
-      if ((rc = do_fb_round(c, opts.pivot_policy, c->colors_split, -1, &st.scc_first, &st.pivot_first,
+      const int gpol = !c->colors_split && opts.pivot_policy == SCC_PIVOT_LEARNED ? SCC_PIVOT_DEGREE_PRODUCT
+                                                                                 : opts.pivot_policy;
+      if ((rc = do_fb_round(c, gpol, c->colors_split, -1, &st.scc_first, &st.pivot_first,
                             &st.fb_levels, c->kernel_timing, tp || c->kernel_timing ? sets : nullptr)))
         break;
       st.fb_f = sets[0];

@@ -539,7 +539,11 @@ __global__ __launch_bounds__(BLOCK) void k_pivot_key(const DevCtl* __restrict__ 
                                          (unsigned long long)(G.rend[v] - G.rbegin[v]);
             score = p > 0x1FFFFFFFFull ? 0x1FFFFFFFFull : p;
           } else if (policy == SCC_PIVOT_LEARNED && scores) {
-            score = scores[v];
+            // the network's estimate in 256 steps, then the plain degree
+            // product among equal estimates (saturated estimates tie often)
+            const unsigned long long p = (unsigned long long)(G.end[v] - G.begin[v]) *
+                                         (unsigned long long)(G.rend[v] - G.rbegin[v]);
+            score = ((unsigned long long)(scores[v] >> 24) << 25) | (p > 0x1FFFFFFull ? 0x1FFFFFFull : p);
           }
           key[k] = pivot_key(score, v);
           slot[k] = slot_of(c, G.n);

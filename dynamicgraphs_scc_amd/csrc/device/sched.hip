@@ -381,33 +381,46 @@ __device__ __forceinline__ void vertex_features(const GraphPtrs& G, const int32_
   x[5] = (od - id) / (od + id + 1.f);
 }
 
-// Partition sizes: pcount[slot] = live members (wave-aggregated atomics).
+// Partition sizes: pcount[slot] = live members.  Each workgroup counts its
+// entries in an LDS table (open addressing, a few probes) and adds each
+// bucket once at the end; an entry whose probes find no room adds directly.
+// Large partitions (the global one, a giant) then cost one atomic per
+// workgroup instead of one per wave and iteration on a single address.
+constexpr int PC_HASH = 1024;  // LDS buckets (a power of two)
+constexpr int PC_PROBES = 8;
+
 __global__ __launch_bounds__(BLOCK) void k_part_count(const DevCtl* __restrict__ dc, int32_t n,
                                                       const int32_t* __restrict__ color,
                                                       int32_t* __restrict__ pcount, const int32_t* __restrict__ L0,
                                                       const int32_t* __restrict__ L1) {
+  __shared__ int32_t hk[PC_HASH], hc[PC_HASH];
+  for (int i = threadIdx.x; i < PC_HASH; i += BLOCK) {
+    hk[i] = -1;
+    hc[i] = 0;
+  }
+  __syncthreads();
   const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
   const int64_t n_in = (int64_t)dc->live.n;
-  const int lane = lane_id();
-  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
-    const int64_t i = base + threadIdx.x;
-    bool pend = false;
-    int32_t slot = 0;
-    if (i < n_in) {
-      const int32_t c = color[list[i]];
-      pend = c != DONE;
-      slot = slot_of(c, n);
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_in; i += (int64_t)gridDim.x * BLOCK) {
+    const int32_t c = color[list[i]];
+    if (c == DONE) continue;
+    const int32_t s = slot_of(c, n);
+    int h = (int)(((uint32_t)s * 2654435761u) >> 22) & (PC_HASH - 1);
+    bool added = false;
+    for (int p = 0; p < PC_PROBES; p++, h = (h + 1) & (PC_HASH - 1)) {
+      int32_t k = hk[h];
+      if (k == -1) k = atomicCAS(&hk[h], -1, s);
+      if (k == -1 || k == s) {
+        atomicAdd(&hc[h], 1);
+        added = true;
+        break;
+      }
     }
-    uint64_t m = __ballot(pend);
-    while (m) {
-      const int leader = __ffsll((unsigned long long)m) - 1;
-      const int32_t S = __shfl(slot, leader);
-      const uint64_t same = __ballot(pend && slot == S);
-      if (lane == leader) atomicAdd(&pcount[S], (int32_t)__popcll(same));
-      if (pend && slot == S) pend = false;
-      m = __ballot(pend);
-    }
+    if (!added) atomicAdd(&pcount[s], 1);
   }
+  __syncthreads();
+  for (int i = threadIdx.x; i < PC_HASH; i += BLOCK)
+    if (hc[i] > 0) atomicAdd(&pcount[hk[i]], hc[i]);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_part_clear(const DevCtl* __restrict__ dc, int32_t n,
@@ -536,7 +549,7 @@ __global__ __launch_bounds__(BLOCK) void k_sched_infer(const DevCtl* __restrict_
     vote += __shfl_xor(vote, o);
     cnt += __shfl_xor(cnt, o);
   }
-  if (l == 0 && cnt > 0.f) {
+  if (acc && l == 0 && cnt > 0.f) {
     atomicAdd(&acc[0], vote);
     atomicAdd(&acc[1], cnt);
   }
@@ -715,16 +728,18 @@ int infer(scc_ctx* c, Sched* s, bool scores, double* vote) {
            c->live[0], c->live[1], s->sslots, (int32_t)m, s->pcount);
   }
   const RoundFeat rf = round_features(c);
-  (void)hipMemsetAsync(s->acc, 0, 8, c->stream);
+  // (scores alone: no vote, no read-back)
+  if (vote) (void)hipMemsetAsync(s->acc, 0, 8, c->stream);
   const unsigned grid = grid_for(std::max<int64_t>(1, (m + 15) / 16) * 64, 2048);
   LAUNCH(k_sched_infer, grid, c->stream, c->dc, gp(c), c->color, s->pcount, rf, s->P,
-         scores ? c->sched_scores : nullptr, s->acc, c->live[0], c->live[1], stride);
+         scores ? c->sched_scores : nullptr, vote ? s->acc : nullptr, c->live[0], c->live[1], stride);
   if (scores) {
     if ((rc = clear_partitions(c, s))) return rc;
   } else {
     LAUNCH(k_part_clear_sampled, (unsigned)std::max<int64_t>(1, (m + BLOCK - 1) / BLOCK), c->stream, s->sslots,
            (int32_t)m, s->pcount);
   }
+  if (!vote) return hip_check(c, hipGetLastError(), "pivot scores");
   if ((rc = hip_check(c, hipMemcpyAsync(s->h_acc, s->acc, 8, hipMemcpyDeviceToHost, c->stream), "vote")) ||
       (rc = hip_check(c, hipStreamSynchronize(c->stream), "vote sync")))
     return rc;
