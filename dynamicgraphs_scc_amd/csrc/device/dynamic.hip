@@ -406,9 +406,8 @@ __global__ __launch_bounds__(BLOCK) void k_gather_heavy(const int32_t* __restric
   }
 }
 
-// Index r of the changed row holding staged entry e: tmpoff[r] <= e < tmpoff[r+1].
-__device__ __forceinline__ int32_t row_of(const int32_t* __restrict__ tmpoff, int32_t nr, int32_t e) {
-  int32_t lo = 0, hi = nr;
+// Index r in [lo, hi) of the changed row holding staged entry e (tmpoff[lo] <= e).
+__device__ __forceinline__ int32_t row_in(const int32_t* __restrict__ tmpoff, int32_t lo, int32_t hi, int32_t e) {
   while (hi - lo > 1) {
     const int32_t mid = (lo + hi) >> 1;
     if (tmpoff[mid] <= e) lo = mid;
@@ -417,17 +416,65 @@ __device__ __forceinline__ int32_t row_of(const int32_t* __restrict__ tmpoff, in
   return lo;
 }
 
-// Stage the old entries of the changed rows (entry-parallel over the grid).
+// The staged entries in chunks of STG_CHUNK per workgroup.  One thread finds
+// the changed rows that hold a chunk's first and last entries (two binary
+// searches over all rows); their offsets are staged in LDS, where every entry
+// of the chunk then finds its row.  (An entry-parallel search over all rows
+// costs ~17 dependent loads per entry, most of an update's time when hub rows
+// hold millions of staged entries.)  A chunk spread over more than STG_ROWS
+// rows (rows with nothing staged) searches that range in memory instead.
+constexpr int32_t STG_CHUNK = 1024;
+constexpr int32_t STG_ROWS = 1024;
+
+template <class Body>
+__device__ __forceinline__ void for_staged(const int32_t* __restrict__ tmpoff, int32_t nr, int32_t total,
+                                           Body&& body) {
+  __shared__ int32_t s_off[STG_ROWS + 1];
+  __shared__ int32_t s_r0, s_nrows;
+  for (int64_t c0 = (int64_t)blockIdx.x * STG_CHUNK; c0 < total; c0 += (int64_t)gridDim.x * STG_CHUNK) {
+    const int32_t E0 = (int32_t)c0, E1 = (int32_t)min<int64_t>(total, c0 + STG_CHUNK);
+    if (threadIdx.x == 0) {
+      const int32_t r0 = row_in(tmpoff, 0, nr, E0);
+      s_r0 = r0;
+      s_nrows = row_in(tmpoff, r0, nr, E1 - 1) - r0 + 1;
+    }
+    __syncthreads();
+    const int32_t r0 = s_r0, nrows = s_nrows;
+    const bool lds = nrows <= STG_ROWS;
+    if (lds)
+      for (int32_t t = threadIdx.x; t <= nrows; t += BLOCK) s_off[t] = tmpoff[r0 + t];
+    __syncthreads();
+    for (int32_t e = E0 + (int32_t)threadIdx.x; e < E1; e += BLOCK) {
+      int32_t r, off;
+      if (lds) {
+        int32_t lo = 0, hi = nrows;
+        while (hi - lo > 1) {
+          const int32_t mid = (lo + hi) >> 1;
+          if (s_off[mid] <= e) lo = mid;
+          else hi = mid;
+        }
+        r = r0 + lo;
+        off = s_off[lo];
+      } else {
+        r = row_in(tmpoff, r0, r0 + nrows, e);
+        off = tmpoff[r];
+      }
+      body(e, r, off);
+    }
+    __syncthreads();  // s_off is restaged by the next chunk
+  }
+}
+
+// Stage the old entries of the changed rows.
 __global__ __launch_bounds__(BLOCK) void k_stage_rows(const int32_t* __restrict__ rows, int32_t nr,
                                                       const int32_t* __restrict__ tmpoff,
                                                       const int32_t* __restrict__ begin,
                                                       const int32_t* __restrict__ adj,
                                                       int32_t* __restrict__ stage) {
   const int32_t total = tmpoff[nr];
-  for (int32_t e = blockIdx.x * BLOCK + threadIdx.x; e < total; e += gridDim.x * BLOCK) {
-    const int32_t r = row_of(tmpoff, nr, e);
-    stage[e] = adj[begin[rows[r]] + (e - tmpoff[r])];
-  }
+  for_staged(tmpoff, nr, total, [&](int32_t e, int32_t r, int32_t off) {
+    stage[e] = adj[begin[rows[r]] + (e - off)];
+  });
 }
 
 // Survivors of the staged rows, written sorted to their new rooms:
@@ -444,20 +491,19 @@ __global__ __launch_bounds__(BLOCK) void k_merge_rows(const int32_t* __restrict_
                                                       const unsigned long long* __restrict__ ins,
                                                       int32_t* __restrict__ adj) {
   const int32_t total = tmpoff[nr];
-  for (int32_t e = blockIdx.x * BLOCK + threadIdx.x; e < total; e += gridDim.x * BLOCK) {
-    const int32_t r = row_of(tmpoff, nr, e);
+  for_staged(tmpoff, nr, total, [&](int32_t e, int32_t r, int32_t off) {
     const int4 g = rng[r];
     const int32_t k = stage[e];
     const unsigned long long key = ((unsigned long long)(uint32_t)rows[r] << 32) | (uint32_t)k;
     int32_t dents = 0;
     if (g.y > g.x) {
       const int32_t dp = (int32_t)lb_u64(del, g.x, g.y, key);
-      if (dp < g.y && del[dp] == key) continue;  // deleted (each copy)
+      if (dp < g.y && del[dp] == key) return;  // deleted (each copy)
       dents = dcum[dp] - dcum[g.x];
     }
     const int32_t ip = g.w > g.z ? (int32_t)lb_u64(ins, g.z, g.w, key) : g.z;
-    adj[newbeg[r] + (e - tmpoff[r]) - dents + (ip - g.z)] = k;
-  }
+    adj[newbeg[r] + (e - off) - dents + (ip - g.z)] = k;
+  });
 }
 
 // Inserted entries at their merged rank in the new room: inserts before it +
@@ -907,7 +953,7 @@ int apply_dir(scc_ctx* c, Dyn* d, bool fwd, const unsigned long long* del, int64
   int32_t* en = fwd ? c->end : c->rend;
   int32_t* adj = fwd ? c->adj : c->radj;
   (void)en;
-  const unsigned sg = grid_for(std::max<int64_t>(staged, 1), 2048);
+  const unsigned sg = (unsigned)std::min<int64_t>(2048, std::max<int64_t>(1, (staged + STG_CHUNK - 1) / STG_CHUNK));
   if (staged) {
     LAUNCH(k_stage_rows, sg, s, d->rows, (int32_t)nr, d->tmpoff, beg, adj, d->stage);
     LAUNCH(k_merge_rows, sg, s, d->rows, (int32_t)nr, d->tmpoff, d->stage, d->newbeg, d->rng, del,
