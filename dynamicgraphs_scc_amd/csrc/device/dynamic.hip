@@ -83,7 +83,7 @@ struct Dyn {
   void* tmp = nullptr;  // library temp storage
   size_t tmp_bytes = 0;
   int32_t n = -1;
-  int32_t* h_stage = nullptr;  // pinned staging for batch upload (2 * cap_b ints)
+  int32_t* h_stage = nullptr;  // pinned staging for batch upload (4 * cap_b ints: inserts, then deletes)
   int64_t rebuilds = 0;        // layout rebuilds of the current update (both directions)
 };
 
@@ -113,10 +113,11 @@ __device__ __forceinline__ int64_t lb_u64(const unsigned long long* __restrict__
 }
 
 __global__ __launch_bounds__(BLOCK) void k_make_keys(const int32_t* __restrict__ s, const int32_t* __restrict__ d,
-                                                     int64_t k, bool drop_loops,
+                                                     int64_t k, bool drop_loops, const int32_t* __restrict__ map,
                                                      unsigned long long* __restrict__ keys) {
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < k; i += (int64_t)gridDim.x * BLOCK) {
-    const int32_t u = s[i], v = d[i];
+    // (map: caller ids to internal ids in the degree layout, validated on the host)
+    const int32_t u = map ? map[s[i]] : s[i], v = map ? map[d[i]] : d[i];
     keys[i] = (drop_loops && u == v) ? BAD_KEY
                                      : (((unsigned long long)(uint32_t)u << 32) | (uint32_t)v);
   }
@@ -742,7 +743,7 @@ int ensure_dyn(scc_ctx* c, int64_t batch) {
       return rc;
     if (d->h_stage) (void)hipHostFree(d->h_stage);
     d->h_stage = nullptr;
-    if ((rc = hip_check(c, hipHostMalloc((void**)&d->h_stage, 8ull * cap, 0), "pinned staging")))
+    if ((rc = hip_check(c, hipHostMalloc((void**)&d->h_stage, 16ull * cap, 0), "pinned staging")))
       return rc;
     // changed rows: at most one per delete plus one per insert
     const int64_t cr = 2 * cap + 1;
@@ -1071,34 +1072,30 @@ extern "C" int scc_update(scc_ctx* c, const int32_t* ins_src, const int32_t* ins
   hipStream_t s = c->stream;
 
   // ---- 1. sorted, filtered batch lists ------------------------------------
-  auto upload_keys = [&](const int32_t* a, const int32_t* b, int64_t k, bool drop_loops) -> int {
+  // (each list has its own pinned region, so no copy waits for the other; the
+  // caller's ids become internal ids on the device, through iperm)
+  auto upload_keys = [&](const int32_t* a, const int32_t* b, int64_t k, bool drop_loops, int region) -> int {
     if (k == 0) return SCC_OK;
-    if (c->relabeled) {  // caller ids to internal ids (validated above)
-      for (int64_t i = 0; i < k; i++) {
-        d->h_stage[i] = c->h_iperm[a[i]];
-        d->h_stage[k + i] = c->h_iperm[b[i]];
-      }
-    } else {
-      std::memcpy(d->h_stage, a, 4ull * k);
-      std::memcpy(d->h_stage + k, b, 4ull * k);
-    }
+    int32_t* h = d->h_stage + 2 * d->cap_b * region;
+    std::memcpy(h, a, 4ull * k);
+    std::memcpy(h + k, b, 4ull * k);
     int r;
-    if ((r = hip_check(c, hipMemcpyAsync(d->src, d->h_stage, 4ull * k, hipMemcpyHostToDevice, s), "batch upload")) ||
-        (r = hip_check(c, hipMemcpyAsync(d->dst, d->h_stage + k, 4ull * k, hipMemcpyHostToDevice, s), "batch upload")))
+    if ((r = hip_check(c, hipMemcpyAsync(d->src, h, 4ull * k, hipMemcpyHostToDevice, s), "batch upload")) ||
+        (r = hip_check(c, hipMemcpyAsync(d->dst, h + k, 4ull * k, hipMemcpyHostToDevice, s), "batch upload")))
       return r;
-    LAUNCH(k_make_keys, grid_for(k), s, d->src, d->dst, k, drop_loops, d->k_in);
-    // the staging buffer is reused right away: wait for the copies
-    return hip_check(c, hipStreamSynchronize(s), "batch upload sync");
+    LAUNCH(k_make_keys, grid_for(k), s, d->src, d->dst, k, drop_loops, c->relabeled ? (const int32_t*)c->iperm : nullptr,
+           d->k_in);
+    return hip_check(c, hipGetLastError(), "batch keys");
   };
   // inserts: sorted keys (all, for the delete filter) then the absent ones
   int64_t ni = 0, nd = 0;
-  if ((rc = upload_keys(ins_src, ins_dst, n_ins, true))) return rc;
+  if ((rc = upload_keys(ins_src, ins_dst, n_ins, true, 0))) return rc;
   if ((rc = sort_keys(c, d, d->k_in, d->ins_sorted, n_ins))) return rc;
   if (n_ins) {
     LAUNCH(k_flag_ins, grid_for(n_ins), s, d->ins_sorted, n_ins, c->begin, row_end(c), c->adj, d->flags);
     if ((rc = select_flagged(c, d, d->ins_sorted, d->ins_f, n_ins, &ni))) return rc;
   }
-  if ((rc = upload_keys(del_src, del_dst, n_del, false))) return rc;
+  if ((rc = upload_keys(del_src, del_dst, n_del, false, 1))) return rc;
   if ((rc = sort_keys(c, d, d->k_in, d->k_tmp, n_del))) return rc;
   if (n_del) {
     LAUNCH(k_flag_del, grid_for(n_del), s, d->k_tmp, n_del, d->ins_sorted, n_ins, c->begin, row_end(c), c->adj,
