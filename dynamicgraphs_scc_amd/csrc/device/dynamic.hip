@@ -1150,13 +1150,22 @@ extern "C" int scc_update(scc_ctx* c, const int32_t* ins_src, const int32_t* ins
   // P = SCC of the max in x out vertex of G', by one forward-backward step over
   // all of G'.  On graphs with a giant SCC this is the new giant, merges into it
   // included, and everything left to do lies outside it.
-  launch_fill_i32(s, c->color, UNSET, c->n);
-  launch_iota(s, c->live[0], c->n);
-  if ((rc = set_live(c, 0, c->n))) return rc;
+  // From the fresh state over G' (every vertex live, no colours), one trim
+  // sweep on row lengths first: a vertex with an empty row in either direction
+  // is a singleton SCC of G' (labelled itself, outside every search below, and
+  // no inserted edge at it can close a cycle).  The sweep also leaves the
+  // step's pivot key and clear reach bitmaps.
   c->colors_split = false;
   c->multi_colors = false;
   c->classes = false;
   c->fresh = false;
+  {
+    const int64_t words = ((int64_t)c->n + 31) / 32;
+    launch_trim1_step(s, grid_for(c->n, (unsigned)c->tune.grid_cap), c->dc, gp(c), c->color, c->label, c->live[0],
+                      c->live[1], 2, 0LL, ++c->trim_epoch, c->pkey, c->vf, 2 * words);
+    if ((rc = pull_ctl(c))) return rc;
+    c->pivot_ready = true;
+  }
   int64_t psize = 0;
   if ((rc = do_fb_round(c, SCC_PIVOT_DEGREE_PRODUCT, false, -1, &psize, nullptr, nullptr, false, nullptr)))
     return rc;
