@@ -16,7 +16,8 @@ the engine's stream), with B_FB = 20|F| + 8E_F + 20|B| + 8E_B + 4|F n B|.
 cpu_baseline: the reference's own method 2 (its phase sources, timed by
 oracle/_ref/ref_bench) when it was built, else the oracle's OpenMP restatement,
 on this host's physical cores: median of 5 runs after a warm-up, a 1-thread
-figure, the CPU model, and one run on the planted graph of 8,388,608 vertices.
+figure, the CPU model, one run on the planted graph of 8,388,608 vertices, and
+R-MAT scale 16 (the reference's CPU-runnable case) timed on the CPU and the GPU.
 
 dynamic: the serving workload (R-MAT scale 23 base resident in HBM, 2,000
 insert/delete batches with sizes log-uniform in [1k, 100k], half uniform
@@ -77,12 +78,37 @@ def host_info() -> dict:
     return {"model": model, "logical_cpus": len(cpus), "physical_cores": max(1, len(cores))}
 
 
+def graphs_mod():
+    from dynamicgraphs_scc_amd import graphs
+    return graphs
+
+
+def gpu_decomposition_ms(g, reps: int = 6):
+    """Median wall time of one synchronised method-2 run on the GPU (degree
+    layout, graph resident), after a warm-up; and the caller-id labels."""
+    from dynamicgraphs_scc_amd.engine import LAYOUT_DEGREE, SccEngine
+    ts = []
+    with SccEngine(device=0) as e:
+        e.set_layout(LAYOUT_DEGREE)
+        e.upload(g)
+        for _ in range(reps):
+            e.reset()
+            e.synchronize()
+            t0 = time.perf_counter()
+            e.run_method2()
+            e.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        lab = e.labels()
+    return statistics.median(ts[1:]), lab
+
+
 def cpu_baseline(g, gpu_labels, gp=None, gp_gpu_ms=None, want_ref: bool = True):
     """The reference's own method 2 on this host's cores, same graph: one load,
     one warm-up and five timed runs (median) on every physical core available
     to the process (at most 16, the host share of one GPU), two more at one
     thread; the labels of the last run are compared with the GPU's.  Then one
-    run on the planted graph (n = 8,388,608).  Falls back to the oracle's
+    run on the planted graph (n = 8,388,608), and the reference's own CPU case
+    (R-MAT scale 16, seed 1) timed on both sides.  Falls back to the oracle's
     restatement when the reference was not built."""
     import numpy as np
     import oracle
@@ -127,6 +153,24 @@ def cpu_baseline(g, gpu_labels, gp=None, gp_gpu_ms=None, want_ref: bool = True):
                                     "gpu_ms_plain": gp_gpu_ms,
                                     "gpu_speedup": (rp["ms"][0] / gp_gpu_ms) if gp_gpu_ms else None}
                 log(f"cpu_baseline: planted 8M reference {rp['ms'][0]:.1f} ms ({time.time() - t:.1f}s)")
+            # the reference's own CPU-runnable case: R-MAT scale 16, seed 1
+            t = time.time()
+            g16 = graphs_mod().rmat(16, 16, seed=1)
+            gpu16, lab16 = gpu_decomposition_ms(g16)
+            p16 = os.path.join(td, "r16.gm")
+            g16.write_gm_binary(p16)
+            r16 = oracle.run_ref_bench(p16, threads, 6, labels_out=os.path.join(td, "lab16.bin"))
+            r16_1 = oracle.run_ref_bench(p16, 1, 6)
+            m16 = statistics.median(r16["ms"][1:])
+            m16_1 = statistics.median(r16_1["ms"][1:])
+            res["rmat16"] = {"n": g16.num_nodes, "m": g16.num_edges, "threads": threads,
+                             "ms": round(m16, 3), "ms_1thread": round(m16_1, 3),
+                             "gteps": g16.num_edges / (m16 / 1e3) / 1e9,
+                             "gpu_ms": round(gpu16, 4), "gpu_speedup": m16 / gpu16,
+                             "labels_equal_gpu": bool(np.array_equal(oracle.canonicalize(r16["labels"]), lab16)),
+                             "sample": "median of 5 runs after a warm-up, each side (GPU: one synchronised run each)"}
+            log(f"cpu_baseline: R-MAT-16 reference {m16:.2f} ms ({threads} threads), {m16_1:.2f} ms (1 thread), "
+                f"GPU {gpu16:.3f} ms ({time.time() - t:.1f}s)")
         except Exception as ex:  # fall back to the restatement
             log(f"cpu_baseline: reference harness failed ({ex}); using the restatement")
             res = {}
