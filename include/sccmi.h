@@ -46,7 +46,10 @@ typedef enum scc_status {
 typedef enum scc_pivot_policy {
   SCC_PIVOT_DEGREE_PRODUCT = 0, /* max(in-degree x out-degree), ties -> min id (plain heuristic) */
   SCC_PIVOT_MIN_ID = 1,         /* min id of the partition (src_CUDA/scc_cuda_fb_global.cu:426-440) */
-  SCC_PIVOT_LEARNED = 2         /* scores from the learned scheduler (scc_sched_* below) */
+  SCC_PIVOT_LEARNED = 2         /* scores from the learned scheduler (scc_sched_* below), the
+                                   degree product among equal scores; scc_run_method2 applies it
+                                   to the partition rounds and keeps the plain rule for its
+                                   global step (scc_fb_global applies it as asked) */
 } scc_pivot_policy;
 
 /* ---- context ------------------------------------------------------------ */
