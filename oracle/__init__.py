@@ -21,6 +21,9 @@ How each function is pinned (tests/test_oracle.py):
   colour; the trimmed set equals an independent Kahn-style degree-peeling.
 * ``trim2_pass`` (src/scc_trim2_new.cc:42-101): every removed pair is a 2-vertex
   SCC by Tarjan; on graphs made only of isolated 2-cycles every pair is found.
+* ``trim2_snapshot``: its pairs are 2-vertex SCCs by Tarjan, a sequential pass
+  removes every one of them, and on a hand-built graph where a pass in id order
+  enables a pair only mid-pass that pair is absent from the snapshot.
 * ``wcc`` (src/scc_weak.cc:65-196): W[v] equals min of the initial tags over the
   same-colour forward-reachable set of v (brute-force DFS), and every SCC lies in
   one class.
@@ -73,6 +76,8 @@ def _load():
     lib.oracle_trim1_fixpoint.argtypes = [C.c_int32, i32p, i32p, i32p, i32p, i32p, i32p]
     lib.oracle_trim2_pass.restype = C.c_int64
     lib.oracle_trim2_pass.argtypes = [C.c_int32, i32p, i32p, i32p, i32p, i32p, i32p]
+    lib.oracle_trim2_snapshot.restype = C.c_int64
+    lib.oracle_trim2_snapshot.argtypes = [C.c_int32, i32p, i32p, i32p, i32p, i32p, u8p]
     lib.oracle_wcc.restype = None
     lib.oracle_wcc.argtypes = [C.c_int32, i32p, i32p, i32p, i32p]
     lib.oracle_reach.restype = None
@@ -140,6 +145,16 @@ def trim2_pass(g, color, label):
     lib = _load()
     b, a, rb, ra = _arrs(g)
     return lib.oracle_trim2_pass(g.num_nodes, b, a, rb, ra, color, label)
+
+
+def trim2_snapshot(g, color):
+    """Vertices of the pairs trim2 accepts against the colours before a pass
+    (src/scc_trim2_new.cc:42-77): every pass removes at least these."""
+    lib = _load()
+    b, a, rb, ra = _arrs(g)
+    out = np.zeros(g.num_nodes, np.uint8)
+    lib.oracle_trim2_snapshot(g.num_nodes, b, a, rb, ra, np.ascontiguousarray(color, np.int32), out)
+    return out.astype(bool)
 
 
 def wcc(g, color):

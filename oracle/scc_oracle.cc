@@ -621,6 +621,34 @@ int64_t oracle_trim2_pass(int32_t n, const int32_t* begin, const int32_t* adj,
   return cnt;
 }
 
+// Phase-level: the pairs trim_2nd_new_main (src/scc_trim2_new.cc:42-77)
+// accepts when every colour is read as it stands before the pass.  out[v] = 1
+// for both members of each such pair.  A pass removes only pairs, so a pair
+// that qualifies before the pass still qualifies whenever it is examined (its
+// members' other same-colour neighbours can only leave): every trim2 pass, in
+// any order or in parallel, removes at least these.  Returns the pair count.
+int64_t oracle_trim2_snapshot(int32_t n, const int32_t* begin, const int32_t* adj,
+                              const int32_t* rbegin, const int32_t* radj, const int32_t* color,
+                              uint8_t* out) {
+  G g{n, begin, adj, rbegin, radj};
+  State s(g, const_cast<int32_t*>(color), nullptr);
+  memset(out, 0, n);
+  int64_t pairs = 0;
+  for (int32_t v = 0; v < n; v++) {
+    const int32_t c = color[v];
+    if (c == DONE) continue;
+    int32_t k, kk;
+    const bool fw = unique_nbr(s, g.begin, g.adj, v, c, k) && v < k && unique_nbr(s, g.begin, g.adj, k, c, kk) &&
+                    kk == v;
+    if (!fw && !(unique_nbr(s, g.rbegin, g.radj, v, c, k) && v < k &&
+                 unique_nbr(s, g.rbegin, g.radj, k, c, kk) && kk == v))
+      continue;
+    out[v] = out[k] = 1;
+    pairs++;
+  }
+  return pairs;
+}
+
 // Phase-level: the WCC fixpoint W over all vertices whose colour is not -2
 // (others untouched).
 void oracle_wcc(int32_t n, const int32_t* begin, const int32_t* adj, int32_t* color, int32_t* W) {

@@ -177,6 +177,7 @@ def test_phase_fb_global_degree_pivot_and_trim2(engine):
     best = live[deg[live] == deg[live].max()].min()
     assert pivot == best
     engine.repeat_global_trim1(0)
+    pre = engine.colors()
     removed = engine.do_global_trim2_new()
     gl = engine.labels(canonical=False)
     gc = engine.colors()
@@ -187,6 +188,56 @@ def test_phase_fb_global_degree_pivot_and_trim2(engine):
     for v in pairs[:2000]:
         members = np.flatnonzero(canon == canon[v])
         assert len(members) == 2 and gl[v] == members.min()
+    # and it removed at least every pair that qualifies against the colours
+    # before the pass (a pass in any order removes those)
+    snap = oracle.trim2_snapshot(g, pre)
+    assert (gc[snap] == -2).all()
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_phase_trim2_removes_every_pair_of_the_snapshot(layout):
+    # sparse random edges plus 3,000 planted 2-cycles; after the trim1 fixpoint
+    # (and after a global step) trim2 removes at least every pair that
+    # qualifies against the colours before the pass, each labelled with one of
+    # its members (the smaller id; in the degree layout the smaller internal
+    # id), and nothing but 2-vertex SCCs
+    rng = np.random.default_rng(21)
+    n = 40_000
+    src = rng.integers(0, n, 40_000)
+    dst = rng.integers(0, n, 40_000)
+    p = rng.permutation(n)[:6000].reshape(3000, 2)
+    src = np.concatenate([src, p[:, 0], p[:, 1]]).astype(np.int32)
+    dst = np.concatenate([dst, p[:, 1], p[:, 0]]).astype(np.int32)
+    g = graphs.from_edges(n, src, dst, dedup=True)
+    canon = scipy_canonical(g)
+    with SccEngine() as engine:
+        engine.set_layout(layout)
+        engine.upload(g)
+        _trim2_snapshot_rounds(engine, g, canon, layout)
+
+
+def _trim2_snapshot_rounds(engine, g, canon, layout):
+    for with_fb in (False, True):
+        engine.reset()
+        engine.repeat_global_trim1(0)
+        if with_fb:
+            engine.do_fw_bw_global_main()
+            engine.repeat_global_trim1(0)
+        pre = engine.colors()
+        snap = oracle.trim2_snapshot(g, pre)
+        assert snap.sum() > 1000
+        removed = engine.do_global_trim2_new()
+        gc = engine.colors()
+        gl = engine.labels(canonical=False)
+        now = (gc == -2) & (pre != -2)
+        assert removed == now.sum() >= snap.sum()
+        assert (gc[snap] == -2).all()
+        for v in np.flatnonzero(now):
+            members = np.flatnonzero(canon == canon[v])
+            assert len(members) == 2 and now[members].all()
+            assert gl[members[0]] == gl[members[1]] and gl[v] in members
+            if layout == 0:
+                assert gl[v] == members.min()
 
 
 def test_phase_wcc_roots_equal_oracle(engine):

@@ -157,6 +157,44 @@ def test_trim2_finds_isolated_pairs_and_only_true_pairs():
     assert done[2 * plain].all()
 
 
+def test_trim2_snapshot_pairs_are_pairs_every_pass_removes():
+    # sparse random graphs with 400 planted 2-cycles: snapshot pairs are 2-vertex
+    # SCCs and a sequential pass removes every one of them
+    for seed in range(4):
+        rng = np.random.default_rng(seed)
+        n = 3000
+        src = rng.integers(0, n, 2400)
+        dst = rng.integers(0, n, 2400)
+        p = rng.permutation(n)[:800].reshape(400, 2)
+        src = np.concatenate([src, p[:, 0], p[:, 1]]).astype(np.int32)
+        dst = np.concatenate([dst, p[:, 1], p[:, 0]]).astype(np.int32)
+        g = graphs.from_edges(n, src, dst, dedup=True)
+        col = np.full(g.num_nodes, -1, np.int32)
+        snap = oracle.trim2_snapshot(g, col)
+        assert snap.sum() > 0
+        canon = scipy_canonical(g)
+        for v in np.flatnonzero(snap):
+            members = np.flatnonzero(canon == canon[v])
+            assert len(members) == 2 and snap[members].all()
+        lab = np.full(g.num_nodes, -1, np.int32)
+        oracle.trim2_pass(g, col, lab)
+        assert (col[snap] == -2).all()
+
+
+def test_trim2_snapshot_excludes_pairs_enabled_mid_pass():
+    # (0,1) qualifies through its out-rows, (2,3) through its in-rows; (4,5) has
+    # an extra out-neighbour 0 and an extra in-neighbour 3, so it qualifies only
+    # once a pass in id order has removed (0,1) and (2,3)
+    src = np.array([0, 1, 4, 2, 3, 3, 4, 5], np.int32)
+    dst = np.array([1, 0, 0, 3, 2, 4, 5, 4], np.int32)
+    g = graphs.from_edges(6, src, dst)
+    col = np.full(6, -1, np.int32)
+    assert oracle.trim2_snapshot(g, col).tolist() == [True, True, True, True, False, False]
+    lab = np.full(6, -1, np.int32)
+    assert oracle.trim2_pass(g, col, lab) == 6
+    assert lab.tolist() == [0, 0, 2, 2, 4, 4]
+
+
 def _wcc_brute(g, color):
     n = g.num_nodes
     tag = np.where(np.diff(g.begin) >= 50, np.arange(n) | 0x20000000, np.arange(n))
