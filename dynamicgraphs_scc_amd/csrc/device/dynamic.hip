@@ -1169,6 +1169,7 @@ extern "C" int scc_update(scc_ctx* c, const int32_t* ins_src, const int32_t* ins
   int64_t psize = 0;
   if ((rc = do_fb_round(c, SCC_PIVOT_DEGREE_PRODUCT, false, -1, &psize, nullptr, nullptr, false, nullptr)))
     return rc;
+  c->pivot_ready = false;  // (the step returns early, key unused, when the sweep left nothing live)
   st.giant = psize;
   LAUNCH(k_unset_live, grid_for(std::max<int64_t>(1, c->n_live)), s, c->dc, c->color, c->live[0], c->live[1]);
   if ((rc = zero_ctr(c))) return rc;
