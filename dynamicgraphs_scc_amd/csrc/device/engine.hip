@@ -431,12 +431,18 @@ int traverse_both(scc_ctx* c, bool multi, int64_t* levels, bool timed, const std
   return level_loop(c, true, true, multi, levels, timed, post);
 }
 
+// Bottom-up entry threshold of a search: frontier edges per candidate (the
+// multi-partition scale on searches that serve many partitions).
+static double bu_ratio(const scc_ctx* c, bool multi) {
+  const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
+  return avg / ((double)c->tune.bu_alpha * (multi ? MULTI_BU_SCALE : 1.0));
+}
+
 int traverse_from(scc_ctx* c, bool fwd, bool multi, const int32_t* seeds,
                   const unsigned long long* d_count, int64_t* levels) {
   const int32_t* off = fwd ? c->begin : c->rbegin;
   uint32_t* V = fwd ? c->vf : c->vb;
-  const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
-  launch_ctl_init(c->stream, c->dc, 8, 0, 0, 0, avg / c->tune.bu_alpha);
+  launch_ctl_init(c->stream, c->dc, 8, 0, 0, 0, bu_ratio(c, multi));
   launch_seed_list(c->stream, grid_for(c->n_live, (unsigned)c->tune.grid_cap), c->dc, off, fwd ? row_end(c) : rrow_end(c), c->color, seeds, d_count, V, c->q[0]);
   return run_levels(c, fwd, multi, levels, false);
 }
@@ -460,11 +466,10 @@ int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int6
   if (pivot_override < 0 && policy == SCC_PIVOT_LEARNED && (rc = sched_pivots(c))) return rc;
   // counters, both search controls and (one partition) its slot; several
   // partitions reset their slots from the live list
-  const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
   // the fused first trim sweep left the global pivot key and clear bitmaps
   const bool ready = !multi && c->pivot_ready && pivot_override < 0 && policy == SCC_PIVOT_DEGREE_PRODUCT;
   c->pivot_ready = false;
-  launch_fb_setup(c->stream, c->dc, c->n, c->pkey, c->rep, multi ? 0 : (ready ? 2 : 1), avg / c->tune.bu_alpha);
+  launch_fb_setup(c->stream, c->dc, c->n, c->pkey, c->rep, multi ? 0 : (ready ? 2 : 1), bu_ratio(c, multi));
   if (multi)
     launch_reset_slots(c->stream, grid, c->dc, c->n, c->color, c->pkey, c->rep, c->live[0], c->live[1], 0);
   const int64_t words = ((int64_t)c->n + 31) / 32;
@@ -552,8 +557,7 @@ static int wcc_fixpoint(scc_ctx* c, bool seed) {
   launch_wcc_store(c->stream, grid, c->dc, c->cw, c->W, c->live[0], c->live[1]);
   c->it_hint = std::min(64, c->h_dc->loop.iters + 1);
   if (seed) {
-    const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
-    launch_ctl_init(c->stream, c->dc, 8, 0, 0, 0, avg / c->tune.bu_alpha);
+    launch_ctl_init(c->stream, c->dc, 8, 0, 0, 0, bu_ratio(c, true));
     launch_fill_u32(c->stream, c->vf, 0u, ((int64_t)c->n + 31) / 32);
   }
   launch_wcc_color(c->stream, grid, c->dc, c->color, c->W, seed ? 1 : 0, c->begin, row_end(c), c->vf, c->q[0],
@@ -734,10 +738,9 @@ static int run_chain(scc_ctx* c, const scc_run_opts& o, scc_run_stats* st, int* 
   if (lastg == 0) {
     launch_chain_gate(c->stream, c->dc, 0, GATE_TRIM, e0, 1, 1, 1);
   } else {  // gate 0 with the global forward-backward step's set-up and seeding
-    const double avg = c->n ? (double)c->m / (double)c->n : 0.0;
     const unsigned grid = grid_at(0);
     const LevelDir F = level_dir(c, true, true), B = level_dir(c, false, true);
-    launch_chain_gate_fb(c->stream, c->dc, e0, 0, c->n, c->pkey, c->rep, avg / c->tune.bu_alpha, F, B, ck(c));
+    launch_chain_gate_fb(c->stream, c->dc, e0, 0, c->n, c->pkey, c->rep, bu_ratio(c, false), F, B, ck(c));
     const int32_t max_chunks = std::max<int32_t>(1, (c->maxdeg + 1023) / 1024);
     for (int i = 0; i < c->chain_levels; i++) {
       const unsigned gi = c->chain_level_grid[i] > 0 ? std::min<unsigned>(grid, (unsigned)c->chain_level_grid[i]) : grid;

@@ -1204,7 +1204,8 @@ __device__ __forceinline__ void level_part(DevCtl* __restrict__ dc, LevelCtl* __
         lv->mode = MODE_BU;
         lv->nh = 0;
       }
-    } else if (oq < lv->prev_nq && (double)oq * dc->tune.td_beta < (double)nlive) {
+    } else if (oq < lv->prev_nq &&
+               (double)oq * (multi ? dc->tune.td_beta * MULTI_TD_SCALE : dc->tune.td_beta) < (double)nlive) {
       lv->mode = MODE_TD;
     }
     lv->out_q = lv->out_c = lv->out_h = lv->out_edges = 0;

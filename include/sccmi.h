@@ -391,8 +391,10 @@ typedef struct scc_tuning {
   int32_t huge_row;  /* top-down frontier rows at least this long are split over the grid (8192) */
   int32_t grid_cap;  /* workgroups per sweep launch (640) */
   int32_t pad;
-  float bu_alpha;    /* top-down -> bottom-up when frontier edges > candidates' edges / bu_alpha (14) */
-  float td_beta;     /* bottom-up -> top-down when frontier < live / td_beta and shrinking (24) */
+  float bu_alpha;    /* top-down -> bottom-up when frontier edges > candidates' edges / bu_alpha (14);
+                        searches serving many partitions at once use bu_alpha * 2/7 */
+  float td_beta;     /* bottom-up -> top-down when frontier < live / td_beta and shrinking (24);
+                        searches serving many partitions at once use td_beta / 3 */
 } scc_tuning;
 int scc_ctx_get_tuning(scc_ctx* ctx, scc_tuning* t);
 /* NULL restores the defaults. */
