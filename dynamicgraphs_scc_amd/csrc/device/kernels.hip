@@ -600,6 +600,9 @@ __global__ void k_force_pivot(int32_t p, int32_t n, const int32_t* __restrict__ 
   }
 }
 
+// Entries per chunk of a row cut over the grid in a top-down level
+constexpr int TD_CHUNK = 1024;  // (2048 measured 7 us slower per R-MAT-22 step, 512 3 us)
+
 // Seeds of the two searches of a step, which run together: each live vertex
 // that is its partition's pivot marks itself in both visited bitmaps and enters
 // both frontiers (or, for a row of at least huge_row entries, the chunked
@@ -660,7 +663,9 @@ __device__ void seed_single_body(DevCtl* __restrict__ dc, int32_t n, const Level
   atomicOr(&F.V[v >> 5], bit);
   atomicOr(&B.V[v >> 5], bit);
   const int32_t df = F.offe[v] - F.off[v], db = B.offe[v] - B.off[v];
-  const int32_t huge = dc->tune.huge_row;
+  // the only frontier vertex: a row longer than one chunk is cut into chunks
+  // over the grid (one wave would otherwise walk all of it alone)
+  const int32_t huge = min(dc->tune.huge_row, (int32_t)TD_CHUNK);
   if (df < huge) F.q0[dc->lv.nq++] = v;
   else F.h0[dc->lv.nh++] = v;
   if (db < huge) B.q0[dc->lv2.nq++] = v;
@@ -863,7 +868,6 @@ __device__ __forceinline__ int td_group(unsigned long long nq, unsigned long lon
   return G;
 }
 
-constexpr int TD_CHUNK = 1024;  // (2048 measured 7 us slower per R-MAT-22 step, 512 3 us)
 
 // One level.  Top-down: expand the frontier (rows of at least HUGE_ROW entries
 // are deferred to the next launch, which cuts them into TD_CHUNK pieces spread
