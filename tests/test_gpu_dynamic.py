@@ -84,6 +84,39 @@ def test_updates_on_a_dag(layout):
         assert np.array_equal(e.labels(), np.arange(n))
 
 
+@pytest.mark.parametrize("layout", [0, 1])
+def test_giant_erodes_then_regrows(layout):
+    # R-MAT-17: five delete-only batches remove 40 % of the edges (the giant
+    # splits and shrinks batch by batch), then five insert-only batches add
+    # uniform random edges (components merge back); labels equal the oracle
+    # after every batch and the device CSR equals the host model at the end
+    base = graphs.rmat(17, 12, seed=31)
+    n = base.num_nodes
+    hs, hd = base.edges()
+    rng = np.random.default_rng(32)
+    empty = np.zeros(0, np.int32)
+    with SccEngine() as e:
+        e.set_layout(layout)
+        e.upload(base)
+        e.run_method2()
+        for b in range(10):
+            if b < 5:
+                pick = rng.choice(len(hs), len(hs) // 10, replace=False)
+                ins, dels = (empty, empty), (hs[pick], hd[pick])
+            else:
+                k = 60_000
+                ins = (rng.integers(0, n, k).astype(np.int32), rng.integers(0, n, k).astype(np.int32))
+                dels = (empty, empty)
+            st = e.update(ins[0], ins[1], dels[0], dels[1])
+            hs, hd, removed = apply_update_host(n, hs, hd, ins, dels)
+            print(f"  batch {b}: giant {st['giant']} affected {st['affected']} sccs {st['num_sccs']}", flush=True)
+            cur = graphs.from_edges(n, hs, hd)
+            want = oracle.scc_canonical(cur, 8)
+            assert np.array_equal(e.labels(), want), f"batch {b}"
+            assert st["num_sccs"] == len(np.unique(want))
+        _check_graph(e, graphs.from_edges(n, hs, hd))
+
+
 def test_insert_merges_and_delete_splits(engine):
     # two 3-cycles joined one way; an inserted back edge merges them, deleting it splits again
     src = np.array([0, 1, 2, 3, 4, 5, 2], np.int32)
