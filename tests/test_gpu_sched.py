@@ -148,7 +148,8 @@ def test_shipped_weights_gain_over_plain():
     """The trained network against the method without it on a held-out planted
     graph (n = 8,388,608, seed 3; the training graphs are planted graphs of
     0.1-1.5 M vertices with other seeds): equal labels, and a gain of at least
-    5 % in the median of 5 runs (measured 1.11-1.12x on one MI355X)."""
+    8 % in the median of 5 runs (measured 1.12-1.13x on one MI355X; runs of
+    the same pair on one box differ by up to ~3 %)."""
     import time
     from dynamicgraphs_scc_amd.engine import PARTS_BOTH
     from dynamicgraphs_scc_amd.sched import load_weights
@@ -175,4 +176,4 @@ def test_shipped_weights_gain_over_plain():
         tl = timed(e, default_opts(partition_mode=PARTS_LEARNED))
         assert np.array_equal(lp, e.labels())
     print(f"  plain {tp * 1e3:.2f} ms, learned {tl * 1e3:.2f} ms, speedup {tp / tl:.3f}", flush=True)
-    assert tp / tl >= 1.05
+    assert tp / tl >= 1.08
