@@ -78,6 +78,7 @@ struct Dyn {
   int4* rng = nullptr;  // per changed row: its delete range, its insert range
   int32_t* stage = nullptr;  // staged old entries of changed rows
   int64_t stage_cap = 0;
+  int32_t* crow = nullptr;   // per chunk of staged entries: its first and last changed row (2 per chunk)
   unsigned long long* dctr = nullptr;    // DynCtr (device)
   unsigned long long* h_dctr = nullptr;  // pinned mirror
   void* tmp = nullptr;  // library temp storage
@@ -417,73 +418,128 @@ __device__ __forceinline__ int32_t row_in(const int32_t* __restrict__ tmpoff, in
   return lo;
 }
 
-// The staged entries in chunks of STG_CHUNK per workgroup.  One thread finds
-// the changed rows that hold a chunk's first and last entries (two binary
-// searches over all rows); their offsets are staged in LDS, where every entry
-// of the chunk then finds its row.  (An entry-parallel search over all rows
-// costs ~17 dependent loads per entry, most of an update's time when hub rows
-// hold millions of staged entries.)  A chunk spread over more than STG_ROWS
-// rows (rows with nothing staged) searches that range in memory instead.
-constexpr int32_t STG_CHUNK = 1024;
+// The staged entries in chunks of STG_CHUNK per workgroup.  The changed rows
+// that hold each chunk's first and last entries come from crow (k_chunk_rows:
+// two binary searches over all rows per chunk, every chunk in parallel); the
+// chunk's row offsets are staged in LDS, where every entry of the chunk then
+// finds its row.  (An entry-parallel search over all rows costs ~17 dependent
+// loads per entry, and one thread searching for its workgroup's chunk before
+// each chunk serialises 34 of them: a 100k-edge batch on R-MAT-23 stages 41 M
+// entries per direction, hub rows hit by deletes being rewritten whole.)  A
+// chunk spread over more than STG_ROWS rows (rows with nothing staged)
+// searches that range in memory instead.
+constexpr int32_t STG_CHUNK = 2048;
 constexpr int32_t STG_ROWS = 1024;
 
+__global__ __launch_bounds__(BLOCK) void k_chunk_rows(const int32_t* __restrict__ tmpoff, int32_t nr,
+                                                      int32_t* __restrict__ crow) {
+  const int32_t total = tmpoff[nr];
+  const int64_t nch = ((int64_t)total + STG_CHUNK - 1) / STG_CHUNK;
+  for (int64_t ch = (int64_t)blockIdx.x * BLOCK + threadIdx.x; ch < nch; ch += (int64_t)gridDim.x * BLOCK) {
+    const int32_t E0 = (int32_t)(ch * STG_CHUNK), E1 = (int32_t)min<int64_t>(total, (ch + 1) * STG_CHUNK);
+    const int32_t r0 = row_in(tmpoff, 0, nr, E0);
+    crow[2 * ch] = r0;
+    crow[2 * ch + 1] = row_in(tmpoff, r0, nr, E1 - 1);
+  }
+}
+
+// One chunk's entries of a thread: e[t] = E0 + thread + t * BLOCK (ok[t] when
+// inside the chunk), r[t] its changed row and off[t] that row's first staged
+// entry; r0 / nrows: the chunk's rows.
+constexpr int32_t STG_PER = STG_CHUNK / BLOCK;
+struct StagedBatch {
+  int32_t e[STG_PER], r[STG_PER], off[STG_PER];
+  bool ok[STG_PER];
+  int32_t r0, nrows;
+};
+
 template <class Body>
-__device__ __forceinline__ void for_staged(const int32_t* __restrict__ tmpoff, int32_t nr, int32_t total,
-                                           Body&& body) {
+__device__ __forceinline__ void for_staged(const int32_t* __restrict__ tmpoff, const int32_t* __restrict__ crow,
+                                           int32_t total, Body&& body) {
   __shared__ int32_t s_off[STG_ROWS + 1];
   __shared__ int32_t s_r0, s_nrows;
   for (int64_t c0 = (int64_t)blockIdx.x * STG_CHUNK; c0 < total; c0 += (int64_t)gridDim.x * STG_CHUNK) {
     const int32_t E0 = (int32_t)c0, E1 = (int32_t)min<int64_t>(total, c0 + STG_CHUNK);
     if (threadIdx.x == 0) {
-      const int32_t r0 = row_in(tmpoff, 0, nr, E0);
+      const int64_t ch = c0 / STG_CHUNK;
+      const int32_t r0 = crow[2 * ch];
       s_r0 = r0;
-      s_nrows = row_in(tmpoff, r0, nr, E1 - 1) - r0 + 1;
+      s_nrows = crow[2 * ch + 1] - r0 + 1;
     }
     __syncthreads();
-    const int32_t r0 = s_r0, nrows = s_nrows;
-    const bool lds = nrows <= STG_ROWS;
+    StagedBatch B;
+    B.r0 = s_r0;
+    B.nrows = s_nrows;
+    const bool lds = B.nrows <= STG_ROWS;
     if (lds)
-      for (int32_t t = threadIdx.x; t <= nrows; t += BLOCK) s_off[t] = tmpoff[r0 + t];
+      for (int32_t t = threadIdx.x; t <= B.nrows; t += BLOCK) s_off[t] = tmpoff[B.r0 + t];
     __syncthreads();
-    for (int32_t e = E0 + (int32_t)threadIdx.x; e < E1; e += BLOCK) {
-      int32_t r, off;
+#pragma unroll
+    for (int t = 0; t < STG_PER; t++) {
+      const int32_t e = E0 + (int32_t)threadIdx.x + t * BLOCK;
+      B.e[t] = e;
+      B.ok[t] = e < E1;
+      B.r[t] = B.r0;
+      B.off[t] = 0;
+      if (!B.ok[t]) continue;
       if (lds) {
-        int32_t lo = 0, hi = nrows;
+        int32_t lo = 0, hi = B.nrows;
         while (hi - lo > 1) {
           const int32_t mid = (lo + hi) >> 1;
           if (s_off[mid] <= e) lo = mid;
           else hi = mid;
         }
-        r = r0 + lo;
-        off = s_off[lo];
+        B.r[t] = B.r0 + lo;
+        B.off[t] = s_off[lo];
       } else {
-        r = row_in(tmpoff, r0, r0 + nrows, e);
-        off = tmpoff[r];
+        B.r[t] = row_in(tmpoff, B.r0, B.r0 + B.nrows, e);
+        B.off[t] = tmpoff[B.r[t]];
       }
-      body(e, r, off);
     }
+    body(B);  // (called by every thread: it may synchronise the workgroup)
     __syncthreads();  // s_off is restaged by the next chunk
   }
 }
 
-// Stage the old entries of the changed rows.
+// Stage the old entries of the changed rows (each thread's loads issued together).
 __global__ __launch_bounds__(BLOCK) void k_stage_rows(const int32_t* __restrict__ rows, int32_t nr,
                                                       const int32_t* __restrict__ tmpoff,
+                                                      const int32_t* __restrict__ crow,
                                                       const int32_t* __restrict__ begin,
                                                       const int32_t* __restrict__ adj,
                                                       int32_t* __restrict__ stage) {
   const int32_t total = tmpoff[nr];
-  for_staged(tmpoff, nr, total, [&](int32_t e, int32_t r, int32_t off) {
-    stage[e] = adj[begin[rows[r]] + (e - off)];
+  for_staged(tmpoff, crow, total, [&](const StagedBatch& B) {
+    int32_t x[STG_PER];
+#pragma unroll
+    for (int t = 0; t < STG_PER; t++) x[t] = B.ok[t] ? adj[begin[rows[B.r[t]]] + (B.e[t] - B.off[t])] : 0;
+#pragma unroll
+    for (int t = 0; t < STG_PER; t++)
+      if (B.ok[t]) stage[B.e[t]] = x[t];
   });
+}
+
+// First index i in [0, n) with a[i] >= key (n when none), over LDS.
+__device__ __forceinline__ int32_t lb_u64_lds(const unsigned long long* a, int32_t n, unsigned long long key) {
+  int32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
 }
 
 // Survivors of the staged rows, written sorted to their new rooms:
 // old index - #entries deleted before it + #inserted before it.  The deleted
 // entries before key x of row r are dcum[first delete key >= x] - dcum[row's
-// first delete key] (every copy of a deleted key counts).
+// first delete key] (every copy of a deleted key counts).  A chunk inside one
+// row (a hub row: most staged entries) with at most MRG_KEYS delete and insert
+// keys searches them in LDS.
+constexpr int32_t MRG_KEYS = 512;
 __global__ __launch_bounds__(BLOCK) void k_merge_rows(const int32_t* __restrict__ rows, int32_t nr,
                                                       const int32_t* __restrict__ tmpoff,
+                                                      const int32_t* __restrict__ crow,
                                                       const int32_t* __restrict__ stage,
                                                       const int32_t* __restrict__ newbeg,
                                                       const int4* __restrict__ rng,
@@ -491,19 +547,51 @@ __global__ __launch_bounds__(BLOCK) void k_merge_rows(const int32_t* __restrict_
                                                       const int32_t* __restrict__ dcum,
                                                       const unsigned long long* __restrict__ ins,
                                                       int32_t* __restrict__ adj) {
+  __shared__ unsigned long long s_del[MRG_KEYS], s_ins[MRG_KEYS];
+  __shared__ int32_t s_dcum[MRG_KEYS + 1];
   const int32_t total = tmpoff[nr];
-  for_staged(tmpoff, nr, total, [&](int32_t e, int32_t r, int32_t off) {
-    const int4 g = rng[r];
-    const int32_t k = stage[e];
-    const unsigned long long key = ((unsigned long long)(uint32_t)rows[r] << 32) | (uint32_t)k;
-    int32_t dents = 0;
-    if (g.y > g.x) {
-      const int32_t dp = (int32_t)lb_u64(del, g.x, g.y, key);
-      if (dp < g.y && del[dp] == key) return;  // deleted (each copy)
-      dents = dcum[dp] - dcum[g.x];
+  for_staged(tmpoff, crow, total, [&](const StagedBatch& B) {
+    int32_t k[STG_PER];
+#pragma unroll
+    for (int t = 0; t < STG_PER; t++) k[t] = B.ok[t] ? stage[B.e[t]] : 0;
+    const int4 g0 = rng[B.r0];
+    const int32_t nd = g0.y - g0.x, ni = g0.w - g0.z;
+    const bool fast = B.nrows == 1 && nd <= MRG_KEYS && ni <= MRG_KEYS;  // (uniform)
+    if (fast) {
+      for (int32_t i = threadIdx.x; i < nd; i += BLOCK) s_del[i] = del[g0.x + i];
+      for (int32_t i = threadIdx.x; i <= nd; i += BLOCK) s_dcum[i] = dcum[g0.x + i] - dcum[g0.x];
+      for (int32_t i = threadIdx.x; i < ni; i += BLOCK) s_ins[i] = ins[g0.z + i];
     }
-    const int32_t ip = g.w > g.z ? (int32_t)lb_u64(ins, g.z, g.w, key) : g.z;
-    adj[newbeg[r] + (e - off) - dents + (ip - g.z)] = k;
+    __syncthreads();
+    if (fast) {
+      const unsigned long long hi = (unsigned long long)(uint32_t)rows[B.r0] << 32;
+      const int32_t nb = newbeg[B.r0];
+#pragma unroll
+      for (int t = 0; t < STG_PER; t++) {
+        if (!B.ok[t]) continue;
+        const unsigned long long key = hi | (uint32_t)k[t];
+        const int32_t dp = nd ? lb_u64_lds(s_del, nd, key) : 0;
+        if (dp < nd && s_del[dp] == key) continue;  // deleted (each copy)
+        const int32_t ip = ni ? lb_u64_lds(s_ins, ni, key) : 0;
+        adj[nb + (B.e[t] - B.off[t]) - s_dcum[dp] + ip] = k[t];
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < STG_PER; t++) {
+        if (!B.ok[t]) continue;
+        const int32_t r = B.r[t];
+        const int4 g = rng[r];
+        const unsigned long long key = ((unsigned long long)(uint32_t)rows[r] << 32) | (uint32_t)k[t];
+        int32_t dents = 0;
+        if (g.y > g.x) {
+          const int32_t dp = (int32_t)lb_u64(del, g.x, g.y, key);
+          if (dp < g.y && del[dp] == key) continue;  // deleted (each copy)
+          dents = dcum[dp] - dcum[g.x];
+        }
+        const int32_t ip = g.w > g.z ? (int32_t)lb_u64(ins, g.z, g.w, key) : g.z;
+        adj[newbeg[r] + (B.e[t] - B.off[t]) - dents + (ip - g.z)] = k[t];
+      }
+    }
   });
 }
 
@@ -781,7 +869,7 @@ uint64_t update_bytes(int64_t n, int64_t m, int64_t m_max, int64_t max_batch) {
   uint64_t b = 4 * 4 * N;                       // per-row counters, marks, old-SCC flags
   b += 2 * 3 * 4 * N;                           // begin / end / room ends of a rebuilt layout
   b += 2 * 2 * 4 * g;                           // adj + radj, old and new during a rebuild
-  b += 4 * (uint64_t)mm;                        // staged rows (at most every entry)
+  b += 4 * (uint64_t)mm + 8 * ((uint64_t)mm / 2048 + 2);  // staged rows (at most every entry), chunk rows
   b += cap_b * (7 * 8 + 1 + 4 + 4);             // keys, flags, src, dst
   b += 2 * 4 * (cap_b + 1);                     // delete copies and their offsets
   b += 5 * 4 * (2 * cap_b + 2);                 // changed-row arrays
@@ -945,9 +1033,13 @@ int apply_dir(scc_ctx* c, Dyn* d, bool fwd, const unsigned long long* del, int64
   const int64_t staged = (int64_t)d->h_dctr[D_STAGED];
   if (staged > d->stage_cap) {
     dev_free(c, d->stage);
+    dev_free(c, d->crow);
     d->stage = nullptr;
+    d->crow = nullptr;
     const int64_t cap = staged + staged / 2 + 4096;
-    if ((rc = dev_alloc(c, (void**)&d->stage, 4ull * (uint64_t)cap, "staged rows"))) return rc;
+    if ((rc = dev_alloc(c, (void**)&d->stage, 4ull * (uint64_t)cap, "staged rows")) ||
+        (rc = dev_alloc(c, (void**)&d->crow, 8ull * (uint64_t)(cap / STG_CHUNK + 2), "staged chunk rows")))
+      return rc;
     d->stage_cap = cap;
   }
   int32_t* beg = fwd ? c->begin : c->rbegin;
@@ -956,8 +1048,9 @@ int apply_dir(scc_ctx* c, Dyn* d, bool fwd, const unsigned long long* del, int64
   (void)en;
   const unsigned sg = (unsigned)std::min<int64_t>(2048, std::max<int64_t>(1, (staged + STG_CHUNK - 1) / STG_CHUNK));
   if (staged) {
-    LAUNCH(k_stage_rows, sg, s, d->rows, (int32_t)nr, d->tmpoff, beg, adj, d->stage);
-    LAUNCH(k_merge_rows, sg, s, d->rows, (int32_t)nr, d->tmpoff, d->stage, d->newbeg, d->rng, del,
+    LAUNCH(k_chunk_rows, grid_for((staged + STG_CHUNK - 1) / STG_CHUNK), s, d->tmpoff, (int32_t)nr, d->crow);
+    LAUNCH(k_stage_rows, sg, s, d->rows, (int32_t)nr, d->tmpoff, d->crow, beg, adj, d->stage);
+    LAUNCH(k_merge_rows, sg, s, d->rows, (int32_t)nr, d->tmpoff, d->crow, d->stage, d->newbeg, d->rng, del,
            (const int32_t*)d->dcum, ins, adj);
   }
   if (ni)
@@ -980,7 +1073,7 @@ void destroy_dynamic(scc_ctx* c) {
   if (!d) return;
   void* ptrs[] = {d->k_in, d->k_tmp, d->del_f, d->del_r, d->ins_f, d->ins_r, d->ins_sorted,
                   d->flags, d->src, d->dst, d->dcopies, d->dcum, d->rdel, d->rins, d->rowmark, d->aff, d->cap_f, d->cap_r,
-                  d->rows, d->tmpoff, d->newbeg, d->newcap, d->lens, d->rng, d->stage, d->dctr, d->tmp};
+                  d->rows, d->tmpoff, d->newbeg, d->newcap, d->lens, d->rng, d->stage, d->crow, d->dctr, d->tmp};
   for (void* p : ptrs) dev_free(c, p);
   if (d->h_stage) (void)hipHostFree(d->h_stage);
   if (d->h_dctr) (void)hipHostFree(d->h_dctr);
