@@ -530,6 +530,8 @@ struct scc_ctx {
   int64_t chain_live[sccmi::CHAIN_GATES] = {-1, -1, -1, -1, -1, -1};
   bool chain_off = false;                 // scc_ctx_set_chain(ctx, 0)
   bool chain_ended = false;               // the last chain ran to its end (no partition steps followed)
+  bool color_pairs = true;                // a colouring round right after another first splits classes by backward reach
+  bool last_color_round = false;          // the last partition step of this decomposition was a colouring round
   int32_t trim_epoch = 0;                 // number of the last trim loop queued
   bool labels_ready = false;              // canon[] valid for the current graph
   bool ran = false;

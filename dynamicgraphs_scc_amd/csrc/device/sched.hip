@@ -751,7 +751,7 @@ int infer(scc_ctx* c, Sched* s, bool scores, double* vote) {
 struct Snap {
   std::vector<int32_t> color, label, live;
   int64_t n_live = 0;
-  bool colors_split = false, fresh = false, multi_colors = false, classes = false;
+  bool colors_split = false, fresh = false, multi_colors = false, classes = false, last_color_round = false;
   size_t trace_len = 0;
   int32_t lv_hint[3] = {0, 0, 0}, it_hint = 0, trim_hint[2] = {0, 0};  // launch-batch hints: both branches start alike
 };
@@ -772,6 +772,7 @@ int take_snap(scc_ctx* c, Snap& sn) {
   sn.fresh = c->fresh;
   sn.multi_colors = c->multi_colors;
   sn.classes = c->classes;
+  sn.last_color_round = c->last_color_round;
   sn.trace_len = c->trace.size();
   for (int i = 0; i < 3; i++) sn.lv_hint[i] = c->lv_hint[i];
   sn.it_hint = c->it_hint;
@@ -790,6 +791,7 @@ int restore_snap(scc_ctx* c, const Snap& sn) {
   c->fresh = sn.fresh;
   c->multi_colors = sn.multi_colors;
   c->classes = sn.classes;
+  c->last_color_round = sn.last_color_round;
   c->trace.resize(sn.trace_len);
   for (int i = 0; i < 3; i++) c->lv_hint[i] = sn.lv_hint[i];
   c->it_hint = sn.it_hint;
