@@ -194,10 +194,12 @@ def cpu_baseline(g, gpu_labels, gp=None, gp_gpu_ms=None, want_ref: bool = True):
 
 def learned_leg(args, eng, g, ms_plain_rmat, gp):
     """Plain vs learned scheduler on R-MAT-22 (the engine's graph, same step as
-    `value`) and on the planted-SCC graph gp (8,388,608 vertices, seed 3), with
-    the network's two heads measured apart: the phase head alone
-    (SCC_PARTS_LEARNED, plain pivots), the pivot head alone (SCC_PIVOT_LEARNED,
-    plain step order) and both.  Labels must equal the plain run's."""
+    `value`) and on the planted-SCC graph gp (8,388,608 vertices, seed 3): the
+    phase head (SCC_PARTS_LEARNED, plain pivots) and the phase head with the
+    pivot head (SCC_PIVOT_LEARNED ranks the pivots of the forward-backward steps
+    it chooses).  Plain is the default schedule (the WCC phase, then colouring
+    steps); the fixed alternation of forward-backward and colouring steps
+    (SCC_PARTS_BOTH) is timed beside it.  Labels must equal the plain run's."""
     import numpy as np
     from dynamicgraphs_scc_amd.engine import (PARTS_BOTH, PARTS_LEARNED, PIVOT_DEGREE_PRODUCT, PIVOT_LEARNED,
                                               SccEngine, default_opts)
@@ -219,19 +221,19 @@ def learned_leg(args, eng, g, ms_plain_rmat, gp):
         return float(np.median(ts))
 
     variants = {"phase": default_opts(partition_mode=PARTS_LEARNED, pivot_policy=PIVOT_DEGREE_PRODUCT),
-                "pivot": default_opts(partition_mode=PARTS_BOTH, pivot_policy=PIVOT_LEARNED),
                 "both": default_opts(partition_mode=PARTS_LEARNED, pivot_policy=PIVOT_LEARNED)}
 
     def heads(e, m, reps):
         # plain and learned runs timed alike: one run per timed call, each
         # between synchronisations (the headline's back-to-back steps overlap
         # one step's launches with the previous one's tail)
-        po = default_opts(partition_mode=PARTS_BOTH, pivot_policy=PIVOT_DEGREE_PRODUCT)
+        po = default_opts(pivot_policy=PIVOT_DEGREE_PRODUCT)
         timed(e, po, 1)
         ms_plain = timed(e, po, reps)
         lab_plain = e.labels()
+        ms_alt = timed(e, default_opts(partition_mode=PARTS_BOTH), reps)
         e.set_sched_weights(w)
-        out = {"ms_plain": round(ms_plain, 4)}
+        out = {"ms_plain": round(ms_plain, 4), "ms_alternating": round(ms_alt, 4)}
         for k, o in variants.items():
             timed(e, o, 1)
             ms = timed(e, o, reps)
@@ -245,6 +247,7 @@ def learned_leg(args, eng, g, ms_plain_rmat, gp):
         out["labels_equal"] = all(out[f"labels_equal_{k}"] for k in variants)
         out["gteps_plain"] = round(m / (ms_plain / 1e3) / 1e9, 4)
         out["learned_gteps"] = round(m / (out["ms_learned"] / 1e3) / 1e9, 4)
+        out["speedup_vs_alternating"] = round(ms_alt / out["ms_learned"], 3)
         return out
 
     res = {"rmat22": heads(eng, g.num_edges, max(5, args.steps))}
@@ -270,7 +273,7 @@ def batched_leg(args):
     del gs
     log(f"batched: {len(off) - 1} graphs, n={u.num_nodes} m={u.num_edges} in {time.time() - t:.1f}s")
 
-    from dynamicgraphs_scc_amd.engine import PARTS_LEARNED, default_opts
+    from dynamicgraphs_scc_amd.engine import PARTS_BOTH, PARTS_LEARNED, default_opts
     from dynamicgraphs_scc_amd.sched import load_weights
 
     def timed(e, reps, **kw):
@@ -291,6 +294,7 @@ def batched_leg(args):
         ms_default = timed(e, 5)
         counts = e.batch_counts()
         lab_plain = e.labels()
+        ms_alt = timed(e, 3, partition_mode=PARTS_BOTH)
         ms_learned = same = None
         if weights is not None:
             e.set_sched_weights(weights)
@@ -306,6 +310,7 @@ def batched_leg(args):
            "graphs": int(len(off) - 1), "n": int(u.num_nodes), "m": int(u.num_edges),
            "num_sccs": int(counts.sum()),
            "ms_default": round(ms_default, 3), "gteps_default": round(u.num_edges / ms_default / 1e6, 4),
+           "ms_alternating": round(ms_alt, 3),
            "ms_tuned": round(ms_tuned, 3), "gteps_tuned": round(u.num_edges / ms_tuned / 1e6, 4),
            "tuning": tune["tuning"]}
     if ms_learned is not None:

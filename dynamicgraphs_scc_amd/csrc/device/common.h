@@ -532,6 +532,7 @@ struct scc_ctx {
   bool chain_ended = false;               // the last chain ran to its end (no partition steps followed)
   bool color_pairs = true;                // a colouring round right after another first splits classes by backward reach
   bool last_color_round = false;          // the last partition step of this decomposition was a colouring round
+  bool w_fresh = false;                   // W holds the labels of the current classes (the WCC phase just ran)
   int32_t trim_epoch = 0;                 // number of the last trim loop queued
   bool labels_ready = false;              // canon[] valid for the current graph
   bool ran = false;

@@ -270,6 +270,7 @@ static void reset_host_state(scc_ctx* c) {
   c->pivot_ready = false;
   c->canon_stale = false;
   c->last_color_round = false;
+  c->w_fresh = false;
 }
 
 int reset_state(scc_ctx* c) {
@@ -298,6 +299,7 @@ int force_pivot(scc_ctx* c, int32_t pivot) {
 // repeat_global_trim1: sweeps until one trims <= stop.
 int do_trim1(scc_ctx* c, int64_t stop, int64_t* trimmed) {
   if (trimmed) *trimmed = 0;
+  c->w_fresh = false;
   c->pivot_ready = false;
   if (c->n_live == 0) return SCC_OK;
   PhaseRec pr(c, SCC_PHASE_TRIM1);
@@ -334,6 +336,7 @@ int do_trim1(scc_ctx* c, int64_t stop, int64_t* trimmed) {
 
 int do_trim2(scc_ctx* c, int64_t* removed) {
   c->fresh = false;
+  c->w_fresh = false;
   if (removed) *removed = 0;
   if (c->n_live == 0) return SCC_OK;
   PhaseRec pr(c, SCC_PHASE_TRIM2);
@@ -453,6 +456,7 @@ int traverse_from(scc_ctx* c, bool fwd, bool multi, const int32_t* seeds,
 int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int64_t* scc,
                 int32_t* pivot_out, int64_t* levels, bool timed, int64_t* sets) {
   c->fresh = false;
+  c->w_fresh = false;
   int rc;
   if (scc) *scc = 0;
   if (c->n_live == 0) return SCC_OK;
@@ -585,6 +589,7 @@ int do_wcc(scc_ctx* c, int64_t* parts) {
   if ((rc = pull_ctl(c))) return rc;
   if (parts) *parts = (int64_t)c->h_dc->ctr[C_ROOTS];
   c->classes = true;
+  c->w_fresh = true;  // (cleared by every other phase)
   pr.finish(c->h_dc->loop.iters);
   return SCC_OK;
 }
@@ -605,7 +610,18 @@ int do_color_round(scc_ctx* c, int64_t* scc) {
   // forward / backward / rest split already refines the classes, it costs more
   // than it saves, and is skipped.
   if (c->color_pairs && c->last_color_round && (rc = wcc_fixpoint(c, false, true))) return rc;
-  if ((rc = wcc_fixpoint(c, true))) return rc;
+  // Right after the WCC phase, its labels are this round's (the same minimum
+  // over the same classes): only the roots' seeding is left to do.
+  if (c->w_fresh) {
+    c->w_fresh = false;
+    launch_ctl_init(c->stream, c->dc, 1 | 8, 0, 0, 0, bu_ratio(c, true));
+    launch_fill_u32(c->stream, c->vf, 0u, ((int64_t)c->n + 31) / 32);
+    launch_wcc_color(c->stream, grid_for(c->n_live, (unsigned)c->tune.grid_cap), c->dc, c->color, c->W, 1, c->begin,
+                     row_end(c), c->vf, c->q[0], c->live[0], c->live[1], ck(c));
+    if ((rc = hip_check(c, hipGetLastError(), "colouring seeds"))) return rc;
+  } else if ((rc = wcc_fixpoint(c, true))) {
+    return rc;
+  }
   const int64_t iters = c->h_dc->loop.iters;
   if ((rc = run_levels(c, true, true, nullptr, false))) return rc;
   launch_extract(c->stream, grid_for(c->n_live, (unsigned)c->tune.grid_cap), c->dc, c->color, c->label, c->vf,
@@ -621,7 +637,7 @@ int do_color_round(scc_ctx* c, int64_t* scc) {
 int do_fb_parts(scc_ctx* c, int policy, int mode, bool trim_between, int64_t* rounds) {
   int64_t r = 0;
   int rc;
-  if (mode == SCC_PARTS_LEARNED && !sched_ready(c)) mode = SCC_PARTS_BOTH;
+  if (mode == SCC_PARTS_LEARNED && !sched_ready(c)) mode = SCC_PARTS_COLOR;
   while (c->n_live > 0) {
     if ((rc = check_cancel(c))) return rc;
     c->round_idx = (int32_t)r;
@@ -1241,7 +1257,7 @@ void scc_run_opts_default(scc_run_opts* o) {
   o->trim_between = 0;
   o->use_wcc = 1;
   o->time_phases = 0;
-  o->partition_mode = SCC_PARTS_BOTH;
+  o->partition_mode = SCC_PARTS_COLOR;
 }
 
 int scc_run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_out) {
@@ -1268,6 +1284,7 @@ int run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_out) 
   st.chain_stop = -1;
   c->labels_ready = false;
   c->last_color_round = false;
+  c->w_fresh = false;
   c->kernel_ms = 0;
   c->kernel_launches = 0;
   const bool tp = opts.time_phases != 0;
@@ -1349,7 +1366,7 @@ int run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_out) 
     st.ms_trim2 = elapsed(2, 3);
     if (c->n_live == 0) break;
     if ((rc = check_cancel(c))) break;
-    if (opts.use_wcc && !(opts.partition_mode == SCC_PARTS_LEARNED && sched_ready(c))) {
+    if (opts.use_wcc) {
       if ((rc = do_wcc(c, &st.wcc_parts))) break;
     }
     mark(4);

@@ -749,9 +749,10 @@ int infer(scc_ctx* c, Sched* s, bool scores, double* vote) {
 
 // ---- snapshot / restore of the decomposition state (counterfactual rounds) ------------
 struct Snap {
-  std::vector<int32_t> color, label, live;
+  std::vector<int32_t> color, label, live, W;
   int64_t n_live = 0;
-  bool colors_split = false, fresh = false, multi_colors = false, classes = false, last_color_round = false;
+  bool colors_split = false, fresh = false, multi_colors = false, classes = false, last_color_round = false,
+       w_fresh = false;
   size_t trace_len = 0;
   int32_t lv_hint[3] = {0, 0, 0}, it_hint = 0, trim_hint[2] = {0, 0};  // launch-batch hints: both branches start alike
 };
@@ -767,6 +768,14 @@ int take_snap(scc_ctx* c, Snap& sn) {
       (c->n_live && (rc = hip_check(c, hipMemcpyAsync(sn.live.data(), live, 4ull * c->n_live, hipMemcpyDeviceToHost, c->stream), "snapshot"))) ||
       (rc = hip_check(c, hipStreamSynchronize(c->stream), "snapshot sync")))
     return rc;
+  // the WCC labels a first colouring step reuses (both branches start alike)
+  sn.w_fresh = c->w_fresh;
+  if (sn.w_fresh) {
+    sn.W.resize(c->n);
+    if ((rc = hip_check(c, hipMemcpyAsync(sn.W.data(), c->W, 4ull * c->n, hipMemcpyDeviceToHost, c->stream), "snapshot")) ||
+        (rc = hip_check(c, hipStreamSynchronize(c->stream), "snapshot sync")))
+      return rc;
+  }
   sn.n_live = c->n_live;
   sn.colors_split = c->colors_split;
   sn.fresh = c->fresh;
@@ -786,6 +795,10 @@ int restore_snap(scc_ctx* c, const Snap& sn) {
       (rc = hip_check(c, hipMemcpyAsync(c->label, sn.label.data(), 4ull * c->n, hipMemcpyHostToDevice, c->stream), "restore")) ||
       (sn.n_live && (rc = hip_check(c, hipMemcpyAsync(c->live[0], sn.live.data(), 4ull * sn.n_live, hipMemcpyHostToDevice, c->stream), "restore"))))
     return rc;
+  if (sn.w_fresh &&
+      (rc = hip_check(c, hipMemcpyAsync(c->W, sn.W.data(), 4ull * c->n, hipMemcpyHostToDevice, c->stream), "restore")))
+    return rc;
+  c->w_fresh = sn.w_fresh;
   if ((rc = set_live(c, 0, sn.n_live))) return rc;
   c->colors_split = sn.colors_split;
   c->fresh = sn.fresh;
@@ -822,8 +835,8 @@ bool sched_ready(const scc_ctx* c) {
   return s && s->have_weights;
 }
 
-// The network overrides the plain rule (alternate, starting with a
-// forward-backward step) only when its mean vote is far from even.
+// The network overrides the plain rule (a colouring step, the default
+// partition mode's) only when its mean vote is far from even.
 constexpr double SCHED_MARGIN = 0.25;
 
 // A step kind repeated right after it resolved less than this share of the
@@ -838,12 +851,12 @@ int sched_choose(scc_ctx* c, int* action) {
   double vote = 0;
   int rc = infer(c, s, false, &vote);
   if (rc) return rc;
-  int plain = 0, last = -1;
+  const int plain = 1;
+  int last = -1;
   double progress = 1.0;
   for (auto it = c->trace.rbegin(); it != c->trace.rend(); ++it) {
     if (it->kind == SCC_PHASE_FB_ROUND || it->kind == SCC_PHASE_COLOR_ROUND) {
       last = it->kind == SCC_PHASE_FB_ROUND ? 0 : 1;
-      plain = 1 - last;
       progress = (double)(it->live_before - it->live_after) / (double)std::max<int64_t>(1, it->live_before);
       break;
     }
@@ -1035,6 +1048,7 @@ int scc_sched_rollout(scc_ctx* c, const scc_run_opts* opts_in, uint64_t seed, fl
   if (opts.use_trim2 && c->n_live) {
     if ((rc = do_trim2(c, nullptr)) || (rc = do_trim1(c, opts.trim_stop, nullptr))) return rc;
   }
+  if (opts.use_wcc && c->n_live && (rc = do_wcc(c, nullptr))) return rc;
   std::vector<int32_t> sample_v;       // sampled vertex per row
   std::vector<float> sample_part;      // its partition size when sampled
   int64_t rows = 0;
@@ -1088,8 +1102,8 @@ int scc_sched_rollout(scc_ctx* c, const scc_run_opts* opts_in, uint64_t seed, fl
     }
     // counterfactual: from the same state, each step followed by the
     // continuation policy until nothing is live (the installed network's
-    // choices, else alternating steps: the other kind next); the better step
-    // is the one whose decomposition finishes first
+    // choices, else colouring steps, the plain rule); the better step is the
+    // one whose decomposition finishes first
     if ((rc = take_snap(c, sn))) return rc;
     double total[2];
     const int first = U(rng) < 0.5f ? 0 : 1;  // random order: neither branch always runs first
@@ -1101,7 +1115,7 @@ int scc_sched_rollout(scc_ctx* c, const scc_run_opts* opts_in, uint64_t seed, fl
         if ((rc = check_cancel(c))) return rc;
         if (guard > 4 * (int64_t)c->n + 64) return set_error(c, SCC_ERR_LIMIT, "rollout did not converge");
         if ((rc = run_action(c, step, false))) return rc;
-        step = 1 - step;
+        step = 1;
         if (learned && c->n_live > 0 && (rc = sched_choose(c, &step))) return rc;
       }
       total[a] = ms_since(t0);

@@ -10,17 +10,18 @@ module drives them:
   by the continuation policy to the end; the trace records which finished
   sooner, and every sampled vertex's SCC size relative to its partition;
 * training: Adam on the device (SccEngine.sched_train), from seeded He
-  initial weights; the first round of traces continues with alternating
-  steps, and further rounds (--iterations) continue with the network trained
+  initial weights; the first round of traces continues with colouring
+  steps (the plain rule), and further rounds (--iterations) continue with the network trained
   so far (policy iteration: its one-step improvements are then measured
   against its own later choices);
-* evaluation: wall time of the plain method (degree-product pivots, a
-  forward-backward step then a colouring step per round) against the learned
-  policy on held-out graphs, labels compared for equality.
+* evaluation: wall time of the plain method (degree-product pivots, the WCC
+  phase, then colouring steps) against the learned policy on held-out graphs,
+  labels compared for equality.
 
 The trained weights ship as sched_weights.npy next to this file (FP32, 5378
 values, loaded with allow_pickle=False).  `python -m dynamicgraphs_scc_amd.sched
---out DIR` regenerates them on a GPU.
+--rollouts 4 --epochs 80 --seed 4 --graph-seed 4100 --out DIR` is how they were
+made on a GPU (the labels come from wall times, so runs differ).
 """
 from __future__ import annotations
 
@@ -33,7 +34,7 @@ import time
 import numpy as np
 
 from . import graphs
-from .engine import (PARTS_BOTH, PARTS_LEARNED, SCHED_FEATURES, SCHED_HEADS, SCHED_HIDDEN, SCHED_NPARAMS,
+from .engine import (PARTS_LEARNED, SCHED_FEATURES, SCHED_HEADS, SCHED_HIDDEN, SCHED_NPARAMS,
                      SccEngine, default_opts)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -151,7 +152,7 @@ def evaluate(weights, specs, reps: int = 5, log=print):
             if off is not None:
                 e.batch_set(off)
             b = off is not None
-            plain_opts = default_opts(partition_mode=PARTS_BOTH)
+            plain_opts = default_opts()
             e.set_sched_weights(None)
             time_runs(e, plain_opts, 1, b)
             t_plain = time_runs(e, plain_opts, reps, b)
@@ -178,6 +179,7 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--iterations", type=int, default=1,
                     help="rounds of traces + training (policy iteration beyond 1)")
+    ap.add_argument("--graph-seed", type=int, default=100, help="seed of the training graphs")
     args = ap.parse_args(argv)
 
     def log(*a):
@@ -187,7 +189,7 @@ def main(argv=None):
     for it in range(args.iterations):
         # each round learns from traces of its own continuation policy only
         t = time.time()
-        X, Y, M = generate(training_graphs(100 + 10 * it), rollouts=args.rollouts, log=log, weights=w)
+        X, Y, M = generate(training_graphs(args.graph_seed + 10 * it), rollouts=args.rollouts, log=log, weights=w)
         log(f"iteration {it}: {X.shape[0]} samples in {time.time() - t:.1f}s")
         w, loss = train(X, Y, M, epochs=args.epochs, seed=args.seed, start=w)
         log(f"iteration {it}: last batch loss {loss:.4f}")

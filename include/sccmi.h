@@ -171,14 +171,15 @@ typedef struct scc_run_opts {
  * remaining partitions at once. */
 typedef enum scc_partition_mode {
   SCC_PARTS_FB = 0,     /* one forward-backward step per partition per round (src/scc_fb_seq2.cc) */
-  SCC_PARTS_COLOR = 1,  /* WCC colouring, then each class's root SCC by a forward search */
+  SCC_PARTS_COLOR = 1,  /* (default) WCC colouring, then each class's root SCC by a forward search;
+                           the first step reuses the WCC phase's labels, and a step right after
+                           another first splits the classes by the minimum over backward reach */
   SCC_PARTS_BOTH = 2,   /* a forward-backward step, then a colouring step, per round */
   SCC_PARTS_LEARNED = 3 /* each step chosen by the learned scheduler (scc_sched_*): a
-                           forward-backward step (after a WCC colouring when the live set is
-                           not yet split into classes) or a colouring step; the separate WCC
-                           phase becomes part of that choice.  The forward-backward steps take
+                           forward-backward step or a colouring step (the plain rule unless the
+                           network's vote is far from even).  The forward-backward steps take
                            pivots by pivot_policy (SCC_PIVOT_LEARNED for the network's
-                           scores).  Without weights: SCC_PARTS_BOTH */
+                           scores).  Without weights: SCC_PARTS_COLOR */
 } scc_partition_mode;
 
 typedef struct scc_run_stats {
@@ -203,7 +204,7 @@ typedef struct scc_run_stats {
 
 /* Default options: degree-product pivot, trim_stop -70 (70 %), trim2 on, no trim between
  * partition rounds (measured faster on planted graphs), WCC on, no phase
- * timing, SCC_PARTS_BOTH. */
+ * timing, SCC_PARTS_COLOR. */
 void scc_run_opts_default(scc_run_opts* o);
 
 /* do_baseline_global_wcc_fb (src/scc_main.cc:272-351): trim1 -> global FB ->

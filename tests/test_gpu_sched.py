@@ -144,12 +144,16 @@ def test_learned_policy_without_weights_falls_back():
             e.set_sched_weights(np.full(5378, np.nan, np.float32))
 
 
-def test_shipped_weights_gain_over_plain():
-    """The trained network against the method without it on a held-out planted
-    graph (n = 8,388,608, seed 3; the training graphs are planted graphs of
-    0.1-1.5 M vertices with other seeds): equal labels, and a gain of at least
-    8 % in the median of 5 runs (measured 1.12-1.13x on one MI355X; runs of
-    the same pair on one box differ by up to ~3 %)."""
+def test_shipped_weights_against_plain_and_alternation():
+    """The trained network on a held-out planted graph (n = 8,388,608, seed 3;
+    the training graphs are planted graphs of 0.1-1.5 M vertices and small
+    batches, other seeds), median of 5 runs each: equal labels; against the
+    method without it (the WCC phase, then colouring steps), no loss beyond
+    run-to-run noise (measured 0.99-1.0x: the colouring steps, which reuse the
+    WCC labels and split classes by backward reach between consecutive steps,
+    leave the network no step choice that pays); against the fixed alternation
+    of forward-backward and colouring steps, a gain of at least 15 % (measured
+    1.25-1.27x on one MI355X)."""
     import time
     from dynamicgraphs_scc_amd.engine import PARTS_BOTH
     from dynamicgraphs_scc_amd.sched import load_weights
@@ -170,10 +174,14 @@ def test_shipped_weights_gain_over_plain():
 
     with SccEngine() as e:
         e.upload(g)
-        tp = timed(e, default_opts(partition_mode=PARTS_BOTH))
+        tp = timed(e, default_opts())
         lp = e.labels()
+        ta = timed(e, default_opts(partition_mode=PARTS_BOTH))
+        assert np.array_equal(lp, e.labels())
         e.set_sched_weights(w)
         tl = timed(e, default_opts(partition_mode=PARTS_LEARNED))
         assert np.array_equal(lp, e.labels())
-    print(f"  plain {tp * 1e3:.2f} ms, learned {tl * 1e3:.2f} ms, speedup {tp / tl:.3f}", flush=True)
-    assert tp / tl >= 1.08
+    print(f"  plain {tp * 1e3:.2f} ms, alternation {ta * 1e3:.2f} ms, learned {tl * 1e3:.2f} ms: "
+          f"{tp / tl:.3f}x plain, {ta / tl:.3f}x alternation", flush=True)
+    assert tp / tl >= 0.93
+    assert ta / tl >= 1.15
