@@ -34,7 +34,7 @@ import time
 import numpy as np
 
 from . import graphs
-from .engine import (PARTS_LEARNED, SCHED_FEATURES, SCHED_HEADS, SCHED_HIDDEN, SCHED_NPARAMS,
+from .engine import (PARTS_BOTH, PARTS_LEARNED, SCHED_FEATURES, SCHED_HEADS, SCHED_HIDDEN, SCHED_NPARAMS,
                      SccEngine, default_opts)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -143,7 +143,9 @@ def time_runs(e, opts, reps: int, batch: bool = False):
 
 
 def evaluate(weights, specs, reps: int = 5, log=print):
-    """Median wall ms of plain and learned runs; labels must be equal."""
+    """Median wall ms of the plain schedule (colouring steps), the
+    forward-backward/colouring alternation and the learned scheduler; labels
+    must be equal."""
     out = []
     with SccEngine() as e:
         for name, make in specs:
@@ -157,15 +159,20 @@ def evaluate(weights, specs, reps: int = 5, log=print):
             time_runs(e, plain_opts, 1, b)
             t_plain = time_runs(e, plain_opts, reps, b)
             lab_plain = e.labels()
+            alt_opts = default_opts(partition_mode=PARTS_BOTH)
+            time_runs(e, alt_opts, 1, b)
+            t_alt = time_runs(e, alt_opts, reps, b)
+            same_alt = bool(np.array_equal(lab_plain, e.labels()))
             e.set_sched_weights(weights)
             learned_opts = default_opts(partition_mode=PARTS_LEARNED)
             time_runs(e, learned_opts, 1, b)
             t_learned = time_runs(e, learned_opts, reps, b)
-            same = bool(np.array_equal(lab_plain, e.labels()))
+            same = same_alt and bool(np.array_equal(lab_plain, e.labels()))
             e.set_sched_weights(None)
             rec = {"graph": name, "n": g.num_nodes, "m": g.num_edges, "ms_plain": round(t_plain, 3),
-                   "ms_learned": round(t_learned, 3), "speedup": round(t_plain / t_learned, 3),
-                   "labels_equal": same}
+                   "ms_alternating": round(t_alt, 3), "ms_learned": round(t_learned, 3),
+                   "speedup": round(t_plain / t_learned, 3),
+                   "speedup_vs_alternating": round(t_alt / t_learned, 3), "labels_equal": same}
             log(f"  eval {rec}")
             out.append(rec)
     return out
