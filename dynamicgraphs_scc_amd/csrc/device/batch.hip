@@ -84,7 +84,7 @@ bool tuning_ok(const scc_tuning* t) {
   const int l = t->light_row;
   return (l == 4 || l == 8 || l == 16 || l == 32 || l == 64) && t->huge_row >= 256 && t->huge_row <= (1 << 30) &&
          t->grid_cap >= 16 && t->grid_cap <= 65536 && t->bu_alpha > 0.f && t->bu_alpha <= 1e6f &&
-         t->td_beta > 0.f && t->td_beta <= 1e6f;
+         t->td_beta > 0.f && t->td_beta <= 1e6f && t->wcc_dense >= 0 && t->wcc_dense <= 64;
 }
 
 void to_api(const Tune& a, scc_tuning* t) {
@@ -94,6 +94,7 @@ void to_api(const Tune& a, scc_tuning* t) {
   t->grid_cap = a.grid_cap;
   t->bu_alpha = a.bu_alpha;
   t->td_beta = a.td_beta;
+  t->wcc_dense = a.wcc_dense;
 }
 
 Tune from_api(const scc_tuning* t) {
@@ -103,7 +104,7 @@ Tune from_api(const scc_tuning* t) {
   a.grid_cap = t->grid_cap;
   a.bu_alpha = t->bu_alpha;
   a.td_beta = t->td_beta;
-  a.pad = 0;
+  a.wcc_dense = t->wcc_dense > 0 ? t->wcc_dense : WCC_DENSE;
   return a;
 }
 
@@ -212,7 +213,7 @@ int scc_ctx_set_tuning(scc_ctx* c, const scc_tuning* t) {
   if (!c) return SCC_ERR_ARG;
   (void)hipSetDevice(c->device);
   if (!t) {
-    c->tune = Tune{LEVEL_LIGHT, HUGE_ROW, 24.f, 14.f, GRID_CAP, 0};
+    c->tune = Tune{LEVEL_LIGHT, HUGE_ROW, 24.f, 14.f, GRID_CAP, WCC_DENSE};
     return push_tuning(c);
   }
   if (!tuning_ok(t)) return set_error(c, SCC_ERR_ARG, "tuning value out of range");
@@ -240,7 +241,8 @@ int scc_autotune(scc_ctx* c, const scc_run_opts* opts, int32_t reps, scc_tuning*
     *ms = ts[ts.size() / 2];
     return SCC_OK;
   };
-  Tune best{LIGHT, HUGE_ROW, 24.f, 14.f, 2048, 0};
+  // (from the context's defaults)
+  Tune best{LEVEL_LIGHT, HUGE_ROW, 24.f, 14.f, GRID_CAP, WCC_DENSE};
   c->tune = best;
   if ((rc = push_tuning(c))) return rc;
   double bms = 0;
@@ -251,10 +253,11 @@ int scc_autotune(scc_ctx* c, const scc_run_opts* opts, int32_t reps, scc_tuning*
   const int grids[] = {512, 1024, 2048, 4096};
   const float alphas[] = {4.f, 8.f, 14.f, 24.f, 48.f};
   const float betas[] = {8.f, 16.f, 24.f, 48.f, 96.f};
+  const int denses[] = {2, 3, 4};
   for (int pass = 0; pass < 2; pass++) {
     bool improved = false;
-    for (int knob = 0; knob < 5; knob++) {
-      const int nc = knob == 2 ? 4 : 5;
+    for (int knob = 0; knob < 6; knob++) {
+      const int nc = knob == 2 ? 4 : (knob == 5 ? 3 : 5);
       for (int i = 0; i < nc; i++) {
         Tune t = best;
         switch (knob) {
@@ -262,7 +265,8 @@ int scc_autotune(scc_ctx* c, const scc_run_opts* opts, int32_t reps, scc_tuning*
           case 1: t.huge_row = huges[i]; break;
           case 2: t.grid_cap = grids[i]; break;
           case 3: t.bu_alpha = alphas[i]; break;
-          default: t.td_beta = betas[i]; break;
+          case 4: t.td_beta = betas[i]; break;
+          default: t.wcc_dense = denses[i]; break;
         }
         if (std::memcmp(&t, &best, sizeof(Tune)) == 0) continue;
         c->tune = t;

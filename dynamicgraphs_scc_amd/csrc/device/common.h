@@ -22,6 +22,8 @@ constexpr int LEVEL_LIGHT = 32; // default of the same for search levels (tunabl
 constexpr int HUGE_ROW = 8192; // default: rows at least this long are expanded by many blocks
 constexpr int GRID_CAP = 640;  // default: workgroups of a sweep launch (2.5 per CU; more only adds
                                // per-workgroup set-up and flushes to the short levels)
+constexpr int WCC_DENSE = 2;   // default: label propagation pulls over the whole live set while
+                               // the frontier holds more than half of it
 
 // Direction switches of searches that serve many partitions at once (partition
 // rounds, colouring-round searches, a batch's first step): bottom-up levels
@@ -108,7 +110,7 @@ struct Tune {
   float td_beta;     // bottom-up -> top-down when the frontier < live / td_beta
   float bu_alpha;    // top-down -> bottom-up when frontier edges > candidates' edges / bu_alpha
   int32_t grid_cap;  // workgroups per sweep launch
-  int32_t pad;
+  int32_t wcc_dense; // propagation iterations pull while the frontier > live / wcc_dense
 };
 
 // A decomposition queued as one chain of launches (no host read-back between
@@ -486,7 +488,7 @@ struct scc_ctx {
   // per-vertex state
   int32_t *color = nullptr, *label = nullptr, *canon = nullptr, *W = nullptr;
   unsigned long long* cw = nullptr;  // WCC iterations: colour << 32 | label
-  sccmi::Tune tune = {sccmi::LEVEL_LIGHT, sccmi::HUGE_ROW, 24.f, 14.f, sccmi::GRID_CAP, 0};  // host copy of dc->tune
+  sccmi::Tune tune = {sccmi::LEVEL_LIGHT, sccmi::HUGE_ROW, 24.f, 14.f, sccmi::GRID_CAP, sccmi::WCC_DENSE};  // host copy of dc->tune
 
   // batched graphs: vertex ranges [boff[g], boff[g+1]) (scc_batch_set)
   int32_t batch_k = 0;
@@ -533,6 +535,15 @@ struct scc_ctx {
   bool color_pairs = true;                // a colouring round right after another first splits classes by backward reach
   bool last_color_round = false;          // the last partition step of this decomposition was a colouring round
   bool w_fresh = false;                   // W holds the labels of the current classes (the WCC phase just ran)
+  // Rows cut to live same-colour neighbours (offsets begin / rbegin, ends fend
+  // / frend): the first pull sweep of every propagation fixpoint writes the
+  // rows it reads, cut, and cut_out / cut_in say which directions hold this
+  // decomposition's cut rows.  Allocated on first use for an edge capacity of
+  // f_cap and f_n vertices.
+  int32_t *fadj = nullptr, *fradj = nullptr, *fend = nullptr, *frend = nullptr;
+  int64_t f_cap = 0;
+  int32_t f_n = 0;
+  bool cut_out = false, cut_in = false;
   int32_t trim_epoch = 0;                 // number of the last trim loop queued
   bool labels_ready = false;              // canon[] valid for the current graph
   bool ran = false;

@@ -1136,6 +1136,7 @@ extern "C" int scc_update(scc_ctx* c, const int32_t* ins_src, const int32_t* ins
                           scc_update_stats* stats) {
   if (!c) return SCC_ERR_ARG;
   if (int r0 = require_graph(c)) return r0;
+  c->cut_out = c->cut_in = false;  // (the rows change)
   if (!c->labels_ready) {
     // phase-by-phase callers: labels are a full decomposition once no vertex is live
     int r0 = do_trim_compact(c);

@@ -95,6 +95,60 @@ GraphPtrs gp(const scc_ctx* c) {
 }
 CanonKeys ck(const scc_ctx* c) { return CanonKeys{c->relabeled ? c->perm : nullptr, c->ckey}; }
 
+// The rows the partition steps follow, per direction: the cut rows once a
+// fixpoint has cut that direction, else the graph's (for searches, traversal
+// copies where they exist).
+struct PartRows {
+  const int32_t *b, *e, *a, *rb, *re, *ra;
+};
+static PartRows part_rows(const scc_ctx* c, bool traversal) {
+  PartRows R;
+  R.b = c->begin;
+  R.rb = c->rbegin;
+  R.e = c->cut_out ? c->fend : row_end(c);
+  R.a = c->cut_out ? c->fadj : (traversal ? tr_adj(c) : c->adj);
+  R.re = c->cut_in ? c->frend : rrow_end(c);
+  R.ra = c->cut_in ? c->fradj : (traversal ? tr_radj(c) : c->radj);
+  return R;
+}
+
+static void drop_cut_rows(scc_ctx* c) {
+  int32_t** arrs[] = {&c->fadj, &c->fradj, &c->fend, &c->frend};
+  for (auto p : arrs) {
+    dev_free(c, *p);
+    *p = nullptr;
+  }
+  c->f_cap = 0;
+  c->f_n = 0;
+  c->cut_out = c->cut_in = false;
+}
+
+// The cut rows' buffers (8 (m + n) bytes), allocated on first use and kept
+// while they are large enough.  False without room in the budget or on the
+// device, and in a call that allocates nothing: the steps then follow the
+// graph's rows.  Also false while fewer than 1/8 of the vertices are live (an
+// update's affected region): the first cut of a direction starts by clearing
+// every row end, which would cost more than the region's sweeps save.
+static bool cut_buffers(scc_ctx* c) {
+  if (c->no_alloc || 8 * c->n_live < (int64_t)c->n) return false;
+  if (c->fadj && c->f_cap >= c->m_cap && c->f_n >= c->n) return true;
+  drop_cut_rows(c);
+  const uint64_t cap = (uint64_t)std::max<int64_t>(c->m_cap, 1);
+  if (c->budget && c->bytes + 8ull * cap + 8ull * (uint64_t)c->n + 1024 > c->budget) return false;
+  if (dev_alloc(c, (void**)&c->fadj, 4ull * cap, "cut rows") ||
+      dev_alloc(c, (void**)&c->fradj, 4ull * cap, "cut rows") ||
+      dev_alloc(c, (void**)&c->fend, 4ull * c->n, "cut row ends") ||
+      dev_alloc(c, (void**)&c->frend, 4ull * c->n, "cut row ends")) {
+    drop_cut_rows(c);
+    (void)hipGetLastError();
+    c->err.clear();
+    return false;
+  }
+  c->f_cap = c->m_cap;
+  c->f_n = c->n;
+  return true;
+}
+
 // Copy the control block to the host and refresh the host copies of the live list.
 int pull_ctl(scc_ctx* c) {
   int rc = hip_check(c, hipMemcpyAsync(c->h_dc, c->dc, sizeof(DevCtl), hipMemcpyDeviceToHost, c->stream),
@@ -185,6 +239,7 @@ int free_graph(scc_ctx* c) {
     dev_free(c, *p);
     *p = nullptr;
   }
+  drop_cut_rows(c);
   dev_free(c, c->vf);  // vb shares this allocation
   dev_free(c, c->pkey);
   dev_free(c, c->ckey);
@@ -271,6 +326,7 @@ static void reset_host_state(scc_ctx* c) {
   c->canon_stale = false;
   c->last_color_round = false;
   c->w_fresh = false;
+  c->cut_out = c->cut_in = false;
 }
 
 int reset_state(scc_ctx* c) {
@@ -353,13 +409,14 @@ int do_trim2(scc_ctx* c, int64_t* removed) {
 // backward search uses its own lists so that it can run alongside the forward one.
 static LevelDir level_dir(scc_ctx* c, bool fwd, bool own_lists) {
   LevelDir d;
-  d.off = fwd ? c->begin : c->rbegin;
-  d.offe = fwd ? row_end(c) : rrow_end(c);
-  d.nb = fwd ? tr_adj(c) : tr_radj(c);
-  d.roff = fwd ? c->rbegin : c->begin;
-  d.roffe = fwd ? rrow_end(c) : row_end(c);
-  d.rnb = fwd ? tr_radj(c) : tr_adj(c);
-  d.rhead = fwd ? c->head_in : c->head_out;
+  const PartRows R = part_rows(c, true);
+  d.off = fwd ? R.b : R.rb;
+  d.offe = fwd ? R.e : R.re;
+  d.nb = fwd ? R.a : R.ra;
+  d.roff = fwd ? R.rb : R.b;
+  d.roffe = fwd ? R.re : R.e;
+  d.rnb = fwd ? R.ra : R.a;
+  d.rhead = fwd ? (c->cut_in ? nullptr : c->head_in) : (c->cut_out ? nullptr : c->head_out);
   d.V = fwd ? c->vf : c->vb;
   const bool b = !fwd && own_lists;
   d.q0 = b ? c->bq[0] : c->q[0];
@@ -547,16 +604,35 @@ static int wcc_fixpoint(scc_ctx* c, bool seed, bool in_dir = false) {
   int rc;
   const unsigned grid = grid_for(c->n_live, (unsigned)c->tune.grid_cap);
   const int64_t words = ((int64_t)c->n + 31) / 32;
-  const int32_t* pb = in_dir ? c->rbegin : c->begin;
-  const int32_t* pe = in_dir ? rrow_end(c) : row_end(c);
-  const int32_t* pa = in_dir ? c->radj : c->adj;
-  const int32_t* qb = in_dir ? c->begin : c->rbegin;
-  const int32_t* qe = in_dir ? row_end(c) : rrow_end(c);
-  const int32_t* qa = in_dir ? c->adj : c->radj;
+  // the first sweep pulls over the rows of the fixpoint's direction and writes
+  // them cut (from the graph's rows the first time in a decomposition, in
+  // place after that); the later iterations follow the cut rows
+  int32_t *fe = nullptr, *fout = nullptr;
+  bool& cut = in_dir ? c->cut_in : c->cut_out;
+  if (cut_buffers(c)) {
+    fe = in_dir ? c->frend : c->fend;
+    fout = in_dir ? c->fradj : c->fadj;
+    // (rows of vertices resolved before the first cut stay empty)
+    if (!cut && (rc = hip_check(c, hipMemcpyAsync(fe, in_dir ? c->rbegin : c->begin, 4ull * c->n,
+                                                  hipMemcpyDeviceToDevice, c->stream),
+                                "cut row ends")))
+      return rc;
+  }
+  const PartRows R0 = part_rows(c, false);
   launch_ctl_init(c->stream, c->dc, 1 | 4 | 8, 0, 0, 0, 0.0);
   launch_fill_u32(c->stream, c->vf, 0u, 2 * words);
-  launch_cw_init(c->stream, c->n, pb, pe, c->color, c->cw);
-  launch_wcc_pull(c->stream, grid, c->dc, pb, pe, pa, c->cw, c->q[0], c->live[0], c->live[1]);
+  // (tags from the graph's row lengths)
+  launch_cw_init(c->stream, c->n, in_dir ? c->rbegin : c->begin, in_dir ? rrow_end(c) : row_end(c), c->color, c->cw);
+  launch_wcc_pull(c->stream, grid, c->dc, in_dir ? R0.rb : R0.b, in_dir ? R0.re : R0.e, in_dir ? R0.ra : R0.a,
+                  c->cw, c->q[0], c->live[0], c->live[1], fe, fout);
+  if (fout) cut = true;
+  const PartRows R = part_rows(c, false);
+  const int32_t* pb = in_dir ? R.rb : R.b;
+  const int32_t* pe = in_dir ? R.re : R.e;
+  const int32_t* pa = in_dir ? R.ra : R.a;
+  const int32_t* qb = in_dir ? R.b : R.rb;
+  const int32_t* qe = in_dir ? R.e : R.re;
+  const int32_t* qa = in_dir ? R.a : R.ra;
   int batch = std::max(2, c->it_hint);
   for (int guard = 0;; guard++) {
     if ((rc = check_cancel(c))) return rc;
@@ -574,8 +650,8 @@ static int wcc_fixpoint(scc_ctx* c, bool seed, bool in_dir = false) {
     launch_ctl_init(c->stream, c->dc, 8, 0, 0, 0, bu_ratio(c, true));
     launch_fill_u32(c->stream, c->vf, 0u, ((int64_t)c->n + 31) / 32);
   }
-  launch_wcc_color(c->stream, grid, c->dc, c->color, c->W, seed ? 1 : 0, c->begin, row_end(c), c->vf, c->q[0],
-                   c->live[0], c->live[1], ck(c));
+  launch_wcc_color(c->stream, grid, c->dc, c->color, c->W, seed ? 1 : 0, R.b, R.e, c->vf, c->q[0], c->live[0],
+                   c->live[1], ck(c));
   c->colors_split = true;
   return hip_check(c, hipGetLastError(), "wcc colour");
 }
@@ -616,8 +692,9 @@ int do_color_round(scc_ctx* c, int64_t* scc) {
     c->w_fresh = false;
     launch_ctl_init(c->stream, c->dc, 1 | 8, 0, 0, 0, bu_ratio(c, true));
     launch_fill_u32(c->stream, c->vf, 0u, ((int64_t)c->n + 31) / 32);
-    launch_wcc_color(c->stream, grid_for(c->n_live, (unsigned)c->tune.grid_cap), c->dc, c->color, c->W, 1, c->begin,
-                     row_end(c), c->vf, c->q[0], c->live[0], c->live[1], ck(c));
+    const PartRows R = part_rows(c, false);
+    launch_wcc_color(c->stream, grid_for(c->n_live, (unsigned)c->tune.grid_cap), c->dc, c->color, c->W, 1, R.b, R.e,
+                     c->vf, c->q[0], c->live[0], c->live[1], ck(c));
     if ((rc = hip_check(c, hipGetLastError(), "colouring seeds"))) return rc;
   } else if ((rc = wcc_fixpoint(c, true))) {
     return rc;
@@ -1285,6 +1362,7 @@ int run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_out) 
   c->labels_ready = false;
   c->last_color_round = false;
   c->w_fresh = false;
+  c->cut_out = c->cut_in = false;
   c->kernel_ms = 0;
   c->kernel_launches = 0;
   const bool tp = opts.time_phases != 0;

@@ -115,7 +115,8 @@ void launch_recolor(hipStream_t st, unsigned grid, DevCtl* dc, int32_t n, int32_
 void launch_cw_init(hipStream_t st, int32_t n, const int32_t* begin, const int32_t* bend, const int32_t* color,
                     unsigned long long* cw);
 void launch_wcc_pull(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* begin, const int32_t* bend,
-                     const int32_t* adj, unsigned long long* cw, int32_t* q0, const int32_t* L0, const int32_t* L1);
+                     const int32_t* adj, unsigned long long* cw, int32_t* q0, const int32_t* L0, const int32_t* L1,
+                     int32_t* fe = nullptr, int32_t* fout = nullptr);
 // later WCC iterations: pull sweeps over the live list while the frontier is
 // large, frontier pushes over the reverse rows after
 void launch_wcc_push(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* begin, const int32_t* bend,

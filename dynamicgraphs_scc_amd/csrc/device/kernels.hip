@@ -130,6 +130,7 @@ __device__ void ctl_init_body(DevCtl* dc, int what, int32_t live_sel, unsigned l
     dc->tune.td_beta = 24.f;
     dc->tune.bu_alpha = 14.f;
     dc->tune.grid_cap = GRID_CAP;
+    dc->tune.wcc_dense = WCC_DENSE;
   }
   if (what & 8) init_level(dc->lv, bu_ratio);
   if (what & 32) init_level(dc->lv2, bu_ratio);
@@ -1452,12 +1453,18 @@ __global__ __launch_bounds__(BLOCK) void k_cw_init(int32_t n, const int32_t* __r
 // to its root's label when the root is not self-rooted.  Vertices whose label
 // dropped are appended to qout (count at *qcount).  All threads of the
 // workgroup; workgroup bid of nblk.
+// With CUT, the sweep also writes each row cut to the neighbours it matched
+// (live, same colour) at fout[begin[v]...fe[v]): colours do not change within
+// a fixpoint and only split between steps, so later sweeps and steps may
+// follow the cut rows.  fout may be adj itself (bend == fe): a row is
+// compacted by the thread or wave that reads it, every store at or behind
+// entries already loaded.
+template <bool CUT>
 __device__ __forceinline__ void wcc_pull_sweep(const int32_t* __restrict__ list, int64_t n_in,
-                                               const int32_t* __restrict__ begin,
-                                               const int32_t* __restrict__ bend,
-                                               const int32_t* __restrict__ adj,
-                                               unsigned long long* __restrict__ cw,
-                                               int32_t* __restrict__ qout, unsigned long long* qcount) {
+                                               const int32_t* __restrict__ begin, const int32_t* bend,
+                                               const int32_t* adj, unsigned long long* __restrict__ cw,
+                                               int32_t* __restrict__ qout, unsigned long long* qcount,
+                                               int32_t* fe, int32_t* fout) {
   const int lane = lane_id();
   using Out = LdsOut<0>;
   for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < n_in; base += (int64_t)gridDim.x * BLOCK) {
@@ -1476,6 +1483,7 @@ __device__ __forceinline__ void wcc_pull_sweep(const int32_t* __restrict__ list,
     const bool act = hv != (uint32_t)DONE;
     const int len = act ? e - s : 0;
     const int lim = len < 32 ? len : 0;
+    int32_t k = s;  // (CUT) next store of v's row
     // four neighbours per step, their loads issued together
     for (int j = 0; j < lim; j += 4) {
       int32_t a[4];
@@ -1486,8 +1494,15 @@ __device__ __forceinline__ void wcc_pull_sweep(const int32_t* __restrict__ list,
       for (int t = 0; t < 4; t++) y[t] = a[t] >= 0 ? cw[a[t]] : 0ull;
 #pragma unroll
       for (int t = 0; t < 4; t++)
-        if (a[t] >= 0 && cw_hi(y[t]) == hv) m = min(m, cw_lo(y[t]));
+        if (a[t] >= 0 && cw_hi(y[t]) == hv) {
+          m = min(m, cw_lo(y[t]));
+          if (CUT) {
+            if (fout != adj || k != s + j + t) fout[k] = a[t];  // (in place: entries not yet moved stay)
+            k++;
+          }
+        }
     }
+    if (CUT && act && len < 32) fe[v] = k;
     uint64_t hm = __ballot(len >= 32);
     while (hm) {
       const int L = __ffsll((unsigned long long)hm) - 1;
@@ -1495,6 +1510,7 @@ __device__ __forceinline__ void wcc_pull_sweep(const int32_t* __restrict__ list,
       const int32_t hs = __shfl(s, L), he = __shfl(e, L);
       const uint32_t hh = __shfl(hv, L);
       int32_t hmin = INT32_MAX;
+      int32_t w = hs;  // (CUT) next store of the row, the same in every lane
       for (int32_t b = hs; b < he; b += 256) {
         int32_t a[4];
 #pragma unroll
@@ -1505,13 +1521,28 @@ __device__ __forceinline__ void wcc_pull_sweep(const int32_t* __restrict__ list,
         unsigned long long y[4];
 #pragma unroll
         for (int t = 0; t < 4; t++) y[t] = a[t] >= 0 ? cw[a[t]] : 0ull;
+        bool keep[4];
 #pragma unroll
-        for (int t = 0; t < 4; t++)
-          if (a[t] >= 0 && cw_hi(y[t]) == hh) hmin = min(hmin, cw_lo(y[t]));
+        for (int t = 0; t < 4; t++) {
+          keep[t] = a[t] >= 0 && cw_hi(y[t]) == hh;
+          if (keep[t]) hmin = min(hmin, cw_lo(y[t]));
+        }
+        if (CUT) {
+#pragma unroll
+          for (int t = 0; t < 4; t++) {
+            const uint64_t km = __ballot(keep[t]);
+            const int32_t pos = w + lanes_below(km);
+            if (keep[t] && (fout != adj || pos != b + t * 64 + lane)) fout[pos] = a[t];
+            w += __popcll(km);
+          }
+        }
       }
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) hmin = min(hmin, __shfl_xor(hmin, o));
-      if (lane == L) m = min(m, hmin);
+      if (lane == L) {
+        m = min(m, hmin);
+        if (CUT) fe[v] = w;
+      }
     }
     bool changed = false;
     if (act) {
@@ -1531,15 +1562,20 @@ __device__ __forceinline__ void wcc_pull_sweep(const int32_t* __restrict__ list,
 }
 
 // First iteration: a pull sweep over the live list; the vertices whose label
-// dropped form the frontier of the later iterations.
+// dropped form the frontier of the later iterations.  With fout, the sweep
+// cuts the rows it reads (wcc_pull_sweep<true>).
 __global__ __launch_bounds__(BLOCK) void k_wcc_pull(DevCtl* __restrict__ dc, const int32_t* __restrict__ begin,
-                                                    const int32_t* __restrict__ bend,
-                                                    const int32_t* __restrict__ adj,
+                                                    const int32_t* bend, const int32_t* adj,
                                                     unsigned long long* __restrict__ cw,
                                                     int32_t* __restrict__ q0, const int32_t* __restrict__ L0,
-                                                    const int32_t* __restrict__ L1) {
+                                                    const int32_t* __restrict__ L1, int32_t* fe, int32_t* fout) {
   LdsOut<0>::init();
-  wcc_pull_sweep(LIVE_IN(dc, L0, L1), (int64_t)dc->live.n, begin, bend, adj, cw, q0, &dc->lv.out_q);
+  if (fout)
+    wcc_pull_sweep<true>(LIVE_IN(dc, L0, L1), (int64_t)dc->live.n, begin, bend, adj, cw, q0, &dc->lv.out_q, fe,
+                         fout);
+  else
+    wcc_pull_sweep<false>(LIVE_IN(dc, L0, L1), (int64_t)dc->live.n, begin, bend, adj, cw, q0, &dc->lv.out_q,
+                          nullptr, nullptr);
   if (last_block2(dc->done) && threadIdx.x == 0) {
     LevelCtl& lv = dc->lv;
     const unsigned long long nq = ld_acq(&lv.out_q);
@@ -1556,7 +1592,7 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_pull(DevCtl* __restrict__ dc, con
 }
 
 // Later iterations.  While the frontier (vertices whose label dropped) holds
-// more than 1/WCC_DENSE of the live set, an iteration is another pull sweep
+// more than 1/wcc_dense of the live set (a tuning knob), an iteration is another pull sweep
 // over the whole live list; below that, every frontier vertex pushes its
 // label to its same-colour in-neighbours (reverse rows) with a 64-bit
 // atomicMin, and a neighbour whose label dropped joins the next frontier once
@@ -1564,7 +1600,6 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_pull(DevCtl* __restrict__ dc, con
 // the claims of its own input frontier, so a bitmap is clean when it is
 // reused).  Ends when an iteration lowers nothing.  The fixpoint is the pull
 // fixpoint: W[v] = min tag over v's same-colour forward-reachable set.
-constexpr int WCC_DENSE = 2;
 __global__ __launch_bounds__(BLOCK) void k_wcc_push(DevCtl* __restrict__ dc, const int32_t* __restrict__ begin,
                                                     const int32_t* __restrict__ bend,
                                                     const int32_t* __restrict__ adj,
@@ -1592,7 +1627,7 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_push(DevCtl* __restrict__ dc, con
   uint32_t* __restrict__ Bout = (s_t & 1) ? B1 : B0;
   const int64_t nq = (int64_t)s_nq;
   const int64_t n_live = (int64_t)dc->live.n;
-  const bool dense = nq * WCC_DENSE > n_live;
+  const bool dense = nq * dc->tune.wcc_dense > n_live;
   const int lane = lane_id();
   using Out = LdsOut<0>;
   Out::init();
@@ -1604,7 +1639,8 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_push(DevCtl* __restrict__ dc, con
     }
   }
   if (dense) {
-    wcc_pull_sweep(LIVE_IN(dc, L0, L1), n_live, begin, bend, adj, cw, qout, &dc->lv.out_q);
+    wcc_pull_sweep<false>(LIVE_IN(dc, L0, L1), n_live, begin, bend, adj, cw, qout, &dc->lv.out_q, nullptr,
+                          nullptr);
   } else {
     // push label x to four in-neighbours u[t] (u[t] < 0: none); add[t] when
     // the push lowered u[t]'s label and claimed it for the next frontier.
@@ -2173,8 +2209,9 @@ void launch_cw_init(hipStream_t st, int32_t n, const int32_t* begin, const int32
   if (n > 0) LAUNCH(k_cw_init, grid_for(n, 4096), st, n, begin, bend, color, cw);
 }
 void launch_wcc_pull(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* begin, const int32_t* bend,
-                     const int32_t* adj, unsigned long long* cw, int32_t* q0, const int32_t* L0, const int32_t* L1) {
-  LAUNCH(k_wcc_pull, grid, st, dc, begin, bend, adj, cw, q0, L0, L1);
+                     const int32_t* adj, unsigned long long* cw, int32_t* q0, const int32_t* L0, const int32_t* L1,
+                     int32_t* fe, int32_t* fout) {
+  LAUNCH(k_wcc_pull, grid, st, dc, begin, bend, adj, cw, q0, L0, L1, fe, fout);
 }
 void launch_wcc_push(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* begin, const int32_t* bend,
                      const int32_t* adj, const int32_t* rbegin, const int32_t* rend, const int32_t* radj,

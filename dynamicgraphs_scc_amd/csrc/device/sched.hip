@@ -799,6 +799,7 @@ int restore_snap(scc_ctx* c, const Snap& sn) {
       (rc = hip_check(c, hipMemcpyAsync(c->W, sn.W.data(), 4ull * c->n, hipMemcpyHostToDevice, c->stream), "restore")))
     return rc;
   c->w_fresh = sn.w_fresh;
+  c->cut_out = c->cut_in = false;  // (rows cut in the other branch may lack edges of this state)
   if ((rc = set_live(c, 0, sn.n_live))) return rc;
   c->colors_split = sn.colors_split;
   c->fresh = sn.fresh;

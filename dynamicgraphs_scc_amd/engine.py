@@ -111,10 +111,10 @@ class RolloutStats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("light_row", C.c_int32), ("huge_row", C.c_int32), ("grid_cap", C.c_int32),
-                ("pad", C.c_int32), ("bu_alpha", C.c_float), ("td_beta", C.c_float)]
+                ("wcc_dense", C.c_int32), ("bu_alpha", C.c_float), ("td_beta", C.c_float)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class UpdateStats(C.Structure):
@@ -452,7 +452,7 @@ class SccEngine:
         return t.as_dict()
 
     def set_tuning(self, **kw) -> None:
-        """Set knobs (light_row, huge_row, grid_cap, bu_alpha, td_beta); none = defaults."""
+        """Set knobs (light_row, huge_row, grid_cap, wcc_dense, bu_alpha, td_beta); none = defaults."""
         if not kw:
             self._check(self.lib.scc_ctx_set_tuning(self.h, None))
             return

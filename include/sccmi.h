@@ -63,7 +63,11 @@ int scc_ctx_destroy(scc_ctx* ctx);
 const char* scc_last_error(const scc_ctx* ctx);
 /* Device bytes currently held by the context. */
 uint64_t scc_ctx_bytes(const scc_ctx* ctx);
-/* Device bytes a graph of n vertices and m edges needs in a context. */
+/* Device bytes a graph of n vertices and m edges needs in a context.  With room
+ * beyond it (no budget, or a budget above it), the partition steps also keep a
+ * copy of the rows cut to each class (8 (m + n) bytes, made at the first label
+ * propagation over at least 1/8 of the vertices); without that room they follow
+ * the graph's rows. */
 uint64_t scc_bytes_needed(int32_t n, int64_t m);
 /* Free and total device memory of a HIP device (hipMemGetInfo). */
 int scc_device_memory(int device, uint64_t* free_bytes, uint64_t* total_bytes);
@@ -391,7 +395,9 @@ typedef struct scc_tuning {
                         wave the rest; one of 4, 8, 16, 32, 64 (32) */
   int32_t huge_row;  /* top-down frontier rows at least this long are split over the grid (8192) */
   int32_t grid_cap;  /* workgroups per sweep launch (640) */
-  int32_t pad;
+  int32_t wcc_dense; /* label propagation iterations are pull sweeps over the whole live set while
+                        the frontier holds more than 1/wcc_dense of it, else frontier pushes (2;
+                        0 = the default) */
   float bu_alpha;    /* top-down -> bottom-up when frontier edges > candidates' edges / bu_alpha (14);
                         searches serving many partitions at once use bu_alpha * 2/7 */
   float td_beta;     /* bottom-up -> top-down when frontier < live / td_beta and shrinking (24);

@@ -6,8 +6,8 @@ import pytest
 
 import oracle
 from dynamicgraphs_scc_amd import graphs
-from dynamicgraphs_scc_amd.engine import (PIVOT_DEGREE_PRODUCT, PIVOT_MIN_ID, SccEngine,
-                                          SccError, default_opts)
+from dynamicgraphs_scc_amd.engine import (PARTS_BOTH, PARTS_COLOR, PIVOT_DEGREE_PRODUCT, PIVOT_MIN_ID,
+                                          SccEngine, SccError, default_opts)
 from helpers import golden_cases, load_golden, random_graph, same_partition, scipy_canonical
 
 pytestmark = pytest.mark.gpu
@@ -103,6 +103,40 @@ def test_planted_small_many_partitions(engine):
     for policy in (PIVOT_DEGREE_PRODUCT, PIVOT_MIN_ID):
         lab, _ = run(engine, g, pivot_policy=policy)
         assert np.array_equal(lab, graphs.canonicalize(comp))
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_cut_rows_same_labels_as_graph_rows(layout):
+    """Once a label propagation has run, the partition steps follow rows cut to
+    each class (8 (m + n) more bytes); a context whose budget has no room for
+    them follows the graph's rows.  Both give the planted partition in both
+    partition modes, on repeated runs (the cut rows are remade per run), and
+    through the phase calls (WCC, which cuts the out-rows, then
+    forward-backward rounds that follow them)."""
+    g, comp = graphs.planted(120000, seed=11, n_medium=3000, with_components=True)
+    want = graphs.canonicalize(comp)
+    with SccEngine() as free, SccEngine() as tight:
+        for e in (free, tight):
+            e.set_layout(layout)
+            e.upload(g)
+        tight.set_budget(tight.bytes + (1 << 20))
+        for mode in (PARTS_COLOR, PARTS_BOTH, PARTS_COLOR):
+            for e in (free, tight):
+                print(f"  layout {layout} mode {mode} budget {e is tight}", flush=True)
+                e.reset()
+                e.run_method2(partition_mode=mode)
+                assert np.array_equal(e.labels(), want), (mode, e is tight)
+        assert free.bytes >= tight.bytes + 8 * g.num_edges
+        for e in (free, tight):
+            e.reset()
+            e.repeat_global_trim1(0)
+            e.do_fw_bw_global_main()
+            e.repeat_global_trim1(0)
+            e.do_global_trim2_new()
+            e.repeat_global_trim1(0)
+            e.do_global_wcc()
+            e.start_workers_fw_bw_dfs()
+            assert np.array_equal(e.labels(), want), e is tight
 
 
 def test_repeated_runs_and_geometry_stable(engine):

@@ -153,7 +153,7 @@ def test_shipped_weights_against_plain_and_alternation():
     WCC labels and split classes by backward reach between consecutive steps,
     leave the network no step choice that pays); against the fixed alternation
     of forward-backward and colouring steps, a gain of at least 15 % (measured
-    1.25-1.27x on one MI355X)."""
+    1.25-1.30x on one MI355X)."""
     import time
     from dynamicgraphs_scc_amd.engine import PARTS_BOTH
     from dynamicgraphs_scc_amd.sched import load_weights
