@@ -519,7 +519,11 @@ struct scc_ctx {
   bool colors_split = false;              // colours other than -1/-2 exist (after FB/WCC/set)
   bool fresh = false;                     // state just reset: all colours -1, nothing done
   int32_t lv_hint[3] = {8, 8, 8};         // level launches per search batch (fw, bw, both)
-  int32_t it_hint = 4;                    // WCC iterations per batch
+  int32_t it_hint = 4;                    // WCC iterations per batch (the last fixpoint's)
+  int32_t it_hints[32] = {};              // the same per fixpoint ordinal of a decomposition (0: none yet)
+  int32_t fix_idx = 0;                    // ordinal of the next fixpoint in this decomposition
+  int32_t lv_hints[32] = {};              // level launches per search ordinal of a decomposition (0: none yet)
+  int32_t search_idx = 0;                 // ordinal of the next search in this decomposition
   int32_t trim_hint[2] = {3, 3};          // trim sweeps per batch (absolute stop, relative stop)
   // the queued chain of a fresh decomposition (ChainCtl), sized by the last one:
   // its last gate (0 first trim loop, 1 global search, 2 second trim loop, 3
@@ -567,6 +571,7 @@ struct scc_ctx {
 
   // timing
   hipEvent_t ev[8] = {};
+  hipEvent_t ev_rb = nullptr;   // end of the last control-block copy (ctl_readback_start)
   std::vector<hipEvent_t> kev;  // per-launch event pairs for kernel timing
   double kernel_ms = 0;
   int64_t kernel_launches = 0;

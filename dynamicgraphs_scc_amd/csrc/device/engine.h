@@ -16,6 +16,9 @@ void dev_free(scc_ctx* c, void* p);
 GraphPtrs gp(const scc_ctx* c);
 CanonKeys ck(const scc_ctx* c);  // canonical-key pointers of the context
 int pull_ctl(scc_ctx* c);   // control block -> host (syncs the stream)
+// pull_ctl in two halves: queue the copy (and an event), later wait for it
+int ctl_readback_start(scc_ctx* c);
+int ctl_readback_wait(scc_ctx* c);
 int zero_ctr(scc_ctx* c);   // zero DevCtl::ctr (queued)
 int set_live(scc_ctx* c, int sel, int64_t n);
 int check_cancel(scc_ctx* c);

@@ -150,6 +150,24 @@ static bool cut_buffers(scc_ctx* c) {
 }
 
 // Copy the control block to the host and refresh the host copies of the live list.
+// pull_ctl in two halves, so that launches can be queued behind the copy
+// before the host waits for it (launches that only read the control block's
+// end state, or return at once once the loop it records is done).
+int ctl_readback_start(scc_ctx* c) {
+  if (int rc = hip_check(c, hipMemcpyAsync(c->h_dc, c->dc, sizeof(DevCtl), hipMemcpyDeviceToHost, c->stream),
+                         "control readback"))
+    return rc;
+  return hip_check(c, hipEventRecord(c->ev_rb, c->stream), "control readback event");
+}
+int ctl_readback_wait(scc_ctx* c) {
+  int rc;
+  if ((rc = hip_check(c, hipEventSynchronize(c->ev_rb), "control readback wait"))) return rc;
+  if ((rc = hip_check(c, hipGetLastError(), "kernel launch"))) return rc;
+  c->cur = c->h_dc->live.sel;
+  c->n_live = (int64_t)c->h_dc->live.n;
+  return SCC_OK;
+}
+
 int pull_ctl(scc_ctx* c) {
   int rc = hip_check(c, hipMemcpyAsync(c->h_dc, c->dc, sizeof(DevCtl), hipMemcpyDeviceToHost, c->stream),
                      "control readback");
@@ -327,6 +345,8 @@ static void reset_host_state(scc_ctx* c) {
   c->last_color_round = false;
   c->w_fresh = false;
   c->cut_out = c->cut_in = false;
+  c->fix_idx = 0;
+  c->search_idx = 0;
 }
 
 int reset_state(scc_ctx* c) {
@@ -439,11 +459,14 @@ static int level_loop(scc_ctx* c, bool both, bool fwd, bool multi, int64_t* leve
   const int32_t max_chunks = std::max<int32_t>(1, (c->maxdeg + 1023) / 1024);
   const LevelDir F = level_dir(c, fwd || both, both), B = level_dir(c, false, true);
   int& hint = c->lv_hint[both ? 2 : (fwd ? 0 : 1)];
-  int batch = std::max(2, hint), rc;
-  for (int guard = 0;; guard++) {
-    if ((rc = check_cancel(c))) return rc;
-    if (timed) (void)hipEventRecord(c->ev[0], c->stream);
-    for (int i = 0; i < batch; i++) {
+  // batches sized by the search of the same ordinal last time, else by the
+  // last search of this kind; outside kernel timing, a few launches are queued
+  // behind each control-block copy (they return at once when the search has
+  // ended), so the device is not idle while the host reads it back
+  const int slot = std::min(c->search_idx++, (int32_t)(sizeof(c->lv_hints) / sizeof(c->lv_hints[0])) - 1);
+  int batch = c->lv_hints[slot] > 0 ? c->lv_hints[slot] : std::max(2, hint), rc;
+  auto levels_q = [&](int k) {
+    for (int i = 0; i < k; i++) {
       if (both)
         launch_level2(c->stream, 2 * grid, c->dc, multi ? 1 : 0, F, B, c->color, c->live[0], c->live[1],
                       max_chunks, c->tune.light);
@@ -451,9 +474,20 @@ static int level_loop(scc_ctx* c, bool both, bool fwd, bool multi, int64_t* leve
         launch_level(c->stream, grid, c->dc, multi ? 1 : 0, F, c->color, c->live[0], c->live[1], max_chunks,
                      c->tune.light);
     }
+  };
+  for (int guard = 0;; guard++) {
+    if ((rc = check_cancel(c))) return rc;
+    if (timed) (void)hipEventRecord(c->ev[0], c->stream);
+    levels_q(batch);
     if (timed) (void)hipEventRecord(c->ev[1], c->stream);
     if (post) post();
-    if ((rc = pull_ctl(c))) return rc;
+    if (timed) {
+      if ((rc = pull_ctl(c))) return rc;
+    } else {
+      if ((rc = ctl_readback_start(c))) return rc;
+      levels_q(std::min(batch, 2));
+      if ((rc = ctl_readback_wait(c))) return rc;
+    }
     if (timed) {
       float ms = 0;
       (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
@@ -470,6 +504,7 @@ static int level_loop(scc_ctx* c, bool both, bool fwd, bool multi, int64_t* leve
   // next time, queue one more launch than this search needed (its last level
   // admits nothing)
   hint = std::min(64, used + 1);
+  c->lv_hints[slot] = hint;
   return SCC_OK;
 }
 
@@ -633,25 +668,38 @@ static int wcc_fixpoint(scc_ctx* c, bool seed, bool in_dir = false) {
   const int32_t* qb = in_dir ? R.b : R.rb;
   const int32_t* qe = in_dir ? R.e : R.re;
   const int32_t* qa = in_dir ? R.a : R.ra;
-  int batch = std::max(2, c->it_hint);
-  for (int guard = 0;; guard++) {
-    if ((rc = check_cancel(c))) return rc;
-    for (int i = 0; i < batch; i++)
+  // Push launches in batches, sized by what the fixpoint of the same ordinal
+  // needed last time (repeated decompositions of a graph repeat their
+  // fixpoints).  The control block is copied after each batch, and a few more
+  // launches are queued behind the copy before the host waits for it, so the
+  // device is not idle while the host reads it back; launches that find the
+  // fixpoint done return at once.
+  const int slot = std::min(c->fix_idx++, (int32_t)(sizeof(c->it_hints) / sizeof(c->it_hints[0])) - 1);
+  int batch = c->it_hints[slot] > 0 ? c->it_hints[slot] : std::max(4, c->it_hint);
+  auto pushes = [&](int k) {
+    for (int i = 0; i < k; i++)
       launch_wcc_push(c->stream, grid, c->dc, pb, pe, pa, qb, qe, qa, c->cw, c->q[0], c->q[1], c->vf, c->vb,
                       c->live[0], c->live[1]);
-    if ((rc = pull_ctl(c))) return rc;
+  };
+  pushes(batch);
+  for (int guard = 0;; guard++) {
+    if ((rc = check_cancel(c))) return rc;
+    if ((rc = ctl_readback_start(c))) return rc;
+    pushes(std::min(batch, 4));
+    if ((rc = ctl_readback_wait(c))) return rc;
     if (c->h_dc->loop.done) break;
-    batch = std::max(4, batch);
+    batch = std::min(2 * batch, 64);
+    pushes(batch);
     if (guard > 4 * (int64_t)c->n + 64) return set_error(c, SCC_ERR_LIMIT, "WCC did not converge");
   }
-  launch_wcc_store(c->stream, grid, c->dc, c->cw, c->W, c->live[0], c->live[1]);
   c->it_hint = std::min(64, c->h_dc->loop.iters + 1);
+  c->it_hints[slot] = c->it_hint;
   if (seed) {
     launch_ctl_init(c->stream, c->dc, 8, 0, 0, 0, bu_ratio(c, true));
     launch_fill_u32(c->stream, c->vf, 0u, ((int64_t)c->n + 31) / 32);
   }
-  launch_wcc_color(c->stream, grid, c->dc, c->color, c->W, seed ? 1 : 0, R.b, R.e, c->vf, c->q[0], c->live[0],
-                   c->live[1], ck(c));
+  launch_wcc_color(c->stream, grid, c->dc, c->color, c->cw, c->W, seed ? 1 : 0, R.b, R.e, c->vf, c->q[0],
+                   c->live[0], c->live[1], ck(c));
   c->colors_split = true;
   return hip_check(c, hipGetLastError(), "wcc colour");
 }
@@ -693,8 +741,8 @@ int do_color_round(scc_ctx* c, int64_t* scc) {
     launch_ctl_init(c->stream, c->dc, 1 | 8, 0, 0, 0, bu_ratio(c, true));
     launch_fill_u32(c->stream, c->vf, 0u, ((int64_t)c->n + 31) / 32);
     const PartRows R = part_rows(c, false);
-    launch_wcc_color(c->stream, grid_for(c->n_live, (unsigned)c->tune.grid_cap), c->dc, c->color, c->W, 1, R.b, R.e,
-                     c->vf, c->q[0], c->live[0], c->live[1], ck(c));
+    launch_wcc_color(c->stream, grid_for(c->n_live, (unsigned)c->tune.grid_cap), c->dc, c->color, nullptr, c->W, 1,
+                     R.b, R.e, c->vf, c->q[0], c->live[0], c->live[1], ck(c));
     if ((rc = hip_check(c, hipGetLastError(), "colouring seeds"))) return rc;
   } else if ((rc = wcc_fixpoint(c, true))) {
     return rc;
@@ -1023,6 +1071,7 @@ int scc_ctx_create(int device, uint64_t hbm_budget_bytes, scc_ctx** out) {
   c->h_ctr = c->h_dc->ctr;
   std::memset(c->h_dc, 0, sizeof(DevCtl));
   for (auto& e : c->ev) (void)hipEventCreate(&e);
+  (void)hipEventCreateWithFlags(&c->ev_rb, hipEventDisableTiming);
   launch_ctl_init(c->stream, c->dc, 1 | 2 | 4 | 8 | 16 | 64, 0, 0, 0, 0.0);
   if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "init"))) {
     scc_ctx_destroy(c);
@@ -1044,6 +1093,7 @@ int scc_ctx_destroy(scc_ctx* c) {
   if (c->h_dc) (void)hipHostFree(c->h_dc);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  if (c->ev_rb) (void)hipEventDestroy(c->ev_rb);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return SCC_OK;
@@ -1363,6 +1413,8 @@ int run_method2(scc_ctx* c, const scc_run_opts* opts_in, scc_run_stats* st_out) 
   c->last_color_round = false;
   c->w_fresh = false;
   c->cut_out = c->cut_in = false;
+  c->fix_idx = 0;
+  c->search_idx = 0;
   c->kernel_ms = 0;
   c->kernel_launches = 0;
   const bool tp = opts.time_phases != 0;

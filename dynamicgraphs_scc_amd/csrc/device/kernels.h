@@ -123,11 +123,11 @@ void launch_wcc_push(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* b
                      const int32_t* adj, const int32_t* rbegin, const int32_t* rend, const int32_t* radj,
                      unsigned long long* cw, int32_t* q0, int32_t* q1, uint32_t* B0, uint32_t* B1,
                      const int32_t* L0, const int32_t* L1);
-void launch_wcc_store(hipStream_t st, unsigned grid, const DevCtl* dc, const unsigned long long* cw, int32_t* W,
-                      const int32_t* L0, const int32_t* L1);
-void launch_wcc_color(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, const int32_t* W,
-                      int seed, const int32_t* off, const int32_t* offe, uint32_t* V, int32_t* q0, const int32_t* L0,
-                      const int32_t* L1, CanonKeys K);
+// colour = class root from the labels (from cw when given, storing them to W,
+// else from W); seed: the roots also seed the extraction search
+void launch_wcc_color(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, const unsigned long long* cw,
+                      int32_t* W, int seed, const int32_t* off, const int32_t* offe, uint32_t* V, int32_t* q0,
+                      const int32_t* L0, const int32_t* L1, CanonKeys K);
 void launch_extract(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, int32_t* label,
                     const uint32_t* V, int32_t* L0, int32_t* L1, CanonKeys K);
 // canonical labels into out, and (perm != NULL) the caller-id labels into

@@ -1631,8 +1631,9 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_push(DevCtl* __restrict__ dc, con
   const int lane = lane_id();
   using Out = LdsOut<0>;
   Out::init();
-  if (!s_pulled) {
+  if (!s_pulled && dense) {
     // the input frontier's claims were made by the previous (push) iteration
+    // (a push iteration clears them as it reads its input, below)
     for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < nq; i += (int64_t)gridDim.x * BLOCK) {
       const int32_t k = qin[i];
       atomicAnd(&Bin[k >> 5], ~(1u << (k & 31)));
@@ -1672,6 +1673,7 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_push(DevCtl* __restrict__ dc, con
       bool act = false;
       if (i < nq) {
         k = qin[i];
+        if (!s_pulled) atomicAnd(&Bin[k >> 5], ~(1u << (k & 31)));
         x = cw[k];
         s = rbegin[k];
         e = rend[k];
@@ -1726,24 +1728,14 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_push(DevCtl* __restrict__ dc, con
   }
 }
 
-// W of the live vertices from the packed words.
-__global__ __launch_bounds__(BLOCK) void k_wcc_store(const DevCtl* __restrict__ dc,
-                                                     const unsigned long long* __restrict__ cw,
-                                                     int32_t* __restrict__ W, const int32_t* __restrict__ L0,
-                                                     const int32_t* __restrict__ L1) {
-  const int32_t* __restrict__ list = LIVE_IN(dc, L0, L1);
-  const int64_t n_in = (int64_t)dc->live.n;
-  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_in; i += (int64_t)gridDim.x * BLOCK) {
-    const int32_t v = list[i];
-    W[v] = cw_lo(cw[v]);
-  }
-}
-
 // src/scc_weak.cc:166-194: each class takes its root's id as colour (a unique
 // colour: the root is a member of its class); counts roots.  With seed != 0 the
 // roots also seed a forward search (V, q0, dc->lv) for root-SCC extraction.
+// With cw, the labels come from the fixpoint's packed words and are stored to
+// W on the way; without, they are read from W.
 __global__ __launch_bounds__(BLOCK) void k_wcc_color(DevCtl* __restrict__ dc, int32_t* __restrict__ color,
-                                                     const int32_t* __restrict__ W, int seed,
+                                                     const unsigned long long* __restrict__ cw,
+                                                     int32_t* __restrict__ W, int seed,
                                                      const int32_t* __restrict__ off, const int32_t* __restrict__ offe,
                                                      uint32_t* __restrict__ V, int32_t* __restrict__ q0,
                                                      const int32_t* __restrict__ L0,
@@ -1760,8 +1752,15 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_color(DevCtl* __restrict__ dc, in
     int32_t v = 0;
     if (i < n_in) {
       v = list[i];
+      int32_t w;
+      if (cw) {
+        w = cw_lo(cw[v]);
+        W[v] = w;
+      } else {
+        w = W[v];
+      }
       if (color[v] != DONE) {
-        const int32_t r = W[v] & WCC_MASK;
+        const int32_t r = w & WCC_MASK;
         color[v] = r;
         root = r == v;
         if (root) {
@@ -2219,14 +2218,10 @@ void launch_wcc_push(hipStream_t st, unsigned grid, DevCtl* dc, const int32_t* b
                      const int32_t* L0, const int32_t* L1) {
   LAUNCH(k_wcc_push, grid, st, dc, begin, bend, adj, rbegin, rend, radj, cw, q0, q1, B0, B1, L0, L1);
 }
-void launch_wcc_store(hipStream_t st, unsigned grid, const DevCtl* dc, const unsigned long long* cw, int32_t* W,
-                      const int32_t* L0, const int32_t* L1) {
-  LAUNCH(k_wcc_store, grid, st, dc, cw, W, L0, L1);
-}
-void launch_wcc_color(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, const int32_t* W,
-                      int seed, const int32_t* off, const int32_t* offe, uint32_t* V, int32_t* q0, const int32_t* L0,
-                      const int32_t* L1, CanonKeys K) {
-  LAUNCH(k_wcc_color, grid, st, dc, color, W, seed, off, offe, V, q0, L0, L1, K);
+void launch_wcc_color(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, const unsigned long long* cw,
+                      int32_t* W, int seed, const int32_t* off, const int32_t* offe, uint32_t* V, int32_t* q0,
+                      const int32_t* L0, const int32_t* L1, CanonKeys K) {
+  LAUNCH(k_wcc_color, grid, st, dc, color, cw, W, seed, off, offe, V, q0, L0, L1, K);
 }
 void launch_extract(hipStream_t st, unsigned grid, DevCtl* dc, int32_t* color, int32_t* label,
                     const uint32_t* V, int32_t* L0, int32_t* L1, CanonKeys K) {

@@ -755,6 +755,7 @@ struct Snap {
        w_fresh = false;
   size_t trace_len = 0;
   int32_t lv_hint[3] = {0, 0, 0}, it_hint = 0, trim_hint[2] = {0, 0};  // launch-batch hints: both branches start alike
+  int32_t fix_idx = 0, search_idx = 0;
 };
 
 int take_snap(scc_ctx* c, Snap& sn) {
@@ -785,6 +786,8 @@ int take_snap(scc_ctx* c, Snap& sn) {
   sn.trace_len = c->trace.size();
   for (int i = 0; i < 3; i++) sn.lv_hint[i] = c->lv_hint[i];
   sn.it_hint = c->it_hint;
+  sn.fix_idx = c->fix_idx;
+  sn.search_idx = c->search_idx;
   for (int i = 0; i < 2; i++) sn.trim_hint[i] = c->trim_hint[i];
   return SCC_OK;
 }
@@ -809,6 +812,8 @@ int restore_snap(scc_ctx* c, const Snap& sn) {
   c->trace.resize(sn.trace_len);
   for (int i = 0; i < 3; i++) c->lv_hint[i] = sn.lv_hint[i];
   c->it_hint = sn.it_hint;
+  c->fix_idx = sn.fix_idx;
+  c->search_idx = sn.search_idx;
   for (int i = 0; i < 2; i++) c->trim_hint[i] = sn.trim_hint[i];
   return hip_check(c, hipStreamSynchronize(c->stream), "restore sync");
 }
