@@ -152,16 +152,21 @@ static bool cut_buffers(scc_ctx* c) {
 // Copy the control block to the host and refresh the host copies of the live list.
 // pull_ctl in two halves, so that launches can be queued behind the copy
 // before the host waits for it (launches that only read the control block's
-// end state, or return at once once the loop it records is done).
+// end state, or return at once once the loop it records is done).  A context
+// without a readback event (the no-alloc decomposition creates none) waits for
+// the whole stream instead, which includes the copy.
 int ctl_readback_start(scc_ctx* c) {
   if (int rc = hip_check(c, hipMemcpyAsync(c->h_dc, c->dc, sizeof(DevCtl), hipMemcpyDeviceToHost, c->stream),
                          "control readback"))
     return rc;
+  if (!c->ev_rb) return SCC_OK;
   return hip_check(c, hipEventRecord(c->ev_rb, c->stream), "control readback event");
 }
 int ctl_readback_wait(scc_ctx* c) {
   int rc;
-  if ((rc = hip_check(c, hipEventSynchronize(c->ev_rb), "control readback wait"))) return rc;
+  if ((rc = hip_check(c, c->ev_rb ? hipEventSynchronize(c->ev_rb) : hipStreamSynchronize(c->stream),
+                      "control readback wait")))
+    return rc;
   if ((rc = hip_check(c, hipGetLastError(), "kernel launch"))) return rc;
   c->cur = c->h_dc->live.sel;
   c->n_live = (int64_t)c->h_dc->live.n;
