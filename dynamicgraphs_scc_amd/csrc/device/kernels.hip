@@ -1591,14 +1591,34 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_pull(DevCtl* __restrict__ dc, con
   }
 }
 
+// Local rounds of a push iteration, and the capacity of each of the two LDS
+// queues they alternate between.
+constexpr int WCC_LOCAL_ROUNDS = 8;
+constexpr int WCC_LOCAL_CAP = 2048;
+struct LocalQ {
+  __device__ static int32_t* buf0() {
+    __shared__ int32_t b[WCC_LOCAL_CAP];
+    return b;
+  }
+  __device__ static int32_t* buf1() {
+    __shared__ int32_t b[WCC_LOCAL_CAP];
+    return b;
+  }
+  __device__ static int* cnt() {
+    __shared__ int c[4];
+    return c;
+  }
+};
+
 // Later iterations.  While the frontier (vertices whose label dropped) holds
 // more than 1/wcc_dense of the live set (a tuning knob), an iteration is another pull sweep
 // over the whole live list; below that, every frontier vertex pushes its
 // label to its same-colour in-neighbours (reverse rows) with a 64-bit
-// atomicMin, and a neighbour whose label dropped joins the next frontier once
-// (claimed in the bitmap of this iteration's parity; each iteration clears
-// the claims of its own input frontier, so a bitmap is clean when it is
-// reused).  Ends when an iteration lowers nothing.  The fixpoint is the pull
+// atomicMin; a neighbour whose label dropped is pushed from again by the same
+// workgroup in its local rounds, or joins the next frontier once (claimed in
+// the bitmap of this iteration's parity; each iteration clears the claims of
+// its own input frontier, so a bitmap is clean when it is reused).  Ends when
+// an iteration leaves nothing to the next frontier.  The fixpoint is the pull
 // fixpoint: W[v] = min tag over v's same-colour forward-reachable set.
 __global__ __launch_bounds__(BLOCK) void k_wcc_push(DevCtl* __restrict__ dc, const int32_t* __restrict__ begin,
                                                     const int32_t* __restrict__ bend,
@@ -1643,10 +1663,27 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_push(DevCtl* __restrict__ dc, con
     wcc_pull_sweep<false>(LIVE_IN(dc, L0, L1), n_live, begin, bend, adj, cw, qout, &dc->lv.out_q, nullptr,
                           nullptr);
   } else {
-    // push label x to four in-neighbours u[t] (u[t] < 0: none); add[t] when
-    // the push lowered u[t]'s label and claimed it for the next frontier.
-    // Each round of loads or atomics is issued for all four together.
-    auto lower4 = [&](const int32_t (&u)[4], unsigned long long x, bool (&add)[4]) {
+    // Local rounds.  A vertex whose label a push lowered goes to this
+    // workgroup's LDS queue, and the workgroup pushes from that queue itself,
+    // round after round, up to WCC_LOCAL_ROUNDS rounds, before the launch
+    // ends: a label then travels several hops per launch instead of one (the
+    // frontier's tail is a long chain of small iterations).  What a full LDS
+    // queue or the last round leaves goes to the next launch's frontier, as a
+    // claim in Bout.  The fixpoint does not depend on the order of the
+    // relaxations: every vertex lowered is pushed again after its last
+    // lowering, here or by a later launch.
+    int32_t* lq0 = LocalQ::buf0();
+    int32_t* lq1 = LocalQ::buf1();
+    int* lc = LocalQ::cnt();  // [0],[1]: reserved in lq0/lq1; [2],[3]: their valid limits
+    if (threadIdx.x == 0) {
+      lc[0] = lc[1] = 0;
+      lc[2] = lc[3] = WCC_LOCAL_CAP;
+    }
+    __syncthreads();
+    // push label x to four in-neighbours u[t] (u[t] < 0: none); low[t] when
+    // the push lowered u[t]'s label.  Each round of loads or atomics is
+    // issued for all four together.
+    auto lower4 = [&](const int32_t (&u)[4], unsigned long long x, bool (&low)[4]) {
       unsigned long long y[4];
 #pragma unroll
       for (int t = 0; t < 4; t++) y[t] = u[t] >= 0 ? cw[u[t]] : 0ull;
@@ -1656,17 +1693,81 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_push(DevCtl* __restrict__ dc, con
       unsigned long long old[4];
 #pragma unroll
       for (int t = 0; t < 4; t++) old[t] = want[t] ? atomicMin(&cw[u[t]], x) : 0ull;
+#pragma unroll
+      for (int t = 0; t < 4; t++) low[t] = want[t] && old[t] > x;
+    };
+    // all lanes of a wave: claim the lowered vertices for the next launch's
+    // frontier (once each)
+    auto to_global = [&](const bool (&g)[4], const int32_t (&u)[4]) {
       uint32_t ob[4];
 #pragma unroll
-      for (int t = 0; t < 4; t++) {
-        want[t] = want[t] && old[t] > x;
-        ob[t] = want[t] ? atomicOr(&Bout[u[t] >> 5], 1u << (u[t] & 31)) : 0xFFFFFFFFu;
-      }
+      for (int t = 0; t < 4; t++) ob[t] = g[t] ? atomicOr(&Bout[u[t] >> 5], 1u << (u[t] & 31)) : 0xFFFFFFFFu;
 #pragma unroll
-      for (int t = 0; t < 4; t++) add[t] = want[t] && !((ob[t] >> (u[t] & 31)) & 1u);
+      for (int t = 0; t < 4; t++) Out::push(g[t] && !((ob[t] >> (u[t] & 31)) & 1u), u[t], qout, &dc->lv.out_q);
+    };
+    // all lanes of a wave: lowered vertices into the local queue `nx`, or,
+    // when it is full or `last`, to the global frontier
+    auto sink4 = [&](const bool (&low)[4], const int32_t (&u)[4], int nx, bool last) {
+      int32_t* lq = nx ? lq1 : lq0;
+      bool g[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        g[t] = low[t];
+        if (last) continue;
+        const uint64_t m = __ballot(low[t]);
+        if (m == 0) continue;
+        const int kk = __popcll(m);
+        int b = 0;
+        if (lane == 0) {
+          b = atomicAdd(&lc[nx], kk);
+          if (b + kk > WCC_LOCAL_CAP) {
+            atomicMin(&lc[2 + nx], b);
+            b = -1;
+          }
+        }
+        b = __shfl(b, 0);
+        if (b >= 0) {
+          if (low[t]) lq[b + lanes_below(m)] = u[t];
+          g[t] = false;
+        }
+      }
+      to_global(g, u);
+    };
+    // all lanes of a wave: push the labels of the lane's vertex k (act) to its
+    // in-neighbours
+    auto push_rows = [&](bool act, int32_t k, unsigned long long x, int32_t s, int32_t e, int nx, bool last) {
+      const int len = act ? e - s : 0;
+      const int lim = len < 32 ? len : 0;
+      const int wl = wave_max(lim);
+      for (int j = 0; j < wl; j += 4) {
+        int32_t u[4];
+        bool low[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) u[t] = j + t < lim ? radj[s + j + t] : -1;
+        lower4(u, x, low);
+        sink4(low, u, nx, last);
+      }
+      uint64_t hm = __ballot(len >= 32);
+      while (hm) {
+        const int L = __ffsll((unsigned long long)hm) - 1;
+        hm &= hm - 1;
+        const int32_t hs = __shfl(s, L), he = __shfl(e, L);
+        const unsigned long long hx = __shfl(x, L);
+        for (int32_t b = hs; b < he; b += 256) {
+          int32_t u[4];
+          bool low[4];
+#pragma unroll
+          for (int t = 0; t < 4; t++) {
+            const int32_t idx = b + t * 64 + lane;
+            u[t] = idx < he ? radj[idx] : -1;
+          }
+          lower4(u, hx, low);
+          sink4(low, u, nx, last);
+        }
+      }
     };
     for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < nq; base += (int64_t)gridDim.x * BLOCK) {
-      Out::maybe_flush(qout, &dc->lv.out_q, BLOCK);
+      Out::maybe_flush(qout, &dc->lv.out_q, 4 * BLOCK);
       const int64_t i = base + threadIdx.x;
       int32_t k = 0, s = 0, e = 0;
       unsigned long long x = 0;
@@ -1679,37 +1780,39 @@ __global__ __launch_bounds__(BLOCK) void k_wcc_push(DevCtl* __restrict__ dc, con
         e = rend[k];
         act = cw_hi(x) != (uint32_t)DONE;
       }
-      const int len = act ? e - s : 0;
-      const int lim = len < 32 ? len : 0;
-      const int wl = wave_max(lim);
-      for (int j = 0; j < wl; j += 4) {
-        int32_t u[4];
-        bool add[4];
-#pragma unroll
-        for (int t = 0; t < 4; t++) u[t] = j + t < lim ? radj[s + j + t] : -1;
-        lower4(u, x, add);
-#pragma unroll
-        for (int t = 0; t < 4; t++) Out::push(add[t], u[t], qout, &dc->lv.out_q);
+      push_rows(act, k, x, s, e, 0, WCC_LOCAL_ROUNDS == 0);
+    }
+    // rounds over the local queue: round r reads queue r & 1 and fills the
+    // other one; the last round sends what it lowers to the global frontier
+    for (int r = 0; r < WCC_LOCAL_ROUNDS; r++) {
+      const int cu = r & 1, nx = cu ^ 1;
+      __syncthreads();
+      const int nc = min(lc[cu], lc[2 + cu]);
+      if (nc == 0) break;  // (the same in every thread: read after the barrier)
+      const bool last = r + 1 == WCC_LOCAL_ROUNDS;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        lc[nx] = 0;
+        lc[2 + nx] = WCC_LOCAL_CAP;
       }
-      uint64_t hm = __ballot(len >= 32);
-      while (hm) {
-        const int L = __ffsll((unsigned long long)hm) - 1;
-        hm &= hm - 1;
-        const int32_t hs = __shfl(s, L), he = __shfl(e, L);
-        const unsigned long long hx = __shfl(x, L);
-        for (int32_t b = hs; b < he; b += 256) {
-          int32_t u[4];
-          bool add[4];
-#pragma unroll
-          for (int t = 0; t < 4; t++) {
-            const int32_t idx = b + t * 64 + lane;
-            u[t] = idx < he ? radj[idx] : -1;
-          }
-          lower4(u, hx, add);
-#pragma unroll
-          for (int t = 0; t < 4; t++) Out::push(add[t], u[t], qout, &dc->lv.out_q);
+      __syncthreads();
+      const int32_t* lq = cu ? lq1 : lq0;
+      for (int b0 = 0; b0 < nc; b0 += BLOCK) {
+        Out::maybe_flush(qout, &dc->lv.out_q, 4 * BLOCK);
+        const int ii = b0 + (int)threadIdx.x;
+        int32_t k = 0, s = 0, e = 0;
+        unsigned long long x = 0;
+        bool act = false;
+        if (ii < nc) {
+          k = lq[ii];
+          x = cw[k];
+          s = rbegin[k];
+          e = rend[k];
+          act = cw_hi(x) != (uint32_t)DONE;
         }
+        push_rows(act, k, x, s, e, nx, last);
       }
+      if (threadIdx.x == 0) lc[cu] = 0;  // (consumed; a later break sees it empty)
     }
     Out::flush(qout, &dc->lv.out_q);
   }
