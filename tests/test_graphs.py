@@ -82,7 +82,7 @@ def test_rmat_deterministic_and_quadrants():
     assert raw.num_edges == 16 * 65536
 
 
-def test_rmat16_sizes_match_survey_shape():
+def test_rmat16_sizes_after_dedup():
     g = graphs.rmat(16, 16, seed=1)
     assert g.num_nodes == 65536
     assert 950_000 < g.num_edges < 960_000  # ~0.91 kept after dedup
