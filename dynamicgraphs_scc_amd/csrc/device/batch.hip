@@ -119,6 +119,9 @@ int start_run(scc_ctx* c) {
   c->multi_colors = true;
   c->colors_split = true;
   c->fresh = false;
+  // every vertex is live and no edge crosses graphs (scc_batch_set checks),
+  // so the first trim sweep needs only the row lengths
+  c->rows_decide = true;
   return hip_check(c, hipGetLastError(), "batch colours");
 }
 

@@ -518,6 +518,8 @@ struct scc_ctx {
   int32_t update_policy = 0;              // pivot policy of dynamic recomputes
   bool colors_split = false;              // colours other than -1/-2 exist (after FB/WCC/set)
   bool fresh = false;                     // state just reset: all colours -1, nothing done
+  bool rows_decide = false;               // just reset, caller colours that no edge crosses: the next
+                                          // trim sweep may decide on row lengths alone
   int32_t lv_hint[3] = {8, 8, 8};         // level launches per search batch (fw, bw, both)
   int32_t it_hint = 4;                    // WCC iterations per batch (the last fixpoint's)
   int32_t it_hints[32] = {};              // the same per fixpoint ordinal of a decomposition (0: none yet)

@@ -1253,6 +1253,7 @@ extern "C" int scc_update(scc_ctx* c, const int32_t* ins_src, const int32_t* ins
   c->multi_colors = false;
   c->classes = false;
   c->fresh = false;
+  c->rows_decide = false;
   {
     const int64_t words = ((int64_t)c->n + 31) / 32;
     launch_trim1_step(s, grid_for(c->n, (unsigned)c->tune.grid_cap), c->dc, gp(c), c->color, c->label, c->live[0],

@@ -343,6 +343,7 @@ static void reset_host_state(scc_ctx* c) {
   c->trace.clear();
   c->round_idx = -1;
   c->fresh = true;
+  c->rows_decide = false;
   c->cur = 0;
   c->n_live = c->n;
   c->pivot_ready = false;
@@ -365,6 +366,7 @@ int reset_state(scc_ctx* c) {
 
 int do_trim_compact(scc_ctx* c) {
   c->fresh = false;
+  c->rows_decide = false;
   if (c->n_live == 0) return SCC_OK;
   int rc;
   if ((rc = zero_ctr(c))) return rc;
@@ -397,10 +399,11 @@ int do_trim1(scc_ctx* c, int64_t stop, int64_t* trimmed) {
   for (int guard = 0;; guard++) {
     if ((rc = check_cancel(c))) return rc;
     for (int i = 0; i < batch; i++) {
-      const int fresh = c->fresh ? (fused && guard == 0 && i == 0 ? 2 : 1) : 0;
+      const int fresh = c->fresh ? (fused && guard == 0 && i == 0 ? 2 : 1) : (c->rows_decide ? 1 : 0);
       launch_trim1_step(c->stream, grid, c->dc, gp(c), c->color, c->label, c->live[0], c->live[1], fresh,
                         (long long)stop, epoch, c->pkey, c->vf, 2 * words);
       c->fresh = false;
+      c->rows_decide = false;
     }
     if ((rc = pull_ctl(c))) return rc;
     if (c->h_dc->loop.done) break;
@@ -417,6 +420,7 @@ int do_trim1(scc_ctx* c, int64_t stop, int64_t* trimmed) {
 
 int do_trim2(scc_ctx* c, int64_t* removed) {
   c->fresh = false;
+  c->rows_decide = false;
   c->w_fresh = false;
   if (removed) *removed = 0;
   if (c->n_live == 0) return SCC_OK;
@@ -553,6 +557,7 @@ int traverse_from(scc_ctx* c, bool fwd, bool multi, const int32_t* seeds,
 int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int64_t* scc,
                 int32_t* pivot_out, int64_t* levels, bool timed, int64_t* sets) {
   c->fresh = false;
+  c->rows_decide = false;
   c->w_fresh = false;
   int rc;
   if (scc) *scc = 0;
@@ -618,6 +623,7 @@ int do_fb_round(scc_ctx* c, int policy, bool multi, int32_t pivot_override, int6
 // classification behind them.
 static int fb_continue(scc_ctx* c, int64_t* scc, int64_t* levels) {
   c->fresh = false;
+  c->rows_decide = false;
   PhaseRec pr(c, SCC_PHASE_FB_GLOBAL);
   const unsigned grid = grid_for(c->n_live, (unsigned)c->tune.grid_cap);
   auto post = [&]() {
@@ -641,6 +647,7 @@ static int fb_continue(scc_ctx* c, int64_t* scc, int64_t* levels) {
 // (rows and reverse rows swap roles).
 static int wcc_fixpoint(scc_ctx* c, bool seed, bool in_dir = false) {
   c->fresh = false;
+  c->rows_decide = false;
   int rc;
   const unsigned grid = grid_for(c->n_live, (unsigned)c->tune.grid_cap);
   const int64_t words = ((int64_t)c->n + 31) / 32;
@@ -879,6 +886,7 @@ static int run_chain(scc_ctx* c, const scc_run_opts& o, scc_run_stats* st, int* 
       launch_trim1_step(c->stream, grid_at(gate - 1), c->dc, gp(c), c->color, c->label, c->live[0], c->live[1], fresh,
                         stop, epoch, c->pkey, c->vf, 2 * words);
       c->fresh = false;
+      c->rows_decide = false;
     }
     return epoch;
   };
@@ -1292,6 +1300,7 @@ int scc_state_set_colors(scc_ctx* c, const int32_t* colors) {
   for (int32_t v = 0; v < c->n && !c->multi_colors; v++) c->multi_colors = colors[v] != -1;
   c->colors_split = c->multi_colors;
   c->fresh = false;
+  c->rows_decide = false;
   if (c->relabeled) return layout_colors_to_internal(c, colors);
   if ((rc = hip_check(c, hipMemcpyAsync(c->color, colors, 4ull * c->n, hipMemcpyHostToDevice, c->stream),
                       "colours upload")))

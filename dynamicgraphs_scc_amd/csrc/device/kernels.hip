@@ -272,8 +272,9 @@ __global__ __launch_bounds__(BLOCK) void k_trim1_step(DevCtl* __restrict__ dc, G
     for (int64_t w = (int64_t)blockIdx.x * BLOCK + threadIdx.x; w < zwords; w += (int64_t)gridDim.x * BLOCK)
       zbits[w] = 0u;
   if (fresh) {
-    // fresh state: the live list is 0..n-1, every colour is -1 and none is
-    // done, so only empty rows decide this sweep (a row holding nothing but
+    // fresh state: the live list is 0..n-1, none is done and every edge joins
+    // two vertices of one colour (all colours -1, or batch colours that no
+    // edge crosses), so only empty rows decide this sweep (a row holding nothing but
     // a self-loop survives it and falls to the next sweep, which reads
     // colours; trimming later is always safe, trimming early never happens).
     // Four vertices per thread and step (base + t * BLOCK + thread: loads
