@@ -289,6 +289,49 @@ def test_phase_wcc_roots_equal_oracle(engine):
     assert parts == int((W[live] & 0x1FFFFFFF == np.flatnonzero(live)).sum())
 
 
+def test_phase_wcc_long_chains_and_hub_rows(engine):
+    """Label propagation whose labels travel far past one push launch's local
+    rounds (three chains of 2-cycles, 20,000 vertices each, ids permuted, each
+    chain reaching the next one way) and whose pushes cross long in-rows (three
+    stars of 3,000 spokes around a hub, each star reaching a chain one way):
+    the WCC labels equal the oracle's."""
+    rng = np.random.default_rng(21)
+    L, S = 20_000, 3_000
+    n = 3 * L + 3 * (S + 1)
+    perm = rng.permutation(n).astype(np.int32)
+    chains = [perm[c * L:(c + 1) * L] for c in range(3)]
+    src, dst = [], []
+    for ids in chains:
+        src += [ids[:-1], ids[1:]]
+        dst += [ids[1:], ids[:-1]]
+    for c in range(2):
+        src.append(chains[c][rng.integers(0, L, 20)])
+        dst.append(chains[c + 1][rng.integers(0, L, 20)])
+    base = 3 * L
+    for s in range(3):
+        hub = perm[base + s * (S + 1)]
+        spokes = perm[base + s * (S + 1) + 1:base + (s + 1) * (S + 1)]
+        src += [spokes, np.full(S, hub, np.int32), spokes[:5]]
+        dst += [np.full(S, hub, np.int32), spokes, chains[s][rng.integers(0, L, 5)]]
+    g = graphs.from_edges(n, np.concatenate(src), np.concatenate(dst), dedup=True)
+    print(f"  chains and stars n={g.num_nodes} m={g.num_edges}", flush=True)
+    engine.upload(g)
+    engine.repeat_global_trim1(0)
+    parts = engine.do_global_wcc()
+    gc = engine.colors()
+    col = np.full(g.num_nodes, -1, np.int32)
+    lab = np.full(g.num_nodes, -1, np.int32)
+    oracle.trim1_fixpoint(g, col, lab)
+    W = oracle.wcc(g, col)
+    live = col != -2
+    assert live.sum() == n
+    assert np.array_equal(gc[live], W[live] & 0x1FFFFFFF)
+    assert parts == int((W[live] & 0x1FFFFFFF == np.flatnonzero(live)).sum())
+    engine.reset()
+    engine.run_method2()
+    assert np.array_equal(engine.labels(canonical=True), oracle.scc_canonical(g, 8))
+
+
 def test_phase_sequence_matches_method2(engine):
     """Calling the scc.h-named phases in method-2 order gives the same labels."""
     g = graphs.rmat(13, 4, seed=15)
